@@ -1,0 +1,140 @@
+/*
+ * ves3d_hip.h -- C ABI of the MI355X (gfx950) implementation of the Ves3D
+ * per-time-step hot path.
+ *
+ * Conventions
+ *   - All pointers are device pointers unless a parameter says "host".
+ *   - Sizes are element counts (not bytes) unless the name ends in _bytes.
+ *   - Every entry point returns 0 on success or a code from the reference's
+ *     Error_t numbering (include/Error.h:46-67 of lu1and10/ves3d).
+ *   - `stream` is a hipStream_t passed as an opaque pointer (NULL = default
+ *     stream).  Work is enqueued on that stream and the call returns without
+ *     synchronising unless it says otherwise.
+ *   - _f64 entries compute in FP64.  _f32 entries take FP32 buffers and compute
+ *     in FP32 (the reference's single-precision build).  _f32c entries take
+ *     FP64 buffers, compute the pair interactions in FP32 with compensated
+ *     (TwoSum) accumulation of per-tile partial sums, and write FP64 results.
+ *
+ * Layouts follow the reference:
+ *   AxisMajor vesicle fields  [ves][xyz][stride]       (Device<> kernels)
+ *   point-major arrays        [point][xyz]             (StokesAlltoAll, PVFMMEval)
+ */
+#ifndef VES3D_HIP_H
+#define VES3D_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *ves3d_stream_t;
+
+/* Error_t values (include/Error.h:46-67). */
+enum {
+  VES3D_SUCCESS = 0,
+  VES3D_UNKNOWN_ERROR = 1,
+  VES3D_ARITHMETIC_ERROR = 2,
+  VES3D_MEMORY_ERROR = 5,
+  VES3D_NOT_IMPLEMENTED = 7,
+  VES3D_INVALID_PARAMETER = 10,
+  VES3D_INVALID_DEVICE = 20,
+  VES3D_ACCURACY_ERROR = 30,
+  VES3D_DIVERGENCE_ERROR = 31
+};
+
+/* ---- device and memory (Device<DT> ctor, Malloc/Free/Memcpy/Memset:
+ *      include/Device.h:95-119) ------------------------------------------ */
+
+/* Selects the device and checks that it is a gfx950 part.  Replaces the
+ * Device<GPU> constructor (src/DeviceGPU.cc:1-21). */
+int ves3d_hip_init(int device_id);
+
+/* Writes the device name, CU count and clock (MHz) into host buffers. */
+int ves3d_hip_device_info(char *name_host, size_t name_len, int *num_cu_host,
+                          int *clock_mhz_host);
+
+int ves3d_hip_malloc(void **ptr_host, size_t nbytes);
+int ves3d_hip_free(void *ptr);
+/* kind: 0 host->host, 1 host->device, 2 device->host, 3 device->device
+ * (Device::MemcpyKind, include/Device.h:87-88).  Synchronous like the
+ * reference's Device<GPU>::Memcpy. */
+int ves3d_hip_memcpy(void *dst, const void *src, size_t nbytes, int kind);
+int ves3d_hip_memset(void *ptr, int value, size_t nbytes);
+int ves3d_hip_synchronize(void);
+
+/* ---- direct Stokes sums ------------------------------------------------ */
+
+/* Device<DT>::DirectStokes (include/Device.h:205-209).  For each vesicle vt
+ * and target index t in [trg_idx_head, trg_idx_tail):
+ *   pot = 1/(8 pi) sum_s [ (w_s d_s)/r + ((w_s d_s).r) r / r^3 ],  r = src-trg,
+ * over the sources of the same vesicle; qw == NULL means w_s = 1
+ * (src/DeviceCPU.cc:513-541).  src,den: [n_surfs][3][src_stride];
+ * trg,pot: [n_surfs][3][trg_stride]; qw: [src_stride]. Coincident points
+ * (r = 0) contribute nothing (src/CPUKernels.cc:37). */
+int ves3d_hip_direct_stokes_f64(const double *src, const double *den,
+                                const double *qw, size_t src_stride,
+                                size_t trg_stride, size_t n_surfs,
+                                const double *trg, size_t trg_idx_head,
+                                size_t trg_idx_tail, double *pot,
+                                ves3d_stream_t stream);
+int ves3d_hip_direct_stokes_f32(const float *src, const float *den,
+                                const float *qw, size_t src_stride,
+                                size_t trg_stride, size_t n_surfs,
+                                const float *trg, size_t trg_idx_head,
+                                size_t trg_idx_tail, float *pot,
+                                ves3d_stream_t stream);
+
+/* Device<DT>::DirectStokesDoubleLayer (include/Device.h:211-215):
+ *   pot = -3/(4 pi) sum_s w_s (r.n_s)(r.d_s) r / r^5,  r = src - trg
+ * (src/CPUKernels.cc:1059-1136).  norm: [n_surfs][3][src_stride]. */
+int ves3d_hip_direct_stokes_dl_f64(const double *src, const double *norm,
+                                   const double *den, const double *qw,
+                                   size_t src_stride, size_t trg_stride,
+                                   size_t n_surfs, const double *trg,
+                                   size_t trg_idx_head, size_t trg_idx_tail,
+                                   double *pot, ves3d_stream_t stream);
+int ves3d_hip_direct_stokes_dl_f32(const float *src, const float *norm,
+                                   const float *den, const float *qw,
+                                   size_t src_stride, size_t trg_stride,
+                                   size_t n_surfs, const float *trg,
+                                   size_t trg_idx_head, size_t trg_idx_tail,
+                                   float *pot, ves3d_stream_t stream);
+
+/* StokesAlltoAll (include/CPUKernels.h:46-49, src/CPUKernels.cc:966-1054),
+ * the VesInteraction<T>::InteractionFun_t without PVFMM
+ * (include/VesInteraction.h:21): single-layer sum over all np point-major
+ * sources/targets, coincident pairs skipped. src, den, pot: [np][3]. */
+int ves3d_hip_stokes_all2all_f64(const double *src, const double *den,
+                                 size_t np, double *pot, ves3d_stream_t stream);
+int ves3d_hip_stokes_all2all_f32c(const double *src, const double *den,
+                                  size_t np, double *pot,
+                                  ves3d_stream_t stream);
+
+/* Direct-sum stand-in for PVFMMEval(src_pos, sl_den, dl_den, n_src, trg_pos,
+ * trg_vel, n_trg, ctx, setup) (include/PVFMMInterface.h:53), with the pair
+ * kernels of src/PVFMMInterface.cc:161-238 and :284-369:
+ *   trg_vel = (1/8pi) sum [f/r + (f.d)d/r^3] + (3/4pi) sum (d.n)(d.g)d/r^5,
+ * d = trg - src, pairs with |d|^2 <= 128*eps(double) skipped.
+ * src_pos: [n_src][3]; sl_den: [n_src][3] or NULL; dl_den: [n_src][6] laid
+ * out [g (3), n (3)] per source (qforce_double, src/StokesVelocity.cc:351-372)
+ * or NULL; trg_pos, trg_vel: [n_trg][3] (trg_vel is overwritten).
+ * Grouping (an extension for the vesicle far field): when src_per_group and
+ * trg_per_group are both > 0, sources and targets are in n_src/src_per_group
+ * consecutive groups (vesicles) and pairs inside one group are skipped --
+ * the own-vesicle part that src/NearSingular.cc:1143-1212 subtracts again. */
+int ves3d_hip_direct_eval_f64(const double *src_pos, const double *sl_den,
+                              const double *dl_den, size_t n_src,
+                              const double *trg_pos, double *trg_vel,
+                              size_t n_trg, size_t src_per_group,
+                              size_t trg_per_group, ves3d_stream_t stream);
+int ves3d_hip_direct_eval_f32c(const double *src_pos, const double *sl_den,
+                               const double *dl_den, size_t n_src,
+                               const double *trg_pos, double *trg_vel,
+                               size_t n_trg, size_t src_per_group,
+                               size_t trg_per_group, ves3d_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VES3D_HIP_H */

@@ -1,0 +1,121 @@
+"""ves3d_amd -- MI355X (gfx950) implementation of the Ves3D per-time-step hot path.
+
+The compute lives in ``lib/libves3d_hip.so`` (hand-written HIP kernels behind
+the C ABI declared in ``include/ves3d_hip.h``).  This module only loads that
+library and gives Python callers typed access to it; device buffers are
+passed as raw pointers (torch tensors on ``cuda`` are accepted and their
+``data_ptr()`` is used).  There is no CPU fallback: if the library is missing
+or fails to load, every call raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libves3d_hip.so")
+
+c_int = ctypes.c_int
+c_size = ctypes.c_size_t
+c_ptr = ctypes.c_void_p
+c_double = ctypes.c_double
+
+ERRORS = {
+    0: "Success", 1: "UnknownError", 2: "ArithmeticError", 5: "MemoryError",
+    7: "NotImplementedError", 10: "InvalidParameterError", 20: "InvalidDeviceError",
+    30: "AccuracyError", 31: "DivergenceError",
+}
+
+
+class Ves3dError(RuntimeError):
+    def __init__(self, fn, code):
+        super().__init__("%s returned %d (%s)" % (fn, code, ERRORS.get(code, "?")))
+        self.code = code
+
+
+# name -> argtypes ; all return int (Error_t)
+_SIGS = {}
+
+
+def _sig(name, *args):
+    _SIGS[name] = args
+
+
+P = c_ptr
+S = c_size
+_sig("ves3d_hip_init", c_int)
+_sig("ves3d_hip_device_info", ctypes.c_char_p, S, ctypes.POINTER(c_int), ctypes.POINTER(c_int))
+_sig("ves3d_hip_malloc", ctypes.POINTER(c_ptr), S)
+_sig("ves3d_hip_free", P)
+_sig("ves3d_hip_memcpy", P, P, S, c_int)
+_sig("ves3d_hip_memset", P, c_int, S)
+_sig("ves3d_hip_synchronize")
+for _t in ("f64", "f32"):
+    _sig("ves3d_hip_direct_stokes_" + _t, P, P, P, S, S, S, P, S, S, P, P)
+    _sig("ves3d_hip_direct_stokes_dl_" + _t, P, P, P, P, S, S, S, P, S, S, P, P)
+_sig("ves3d_hip_stokes_all2all_f64", P, P, S, P, P)
+_sig("ves3d_hip_stokes_all2all_f32c", P, P, S, P, P)
+_sig("ves3d_hip_direct_eval_f64", P, P, P, S, P, P, S, S, S, P)
+_sig("ves3d_hip_direct_eval_f32c", P, P, P, S, P, P, S, S, S, P)
+
+_lib = None
+
+
+def load():
+    """Loads the HIP library (once) and attaches the C signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("ves3d_amd: %s is not built (run python -m ves3d_amd.build)" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = list(args)
+        fn.restype = c_int
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    """Names of the C entry points this module binds."""
+    return sorted(_SIGS)
+
+
+def ptr(x):
+    """Raw pointer of a torch tensor / int / None."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def stream_ptr(stream=None):
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+def call(name, *args):
+    """Calls a C entry point and raises Ves3dError on a non-zero status."""
+    fn = getattr(load(), name)
+    rc = fn(*args)
+    if rc != 0:
+        raise Ves3dError(name, rc)
+    return rc
+
+
+def init(device=0):
+    call("ves3d_hip_init", device)
+
+
+def device_info():
+    name = ctypes.create_string_buffer(256)
+    cu, clk = c_int(0), c_int(0)
+    call("ves3d_hip_device_info", name, 256, ctypes.byref(cu), ctypes.byref(clk))
+    return name.value.decode(), cu.value, clk.value
+
+
+from . import stokes  # noqa: E402,F401

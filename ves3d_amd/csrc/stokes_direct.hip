@@ -1,0 +1,515 @@
+// stokes_direct.hip -- direct Stokes single/double-layer sums on gfx950.
+//
+// Two kernel families:
+//
+//  1. k_far_*: the all-pairs / far-field sum that stands in for the FMM
+//     (StokesAlltoAll, src/CPUKernels.cc:966-1054, and the PVFMM pair kernels
+//     src/PVFMMInterface.cc:161-238, :284-369).  One target per lane, 256
+//     targets per workgroup; sources are staged through LDS in tiles of 256
+//     and read back as wave-wide broadcasts.  The source range is cut into
+//     `nsplit` slices; the slice index is the fastest-varying part of the
+//     block index, so with nsplit a multiple of 8 every XCD (blocks are dealt
+//     round-robin over the 8 XCDs) streams only its own slice of sources
+//     through its L2.  Each slice writes an FP64 partial, and k_sum_slices adds
+//     the partials in a fixed order (deterministic, no atomics).
+//     The FP32 ("f32c") form stages each source tile relative to an FP64
+//     anchor (the tile's first point) and shifts the lane's target by the same
+//     anchor in FP64 before rounding, so coordinate differences keep FP32
+//     relative accuracy independent of the box size; pair terms are summed in
+//     FP32 over 64 sources and merged into an FP32 (hi, lo) TwoSum pair.
+//
+//  2. k_vesicle_*: the per-vesicle Device<>::DirectStokes /
+//     DirectStokesDoubleLayer sums (src/CPUKernels.cc:8-118, :1059-1136) on
+//     AxisMajor [ves][xyz][stride] data; one wave per 64 targets of one
+//     vesicle, sources of that vesicle staged through LDS.
+#include "common.hpp"
+#include "runtime.hpp"
+
+namespace v3d {
+
+constexpr int FF_BLOCK = 256;
+constexpr int FF_TILE = 256;
+constexpr int FF_SUB = 64;  // FP32 form: TwoSum merge period (sources)
+
+struct __align__(16) SrcRec64 {
+  double x, y, z, _p0;
+  double fx, fy, fz, _p1;
+  double gx, gy, gz, _p2;
+  double nx, ny, nz, _p3;
+};
+struct __align__(16) SrcRec32 {
+  float x, y, z, _p0;
+  float fx, fy, fz, _p1;
+  float gx, gy, gz, _p2;
+  float nx, ny, nz, _p3;
+};
+
+// Source range of one slice: whole groups when grouped, else an even split.
+__device__ __forceinline__ void slice_range(long slice, int nsplit, long n_src,
+                                            long spg, long n_groups, long &g0,
+                                            long &g1) {
+  if (spg > 0) {
+    g0 = slice * n_groups / nsplit;
+    g1 = (slice + 1) * n_groups / nsplit;
+  } else {
+    g0 = 0;
+    g1 = 1;
+  }
+}
+
+template <bool SL, bool DL>
+__global__ __launch_bounds__(FF_BLOCK) void k_far_f64(
+    const double *__restrict__ spos, const double *__restrict__ sld,
+    const double *__restrict__ dld, long n_src, const double *__restrict__ tpos,
+    long n_trg, long spg, long tpg, long n_groups, int nsplit, double eps2,
+    double csl, double cdl, double *__restrict__ part) {
+  __shared__ SrcRec64 sh[FF_TILE];
+  const long slice = blockIdx.x % nsplit;
+  const long tb = blockIdx.x / nsplit;
+  const long t = tb * FF_BLOCK + threadIdx.x;
+  const bool tvalid = t < n_trg;
+  double tx = 0, ty = 0, tz = 0;
+  if (tvalid) {
+    tx = tpos[3 * t];
+    ty = tpos[3 * t + 1];
+    tz = tpos[3 * t + 2];
+  }
+  const bool grouped = spg > 0 && tpg > 0;
+  const long my_g = (grouped && tvalid) ? t / tpg : -1;
+  const long tfirst = tb * FF_BLOCK;
+  const long tlast = min(tfirst + FF_BLOCK, n_trg) - 1;
+
+  double Sx = 0, Sy = 0, Sz = 0, Qx = 0, Qy = 0, Qz = 0;
+  long g0, g1;
+  slice_range(slice, nsplit, n_src, grouped ? spg : 0, n_groups, g0, g1);
+  for (long g = g0; g < g1; ++g) {
+    long sb, se;
+    if (grouped) {
+      if (tfirst / tpg == g && tlast / tpg == g) continue;  // block-uniform skip
+      sb = g * spg;
+      se = sb + spg;
+    } else {
+      sb = slice * n_src / nsplit;
+      se = (slice + 1) * n_src / nsplit;
+    }
+    const bool lane_on = tvalid && (my_g != g);
+    for (long c0 = sb; c0 < se; c0 += FF_TILE) {
+      const int cnt = (int)min((long)FF_TILE, se - c0);
+      __syncthreads();
+      for (int i = threadIdx.x; i < cnt; i += FF_BLOCK) {
+        const long s = c0 + i;
+        SrcRec64 r;
+        r.x = spos[3 * s];
+        r.y = spos[3 * s + 1];
+        r.z = spos[3 * s + 2];
+        if (SL) {
+          r.fx = sld[3 * s];
+          r.fy = sld[3 * s + 1];
+          r.fz = sld[3 * s + 2];
+        }
+        if (DL) {
+          r.gx = dld[6 * s];
+          r.gy = dld[6 * s + 1];
+          r.gz = dld[6 * s + 2];
+          r.nx = dld[6 * s + 3];
+          r.ny = dld[6 * s + 4];
+          r.nz = dld[6 * s + 5];
+        }
+        sh[i] = r;
+      }
+      __syncthreads();
+      if (lane_on) {
+#pragma unroll 4
+        for (int j = 0; j < cnt; ++j) {
+          const double dx = tx - sh[j].x, dy = ty - sh[j].y, dz = tz - sh[j].z;
+          const double r2 = dx * dx + dy * dy + dz * dz;
+          const double ri = (r2 > eps2) ? rsqrt(r2) : 0.0;
+          const double ri2 = ri * ri;
+          if (SL) {
+            const double ip =
+                (sh[j].fx * dx + sh[j].fy * dy + sh[j].fz * dz) * ri2;
+            Sx += ri * (sh[j].fx + dx * ip);
+            Sy += ri * (sh[j].fy + dy * ip);
+            Sz += ri * (sh[j].fz + dz * ip);
+          }
+          if (DL) {
+            const double p = (sh[j].nx * dx + sh[j].ny * dy + sh[j].nz * dz) *
+                             (sh[j].gx * dx + sh[j].gy * dy + sh[j].gz * dz) *
+                             (ri2 * ri2 * ri);
+            Qx += dx * p;
+            Qy += dy * p;
+            Qz += dz * p;
+          }
+        }
+      }
+    }
+  }
+  if (tvalid) {
+    double *o = part + (slice * n_trg + t) * 3;
+    o[0] = csl * Sx + cdl * Qx;
+    o[1] = csl * Sy + cdl * Qy;
+    o[2] = csl * Sz + cdl * Qz;
+  }
+}
+
+template <bool SL, bool DL>
+__global__ __launch_bounds__(FF_BLOCK) void k_far_f32c(
+    const double *__restrict__ spos, const double *__restrict__ sld,
+    const double *__restrict__ dld, long n_src, const double *__restrict__ tpos,
+    long n_trg, long spg, long tpg, long n_groups, int nsplit, float eps2,
+    double csl, double cdl, double *__restrict__ part) {
+  __shared__ SrcRec32 sh[FF_TILE];
+  const long slice = blockIdx.x % nsplit;
+  const long tb = blockIdx.x / nsplit;
+  const long t = tb * FF_BLOCK + threadIdx.x;
+  const bool tvalid = t < n_trg;
+  double tx = 0, ty = 0, tz = 0;
+  if (tvalid) {
+    tx = tpos[3 * t];
+    ty = tpos[3 * t + 1];
+    tz = tpos[3 * t + 2];
+  }
+  const bool grouped = spg > 0 && tpg > 0;
+  const long my_g = (grouped && tvalid) ? t / tpg : -1;
+  const long tfirst = tb * FF_BLOCK;
+  const long tlast = min(tfirst + FF_BLOCK, n_trg) - 1;
+
+  // (hi, lo) compensated running sums
+  float Sh[3] = {0, 0, 0}, Sl[3] = {0, 0, 0};
+  float Qh[3] = {0, 0, 0}, Ql[3] = {0, 0, 0};
+  long g0, g1;
+  slice_range(slice, nsplit, n_src, grouped ? spg : 0, n_groups, g0, g1);
+  for (long g = g0; g < g1; ++g) {
+    long sb, se;
+    if (grouped) {
+      if (tfirst / tpg == g && tlast / tpg == g) continue;
+      sb = g * spg;
+      se = sb + spg;
+    } else {
+      sb = slice * n_src / nsplit;
+      se = (slice + 1) * n_src / nsplit;
+    }
+    const bool lane_on = tvalid && (my_g != g);
+    for (long c0 = sb; c0 < se; c0 += FF_TILE) {
+      const int cnt = (int)min((long)FF_TILE, se - c0);
+      __syncthreads();
+      const double ax = spos[3 * c0], ay = spos[3 * c0 + 1], az = spos[3 * c0 + 2];
+      for (int i = threadIdx.x; i < cnt; i += FF_BLOCK) {
+        const long s = c0 + i;
+        SrcRec32 r;
+        r.x = (float)(spos[3 * s] - ax);
+        r.y = (float)(spos[3 * s + 1] - ay);
+        r.z = (float)(spos[3 * s + 2] - az);
+        if (SL) {
+          r.fx = (float)sld[3 * s];
+          r.fy = (float)sld[3 * s + 1];
+          r.fz = (float)sld[3 * s + 2];
+        }
+        if (DL) {
+          r.gx = (float)dld[6 * s];
+          r.gy = (float)dld[6 * s + 1];
+          r.gz = (float)dld[6 * s + 2];
+          r.nx = (float)dld[6 * s + 3];
+          r.ny = (float)dld[6 * s + 4];
+          r.nz = (float)dld[6 * s + 5];
+        }
+        sh[i] = r;
+      }
+      __syncthreads();
+      if (lane_on) {
+        const float lx = (float)(tx - ax), ly = (float)(ty - ay), lz = (float)(tz - az);
+        for (int j0 = 0; j0 < cnt; j0 += FF_SUB) {
+          const int j1 = min(cnt, j0 + FF_SUB);
+          float sx = 0, sy = 0, sz = 0, qx = 0, qy = 0, qz = 0;
+#pragma unroll 4
+          for (int j = j0; j < j1; ++j) {
+            const float dx = lx - sh[j].x, dy = ly - sh[j].y, dz = lz - sh[j].z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            const float ri = (r2 > eps2) ? __frsqrt_rn(r2) : 0.0f;
+            const float ri2 = ri * ri;
+            if (SL) {
+              const float ip = (sh[j].fx * dx + sh[j].fy * dy + sh[j].fz * dz) * ri2;
+              sx += ri * (sh[j].fx + dx * ip);
+              sy += ri * (sh[j].fy + dy * ip);
+              sz += ri * (sh[j].fz + dz * ip);
+            }
+            if (DL) {
+              const float p = (sh[j].nx * dx + sh[j].ny * dy + sh[j].nz * dz) *
+                              (sh[j].gx * dx + sh[j].gy * dy + sh[j].gz * dz) *
+                              (ri2 * ri2 * ri);
+              qx += dx * p;
+              qy += dy * p;
+              qz += dz * p;
+            }
+          }
+          float s_, e_;
+          if (SL) {
+            two_sum(Sh[0], sx, s_, e_); Sh[0] = s_; Sl[0] += e_;
+            two_sum(Sh[1], sy, s_, e_); Sh[1] = s_; Sl[1] += e_;
+            two_sum(Sh[2], sz, s_, e_); Sh[2] = s_; Sl[2] += e_;
+          }
+          if (DL) {
+            two_sum(Qh[0], qx, s_, e_); Qh[0] = s_; Ql[0] += e_;
+            two_sum(Qh[1], qy, s_, e_); Qh[1] = s_; Ql[1] += e_;
+            two_sum(Qh[2], qz, s_, e_); Qh[2] = s_; Ql[2] += e_;
+          }
+        }
+      }
+    }
+  }
+  if (tvalid) {
+    double *o = part + (slice * n_trg + t) * 3;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      o[c] = csl * ((double)Sh[c] + (double)Sl[c]) + cdl * ((double)Qh[c] + (double)Ql[c]);
+  }
+}
+
+__global__ void k_sum_slices(const double *__restrict__ part, int nsplit, long n,
+                             double *__restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0;
+  for (int k = 0; k < nsplit; ++k) s += part[(long)k * n + i];
+  out[i] = s;
+}
+
+// Number of source slices: a multiple of 8 (one per XCD), raised until the
+// grid holds at least ~16 waves per SIMD of the 256-CU part.
+static int choose_nsplit(long n_trg, long n_src, long n_groups) {
+  const long tblocks = (n_trg + FF_BLOCK - 1) / FF_BLOCK;
+  int ns = 8;
+  while (ns < 64 && tblocks * ns * (FF_BLOCK / 64) < 16 * 1024) ns *= 2;
+  long cap = n_groups > 0 ? n_groups : (n_src + FF_TILE - 1) / FF_TILE;
+  if (cap < 1) cap = 1;
+  if (ns > cap) ns = (int)cap;
+  return ns;
+}
+
+template <bool F32, bool SL, bool DL>
+static void launch_far(dim3 grid, hipStream_t st, const double *spos,
+                       const double *sld, const double *dld, long n_src,
+                       const double *tpos, long n_trg, long sp, long tp,
+                       long n_groups, int ns, double eps2, double csl,
+                       double cdl, double *part) {
+  if (F32)
+    k_far_f32c<SL, DL><<<grid, FF_BLOCK, 0, st>>>(spos, sld, dld, n_src, tpos,
+                                                  n_trg, sp, tp, n_groups, ns,
+                                                  (float)eps2, csl, cdl, part);
+  else
+    k_far_f64<SL, DL><<<grid, FF_BLOCK, 0, st>>>(spos, sld, dld, n_src, tpos,
+                                                 n_trg, sp, tp, n_groups, ns,
+                                                 eps2, csl, cdl, part);
+}
+
+// Shared driver of the all-pairs and far-field entry points.
+template <bool F32>
+static int far_eval(const double *spos, const double *sld, const double *dld,
+                    size_t n_src, const double *tpos, double *tvel, size_t n_trg,
+                    size_t spg, size_t tpg, double eps2, double csl, double cdl,
+                    hipStream_t st) {
+  if (n_trg == 0) return VES3D_SUCCESS;
+  if (!tpos || !tvel) return VES3D_INVALID_PARAMETER;
+  if (n_src == 0 || (!sld && !dld)) {
+    V3D_HIP(hipMemsetAsync(tvel, 0, n_trg * 3 * sizeof(double), st));
+    return VES3D_SUCCESS;
+  }
+  if (!spos) return VES3D_INVALID_PARAMETER;
+  const bool grouped = spg > 0 && tpg > 0;
+  long n_groups = 0;
+  if (grouped) {
+    if (n_src % spg || n_trg % tpg || n_src / spg != n_trg / tpg)
+      return VES3D_INVALID_PARAMETER;
+    n_groups = (long)(n_src / spg);
+  }
+  const int ns = choose_nsplit((long)n_trg, (long)n_src, n_groups);
+  double *part = (double *)workspace(kWsFarPartial, (size_t)ns * n_trg * 3 * sizeof(double));
+  if (!part) return VES3D_MEMORY_ERROR;
+  const long tblocks = ((long)n_trg + FF_BLOCK - 1) / FF_BLOCK;
+  const dim3 grid((unsigned)(tblocks * ns));
+  const long sp = grouped ? (long)spg : 0, tp = grouped ? (long)tpg : 0;
+  if (sld && dld)
+    launch_far<F32, true, true>(grid, st, spos, sld, dld, n_src, tpos, n_trg, sp, tp, n_groups, ns, eps2, csl, cdl, part);
+  else if (sld)
+    launch_far<F32, true, false>(grid, st, spos, sld, dld, n_src, tpos, n_trg, sp, tp, n_groups, ns, eps2, csl, cdl, part);
+  else
+    launch_far<F32, false, true>(grid, st, spos, sld, dld, n_src, tpos, n_trg, sp, tp, n_groups, ns, eps2, csl, cdl, part);
+  V3D_CHECK(launch_status());
+  const long n = (long)n_trg * 3;
+  k_sum_slices<<<cdiv(n, 256), 256, 0, st>>>(part, ns, n, tvel);
+  return launch_status();
+}
+
+// ---------------------------------------------------------------------------
+// Per-vesicle Device<>::DirectStokes / DirectStokesDoubleLayer.
+constexpr int VS_BLOCK = 64;
+constexpr int VS_TILE = 256;
+
+// DLK=false: single layer, 1/(8pi) [w d / r + (w d . r) r / r^3]
+// DLK=true : double layer, -3/(4pi) w (r.n)(r.d) r / r^5        (r = src - trg)
+template <typename Real, bool DLK>
+__global__ __launch_bounds__(VS_BLOCK) void k_vesicle_stokes(
+    const Real *__restrict__ src, const Real *__restrict__ nrm,
+    const Real *__restrict__ den, const Real *__restrict__ qw, long ss, long ts,
+    long head, long tail, const Real *__restrict__ trg, Real *__restrict__ pot,
+    Real scal) {
+  __shared__ Real sh[VS_TILE][DLK ? 10 : 6];
+  const long vt = blockIdx.y;
+  const long t = head + (long)blockIdx.x * VS_BLOCK + threadIdx.x;
+  const bool tvalid = t < tail;
+  const Real *S = src + 3 * ss * vt;
+  const Real *D = den + 3 * ss * vt;
+  const Real *N = DLK ? nrm + 3 * ss * vt : nullptr;
+  Real tx = 0, ty = 0, tz = 0;
+  if (tvalid) {
+    const Real *T = trg + 3 * ts * vt;
+    tx = T[t];
+    ty = T[ts + t];
+    tz = T[2 * ts + t];
+  }
+  Real px = 0, py = 0, pz = 0;
+  for (long c0 = 0; c0 < ss; c0 += VS_TILE) {
+    const int cnt = (int)min((long)VS_TILE, ss - c0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += VS_BLOCK) {
+      const long s = c0 + i;
+      const Real w = qw ? qw[s] : Real(1);
+      sh[i][0] = S[s];
+      sh[i][1] = S[ss + s];
+      sh[i][2] = S[2 * ss + s];
+      if (DLK) {
+        sh[i][3] = D[s];
+        sh[i][4] = D[ss + s];
+        sh[i][5] = D[2 * ss + s];
+        sh[i][6] = N[s];
+        sh[i][7] = N[ss + s];
+        sh[i][8] = N[2 * ss + s];
+        sh[i][9] = w;
+      } else {
+        sh[i][3] = D[s] * w;
+        sh[i][4] = D[ss + s] * w;
+        sh[i][5] = D[2 * ss + s] * w;
+      }
+    }
+    __syncthreads();
+    if (tvalid) {
+#pragma unroll 4
+      for (int j = 0; j < cnt; ++j) {
+        const Real dx = sh[j][0] - tx, dy = sh[j][1] - ty, dz = sh[j][2] - tz;
+        const Real r2 = dx * dx + dy * dy + dz * dz;
+        const Real ri = (r2 != Real(0)) ? Real(1) / sqrt(r2) : Real(0);
+        if (DLK) {
+          const Real ri2 = ri * ri;
+          const Real rn = sh[j][6] * dx + sh[j][7] * dy + sh[j][8] * dz;
+          const Real rf = sh[j][3] * dx + sh[j][4] * dy + sh[j][5] * dz;
+          const Real c = rn * rf * (ri2 * ri2 * ri) * sh[j][9];
+          px += dx * c;
+          py += dy * c;
+          pz += dz * c;
+        } else {
+          const Real cc = (dx * sh[j][3] + dy * sh[j][4] + dz * sh[j][5]) * ri * ri;
+          px += (sh[j][3] + cc * dx) * ri;
+          py += (sh[j][4] + cc * dy) * ri;
+          pz += (sh[j][5] + cc * dz) * ri;
+        }
+      }
+      Real *P = pot + 3 * ts * vt;
+      P[t] = px * scal;
+      P[ts + t] = py * scal;
+      P[2 * ts + t] = pz * scal;
+    }
+  }
+}
+
+template <typename Real, bool DLK>
+static int vesicle_stokes(const Real *src, const Real *nrm, const Real *den,
+                          const Real *qw, size_t ss, size_t ts, size_t nv,
+                          const Real *trg, size_t head, size_t tail, Real *pot,
+                          hipStream_t st) {
+  if (tail < head || tail > ts) return VES3D_INVALID_PARAMETER;
+  if (nv == 0 || tail == head) return VES3D_SUCCESS;
+  if (!src || !den || !trg || !pot || (DLK && !nrm)) return VES3D_INVALID_PARAMETER;
+  if (nv > 65535) return VES3D_INVALID_PARAMETER;
+  const Real scal = DLK ? Real(-3.0 / (4.0 * kPi)) : Real(1.0 / kPi / 8.0);
+  dim3 grid(cdiv(tail - head, VS_BLOCK), (unsigned)nv);
+  k_vesicle_stokes<Real, DLK><<<grid, VS_BLOCK, 0, st>>>(
+      src, nrm, den, qw, (long)ss, (long)ts, (long)head, (long)tail, trg, pot, scal);
+  return launch_status();
+}
+
+}  // namespace v3d
+
+// ---------------------------------------------------------------------------
+// C ABI
+using namespace v3d;
+
+extern "C" {
+
+int ves3d_hip_direct_stokes_f64(const double *src, const double *den,
+                                const double *qw, size_t src_stride,
+                                size_t trg_stride, size_t n_surfs,
+                                const double *trg, size_t head, size_t tail,
+                                double *pot, ves3d_stream_t stream) {
+  return vesicle_stokes<double, false>(src, nullptr, den, qw, src_stride, trg_stride,
+                                       n_surfs, trg, head, tail, pot, (hipStream_t)stream);
+}
+int ves3d_hip_direct_stokes_f32(const float *src, const float *den,
+                                const float *qw, size_t src_stride,
+                                size_t trg_stride, size_t n_surfs,
+                                const float *trg, size_t head, size_t tail,
+                                float *pot, ves3d_stream_t stream) {
+  return vesicle_stokes<float, false>(src, nullptr, den, qw, src_stride, trg_stride,
+                                      n_surfs, trg, head, tail, pot, (hipStream_t)stream);
+}
+int ves3d_hip_direct_stokes_dl_f64(const double *src, const double *norm,
+                                   const double *den, const double *qw,
+                                   size_t src_stride, size_t trg_stride,
+                                   size_t n_surfs, const double *trg,
+                                   size_t head, size_t tail, double *pot,
+                                   ves3d_stream_t stream) {
+  return vesicle_stokes<double, true>(src, norm, den, qw, src_stride, trg_stride,
+                                      n_surfs, trg, head, tail, pot, (hipStream_t)stream);
+}
+int ves3d_hip_direct_stokes_dl_f32(const float *src, const float *norm,
+                                   const float *den, const float *qw,
+                                   size_t src_stride, size_t trg_stride,
+                                   size_t n_surfs, const float *trg,
+                                   size_t head, size_t tail, float *pot,
+                                   ves3d_stream_t stream) {
+  return vesicle_stokes<float, true>(src, norm, den, qw, src_stride, trg_stride,
+                                     n_surfs, trg, head, tail, pot, (hipStream_t)stream);
+}
+
+int ves3d_hip_stokes_all2all_f64(const double *src, const double *den,
+                                 size_t np, double *pot, ves3d_stream_t stream) {
+  return far_eval<false>(src, den, nullptr, np, src, pot, np, 0, 0, 0.0,
+                         1.0 / kPi / 8.0, 0.0, (hipStream_t)stream);
+}
+int ves3d_hip_stokes_all2all_f32c(const double *src, const double *den,
+                                  size_t np, double *pot, ves3d_stream_t stream) {
+  return far_eval<true>(src, den, nullptr, np, src, pot, np, 0, 0, 0.0,
+                        1.0 / kPi / 8.0, 0.0, (hipStream_t)stream);
+}
+
+static const double kEvalEps2 = 128.0 * 2.220446049250313e-16;
+
+int ves3d_hip_direct_eval_f64(const double *src_pos, const double *sl_den,
+                              const double *dl_den, size_t n_src,
+                              const double *trg_pos, double *trg_vel,
+                              size_t n_trg, size_t spg, size_t tpg,
+                              ves3d_stream_t stream) {
+  return far_eval<false>(src_pos, sl_den, dl_den, n_src, trg_pos, trg_vel, n_trg,
+                         spg, tpg, kEvalEps2, 1.0 / (8.0 * kPi),
+                         3.0 / (4.0 * kPi), (hipStream_t)stream);
+}
+int ves3d_hip_direct_eval_f32c(const double *src_pos, const double *sl_den,
+                               const double *dl_den, size_t n_src,
+                               const double *trg_pos, double *trg_vel,
+                               size_t n_trg, size_t spg, size_t tpg,
+                               ves3d_stream_t stream) {
+  return far_eval<true>(src_pos, sl_den, dl_den, n_src, trg_pos, trg_vel, n_trg,
+                        spg, tpg, kEvalEps2, 1.0 / (8.0 * kPi),
+                        3.0 / (4.0 * kPi), (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,71 @@
+"""Synthetic vesicle suspensions (host-side input generation, numpy only).
+
+Grids follow the reference's SphericalHarmonics convention: latitudes are the
+p+1 Gauss-Legendre nodes x_i in ascending order, longitudes phi_j = pi j / p
+(j < 2p), and the point (i, j) of the unit sphere is
+(x_i, sqrt(1-x_i^2) sin phi_j, sqrt(1-x_i^2) cos phi_j)
+(src/SphericalHarmonics.cc:785-790).  The point index is i*2p + j.
+"""
+import numpy as np
+
+SEED_BASE = 20261202
+
+
+def gl_nodes(p):
+    """Gauss-Legendre nodes (ascending) and weights with p+1 points."""
+    x, w = np.polynomial.legendre.leggauss(p + 1)
+    return x, w
+
+
+def sphere_grid(p):
+    """Unit-sphere grid [3][2p(p+1)] and the smooth quadrature weight per
+    point (w_i * pi/p, which integrates dS exactly for polynomials)."""
+    x, w = gl_nodes(p)
+    phi = np.pi * np.arange(2 * p) / p
+    s = np.sqrt(1.0 - x * x)
+    X = np.empty((3, p + 1, 2 * p))
+    X[0] = x[:, None]
+    X[1] = s[:, None] * np.sin(phi)[None, :]
+    X[2] = s[:, None] * np.cos(phi)[None, :]
+    qw = np.repeat(w * np.pi / p, 2 * p)
+    return X.reshape(3, -1), qw
+
+
+def random_centres(nv, volfrac, radius=1.0, seed=SEED_BASE, gap=0.0):
+    """Centres uniform in a cube sized for `volfrac` (sphere volume fraction),
+    overlapping spheres rejected (centre distance < 2 radius + gap)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    side = (nv * 4.0 / 3.0 * np.pi * radius ** 3 / volfrac) ** (1.0 / 3.0)
+    out = np.empty((nv, 3))
+    k = 0
+    dmin2 = (2 * radius + gap) ** 2
+    tries = 0
+    while k < nv:
+        c = rng.uniform(radius, side - radius, size=3)
+        if k == 0 or np.min(np.sum((out[:k] - c) ** 2, axis=1)) >= dmin2:
+            out[k] = c
+            k += 1
+        tries += 1
+        if tries > 1000000:
+            raise RuntimeError("could not place %d spheres at volume fraction %g" % (nv, volfrac))
+    return out, side
+
+
+def stokes_direct_sum_case(nv=512, p=16, volfrac=0.15, seed=SEED_BASE + 1):
+    """The 'Stokes direct summation alone' workload: nv unit spheres at SH
+    order p (2p(p+1) points each) with centres uniform in a cube at the given
+    volume fraction; single-layer densities U[-1,1]^3 times the quadrature
+    weight, double-layer densities the same with unit normals.
+
+    Returns point-major arrays: pos [N,3], sl [N,3], dl [N,6] (=[g, n])."""
+    X, qw = sphere_grid(p)
+    c, _ = random_centres(nv, volfrac, 1.0, seed)
+    npv = X.shape[1]
+    pos = (c[:, None, :] + X.T[None, :, :]).reshape(-1, 3)
+    rng = np.random.Generator(np.random.PCG64(seed + 1000))
+    w = np.tile(qw, nv)[:, None]
+    sl = rng.uniform(-1, 1, size=(nv * npv, 3)) * w
+    g = rng.uniform(-1, 1, size=(nv * npv, 3)) * w
+    n = np.tile(X.T, (nv, 1))
+    dl = np.concatenate([g, n], axis=1)
+    return np.ascontiguousarray(pos), np.ascontiguousarray(sl), np.ascontiguousarray(dl), npv
