@@ -134,6 +134,74 @@ int ves3d_hip_direct_eval_f32c(const double *src_pos, const double *sl_den,
                                size_t n_trg, size_t src_per_group,
                                size_t trg_per_group, ves3d_stream_t stream);
 
+/* ---- spherical-harmonic transforms (SphericalHarmonics<Real>,
+ *      include/SphericalHarmonics.h:14-24, src/SphericalHarmonics.cc:3-360) --
+ * Grid layout: per function (p+1) x 2p values, latitude i = i-th Gauss-Legendre
+ * node (ascending), longitude j = pi j / p.  Coefficient layout: per function
+ * p(p+2) values in blocks j = 0..2p-1 of length p+1-(j+1)/2 (order (j+1)/2,
+ * cosine for odd j and j=0, sine for even j>0, degree ascending).  N is the
+ * number of functions.  The f32 forms run the same transforms in FP32. */
+
+/* SHC2Grid: S [N][p0(p0+2)] -> X [N][(p1+1) 2p1].  X_theta (longitude
+ * derivative, MatFourierGrad) and X_phi (latitude derivative, MatLegendreGrad)
+ * are optional outputs (NULL to skip), named as in the reference. */
+int ves3d_hip_shc2grid_f64(const double *S, size_t N, int p0, int p1, double *X,
+                           double *X_theta, double *X_phi, ves3d_stream_t stream);
+int ves3d_hip_shc2grid_f32(const float *S, size_t N, int p0, int p1, float *X,
+                           float *X_theta, float *X_phi, ves3d_stream_t stream);
+/* Grid2SHC: X [N][(p0+1) 2p0] -> S [N][p1(p1+2)]. */
+int ves3d_hip_grid2shc_f64(const double *X, size_t N, int p0, int p1, double *S,
+                           ves3d_stream_t stream);
+int ves3d_hip_grid2shc_f32(const float *X, size_t N, int p0, int p1, float *S,
+                           ves3d_stream_t stream);
+/* SHC2GridTranspose (the adjoint of SHC2Grid(p1 -> p0)):
+ * X [N][(p0+1) 2p0] -> S [N][p1(p1+2)]. */
+int ves3d_hip_shc2grid_transpose_f64(const double *X, size_t N, int p0, int p1,
+                                     double *S, ves3d_stream_t stream);
+int ves3d_hip_shc2grid_transpose_f32(const float *X, size_t N, int p0, int p1,
+                                     float *S, ves3d_stream_t stream);
+/* SHC2Pole: S [N][p0(p0+2)] -> P [N][2], values at x = -1 and x = +1. */
+int ves3d_hip_shc2pole_f64(const double *S, size_t N, int p0, double *P,
+                           ves3d_stream_t stream);
+int ves3d_hip_shc2pole_f32(const float *S, size_t N, int p0, float *P,
+                           ves3d_stream_t stream);
+
+/* ---- rotations and the singular self-interaction ------------------------ */
+
+/* RotateAll (src/SphericalHarmonics.cc:362-449): S [N*dof][p(p+2)] ->
+ * S_out [N][p+1][p][dof][p(p+2)], the coefficients rotated so that grid point
+ * (t, j) sits at the pole. */
+int ves3d_hip_rotate_all_f64(const double *S, size_t N, int p, int dof,
+                             double *S_out, ves3d_stream_t stream);
+/* RotateTranspose (src/SphericalHarmonics.cc:451-541): the adjoint, same
+ * [N][p+1][p][dof][p(p+2)] layout in and out. */
+int ves3d_hip_rotate_transpose_f64(const double *S_in, size_t N, int p, int dof,
+                                   double *S_out, ves3d_stream_t stream);
+
+/* StokesSingularInteg (include/SphericalHarmonics.h:40,
+ * src/SphericalHarmonics.cc:862-886, :952-1180): per-vesicle single- and
+ * double-layer self-interaction operators on SH coefficients.
+ * X0: vesicle positions [nves][3][(p0+1) 2p0] (FP64); p1 the upsampled order
+ * of the quadrature (the reference uses 2 p0).  SL, DL: [nves][3 Nc][3 Nc],
+ * Nc = p0(p0+2), row = density coefficient (component-major), column =
+ * velocity coefficient; either may be NULL.  The f32 form runs the transform
+ * GEMMs in FP32 and the pole-kernel evaluation in FP64, and stores FP32. */
+int ves3d_hip_singular_build_f64(const double *X0, size_t nves, int p0, int p1,
+                                 double *SL, double *DL, ves3d_stream_t stream);
+int ves3d_hip_singular_build_f32(const double *X0, size_t nves, int p0, int p1,
+                                 float *SL, float *DL, ves3d_stream_t stream);
+
+/* Self-interaction apply (src/StokesVelocity.cc:397-447):
+ * V[v] = F[v] SL[v] + G[v] DL[v] (row vector times matrix) for every vesicle;
+ * F, G, V: [nves][3 Nc] FP64 coefficient vectors; SL or DL (and its vector)
+ * may be NULL.  The f32 form reads FP32 matrices, accumulates in FP64. */
+int ves3d_hip_self_apply_f64(const double *SL, const double *F, const double *DL,
+                             const double *G, size_t nves, int p, double *V,
+                             ves3d_stream_t stream);
+int ves3d_hip_self_apply_f32(const float *SL, const double *F, const float *DL,
+                             const double *G, size_t nves, int p, double *V,
+                             ves3d_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

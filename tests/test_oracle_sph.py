@@ -1,0 +1,166 @@
+"""Pins the numpy oracle of SphericalHarmonics (oracle/sph.py).
+
+  quadrature   : numpy leggauss; the reference's own precomputed weight files
+                 (tests/golden/*.txt, copied from precomputed/):
+                 quad_weights_{6,12} = LegendreWeights on the grid,
+                 sing_quad_weights_{6,12} = SingularWeights, latitudes reversed.
+  Legendre     : scipy.special.lpmv with the orthonormal factor.
+  transforms   : Grid2SHC o SHC2Grid = identity; upsample -> downsample;
+                 SHC2GridTranspose is the adjoint of SHC2Grid; Y_1^0 values.
+  rotation     : per-degree blocks preserve the weighted energy
+                 (Mr D Mr^T = D, D = 2 for m = 0 else 1).
+  singular op  : unit-sphere closed forms  SL[c] = (2/3) c,  SL[n] = 0,
+                 DL[c] = c/2 (this grid's normal orientation), and spectral
+                 convergence on a perturbed shape.
+"""
+import os
+
+import numpy as np
+import pytest
+from scipy.special import lpmv, factorial
+
+from oracle import sph
+from ves3d_amd import workloads as W
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+RNG = np.random.default_rng(3)
+
+
+@pytest.mark.parametrize("p", [1, 6, 12, 16, 32, 48])
+def test_gauss_legendre(p):
+    x, w = sph.legendre_nodes_weights(p)
+    x2, w2 = np.polynomial.legendre.leggauss(p + 1)
+    assert np.max(np.abs(x - x2)) < 1e-14 and np.max(np.abs(w - w2)) < 1e-13
+    # exact for degree 2p+1
+    for d in range(2 * p + 2):
+        assert abs(np.sum(w * x ** d) - (2.0 / (d + 1) if d % 2 == 0 else 0.0)) < 1e-13
+
+
+@pytest.mark.parametrize("p,prec,tol", [(6, "double", 1e-14), (12, "double", 1e-14)])
+def test_quad_weights_fixture(p, prec, tol):
+    f = np.loadtxt(os.path.join(GOLD, "quad_weights_%d_%s.txt" % (p, prec)))
+    qw = sph.legendre_weights(p)
+    assert f.shape == (2 * p * (p + 1),)
+    assert np.max(np.abs(f - np.repeat(qw, 2 * p))) < tol
+
+
+@pytest.mark.parametrize("p,prec,tol", [(6, "double", 1e-14), (12, "single", 1e-6)])
+def test_singular_weights_fixture(p, prec, tol):
+    f = np.loadtxt(os.path.join(GOLD, "sing_quad_weights_%d_%s.txt" % (p, prec)))
+    sw = sph.singular_weights(p)
+    expect = np.repeat(sw[::-1], 2 * p)   # flipud over latitudes (matlab/Generate_Cpp_data.m:83-86)
+    assert np.max(np.abs(f - expect)) / np.max(np.abs(expect)) < tol
+
+
+def test_legpoly_vs_scipy():
+    deg = 12
+    x = np.linspace(-0.99, 0.99, 17)
+    P = sph.legpoly(x, deg)
+    row = 0
+    for m in range(deg + 1):
+        for n in range(m, deg + 1):
+            norm = np.sqrt((2 * n + 1) / (4 * np.pi) * factorial(n - m) / factorial(n + m))
+            assert np.allclose(P[row], norm * lpmv(m, n, x), atol=1e-12), (m, n)
+            row += 1
+
+
+def test_legpoly_deriv_finite_difference():
+    deg = 10
+    x = np.linspace(-0.9, 0.9, 13)
+    h = 1e-6
+    D = sph.legpoly_deriv(x, deg)
+    # H = dP/dtheta with x = -cos(theta') orientation: check against d/dx * dx/dtheta
+    dPdx = (sph.legpoly(x + h, deg) - sph.legpoly(x - h, deg)) / (2 * h)
+    s = np.sqrt(1 - x * x)
+    assert np.allclose(D, dPdx * s, atol=1e-7)
+
+
+@pytest.mark.parametrize("p0,p1", [(6, 6), (8, 16), (12, 24), (16, 32)])
+def test_transforms_roundtrip_and_adjoint(p0, p1):
+    S = RNG.standard_normal((4, p0 * (p0 + 2)))
+    X = sph.shc2grid(S, p0, p0)
+    assert np.max(np.abs(sph.grid2shc(X, p0, p0) - S)) < 1e-12
+    Xu = sph.shc2grid(S, p0, p1)
+    assert np.max(np.abs(sph.grid2shc(Xu, p1, p0) - S)) < 1e-12
+    # <SHC2Grid(p0->p1) S, Y> = <S, SHC2GridTranspose(Y)>
+    Y = RNG.standard_normal((4, 2 * p1 * (p1 + 1)))
+    lhs = np.sum(Xu * Y)
+    rhs = np.sum(S * sph.shc2grid_transpose(Y, p1, p0))
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+def test_y10_and_derivatives():
+    """Y_1^0 = sqrt(3/4pi) x -> coefficient of (n=1, m=0) is 1/sqrt(2pi); its
+    Legendre-direction derivative is sqrt(3/4pi) sqrt(1-x^2)."""
+    p = 6
+    S = np.zeros((1, p * (p + 2)))
+    S[0, 1] = 1.0 / np.sqrt(2 * np.pi)
+    X, Xt, Xp = sph.shc2grid(S, p, p, True, True)
+    x = sph.legendre_nodes(p)
+    assert np.allclose(X.reshape(p + 1, 2 * p), np.sqrt(3 / (4 * np.pi)) * x[:, None], atol=1e-14)
+    assert np.allclose(Xt, 0, atol=1e-14)
+    assert np.allclose(Xp.reshape(p + 1, 2 * p), -np.sqrt(3 / (4 * np.pi)) * np.sqrt(1 - x * x)[:, None] * -1,
+                       atol=1e-13)
+    P = sph.shc2pole(S, p)
+    assert np.allclose(P, [[-np.sqrt(3 / (4 * np.pi)), np.sqrt(3 / (4 * np.pi))]], atol=1e-14)
+
+
+@pytest.mark.parametrize("p", [6, 12])
+def test_rotation_blocks_preserve_energy(p):
+    Mr = sph.mat_rotate(p)
+    perm = sph.coeff_perm(p)
+    for l in range(p + 1):
+        for n in range(p):  # degree p lacks its m=p sine on the 2p grid: not rotation-closed
+            M = Mr[l * (p + 1) + n]
+            # weight 2 for the m=0 coefficient (j = 0), 1 otherwise
+            D = np.diag([2.0 if j == 0 else 1.0 for j in range(len(perm[n]))])
+            # row r of M holds the coefficients of the rotated basis function r
+            # MatRotate samples the rotated basis on the rotated grid, where one
+            # grid point lands on the pole up to rounding: x = -1 + O(eps) gives
+            # sqrt(1-x^2) ~ 1.5e-8 and an arbitrary atan2 angle, so m >= 1 terms
+            # carry an error of that size (src/SphericalHarmonics.cc:785-810).
+            assert np.allclose(M @ D @ M.T, D, atol=5e-8), (l, n)
+
+
+def test_rotate_transpose_is_adjoint():
+    p, dof = 6, 2
+    S = RNG.standard_normal((2 * dof, p * (p + 2)))
+    R = sph.rotate_all(S, p, dof)
+    Y = RNG.standard_normal(R.shape)
+    lhs = np.sum(R * Y)
+    Yt = sph.rotate_transpose(Y, p, dof)
+    # RotateTranspose returns per-rotation vectors; the adjoint of RotateAll sums them
+    rhs = np.sum(S.reshape(2, 1, 1, dof, -1) * Yt)
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+@pytest.mark.parametrize("p0", [6, 8, 12])
+def test_singular_closed_forms(p0):
+    X, _ = W.sphere_grid(p0)
+    SL, DL = sph.stokes_singular_integ(X[None], p0, 2 * p0)
+    nc = p0 * (p0 + 2)
+    c = np.array([0.3, -1.1, 0.7])
+    f = np.repeat(c, X.shape[1]).reshape(3, -1)
+    F = sph.grid2shc(f, p0, p0).reshape(-1)
+    V = sph.shc2grid((F @ SL[0]).reshape(3, nc), p0, p0)
+    Vd = sph.shc2grid((F @ DL[0]).reshape(3, nc), p0, p0)
+    Fn = sph.grid2shc(X, p0, p0).reshape(-1)
+    Vn = sph.shc2grid((Fn @ SL[0]).reshape(3, nc), p0, p0)
+    assert np.max(np.abs(V - 2.0 / 3.0 * f)) < 1e-9
+    assert np.max(np.abs(Vd - 0.5 * f)) < 1e-9
+    assert np.max(np.abs(Vn)) < 1e-9
+
+
+def test_singular_convergence_on_perturbed_shape():
+    """On a perturbed sphere the SL velocity of a smooth density converges
+    spectrally in the quadrature order p1 (p0 fixed)."""
+    p0 = 8
+    X = W.perturbed_shapes(1, p0, amp=0.05, seed=4)
+    f = np.sin(X[0]) + 0.3
+    F = sph.grid2shc(f, p0, p0).reshape(-1)
+    vals = []
+    for p1 in (12, 16, 24, 32):
+        SL, _ = sph.stokes_singular_integ(X, p0, p1, want_dl=False)
+        vals.append(F @ SL[0])
+    e = [np.max(np.abs(v - vals[-1])) for v in vals[:-1]]
+    assert e[0] > e[1] > e[2] and e[2] < 1e-6, e

@@ -55,6 +55,15 @@ _sig("ves3d_hip_stokes_all2all_f64", P, P, S, P, P)
 _sig("ves3d_hip_stokes_all2all_f32c", P, P, S, P, P)
 _sig("ves3d_hip_direct_eval_f64", P, P, P, S, P, P, S, S, S, P)
 _sig("ves3d_hip_direct_eval_f32c", P, P, P, S, P, P, S, S, S, P)
+for _t in ("f64", "f32"):
+    _sig("ves3d_hip_shc2grid_" + _t, P, S, c_int, c_int, P, P, P, P)
+    _sig("ves3d_hip_grid2shc_" + _t, P, S, c_int, c_int, P, P)
+    _sig("ves3d_hip_shc2grid_transpose_" + _t, P, S, c_int, c_int, P, P)
+    _sig("ves3d_hip_shc2pole_" + _t, P, S, c_int, P, P)
+    _sig("ves3d_hip_singular_build_" + _t, P, S, c_int, c_int, P, P, P)
+    _sig("ves3d_hip_self_apply_" + _t, P, P, P, P, S, c_int, P, P)
+_sig("ves3d_hip_rotate_all_f64", P, S, c_int, c_int, P, P)
+_sig("ves3d_hip_rotate_transpose_f64", P, S, c_int, c_int, P, P)
 
 _lib = None
 
@@ -118,4 +127,4 @@ def device_info():
     return name.value.decode(), cu.value, clk.value
 
 
-from . import stokes  # noqa: E402,F401
+from . import sph, stokes  # noqa: E402,F401

@@ -1,0 +1,31 @@
+// sht.hpp -- the SphericalHarmonics transforms on the GPU
+// (SHC2Grid, Grid2SHC, SHC2GridTranspose, SHC2Pole; src/SphericalHarmonics.cc:3-360)
+// expressed as two grouped GEMM launches each (Legendre stage per order,
+// Fourier stage as one dense product), with all re-layouts folded into the
+// operand strides.
+#pragma once
+#include "gemm.hpp"
+#include "sh_tables.hpp"
+
+namespace v3d {
+
+// Workspace element counts the transforms need.
+inline long shc2grid_work(long N, int p0, int p1) { return 2L * p0 * N * (p1 + 1); }
+inline long grid2shc_work(long N, int p0, int p1) { return 2L * p1 * N * (p0 + 1); }
+
+// S [N][p0(p0+2)] -> X [N][(p1+1)2p1]; X_theta / X_phi optional (may be null).
+template <typename T>
+int sht_shc2grid(const T *S, long N, int p0, int p1, T *X, T *X_theta, T *X_phi,
+                 T *work, hipStream_t st);
+// X [N][(p0+1)2p0] -> S [N][p1(p1+2)].
+template <typename T>
+int sht_grid2shc(const T *X, long N, int p0, int p1, T *S, T *work, hipStream_t st);
+// Adjoint of SHC2Grid(p1 -> p0): X [N][(p0+1)2p0] -> S [N][p1(p1+2)].
+template <typename T>
+int sht_shc2grid_transpose(const T *X, long N, int p0, int p1, T *S, T *work,
+                           hipStream_t st);
+// S [N][p0(p0+2)] -> P [N][2] at x = -1, +1.
+template <typename T>
+int sht_shc2pole(const T *S, long N, int p0, T *P, hipStream_t st);
+
+}  // namespace v3d

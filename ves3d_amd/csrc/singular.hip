@@ -1,0 +1,516 @@
+// singular.hip -- rotations of SH coefficients to every grid point
+// (RotateAll / RotateTranspose, src/SphericalHarmonics.cc:362-541), the
+// singular self-interaction operator build (StokesSingularInteg_,
+// :952-1180) and its per-vesicle application (the self term of
+// StokesVelocity::operator(), src/StokesVelocity.cc:385-447).
+//
+// Rotation: the per-degree block products Bo[l] = Bi[l] * Mr[t][l] run as one
+// grouped GEMM launch (descriptor per (t, l), batch over vesicles) on a
+// degree-major copy of the coefficients; the longitude phase and the
+// degree-major gather/scatter are separate elementwise kernels.
+//
+// Singular build, per chunk of vesicles:
+//   Grid2SHC -> RotateAll -> SHC2Grid(p0 -> p1, +derivatives) & SHC2Pole
+//   -> pole kernel (FP64) -> SHC2GridTranspose(p1 -> p0) -> RotateTranspose
+//   -> rearrange -> Grid2SHC = [Nves][3 Ncoef][3 Ncoef]
+// The FP32 form runs every GEMM stage in FP32 and keeps the pole kernel
+// (singular weights, 1/r near the rotated pole) in FP64.
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "runtime.hpp"
+#include "sht.hpp"
+
+namespace v3d {
+
+namespace {
+
+std::mutex g_mu;
+std::map<std::tuple<int, int, int, int, long>, std::pair<GemmDesc *, int>> g_rplans;
+
+// Grouped GEMM plan of the per-degree rotation products.
+// transpose=false: RotateAll, rows (j, d) of vesicle v, output slab t.
+// transpose=true : RotateTranspose, rows (j, d) of (v, t), Mr read transposed.
+const std::pair<GemmDesc *, int> *rot_plan(int p, int dof, bool transpose,
+                                           const RotMatsHost &h) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto key = std::make_tuple(dev, p, dof, (int)transpose, 0L);
+  auto it = g_rplans.find(key);
+  if (it != g_rplans.end()) return &it->second;
+  const long Ncoef = sh_ncoef(p);
+  const long slab = (long)p * dof * Ncoef;  // one (v, t) slab
+  std::vector<GemmDesc> v;
+  for (int t = 0; t <= p; ++t)
+    for (int l = 0; l <= p; ++l) {
+      GemmDesc d{};
+      const int len = h.len[l];
+      d.M = p * dof;
+      d.N = len;
+      d.K = len;
+      d.a_rs = Ncoef;
+      d.a_cs = 1;
+      d.b_off = h.off[t * (p + 1) + l];
+      d.b_rs = transpose ? 1 : len;
+      d.b_cs = transpose ? len : 1;
+      d.b_bs = 0;
+      d.c_rs = Ncoef;
+      d.c_cs = 1;
+      if (!transpose) {
+        d.a_off = h.doff[l];
+        d.a_bs = slab;
+        d.c_off = t * slab + h.doff[l];
+        d.c_bs = (p + 1) * slab;
+      } else {
+        d.a_off = t * slab + h.doff[l];
+        d.a_bs = (p + 1) * slab;
+        d.c_off = t * slab + h.doff[l];
+        d.c_bs = (p + 1) * slab;
+      }
+      v.push_back(d);
+    }
+  GemmDesc *dd = upload_descs(v.data(), (int)v.size());
+  if (!dd) return nullptr;
+  return &(g_rplans[key] = std::make_pair(dd, (int)v.size()));
+}
+
+template <typename T>
+const T *rot_ptr(const RotMatsDev *r);
+template <>
+const double *rot_ptr<double>(const RotMatsDev *r) { return r->d64; }
+template <>
+const float *rot_ptr<float>(const RotMatsDev *r) { return r->d32; }
+
+// B0p[v][j][d][pos] = phase_j(S0[v][d])[perm[pos]] with exp(-i k j pi / p)
+// (src/SphericalHarmonics.cc:398-418).
+template <typename T>
+__global__ void k_rot_phase_gather(const T *__restrict__ S0, long Nv, int p, int dof,
+                                   int ncoef, const int *__restrict__ perm,
+                                   const int *__restrict__ partner,
+                                   const int *__restrict__ order,
+                                   const int *__restrict__ is_sin, T *__restrict__ out) {
+  const long total = Nv * p * dof * (long)ncoef;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long)gridDim.x * blockDim.x) {
+    const int pos = (int)(i % ncoef);
+    long r = i / ncoef;
+    const int d = (int)(r % dof);
+    r /= dof;
+    const int j = (int)(r % p);
+    const long v = r / p;
+    const int c = perm[pos];
+    const T *s = S0 + (v * dof + d) * ncoef;
+    const int k = order[c];
+    T val;
+    if (k == 0 || k == p) {
+      val = s[c];
+    } else {
+      const bool sn = is_sin[c];
+      const T x0 = s[sn ? partner[c] : c], x1 = s[sn ? c : partner[c]];
+      double sv, cv;
+      sincos((double)k * j * kPi / p, &sv, &cv);
+      const T r0 = (T)cv, r1 = (T)(-sv);
+      val = sn ? x0 * r1 + x1 * r0 : x0 * r0 - x1 * r1;
+    }
+    out[i] = val;
+  }
+}
+
+// out[row][c] = in[row][iperm[c]]  (degree-major -> coefficient order)
+template <typename T>
+__global__ void k_scatter_coef(const T *__restrict__ in, long rows, int ncoef,
+                               const int *__restrict__ iperm, T *__restrict__ out) {
+  const long total = rows * ncoef;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % ncoef);
+    out[i] = in[(i - c) + iperm[c]];
+  }
+}
+
+// out[row][pos] = in[row][perm[pos]]  (coefficient order -> degree-major)
+template <typename T>
+__global__ void k_gather_coef(const T *__restrict__ in, long rows, int ncoef,
+                              const int *__restrict__ perm, T *__restrict__ out) {
+  const long total = rows * ncoef;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long)gridDim.x * blockDim.x) {
+    const int pos = (int)(i % ncoef);
+    out[i] = in[(i - pos) + perm[pos]];
+  }
+}
+
+// out[v][t][j][d][c] = phase_j^+(in)[c] reading the degree-major input
+// (src/SphericalHarmonics.cc:511-537).
+template <typename T>
+__global__ void k_rot_scatter_phase(const T *__restrict__ in, long Nv, int p, int dof,
+                                    int ncoef, const int *__restrict__ iperm,
+                                    const int *__restrict__ partner,
+                                    const int *__restrict__ order,
+                                    const int *__restrict__ is_sin, T *__restrict__ out) {
+  const long total = Nv * (p + 1) * (long)p * dof * ncoef;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % ncoef);
+    const long row = i / ncoef;
+    const int j = (int)((row / dof) % p);
+    const T *s = in + row * ncoef;
+    const int k = order[c];
+    T val;
+    if (k == 0 || k == p) {
+      val = s[iperm[c]];
+    } else {
+      const bool sn = is_sin[c];
+      const T x0 = s[iperm[sn ? partner[c] : c]], x1 = s[iperm[sn ? c : partner[c]]];
+      double sv, cv;
+      sincos((double)k * j * kPi / p, &sv, &cv);
+      const T r0 = (T)cv, r1 = (T)sv;
+      val = sn ? x0 * r1 + x1 * r0 : x0 * r0 - x1 * r1;
+    }
+    out[i] = val;
+  }
+}
+
+inline unsigned grid_for(long n) {
+  long b = (n + 255) / 256;
+  if (b > 65536L * 16) b = 65536L * 16;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+}  // namespace
+
+// RotateAll: S [Nv*dof][Ncoef] -> out [Nv][p+1][p][dof][Ncoef]; work: Nv*p*dof*Ncoef
+// + Nv*(p+1)*p*dof*Ncoef elements.
+template <typename T>
+int rotate_all(const T *S, long Nv, int p, int dof, T *out, T *work, hipStream_t st) {
+  const RotMatsDev *r = rot_mats(p);
+  if (!r) return VES3D_MEMORY_ERROR;
+  const auto *pl = rot_plan(p, dof, false, *r->h);
+  if (!pl) return VES3D_MEMORY_ERROR;
+  const int nc = sh_ncoef(p);
+  T *B0p = work;
+  T *S1p = work + Nv * p * dof * (long)nc;
+  const long n0 = Nv * p * dof * (long)nc;
+  k_rot_phase_gather<T><<<grid_for(n0), 256, 0, st>>>(S, Nv, p, dof, nc, r->perm, r->partner,
+                                                      r->order, r->is_sin, B0p);
+  V3D_CHECK(launch_status());
+  V3D_CHECK(gemm_grouped<T>(pl->first, pl->second, (int)Nv, p * dof, 2 * p, B0p,
+                            rot_ptr<T>(r), S1p, T(1), T(0), st));
+  const long rows = Nv * (p + 1) * (long)p * dof;
+  k_scatter_coef<T><<<grid_for(rows * nc), 256, 0, st>>>(S1p, rows, nc, r->iperm, out);
+  return launch_status();
+}
+
+// RotateTranspose: in/out [Nv][p+1][p][dof][Ncoef]; work: 2 * that size.
+template <typename T>
+int rotate_transpose(const T *in, long Nv, int p, int dof, T *out, T *work, hipStream_t st) {
+  const RotMatsDev *r = rot_mats(p);
+  if (!r) return VES3D_MEMORY_ERROR;
+  const auto *pl = rot_plan(p, dof, true, *r->h);
+  if (!pl) return VES3D_MEMORY_ERROR;
+  const int nc = sh_ncoef(p);
+  const long rows = Nv * (p + 1) * (long)p * dof;
+  T *Ap = work, *Bp = work + rows * nc;
+  k_gather_coef<T><<<grid_for(rows * nc), 256, 0, st>>>(in, rows, nc, r->perm, Ap);
+  V3D_CHECK(launch_status());
+  V3D_CHECK(gemm_grouped<T>(pl->first, pl->second, (int)Nv, p * dof, 2 * p, Ap,
+                            rot_ptr<T>(r), Bp, T(1), T(0), st));
+  k_rot_scatter_phase<T><<<grid_for(rows * nc), 256, 0, st>>>(Bp, Nv, p, dof, nc, r->iperm,
+                                                              r->partner, r->order,
+                                                              r->is_sin, out);
+  return launch_status();
+}
+
+// ------------------------------------------------------------- pole kernel
+// For each rotation r, pole t (0: x=-1, 1: x=+1) and upsampled source s:
+// the 6 symmetric tensor components of the SL (and/or DL) kernel times the
+// singular quadrature weight (src/SphericalHarmonics.cc:970-1073).
+template <typename T>
+__global__ void k_pole_kernel(const T *__restrict__ X, const T *__restrict__ Xt,
+                              const T *__restrict__ Xp, const T *__restrict__ trg,
+                              const double *__restrict__ qw, long nrot, int p1,
+                              T *__restrict__ SL0, T *__restrict__ DL0) {
+  const int M1 = 2 * p1 * (p1 + 1);
+  const long total = nrot * 2 * (long)M1;
+  const double eps = 2.220446049250313e-16;
+  const double csl = 1.0 / (8.0 * kPi), cdl = 3.0 / (4.0 * kPi);
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long)gridDim.x * blockDim.x) {
+    const int s = (int)(i % M1);
+    const long rt = i / M1;
+    const int t = (int)(rt & 1);
+    const long r = rt >> 1;
+    const int j0 = s / (2 * p1);
+    const double w = qw[t ? j0 : p1 - j0];
+    double tx = trg[(r * 3 + 0) * 2 + t], ty = trg[(r * 3 + 1) * 2 + t], tz = trg[(r * 3 + 2) * 2 + t];
+    const long b = r * 3 * (long)M1 + s;
+    const double dx = tx - (double)X[b], dy = ty - (double)X[b + M1], dz = tz - (double)X[b + 2 * M1];
+    const double xt = Xt[b], yt = Xt[b + M1], zt = Xt[b + 2 * M1];
+    const double xp = Xp[b], yp = Xp[b + M1], zp = Xp[b + 2 * M1];
+    const double nx = yt * zp - zt * yp, ny = zt * xp - xt * zp, nz = xt * yp - yt * xp;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    double rinv = 1.0 / sqrt(r2);
+    if (r2 <= eps) rinv = 0;
+    const double rinv2 = rinv * rinv;
+    const double dd[6] = {dx * dx, dx * dy, dx * dz, dy * dy, dy * dz, dz * dz};
+    const long o = (rt * 6) * (long)M1 + s;
+    if (DL0) {
+      const double rn5 = cdl * w * (nx * dx + ny * dy + nz * dz) * rinv2 * rinv2 * rinv;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) DL0[o + c * (long)M1] = (T)(dd[c] * rn5);
+    }
+    if (SL0) {
+      const double area = sqrt(nx * nx + ny * ny + nz * nz);
+      const double ar = csl * w * area * rinv;
+      const double ar2 = ar * rinv2;
+#pragma unroll
+      for (int c = 0; c < 6; ++c)
+        SL0[o + c * (long)M1] = (T)(dd[c] * ar2 + ((c == 0 || c == 3 || c == 5) ? ar : 0.0));
+    }
+  }
+}
+
+// M1arr[v][a*Ncoef+k][b][lat][lon] from K2[v][j1][j2][t2*6+c][k]
+// (src/SphericalHarmonics.cc:1090-1131): t2=0 pole -> grid (j1, j2),
+// t2=1 pole -> grid (p0-j1, p0+j2); c = sym(a, b).
+template <typename T>
+__global__ void k_singular_rearrange(const T *__restrict__ K2, long Nv, int p0,
+                                     T *__restrict__ M1) {
+  const int Ncoef = p0 * (p0 + 2), Ngrid = 2 * p0 * (p0 + 1);
+  const long total = Nv * 3L * Ncoef * 3L * Ngrid;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long)gridDim.x * blockDim.x) {
+    const int g = (int)(i % Ngrid);
+    long r = i / Ngrid;
+    const int b = (int)(r % 3);
+    r /= 3;
+    const int k = (int)(r % Ncoef);
+    r /= Ncoef;
+    const int a = (int)(r % 3);
+    const long v = r / 3;
+    const int lat = g / (2 * p0), lon = g % (2 * p0);
+    int j1, j2, t2;
+    if (lon < p0) {
+      j1 = lat; j2 = lon; t2 = 0;
+    } else {
+      j1 = p0 - lat; j2 = lon - p0; t2 = 1;
+    }
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    const int c = lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
+    M1[i] = K2[((((v * (p0 + 1) + j1) * (long)p0 + j2) * 12 + t2 * 6 + c) * (long)Ncoef) + k];
+  }
+}
+
+template <typename T>
+__global__ void k_copy_cast_grid(const double *__restrict__ in, long n, T *__restrict__ out) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i] = (T)in[i];
+}
+
+namespace {
+std::map<std::pair<int, int>, double *> g_sw;  // singular weights per (device, p1)
+const double *sing_weights_dev(int p1) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto key = std::make_pair(dev, p1);
+  auto it = g_sw.find(key);
+  if (it != g_sw.end()) return it->second;
+  std::vector<double> w;
+  singular_weights(p1, w);
+  double *d = nullptr;
+  if (hipMalloc(&d, sizeof(double) * w.size()) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return nullptr;
+  g_sw[key] = d;
+  return d;
+}
+}  // namespace
+
+// Elements of scratch the build needs per vesicle of a chunk.
+static long singular_elems_per_ves(int p0, int p1) {
+  const long Nc = sh_ncoef(p0), Ng = sh_ngrid(p0), M1 = sh_ngrid(p1);
+  const long nr = (long)(p0 + 1) * p0;  // rotations per vesicle
+  long e = 0;
+  e += 3 * Ng;                          // X0 in T
+  e += 3 * Nc;                          // S0
+  e += nr * 3 * Nc;                     // S (rotated)
+  e += 3 * nr * 3 * M1;                 // X, Xt, Xp
+  e += nr * 3 * 2;                      // poles
+  e += nr * 12 * M1;                    // K0
+  e += nr * 12 * Nc;                    // K1
+  e += nr * 12 * Nc;                    // K2
+  e += 3 * Nc * 3 * Ng;                 // M1 rearranged
+  long w1 = std::max(shc2grid_work(nr * 3, p0, p1), grid2shc_work(nr * 12, p1, p0));
+  w1 = std::max(w1, grid2shc_work(9 * Nc, p0, p0));
+  w1 = std::max(w1, 3 * nr * 12 * Nc);  // rotation scratch
+  e += w1;
+  return e;
+}
+
+template <typename T>
+int singular_build(const double *X0, long Nv, int p0, int p1, T *SL, T *DL, hipStream_t st) {
+  if (p0 < 1 || p1 < 1 || p0 > 128 || p1 > 255 || Nv < 0) return VES3D_INVALID_PARAMETER;
+  if (Nv == 0 || (!SL && !DL)) return VES3D_SUCCESS;
+  if (!X0) return VES3D_INVALID_PARAMETER;
+  const long Nc = sh_ncoef(p0), Ng = sh_ngrid(p0), M1 = sh_ngrid(p1);
+  const long nr = (long)(p0 + 1) * p0;
+  const double *qw = sing_weights_dev(p1);
+  if (!qw) return VES3D_MEMORY_ERROR;
+  const long per = singular_elems_per_ves(p0, p1);
+  const long budget = (6L << 30) / (long)sizeof(T);
+  long chunk = std::max(1L, std::min(Nv, budget / per));
+  T *ws = (T *)workspace(kWsSing0, sizeof(T) * per * chunk);
+  if (!ws) return VES3D_MEMORY_ERROR;
+  for (long v0 = 0; v0 < Nv; v0 += chunk) {
+    const long cv = std::min(chunk, Nv - v0);
+    const long rot = cv * nr;
+    T *p = ws;
+    T *x0 = p; p += cv * 3 * Ng;
+    T *S0 = p; p += cv * 3 * Nc;
+    T *S = p; p += rot * 3 * Nc;
+    T *X = p; p += rot * 3 * M1;
+    T *Xt = p; p += rot * 3 * M1;
+    T *Xp = p; p += rot * 3 * M1;
+    T *P = p; p += rot * 3 * 2;
+    T *K0 = p; p += rot * 12 * M1;
+    T *K1 = p; p += rot * 12 * Nc;
+    T *K2 = p; p += rot * 12 * Nc;
+    T *M1a = p; p += cv * 3 * Nc * 3 * Ng;
+    T *w = p;
+    const long nx0 = cv * 3 * Ng;
+    k_copy_cast_grid<T><<<grid_for(nx0), 256, 0, st>>>(X0 + v0 * 3 * Ng, nx0, x0);
+    V3D_CHECK(launch_status());
+    V3D_CHECK(sht_grid2shc<T>(x0, cv * 3, p0, p0, S0, w, st));
+    V3D_CHECK(rotate_all<T>(S0, cv, p0, 3, S, w, st));
+    V3D_CHECK(sht_shc2grid<T>(S, rot * 3, p0, p1, X, Xt, Xp, w, st));
+    V3D_CHECK(sht_shc2pole<T>(S, rot * 3, p0, P, st));
+    for (int layer = 0; layer < 2; ++layer) {
+      T *dst = layer == 0 ? SL : DL;
+      if (!dst) continue;
+      k_pole_kernel<T><<<grid_for(rot * 2 * M1), 256, 0, st>>>(
+          X, Xt, Xp, P, qw, rot, p1, layer == 0 ? K0 : nullptr, layer == 1 ? K0 : nullptr);
+      V3D_CHECK(launch_status());
+      V3D_CHECK(sht_shc2grid_transpose<T>(K0, rot * 12, p1, p0, K1, w, st));
+      V3D_CHECK(rotate_transpose<T>(K1, cv, p0, 12, K2, w, st));
+      k_singular_rearrange<T><<<grid_for(cv * 9 * Nc * Ng), 256, 0, st>>>(K2, cv, p0, M1a);
+      V3D_CHECK(launch_status());
+      V3D_CHECK(sht_grid2shc<T>(M1a, cv * 9 * Nc, p0, p0, dst + v0 * 9 * Nc * Nc, w, st));
+    }
+  }
+  return VES3D_SUCCESS;
+}
+
+// ------------------------------------------------------------- self apply
+// V[v][c] = sum_r F[v][r] SL[v][r][c] (+ G[v][r] DL[v][r][c]), r, c < 3 Ncoef
+// (src/StokesVelocity.cc:397-447).  One workgroup per (vesicle, 256 columns);
+// matrix rows stream through coalesced, the density vector sits in LDS.
+template <typename TM>
+__global__ __launch_bounds__(256) void k_self_apply(const TM *__restrict__ SL,
+                                                     const double *__restrict__ F,
+                                                     const TM *__restrict__ DL,
+                                                     const double *__restrict__ G, int n,
+                                                     double *__restrict__ V) {
+  extern __shared__ double sh[];
+  const long v = blockIdx.y;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  double *f = sh, *g = sh + n;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    f[i] = SL ? F[v * n + i] : 0.0;
+    g[i] = DL ? G[v * n + i] : 0.0;
+  }
+  __syncthreads();
+  if (c >= n) return;
+  double acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+  if (SL) {
+    const TM *M = SL + v * (long)n * n + c;
+    int r = 0;
+    for (; r + 3 < n; r += 4) {
+      acc0 += f[r] * (double)M[(long)r * n];
+      acc1 += f[r + 1] * (double)M[(long)(r + 1) * n];
+      acc2 += f[r + 2] * (double)M[(long)(r + 2) * n];
+      acc3 += f[r + 3] * (double)M[(long)(r + 3) * n];
+    }
+    for (; r < n; ++r) acc0 += f[r] * (double)M[(long)r * n];
+  }
+  if (DL) {
+    const TM *M = DL + v * (long)n * n + c;
+    int r = 0;
+    for (; r + 3 < n; r += 4) {
+      acc0 += g[r] * (double)M[(long)r * n];
+      acc1 += g[r + 1] * (double)M[(long)(r + 1) * n];
+      acc2 += g[r + 2] * (double)M[(long)(r + 2) * n];
+      acc3 += g[r + 3] * (double)M[(long)(r + 3) * n];
+    }
+    for (; r < n; ++r) acc0 += g[r] * (double)M[(long)r * n];
+  }
+  V[v * n + c] = (acc0 + acc1) + (acc2 + acc3);
+}
+
+template <typename TM>
+int self_apply(const TM *SL, const double *F, const TM *DL, const double *G, long Nv, int p,
+               double *V, hipStream_t st) {
+  if (p < 1 || Nv < 0 || Nv > 65535) return VES3D_INVALID_PARAMETER;
+  if (Nv == 0) return VES3D_SUCCESS;
+  if (!V || (SL && !F) || (DL && !G)) return VES3D_INVALID_PARAMETER;
+  const int n = 3 * sh_ncoef(p);
+  if (!SL && !DL) return hip_status(hipMemsetAsync(V, 0, sizeof(double) * n * Nv, st));
+  dim3 grid(cdiv(n, 256), (unsigned)Nv);
+  k_self_apply<TM><<<grid, 256, 2 * n * sizeof(double), st>>>(SL, F, DL, G, n, V);
+  return launch_status();
+}
+
+template int rotate_all<double>(const double *, long, int, int, double *, double *, hipStream_t);
+template int rotate_transpose<double>(const double *, long, int, int, double *, double *, hipStream_t);
+template int singular_build<double>(const double *, long, int, int, double *, double *, hipStream_t);
+template int singular_build<float>(const double *, long, int, int, float *, float *, hipStream_t);
+template int self_apply<double>(const double *, const double *, const double *, const double *,
+                                long, int, double *, hipStream_t);
+template int self_apply<float>(const float *, const double *, const float *, const double *,
+                               long, int, double *, hipStream_t);
+
+}  // namespace v3d
+
+using namespace v3d;
+
+extern "C" {
+int ves3d_hip_rotate_all_f64(const double *S, size_t N, int p, int dof, double *S_out,
+                             ves3d_stream_t stream) {
+  if (!S || !S_out || dof < 1 || p < 1) return VES3D_INVALID_PARAMETER;
+  const long nc = sh_ncoef(p);
+  double *w = (double *)workspace(kWsSing1, sizeof(double) * (long)N * p * dof * nc * (p + 2));
+  if (!w) return VES3D_MEMORY_ERROR;
+  return rotate_all<double>(S, (long)N, p, dof, S_out, w, (hipStream_t)stream);
+}
+int ves3d_hip_rotate_transpose_f64(const double *S_in, size_t N, int p, int dof, double *S_out,
+                                   ves3d_stream_t stream) {
+  if (!S_in || !S_out || dof < 1 || p < 1) return VES3D_INVALID_PARAMETER;
+  const long nc = sh_ncoef(p);
+  double *w = (double *)workspace(kWsSing1, sizeof(double) * 2L * N * (p + 1) * p * dof * nc);
+  if (!w) return VES3D_MEMORY_ERROR;
+  return rotate_transpose<double>(S_in, (long)N, p, dof, S_out, w, (hipStream_t)stream);
+}
+int ves3d_hip_singular_build_f64(const double *X0, size_t nves, int p0, int p1, double *SL,
+                                 double *DL, ves3d_stream_t stream) {
+  return singular_build<double>(X0, (long)nves, p0, p1, SL, DL, (hipStream_t)stream);
+}
+int ves3d_hip_singular_build_f32(const double *X0, size_t nves, int p0, int p1, float *SL,
+                                 float *DL, ves3d_stream_t stream) {
+  return singular_build<float>(X0, (long)nves, p0, p1, SL, DL, (hipStream_t)stream);
+}
+int ves3d_hip_self_apply_f64(const double *SL, const double *F, const double *DL,
+                             const double *G, size_t nves, int p, double *V,
+                             ves3d_stream_t stream) {
+  return self_apply<double>(SL, F, DL, G, (long)nves, p, V, (hipStream_t)stream);
+}
+int ves3d_hip_self_apply_f32(const float *SL, const double *F, const float *DL,
+                             const double *G, size_t nves, int p, double *V,
+                             ves3d_stream_t stream) {
+  return self_apply<float>(SL, F, DL, G, (long)nves, p, V, (hipStream_t)stream);
+}
+}

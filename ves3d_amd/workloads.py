@@ -51,6 +51,61 @@ def random_centres(nv, volfrac, radius=1.0, seed=SEED_BASE, gap=0.0):
     return out, side
 
 
+def _radial_perturbation(p, amp, rng, lmax=None):
+    """Band-limited random field on the grid: sum over l=1..lmax, m=0..l of
+    (a cos m phi + b sin m phi) Pbar_l^m(x), a, b ~ N(0, (amp/(l+1)^2)^2),
+    Pbar orthonormal (scipy lpmv with the normalisation factor)."""
+    from scipy.special import lpmv, factorial
+    lmax = p if lmax is None else lmax
+    x, _ = gl_nodes(p)
+    phi = np.pi * np.arange(2 * p) / p
+    r = np.zeros((p + 1, 2 * p))
+    for l in range(1, lmax + 1):
+        sd = amp / (l + 1) ** 2
+        for m in range(0, l + 1):
+            nrm = np.sqrt((2 * l + 1) / (4 * np.pi) * factorial(l - m) / factorial(l + m))
+            P = nrm * lpmv(m, l, x)
+            a, b = rng.normal(0, sd, 2)
+            r += P[:, None] * (a * np.cos(m * phi) + (b * np.sin(m * phi) if m > 0 else 0.0))[None, :]
+    return r.reshape(-1)
+
+
+def perturbed_shapes(nves, p, amp=0.05, seed=SEED_BASE, radius=1.0):
+    """Unit spheres with a seeded radial perturbation (coefficients
+    N(0, (amp/(l+1)^2)^2), l = 1..p).  Returns [nves][3][2p(p+1)] centred at 0."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    X, _ = sphere_grid(p)
+    out = np.empty((nves, 3, X.shape[1]))
+    for v in range(nves):
+        rr = radius * (1.0 + _radial_perturbation(p, amp, rng))
+        out[v] = X * rr[None, :]
+    return out
+
+
+def ellipsoid_shapes(nves, p, seed=SEED_BASE, c_mean=0.6, c_jitter=0.05, amp=0.0):
+    """Ellipsoids with semi-axes (1, 1, c), c = c_mean +- c_jitter uniform,
+    the short axis along the grid's polar (first) coordinate."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    X, _ = sphere_grid(p)
+    out = np.empty((nves, 3, X.shape[1]))
+    for v in range(nves):
+        c = c_mean + rng.uniform(-c_jitter, c_jitter)
+        rr = 1.0 + (_radial_perturbation(p, amp, rng) if amp > 0 else 0.0)
+        out[v] = X * rr[None, :]
+        out[v, 0] *= c
+    return out
+
+
+def suspension(shapes, volfrac, seed=SEED_BASE, gap=0.05):
+    """Places the given vesicle shapes [nv][3][np] at rejection-sampled
+    centres in a cube at volume fraction `volfrac` (bounding radius used
+    for overlap rejection).  Returns positions [nv][3][np]."""
+    nv = shapes.shape[0]
+    rad = float(np.max(np.sqrt(np.sum(shapes ** 2, axis=1))))
+    c, _ = random_centres(nv, volfrac * rad ** 3, rad, seed, gap)
+    return shapes + c[:, :, None]
+
+
 def stokes_direct_sum_case(nv=512, p=16, volfrac=0.15, seed=SEED_BASE + 1):
     """The 'Stokes direct summation alone' workload: nv unit spheres at SH
     order p (2p(p+1) points each) with centres uniform in a cube at the given
