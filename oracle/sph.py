@@ -130,6 +130,61 @@ def legpoly_deriv(X, degree):
     return out
 
 
+def legpoly_ladder(X, degree):
+    """First and second latitude derivatives of sqrt-normalised Legendre
+    functions by the ladder of matlab/Ynm.m:18-40 (the generator of the
+    reference's d1legTrans / d2legTrans files):
+        DP_m  = c_m P_{m-1} - c_{m+1} P_{m+1},  c_k = sqrt((n+k)(n+1-k))/2,
+        D2P_m = c_m DP_{m-1} - c_{m+1} DP_{m+1},
+    with P_{-m} = (-1)^m P_m.  Returns (H, W) = (-DP, D2P) in the LegPoly row
+    order; H equals -legpoly_deriv."""
+    X = np.asarray(X, dtype=np.float64)
+    P = legpoly(X, degree)
+    H = np.zeros_like(P)
+    Wd = np.zeros_like(P)
+
+    def idx(m, n):
+        return ((degree * 2 - m + 1) * m) // 2 + n
+
+    for n in range(degree + 1):
+        full = {}
+        for m in range(-n - 1, n + 2):
+            if abs(m) > n:
+                full[m] = np.zeros_like(X)
+            elif m >= 0:
+                full[m] = P[idx(m, n)]
+            else:
+                full[m] = (-1) ** (-m) * P[idx(-m, n)]
+        c = {k: np.sqrt(max((n + k) * (n + 1 - k), 0)) / 2.0 for k in range(-n - 1, n + 3)}
+        DP = {}
+        for m in range(-n - 1, n + 2):
+            DP[m] = np.zeros_like(X) if abs(m) > n else c[m] * full[m - 1] - c[m + 1] * full[m + 1]
+        for m in range(0, n + 1):
+            H[idx(m, n)] = -DP[m]
+            Wd[idx(m, n)] = c[m] * DP[m - 1] - c[m + 1] * DP[m + 1]
+    return H, Wd
+
+
+@functools.lru_cache(maxsize=None)
+def mat_legendre_d2(p0, p1):
+    """Second latitude-derivative analogue of MatLegendre (the d2legTrans
+    matrices of src/OperatorsMats.cc:26-93, generated by matlab/Ynm.m)."""
+    return tuple(_split_blocks(legpoly_ladder(legendre_nodes(p1), p0)[1], p0))
+
+
+@functools.lru_cache(maxsize=None)
+def mat_fourier_d2(p0, p1):
+    """Second longitude derivative of MatFourier (src/SHTMats.cc:160-187 up to
+    the sqrt(2 pi) factor SphericalHarmonics carries in the Fourier matrix)."""
+    M = np.zeros((2 * p0, 2 * p1))
+    j = np.arange(2 * p1)
+    for k in range(1, p0):
+        M[2 * k - 1] = -SQRT2PI * k * k * np.cos(j * k * np.pi / p1)
+        M[2 * k] = -SQRT2PI * k * k * np.sin(j * k * np.pi / p1)
+    M[2 * p0 - 1] = -SQRT2PI * p0 * p0 * np.cos(j * p0 * np.pi / p1)
+    return M
+
+
 # ---------------------------------------------------------------- matrices
 @functools.lru_cache(maxsize=None)
 def mat_fourier(p0, p1):
@@ -242,6 +297,21 @@ def shc2grid(S, p0, p1, want_theta=False, want_phi=False):
         B1d = np.stack(_legendre_stage(B0, mat_legendre_grad(p0, p1), N), axis=0)
         res.append((B1d.reshape(2 * p0, -1).T @ Mf).reshape(N, -1))
     return res[0] if len(res) == 1 else tuple(res)
+
+
+def shc2grid_d(S, p0, p1, dleg=0, dfour=0):
+    """SHC2Grid with the Legendre stage differentiated dleg times (latitude
+    direction) and the Fourier stage dfour times (longitude), dleg, dfour in
+    {0, 1, 2}: the synthesis behind SHTrans::backward, backward_du / dv /
+    d2u / d2v / duv (src/SHTrans.cc:123-170)."""
+    S = np.asarray(S).reshape(-1, p0 * (p0 + 2))
+    N = S.shape[0]
+    off, L = block_offsets(p0)
+    Ml = (mat_legendre, mat_legendre_grad, mat_legendre_d2)[dleg](p0, p1)
+    Mf = (mat_fourier, mat_fourier_grad, mat_fourier_d2)[dfour](p0, p1)
+    B0 = [S[:, off[j]:off[j] + L[j]] for j in range(2 * p0)]
+    B1 = np.stack(_legendre_stage(B0, Ml, N), axis=0)
+    return (B1.reshape(2 * p0, -1).T @ Mf).reshape(N, -1)
 
 
 def _grid2shc_with(X, p0, p1, Mf, Ml):

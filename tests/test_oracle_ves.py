@@ -1,0 +1,163 @@
+"""Pins the oracle of the implicit-step layer (oracle/ves.py).
+
+  SHTrans matrices : the reference's own precomputed legTrans, legTransInv,
+                     d1legTrans, d2legTrans files at p = 6, 12 (tests/golden).
+  geometry         : unit sphere H = +1 (x_u x x_v points inward on this
+                     grid), K = 1,
+                     area 4 pi, volume 4 pi / 3; perturbed shape converges.
+  forces           : bending force of a sphere vanishes; linearised mean
+                     curvature of the position itself is the mean curvature.
+  Stokes operator  : SL of a constant density on a lone unit sphere = (2/3) c.
+  matvec           : exactly linear (test/InterfacialVelocityTest.cc:93-112).
+  BiCGStab         : the diagonal system of test/BiCGStabTest.cc and random
+                     nonsymmetric systems against numpy.linalg.solve.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import sph, ves
+from ves3d_amd import workloads as W
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _file_blocks(name, p, inverse):
+    """Unpacks a legTrans-style file into per-order [deg][lat] blocks."""
+    v = np.loadtxt(os.path.join(GOLD, name))
+    out = []
+    pos = 0
+    for m in range(p + 1):
+        nd = p + 1 - m
+        blk = v[pos:pos + nd * (p + 1)]
+        pos += nd * (p + 1)
+        if inverse:   # stored (lat x deg) column-major
+            out.append(blk.reshape(nd, p + 1))
+        else:         # stored (deg x lat) column-major
+            out.append(blk.reshape(p + 1, nd).T)
+    assert pos == v.size
+    return out
+
+
+@pytest.mark.parametrize("p", [6, 12])
+def test_shtrans_matrices_match_reference_files(p):
+    x, w = sph.legendre_nodes_weights(p)
+    s2p = np.sqrt(2 * np.pi)
+    P = sph._split_blocks(sph.legpoly(x, p), p)
+    H = sph._split_blocks(sph.legpoly_deriv(x, p), p)
+    Hl, Wl = sph.legpoly_ladder(x, p)
+    Hl = sph._split_blocks(Hl, p)
+    Wl = sph._split_blocks(Wl, p)
+    fwd = _file_blocks("legTrans%d_double.txt" % p, p, False)
+    inv = _file_blocks("legTransInv%d_double.txt" % p, p, True)
+    d1 = _file_blocks("d1legTrans%d_double.txt" % p, p, True)
+    d2 = _file_blocks("d2legTrans%d_double.txt" % p, p, True)
+    for m in range(p + 1):
+        assert np.allclose(fwd[m], s2p * P[m] * w[None, :], atol=1e-13), m
+        assert np.allclose(inv[m], s2p * P[m], atol=1e-13), m
+        # d1legTrans holds matlab/Ynm.m's Hn = -DPn, which is -LegPolyDeriv:
+        # SHTrans's latitude derivative has the opposite sign of
+        # SphericalHarmonics' (two independent derivations agree on it)
+        assert np.allclose(d1[m], -s2p * H[m], atol=1e-12), m
+        assert np.allclose(d1[m], s2p * Hl[m], atol=1e-12), m
+        assert np.allclose(d2[m], s2p * Wl[m], atol=1e-11), m
+
+
+@pytest.mark.parametrize("p", [6, 12, 16])
+def test_sphere_geometry(p):
+    X, _ = W.sphere_grid(p)
+    S = ves.Surface(X[None], p)
+    assert np.allclose(S.H, 1.0, atol=1e-11)
+    assert np.allclose(S.K, 1.0, atol=1e-11)
+    assert abs(S.area()[0] - 4 * np.pi) < 1e-11
+    assert abs(abs(S.volume()[0]) - 4 * np.pi / 3) < 1e-11
+
+
+def test_sphere_forces():
+    p = 8
+    X, _ = W.sphere_grid(p)
+    S = ves.Surface(X[None] * 1.0, p)
+    F = ves.Forces(S, 2 * p, 0.01)
+    assert np.max(np.abs(F.bending())) < 1e-11
+    h = S.linearized_mean_curv(S.X)
+    assert np.allclose(h, S.H, atol=1e-12)
+    # tension force of a uniform tension on a sphere: 2 sigma H n
+    sig = np.full((1, X.shape[1]), 0.7)
+    Fs = F.tensile(sig)
+    assert np.allclose(Fs, 2 * 0.7 * 1.0 * S.normal, atol=1e-10)
+
+
+def test_perturbed_geometry_converges():
+    """Area of a band-limited perturbed shape is resolved spectrally."""
+    Xs = W.perturbed_shapes(1, 6, amp=0.1, seed=2)
+    a = []
+    for q in (6, 12, 24):
+        Xq = ves.resample_vec(Xs, 6, q)
+        a.append(ves.Surface(Xq, q).area()[0])
+    assert abs(a[1] - a[2]) < 1e-3 * abs(a[0] - a[2]) + 1e-12 or abs(a[1] - a[2]) < 1e-9
+
+
+def test_stokes_surface_lone_sphere():
+    p = 8
+    X, _ = W.sphere_grid(p)
+    st = ves.StokesSurface(X[None], p, 2 * p)
+    c = np.array([0.3, -1.1, 0.7])
+    f = np.broadcast_to(c[None, :, None], (1, 3, X.shape[1])).copy()
+    assert np.allclose(st(f), 2.0 / 3.0 * f, atol=1e-9)
+
+
+def _small_suspension(nv=2, p=6, seed=3):
+    shapes = W.perturbed_shapes(nv, p, amp=0.05, seed=seed)
+    return W.suspension(shapes, 0.05, seed=seed)
+
+
+def test_matvec_is_linear():
+    X = _small_suspension()
+    imp = ves.Implicit(X, 6, dt=1e-3)
+    n = 4 * imp.nv * imp.nc
+    rng = np.random.default_rng(0)
+    v1, v2 = rng.standard_normal(n), rng.standard_normal(n)
+    assert np.max(np.abs(imp.matvec(np.zeros(n)))) < 1e-14
+    a = imp.matvec(v1) + 2 * imp.matvec(v2)
+    b = imp.matvec(v1 + 2 * v2)
+    assert np.max(np.abs(a - b)) < 1e-10 * np.max(np.abs(a))
+
+
+def test_bicgstab_diag():
+    n = 4 * 12 * 14
+    d = (np.arange(n) + 1.0) ** 2
+    b = np.random.default_rng(1).uniform(size=n)
+    # the reference test only prints the residual of this ill-conditioned
+    # system (cond ~7e6); with the exact diagonal preconditioner it converges
+    x, res, it, st = ves.bicgstab(lambda v: d * v, b, np.zeros(n), 1e-6, 1000, lambda v: v / d)
+    assert st == "success" and res < 1e-6 and it <= 2
+    assert np.linalg.norm(d * x - b) / np.linalg.norm(b) < 1e-6
+    d2 = (np.arange(60) + 1.0) ** 2
+    x, res, it, st = ves.bicgstab(lambda v: d2 * v, b[:60], np.zeros(60), 1e-6, 1000)
+    assert st == "success" and np.linalg.norm(d2 * x - b[:60]) / np.linalg.norm(b[:60]) < 1e-6
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_bicgstab_random(seed):
+    rng = np.random.default_rng(seed)
+    n = 60
+    A = np.eye(n) * 4 + rng.standard_normal((n, n)) * 0.3
+    b = rng.standard_normal(n)
+    x, res, it, st = ves.bicgstab(lambda v: A @ v, b, np.zeros(n), 1e-12, 500)
+    assert st == "success"
+    assert np.allclose(x, np.linalg.solve(A, b), atol=1e-9)
+    P = np.diag(1.0 / np.diag(A))
+    x2, res2, it2, st2 = ves.bicgstab(lambda v: A @ v, b, np.zeros(n), 1e-12, 500, lambda v: P @ v)
+    assert st2 == "success" and np.allclose(x2, np.linalg.solve(A, b), atol=1e-9)
+
+
+def test_implicit_solve_small():
+    """One implicit solve on two vesicles at p=6 reaches the tolerance and its
+    true residual confirms it (src/InterfacialVelocity.cc:181-188)."""
+    X = _small_suspension()
+    imp = ves.Implicit(X, 6, dt=1e-3, shear=1.0)
+    b = imp.rhs()
+    x, res, it, st = ves.bicgstab(imp.matvec, b, np.zeros_like(b), 1e-6, 200, imp.precond)
+    assert st == "success"
+    assert np.linalg.norm(imp.matvec(x) - b) / np.linalg.norm(b) < 1e-6
