@@ -202,6 +202,80 @@ int ves3d_hip_self_apply_f32(const float *SL, const double *F, const float *DL,
                              const double *G, size_t nves, int p, double *V,
                              ves3d_stream_t stream);
 
+/* ---- implicit solver and time step -------------------------------------- */
+
+/* Linear operator / preconditioner callback on device vectors, the device
+ * counterpart of ParallelLinOp<T>::apply_type and
+ * ParallelLinSolver<T>::precond_type (include/ParallelLinSolverInterface.h:111,
+ * :160): y = Op(x); returns 0 on success. */
+typedef int (*ves3d_apply_fn)(const void *ctx, const double *x, double *y);
+
+/* BiCGStab<Container, MatVec, Precond>::operator() (include/BiCGStab.h:62-65,
+ * src/BiCGStab.cc:45-138): right-preconditioned BiCGStab on device vectors of
+ * length n with FP64 inner products.  P may be NULL (identity).  On return
+ * *max_iter holds the iterations used, *tol the achieved relative residual
+ * and *bicgs_status a BiCGSReturn code (100 success, 101 max iterations,
+ * 102 rho breakdown, 103 omega breakdown, 104 NaN residual). */
+int ves3d_hip_bicgstab(size_t n, ves3d_apply_fn A, const void *A_ctx,
+                       ves3d_apply_fn P, const void *P_ctx, double *x,
+                       const double *b, int *max_iter, double *tol,
+                       int *bicgs_status, ves3d_stream_t stream);
+
+/* Parameters of the globally implicit interfacial-velocity step
+ * (InterfacialVelocity::updateImplicit with solve_for_velocity,
+ * src/InterfacialVelocity.cc:203-560): shear background flow
+ * u = (shear_rate z, 0, 0), no viscosity contrast, no gravity. */
+typedef struct ves3d_step_params {
+  size_t nves;            /* number of vesicles */
+  int p;                  /* SH order of the surfaces */
+  int p_up;               /* upsampled order (0: 2p, the reference default) */
+  int filter_freq;        /* low-pass filter frequency (0: 2p/3) */
+  double dt;              /* time step */
+  double shear_rate;      /* background shear rate */
+  double bending_modulus; /* kappa_b, the same for every vesicle */
+  double tol;             /* BiCGStab relative residual tolerance */
+  int max_iter;           /* BiCGStab iteration limit */
+  int precision;          /* 0: FP64; 1: FP32 singular operator + FP32-compensated far field */
+  int precond;            /* 0: none; 1: diagonal spectral (ConfigurePrecond); 2: learned */
+} ves3d_step_params;
+
+typedef struct ves3d_step_stats {
+  int iterations;         /* BiCGStab iterations */
+  int matvecs;            /* operator applications (incl. residual checks) */
+  int status;             /* BiCGSReturn code */
+  int fell_back;          /* 1 if the learned preconditioner was abandoned */
+  double relres;          /* recursive relative residual at exit */
+  double true_relres;     /* ||b - A x|| / ||b|| of the returned solution */
+  double ms_setup, ms_rhs, ms_solve, ms_total;
+} ves3d_step_stats;
+
+/* Creates / destroys a stepper (device state for nves vesicles). */
+int ves3d_hip_stepper_create(const ves3d_step_params *params, void **handle_host);
+int ves3d_hip_stepper_destroy(void *handle);
+/* Length of the Galerkin solver vector, 4 nves p(p+2) (velocity block of
+ * 3 nves functions then tension block of nves functions, each in the
+ * order-major SHTrans layout, src/InterfacialVelocity.cc:562-573). */
+int ves3d_hip_stepper_size(void *handle, size_t *n_host);
+/* Copies the state in / out: positions X [nves][3][(p+1)2p], interfacial
+ * velocity U (same layout) and tension sigma [nves][(p+1)2p]; any may be NULL. */
+int ves3d_hip_stepper_set_state(void *handle, const double *X, const double *U,
+                                const double *sigma);
+int ves3d_hip_stepper_get_state(void *handle, double *X, double *U, double *sigma);
+/* Geometry, singular operators and far-field setup for the current positions
+ * (Prepare + StokesVelocity setup, src/InterfacialVelocity.cc:576-611). */
+int ves3d_hip_stepper_prepare(void *handle);
+/* AssembleRhsVel (src/InterfacialVelocity.cc:715-787) into b [n]. */
+int ves3d_hip_stepper_rhs(void *handle, double *b);
+/* ImplicitApply (src/InterfacialVelocity.cc:981-1052): y = A x. */
+int ves3d_hip_stepper_matvec(void *handle, const double *x, double *y);
+/* ImplicitPrecond with the diagonal spectral preconditioner
+ * (src/InterfacialVelocity.cc:1054-1109): y = D x. */
+int ves3d_hip_stepper_precond(void *handle, const double *x, double *y);
+/* One full implicit step: prepare, right-hand side, BiCGStab from the
+ * previous (velocity, tension) as initial guess, true-residual check,
+ * X += dt U.  stats_host may be NULL. */
+int ves3d_hip_stepper_step(void *handle, ves3d_step_stats *stats_host);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,17 @@ for _t in ("f64", "f32"):
     _sig("ves3d_hip_self_apply_" + _t, P, P, P, P, S, c_int, P, P)
 _sig("ves3d_hip_rotate_all_f64", P, S, c_int, c_int, P, P)
 _sig("ves3d_hip_rotate_transpose_f64", P, S, c_int, c_int, P, P)
+_sig("ves3d_hip_bicgstab", S, P, P, P, P, P, P, P, P, P, P)
+_sig("ves3d_hip_stepper_create", P, P)
+_sig("ves3d_hip_stepper_destroy", P)
+_sig("ves3d_hip_stepper_size", P, P)
+_sig("ves3d_hip_stepper_set_state", P, P, P, P)
+_sig("ves3d_hip_stepper_get_state", P, P, P, P)
+_sig("ves3d_hip_stepper_prepare", P)
+_sig("ves3d_hip_stepper_rhs", P, P)
+_sig("ves3d_hip_stepper_matvec", P, P, P)
+_sig("ves3d_hip_stepper_precond", P, P, P)
+_sig("ves3d_hip_stepper_step", P, P)
 
 _lib = None
 
@@ -127,4 +138,4 @@ def device_info():
     return name.value.decode(), cu.value, clk.value
 
 
-from . import sph, stokes  # noqa: E402,F401
+from . import implicit, sph, stokes  # noqa: E402,F401

@@ -1,0 +1,469 @@
+// implicit.hip -- Surface geometry, interfacial forces and the surface
+// Stokes operator of the implicit step (see implicit.hpp).  All fields are
+// AxisMajor: vectors [nv][3][ng], scalars [nv][ng]; coefficient arrays are
+// function-major [nf][p(p+2)] (the order-major Galerkin layout is produced
+// only when packing the solver vector).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "implicit.hpp"
+#include "runtime.hpp"
+#include "sh_tables.hpp"
+#include "sht.hpp"
+#include "singular.hpp"
+
+extern "C" int ves3d_hip_direct_eval_f64(const double *, const double *, const double *, size_t,
+                                         const double *, double *, size_t, size_t, size_t, void *);
+extern "C" int ves3d_hip_direct_eval_f32c(const double *, const double *, const double *, size_t,
+                                          const double *, double *, size_t, size_t, size_t, void *);
+
+namespace v3d {
+
+int DBuf::alloc(long count) {
+  release();
+  if (count <= 0) count = 1;
+  if (hipMalloc(&p, sizeof(double) * count) != hipSuccess) {
+    p = nullptr;
+    return VES3D_MEMORY_ERROR;
+  }
+  n = count;
+  return VES3D_SUCCESS;
+}
+void DBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  n = 0;
+}
+
+namespace {
+inline unsigned gblocks(long n) {
+  long b = (n + 255) / 256;
+  if (b > 65536) b = 65536;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+#define GRID_LOOP(i, n) \
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (long)gridDim.x * blockDim.x)
+
+// First fundamental forms, area element, normal and the grad/div
+// coefficients cu, cv (src/Surface.cc:211-262).
+__global__ void k_first_forms(const double *__restrict__ du, const double *__restrict__ dv,
+                              long nv, long ng, double *E, double *F, double *G, double *W,
+                              double *cu, double *cv, double *nrm) {
+  GRID_LOOP(i, nv * ng) {
+    const long v = i / ng, g = i % ng;
+    const long b = v * 3 * ng + g;
+    const double ux = du[b], uy = du[b + ng], uz = du[b + 2 * ng];
+    const double vx = dv[b], vy = dv[b + ng], vz = dv[b + 2 * ng];
+    const double e = ux * ux + uy * uy + uz * uz;
+    const double f = ux * vx + uy * vy + uz * vz;
+    const double gg = vx * vx + vy * vy + vz * vz;
+    const double w2 = e * gg - f * f;
+    const double es = e / w2, fs = f / w2, gs = gg / w2;
+    const double w = sqrt(w2);
+    E[i] = es;
+    F[i] = fs;
+    G[i] = gs;
+    W[i] = w;
+    cu[b] = gs * ux - fs * vx;
+    cu[b + ng] = gs * uy - fs * vy;
+    cu[b + 2 * ng] = gs * uz - fs * vz;
+    cv[b] = es * vx - fs * ux;
+    cv[b + ng] = es * vy - fs * uy;
+    cv[b + 2 * ng] = es * vz - fs * uz;
+    nrm[b] = (uy * vz - uz * vy) / w;
+    nrm[b + ng] = (uz * vx - ux * vz) / w;
+    nrm[b + 2 * ng] = (ux * vy - uy * vx) / w;
+  }
+}
+
+// Mean and Gaussian curvature before filtering (src/Surface.cc:264-321).
+__global__ void k_second_forms(const double *__restrict__ duv, const double *__restrict__ d2u,
+                               const double *__restrict__ d2v, const double *__restrict__ nrm,
+                               const double *E, const double *F, const double *G,
+                               const double *W, long nv, long ng, double *H, double *K) {
+  GRID_LOOP(i, nv * ng) {
+    const long v = i / ng, g = i % ng;
+    const long b = v * 3 * ng + g;
+    const double nx = nrm[b], ny = nrm[b + ng], nz = nrm[b + 2 * ng];
+    const double M = duv[b] * nx + duv[b + ng] * ny + duv[b + 2 * ng] * nz;
+    const double L = d2u[b] * nx + d2u[b + ng] * ny + d2u[b + 2 * ng] * nz;
+    const double N = d2v[b] * nx + d2v[b + ng] * ny + d2v[b + 2 * ng] * nz;
+    H[i] = -F[i] * M + 0.5 * G[i] * L + 0.5 * E[i] * N;
+    K[i] = (L * N - M * M) / W[i] / W[i];
+  }
+}
+
+// Linearised mean curvature before filtering (src/Surface.cc:529-566).
+__global__ void k_lin_mean_curv(const double *__restrict__ d2u, const double *__restrict__ duv,
+                                const double *__restrict__ d2v, const double *__restrict__ nrm,
+                                const double *E, const double *F, const double *G, long nv,
+                                long ng, double *out) {
+  GRID_LOOP(i, nv * ng) {
+    const long v = i / ng, g = i % ng;
+    const long b = v * 3 * ng + g;
+    const double nx = nrm[b], ny = nrm[b + ng], nz = nrm[b + 2 * ng];
+    const double a = d2u[b] * nx + d2u[b + ng] * ny + d2u[b + 2 * ng] * nz;
+    const double m = duv[b] * nx + duv[b + ng] * ny + duv[b + 2 * ng] * nz;
+    const double c = d2v[b] * nx + d2v[b + ng] * ny + d2v[b + 2 * ng] * nz;
+    out[i] = 0.5 * (G[i] * a - 2.0 * F[i] * m + E[i] * c);
+  }
+}
+
+// grad f = f_u cu + f_v cv  (src/Surface.cc:323-346)
+__global__ void k_grad_comb(const double *__restrict__ fu, const double *__restrict__ fv,
+                            const double *__restrict__ cu, const double *__restrict__ cv,
+                            long nv, long ng, double *out) {
+  GRID_LOOP(i, nv * ng) {
+    const long v = i / ng, g = i % ng;
+    const long b = v * 3 * ng + g;
+    const double a = fu[i], c = fv[i];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) out[b + d * ng] = a * cu[b + d * ng] + c * cv[b + d * ng];
+  }
+}
+
+// div V = V_u . cu + V_v . cv  (src/Surface.cc:348-377)
+__global__ void k_div_comb(const double *__restrict__ Vu, const double *__restrict__ Vv,
+                           const double *__restrict__ cu, const double *__restrict__ cv, long nv,
+                           long ng, double *out) {
+  GRID_LOOP(i, nv * ng) {
+    const long v = i / ng, g = i % ng;
+    const long b = v * 3 * ng + g;
+    double s = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) s += Vv[b + d * ng] * cv[b + d * ng];
+    double t = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) t += Vu[b + d * ng] * cu[b + d * ng];
+    out[i] = s + t;
+  }
+}
+
+__global__ void k_scale_rows(double *S, long nf, int nc, const double *__restrict__ scale) {
+  GRID_LOOP(i, nf * nc) S[i] *= scale[i % nc];
+}
+
+// s2 = a * (H^2 - K)
+__global__ void k_bend_s2(const double *__restrict__ a, const double *__restrict__ H,
+                          const double *__restrict__ K, long n, double *s2) {
+  GRID_LOOP(i, n) s2[i] = a[i] * (H[i] * H[i] - K[i]);
+}
+
+// F = kb (s1 + 2 s2) n   (src/InterfacialForce.cc:37-40)
+__global__ void k_bend_force(const double *__restrict__ s1, const double *__restrict__ s2,
+                             const double *__restrict__ nrm, double kb, long nv, long ng,
+                             double *Fo) {
+  GRID_LOOP(i, nv * ng) {
+    const long v = i / ng, g = i % ng;
+    const long b = v * 3 * ng + g;
+    const double s = kb * (s1[i] + 2.0 * s2[i]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) Fo[b + d * ng] = s * nrm[b + d * ng];
+  }
+}
+
+// F = 2 sigma H n + grad sigma   (src/InterfacialForce.cc:109-112)
+__global__ void k_tens_force(const double *__restrict__ sig, const double *__restrict__ H,
+                             const double *__restrict__ nrm, const double *__restrict__ gs,
+                             long nv, long ng, double *Fo) {
+  GRID_LOOP(i, nv * ng) {
+    const long v = i / ng, g = i % ng;
+    const long b = v * 3 * ng + g;
+    const double s = 2.0 * sig[i] * H[i];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) Fo[b + d * ng] = s * nrm[b + d * ng] + gs[b + d * ng];
+  }
+}
+
+// Far-field source setup (src/StokesVelocity.cc:196-277, :307-337):
+// far[v][0..1] = poles, far[v][2+j] = upsampled points; aqw = |x_t x x_p| w(lat).
+__global__ void k_far_setup(const double *__restrict__ X, const double *__restrict__ Xt,
+                            const double *__restrict__ Xp, const double *__restrict__ pole,
+                            const double *__restrict__ qw, long nv, int pu, double *far,
+                            double *aqw) {
+  const long mu = 2L * pu * (pu + 1);
+  GRID_LOOP(i, nv * (mu + 2)) {
+    const long v = i / (mu + 2), j = i % (mu + 2);
+    double *o = far + i * 3;
+    if (j < 2) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) o[d] = pole[(v * 3 + d) * 2 + j];
+    } else {
+      const long s = j - 2;
+      const long b = v * 3 * mu + s;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) o[d] = X[b + d * mu];
+      const double xt = Xt[b], yt = Xt[b + mu], zt = Xt[b + 2 * mu];
+      const double xp = Xp[b], yp = Xp[b + mu], zp = Xp[b + 2 * mu];
+      const double nx = yt * zp - zt * yp, ny = zt * xp - xt * zp, nz = xt * yp - yt * xp;
+      aqw[v * mu + s] = sqrt(nx * nx + ny * ny + nz * nz) * qw[s / (2 * pu)];
+    }
+  }
+}
+
+// AxisMajor [nv][3][ng] -> point-major [nv*ng][3]
+__global__ void k_to_points(const double *__restrict__ X, long nv, long ng, double *P) {
+  GRID_LOOP(i, nv * ng) {
+    const long v = i / ng, g = i % ng;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) P[i * 3 + d] = X[(v * 3 + d) * ng + g];
+  }
+}
+
+// q[v][0..1] = 0, q[v][2+s] = f_up[v][:][s] * aqw[v][s]
+__global__ void k_pack_density(const double *__restrict__ fu, const double *__restrict__ aqw,
+                               long nv, long mu, double *q) {
+  GRID_LOOP(i, nv * (mu + 2)) {
+    const long v = i / (mu + 2), j = i % (mu + 2);
+    double *o = q + i * 3;
+    if (j < 2) {
+      o[0] = o[1] = o[2] = 0.0;
+    } else {
+      const long s = j - 2;
+      const double w = aqw[v * mu + s];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) o[d] = fu[(v * 3 + d) * mu + s] * w;
+    }
+  }
+}
+
+// vel[v][d][g] += far[v*ng+g][d]
+__global__ void k_add_points(const double *__restrict__ P, long nv, long ng, double *vel) {
+  GRID_LOOP(i, nv * ng) {
+    const long v = i / ng, g = i % ng;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) vel[(v * 3 + d) * ng + g] += P[i * 3 + d];
+  }
+}
+
+std::vector<double> box_filter(int p, int f) {  // src/SHTrans.cc:23-31
+  std::vector<double> b(sh_ncoef(p));
+  for (int j = 0; j < 2 * p; ++j) {
+    const int m = (j + 1) / 2, off = sh_blockoff(p, j), len = sh_blocklen(p, j);
+    for (int k = 0; k < len; ++k) b[off + k] = (m + k) > f ? 0.0 : 1.0;
+  }
+  return b;
+}
+
+int upload(DBuf &d, const std::vector<double> &h) {
+  V3D_CHECK(d.alloc((long)h.size()));
+  return hip_status(hipMemcpy(d.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+}
+}  // namespace
+
+// ------------------------------------------------------------- SurfaceG
+int SurfaceG::init(int p_, int f_, long nv_) {
+  p = p_;
+  f = f_;
+  nv = nv_;
+  ng = sh_ngrid(p);
+  nc = sh_ncoef(p);
+  for (DBuf *b : {&E, &F, &G, &W, &H, &K, &s1}) V3D_CHECK(b->alloc(nv * ng));
+  for (DBuf *b : {&cu, &cv, &nrm, &t1, &t2, &t3}) V3D_CHECK(b->alloc(3 * nv * ng));
+  V3D_CHECK(shc.alloc(3 * nv * nc));
+  V3D_CHECK(work.alloc(std::max(shc2grid_work(3 * nv, p, p), grid2shc_work(3 * nv, p, p))));
+  return upload(box, box_filter(p, f));
+}
+
+int SurfaceG::forward(const double *X, long nf, double *S, hipStream_t st) {
+  return sht_grid2shc<double>(X, nf, p, p, S, work, st);
+}
+
+// SHTrans::backward / backward_du / dv / d2u / d2v / duv: du, dv = number of
+// latitude / longitude derivatives.  SHTrans's first latitude derivative is
+// the negative of SphericalHarmonics' LegPolyDeriv (d1legTrans files).
+int SurfaceG::backward(const double *S, long nf, int du, int dv, double *X, hipStream_t st) {
+  return sht_synth<double>(S, nf, p, p, du, dv, X, work, st, du == 1 ? -1.0 : 1.0);
+}
+
+int SurfaceG::low_pass(double *field, long nf, hipStream_t st) {
+  V3D_CHECK(forward(field, nf, shc, st));
+  k_scale_rows<<<gblocks(nf * nc), 256, 0, st>>>(shc, nf, (int)nc, box);
+  V3D_CHECK(launch_status());
+  return backward(shc, nf, 0, 0, field, st);
+}
+
+int SurfaceG::update(const double *X, hipStream_t st) {
+  V3D_CHECK(forward(X, 3 * nv, shc, st));
+  V3D_CHECK(backward(shc, 3 * nv, 1, 0, t1, st));
+  V3D_CHECK(backward(shc, 3 * nv, 0, 1, t2, st));
+  k_first_forms<<<gblocks(nv * ng), 256, 0, st>>>(t1, t2, nv, ng, E, F, G, W, cu, cv, nrm);
+  V3D_CHECK(launch_status());
+  V3D_CHECK(backward(shc, 3 * nv, 1, 1, t1, st));
+  V3D_CHECK(backward(shc, 3 * nv, 2, 0, t2, st));
+  V3D_CHECK(backward(shc, 3 * nv, 0, 2, t3, st));
+  k_second_forms<<<gblocks(nv * ng), 256, 0, st>>>(t1, t2, t3, nrm, E, F, G, W, nv, ng, H, K);
+  V3D_CHECK(launch_status());
+  V3D_CHECK(low_pass(K, nv, st));
+  return low_pass(H, nv, st);
+}
+
+int SurfaceG::grad(const double *fin, double *out, hipStream_t st) {
+  V3D_CHECK(forward(fin, nv, shc, st));
+  V3D_CHECK(backward(shc, nv, 1, 0, t1, st));
+  V3D_CHECK(backward(shc, nv, 0, 1, t2, st));
+  k_grad_comb<<<gblocks(nv * ng), 256, 0, st>>>(t1, t2, cu, cv, nv, ng, out);
+  V3D_CHECK(launch_status());
+  return low_pass(out, 3 * nv, st);
+}
+
+int SurfaceG::div(const double *V, double *out, hipStream_t st) {
+  V3D_CHECK(forward(V, 3 * nv, shc, st));
+  V3D_CHECK(backward(shc, 3 * nv, 1, 0, t1, st));
+  V3D_CHECK(backward(shc, 3 * nv, 0, 1, t2, st));
+  k_div_comb<<<gblocks(nv * ng), 256, 0, st>>>(t1, t2, cu, cv, nv, ng, out);
+  V3D_CHECK(launch_status());
+  return low_pass(out, nv, st);
+}
+
+int SurfaceG::lin_mean_curv(const double *x, double *out, hipStream_t st) {
+  V3D_CHECK(forward(x, 3 * nv, shc, st));
+  V3D_CHECK(backward(shc, 3 * nv, 2, 0, t1, st));
+  V3D_CHECK(backward(shc, 3 * nv, 1, 1, t2, st));
+  V3D_CHECK(backward(shc, 3 * nv, 0, 2, t3, st));
+  k_lin_mean_curv<<<gblocks(nv * ng), 256, 0, st>>>(t1, t2, t3, nrm, E, F, G, nv, ng, out);
+  V3D_CHECK(launch_status());
+  return low_pass(out, nv, st);
+}
+
+// ------------------------------------------------------------- ForcesG
+int ForcesG::init(int p_, int pu_, int f, long nv_, double kb_) {
+  p = p_;
+  pu = pu_;
+  nv = nv_;
+  kb = kb_;
+  V3D_CHECK(S.init(p, f, nv));
+  V3D_CHECK(Su.init(pu, f * pu / p, nv));  // filter kept relative (src/Surface.cc:589-590)
+  const long mu = sh_ngrid(pu);
+  for (DBuf *b : {&xu, &gu, &fu}) V3D_CHECK(b->alloc(3 * nv * mu));
+  for (DBuf *b : {&su1, &su2}) V3D_CHECK(b->alloc(nv * mu));
+  const int pm = std::min(p, pu);
+  V3D_CHECK(rcoef.alloc(3 * nv * sh_ncoef(pm)));
+  long w = std::max({grid2shc_work(3 * nv, p, pm), grid2shc_work(3 * nv, pu, pm),
+                     shc2grid_work(3 * nv, pm, p), shc2grid_work(3 * nv, pm, pu)});
+  return rwork.alloc(w);
+}
+
+// Resample (src/HelperFuns.cc:436-493): truncate / zero-pad the SH expansion.
+int ForcesG::resample(const double *in, long nf, int pf, int pt, double *out, hipStream_t st) {
+  if (pf == pt) return hip_status(hipMemcpyAsync(out, in, sizeof(double) * nf * sh_ngrid(pf),
+                                                 hipMemcpyDeviceToDevice, st));
+  const int pm = std::min(pf, pt);
+  V3D_CHECK(sht_grid2shc<double>(in, nf, pf, pm, rcoef, rwork, st));
+  return sht_shc2grid<double>(rcoef, nf, pm, pt, out, nullptr, nullptr, rwork, st);
+}
+
+int ForcesG::update(const double *X, hipStream_t st) {
+  V3D_CHECK(S.update(X, st));
+  V3D_CHECK(resample(X, 3 * nv, p, pu, xu, st));
+  return Su.update(xu, st);
+}
+
+int ForcesG::bending(double *Fb, hipStream_t st) {  // src/InterfacialForce.cc:20-50
+  const long mu = Su.ng;
+  k_bend_s2<<<gblocks(nv * mu), 256, 0, st>>>(Su.H, Su.H, Su.K, nv * mu, su2);
+  V3D_CHECK(launch_status());
+  V3D_CHECK(Su.grad(Su.H, gu, st));
+  V3D_CHECK(Su.div(gu, su1, st));
+  k_bend_force<<<gblocks(nv * mu), 256, 0, st>>>(su1, su2, Su.nrm, kb, nv, mu, fu);
+  V3D_CHECK(launch_status());
+  return resample(fu, 3 * nv, pu, p, Fb, st);
+}
+
+int ForcesG::linear_bending(const double *x_new, double *Fb, hipStream_t st) {  // :52-92
+  const long mu = Su.ng;
+  V3D_CHECK(resample(x_new, 3 * nv, p, pu, xu, st));
+  V3D_CHECK(Su.lin_mean_curv(xu, su1, st));
+  k_bend_s2<<<gblocks(nv * mu), 256, 0, st>>>(su1, Su.H, Su.K, nv * mu, su2);
+  V3D_CHECK(launch_status());
+  V3D_CHECK(Su.grad(su1, gu, st));
+  V3D_CHECK(Su.div(gu, su1, st));
+  k_bend_force<<<gblocks(nv * mu), 256, 0, st>>>(su1, su2, Su.nrm, kb, nv, mu, fu);
+  V3D_CHECK(launch_status());
+  return resample(fu, 3 * nv, pu, p, Fb, st);
+}
+
+int ForcesG::tensile(const double *sigma, double *Fs, hipStream_t st) {  // :94-124
+  const long mu = Su.ng;
+  V3D_CHECK(resample(sigma, nv, p, pu, su1, st));
+  V3D_CHECK(Su.grad(su1, gu, st));
+  k_tens_force<<<gblocks(nv * mu), 256, 0, st>>>(su1, Su.H, Su.nrm, gu, nv, mu, fu);
+  V3D_CHECK(launch_status());
+  return resample(fu, 3 * nv, pu, p, Fs, st);
+}
+
+// ------------------------------------------------------------- StokesG
+StokesG::~StokesG() {
+  if (SL32) (void)hipFree(SL32);
+}
+
+int StokesG::init(int p_, int pu_, long nv_, int precision_) {
+  p = p_;
+  pu = pu_;
+  nv = nv_;
+  precision = precision_;
+  const long nc = sh_ncoef(p), ng = sh_ngrid(p), mu = sh_ngrid(pu);
+  const long nmat = nv * 9 * nc * nc;
+  if (precision == 0) {
+    V3D_CHECK(SL.alloc(nmat));
+  } else {
+    if (hipMalloc(&SL32, sizeof(float) * nmat) != hipSuccess) return VES3D_MEMORY_ERROR;
+  }
+  V3D_CHECK(far_src.alloc(nv * (mu + 2) * 3));
+  V3D_CHECK(q.alloc(nv * (mu + 2) * 3));
+  V3D_CHECK(trg.alloc(nv * ng * 3));
+  V3D_CHECK(farv.alloc(nv * ng * 3));
+  V3D_CHECK(aqw.alloc(nv * mu));
+  for (DBuf *b : {&F, &V, &shc}) V3D_CHECK(b->alloc(3 * nv * nc));
+  for (DBuf *b : {&up, &tmpx, &tmpt, &tmpp}) V3D_CHECK(b->alloc(3 * nv * mu));
+  V3D_CHECK(pole.alloc(3 * nv * 2));
+  return work.alloc(std::max({shc2grid_work(3 * nv, p, pu), grid2shc_work(3 * nv, p, p),
+                              shc2grid_work(3 * nv, p, p)}));
+}
+
+int StokesG::setup(const double *X, hipStream_t st) {
+  const long ng = sh_ngrid(p);
+  if (precision == 0)
+    V3D_CHECK(singular_build<double>(X, nv, p, pu, SL.p, (double *)nullptr, st));
+  else
+    V3D_CHECK(singular_build<float>(X, nv, p, pu, SL32, (float *)nullptr, st));
+  V3D_CHECK(sht_grid2shc<double>(X, 3 * nv, p, p, shc, work, st));
+  V3D_CHECK(sht_shc2grid<double>(shc, 3 * nv, p, pu, tmpx, tmpt, tmpp, work, st));
+  V3D_CHECK(sht_shc2pole<double>(shc, 3 * nv, p, pole, st));
+  std::vector<double> qw;
+  legendre_weights(pu, qw);
+  DBuf dqw;
+  V3D_CHECK(upload(dqw, qw));
+  k_far_setup<<<gblocks(nv * (sh_ngrid(pu) + 2)), 256, 0, st>>>(tmpx, tmpt, tmpp, pole, dqw, nv,
+                                                                  pu, far_src, aqw);
+  V3D_CHECK(launch_status());
+  // targets: the re-synthesised p-grid positions (src/StokesVelocity.cc:227-239)
+  V3D_CHECK(sht_shc2grid<double>(shc, 3 * nv, p, p, tmpx, nullptr, nullptr, work, st));
+  k_to_points<<<gblocks(nv * ng), 256, 0, st>>>(tmpx, nv, ng, trg);
+  V3D_CHECK(launch_status());
+  return hip_status(hipStreamSynchronize(st));  // dqw is freed on return
+}
+
+int StokesG::apply(const double *f, double *vel, hipStream_t st) {
+  const long ng = sh_ngrid(p), mu = sh_ngrid(pu);
+  V3D_CHECK(sht_grid2shc<double>(f, 3 * nv, p, p, F, work, st));
+  if (precision == 0)
+    V3D_CHECK(self_apply<double>(SL.p, F, nullptr, nullptr, nv, p, V, st));
+  else
+    V3D_CHECK(self_apply<float>(SL32, F, nullptr, nullptr, nv, p, V, st));
+  V3D_CHECK(sht_shc2grid<double>(V, 3 * nv, p, p, vel, nullptr, nullptr, work, st));
+  V3D_CHECK(sht_shc2grid<double>(F, 3 * nv, p, pu, up, nullptr, nullptr, work, st));
+  k_pack_density<<<gblocks(nv * (mu + 2)), 256, 0, st>>>(up, aqw, nv, mu, q);
+  V3D_CHECK(launch_status());
+  if (precision == 0)
+    V3D_CHECK(ves3d_hip_direct_eval_f64(far_src, q, nullptr, nv * (mu + 2), trg, farv, nv * ng,
+                                        mu + 2, ng, st));
+  else
+    V3D_CHECK(ves3d_hip_direct_eval_f32c(far_src, q, nullptr, nv * (mu + 2), trg, farv, nv * ng,
+                                         mu + 2, ng, st));
+  k_add_points<<<gblocks(nv * ng), 256, 0, st>>>(farv, nv, ng, vel);
+  return launch_status();
+}
+
+}  // namespace v3d

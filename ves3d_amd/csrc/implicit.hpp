@@ -1,0 +1,80 @@
+// implicit.hpp -- the globally implicit interfacial-velocity step on the GPU:
+// Surface geometry (src/Surface.cc), interfacial forces
+// (src/InterfacialForce.cc), the surface Stokes operator (StokesVelocity,
+// src/StokesVelocity.cc), the Galerkin matvec / right-hand side / diagonal
+// spectral preconditioner (src/InterfacialVelocity.cc:664-1172) and
+// BiCGStab (src/BiCGStab.cc:45-138).
+#pragma once
+#include <vector>
+
+#include "common.hpp"
+
+namespace v3d {
+
+// Device buffer owned by the step objects.
+struct DBuf {
+  double *p = nullptr;
+  long n = 0;
+  int alloc(long count);
+  void release();
+  ~DBuf() { release(); }
+  operator double *() const { return p; }
+};
+
+// Surface<Sca, Vec> at order p: first and second fundamental forms,
+// grad / div / linearised mean curvature with the low-pass filter of
+// frequency f (src/Surface.cc:211-377, :529-566).
+class SurfaceG {
+ public:
+  int init(int p, int filter_freq, long nv);
+  int update(const double *X, hipStream_t st);  // X: [nv][3][ng]
+  int grad(const double *f, double *out, hipStream_t st);
+  int div(const double *V, double *out, hipStream_t st);
+  int lin_mean_curv(const double *x, double *out, hipStream_t st);
+  int low_pass(double *field, long nf, hipStream_t st);  // in place, [nf][ng]
+  int forward(const double *X, long nf, double *S, hipStream_t st);
+  int backward(const double *S, long nf, int du, int dv, double *X, hipStream_t st);
+
+  int p = 0, f = 0;
+  long nv = 0, ng = 0, nc = 0;
+  DBuf E, F, G, W, cu, cv, nrm, H, K;
+  DBuf shc, t1, t2, t3, s1, work, box;
+};
+
+// InterfacialForce at order p with the surface upsampled to p_up.
+class ForcesG {
+ public:
+  int init(int p, int p_up, int filter_freq, long nv, double kb);
+  int update(const double *X, hipStream_t st);
+  int bending(double *Fb, hipStream_t st);                            // explicit
+  int linear_bending(const double *x_new, double *Fb, hipStream_t st);
+  int tensile(const double *sigma, double *Fs, hipStream_t st);
+  // resample [nf][grid(p_from)] -> [nf][grid(p_to)]
+  int resample(const double *in, long nf, int p_from, int p_to, double *out, hipStream_t st);
+
+  SurfaceG S, Su;
+  int p = 0, pu = 0;
+  long nv = 0;
+  double kb = 0;
+  DBuf xu, gu, fu, su1, su2, rcoef, rwork;
+};
+
+// StokesVelocity<Real>::operator() for surface targets, single layer, no
+// viscosity contrast: self (singular operator) + far field (direct sum over
+// the other vesicles' upsampled sources).  precision 0: FP64 everywhere;
+// 1: FP32 singular operator and FP32-compensated far field.
+class StokesG {
+ public:
+  int init(int p, int p_up, long nv, int precision);
+  int setup(const double *X, hipStream_t st);
+  int apply(const double *f, double *vel, hipStream_t st);
+
+  int p = 0, pu = 0, precision = 0;
+  long nv = 0;
+  DBuf SL;            // FP64 operator (precision 0)
+  float *SL32 = nullptr;
+  DBuf far_src, trg, aqw, F, V, up, q, farv, shc, work, tmpx, tmpt, tmpp, pole;
+  ~StokesG();
+};
+
+}  // namespace v3d

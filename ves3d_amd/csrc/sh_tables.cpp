@@ -103,6 +103,46 @@ void legpoly_deriv(const double *X, long N, int degree, double *out) {
     }
 }
 
+// D P_m = c_m P_{m-1} - c_{m+1} P_{m+1}, c_k = sqrt((n+k)(n+1-k))/2, applied
+// twice, with P_{-m} = (-1)^m P_m (orthonormal functions, Condon-Shortley).
+void legpoly_d2(const double *X, long N, int degree, double *out) {
+  std::vector<double> lp((long)(degree + 1) * (degree + 2) / 2 * N);
+  legpoly(X, N, degree, lp.data());
+  auto idx = [degree](int m, int n) { return (long)((degree * 2 - m + 1) * m) / 2 + n; };
+  std::vector<double> full(2 * degree + 5), dp(2 * degree + 5);
+  for (long i = 0; i < N; ++i)
+    for (int n = 0; n <= degree; ++n) {
+      // index k + (n+2) holds order k, k in [-n-2, n+2]
+      auto at = [n](int k) { return k + n + 2; };
+      auto c = [n](int k) {
+        double v = (double)(n + k) * (n + 1 - k);
+        return v > 0 ? std::sqrt(v) / 2.0 : 0.0;
+      };
+      for (int k = -n - 2; k <= n + 2; ++k) {
+        double v = 0;
+        if (k >= 0 && k <= n) v = lp[N * idx(k, n) + i];
+        else if (k < 0 && -k <= n) v = ((-k) % 2 ? -1.0 : 1.0) * lp[N * idx(-k, n) + i];
+        full[at(k)] = v;
+      }
+      for (int k = -n - 2; k <= n + 2; ++k) dp[at(k)] = 0;
+      for (int k = -n; k <= n; ++k) dp[at(k)] = c(k) * full[at(k - 1)] - c(k + 1) * full[at(k + 1)];
+      for (int m = 0; m <= n; ++m)
+        out[N * idx(m, n) + i] = c(m) * dp[at(m - 1)] - c(m + 1) * dp[at(m + 1)];
+    }
+}
+
+static void fill_fourier_d2(int p0, int p1, double *M) {
+  const double s2p = std::sqrt(2 * kPI);
+  for (int j = 0; j < 2 * p1; ++j) {
+    M[j] = 0.0;
+    for (int k = 1; k < p0; ++k) {
+      M[(2 * k - 1) * 2 * p1 + j] = -s2p * k * k * std::cos(j * k * kPI / p1);
+      M[(2 * k) * 2 * p1 + j] = -s2p * k * k * std::sin(j * k * kPI / p1);
+    }
+    M[(2 * p0 - 1) * 2 * p1 + j] = -s2p * p0 * p0 * std::cos(j * p0 * kPI / p1);
+  }
+}
+
 static void fill_fourier(int p0, int p1, double *M) {  // src/SphericalHarmonics.cc:606-627
   const double s2p = std::sqrt(2 * kPI);
   for (int j = 0; j < 2 * p1; ++j) {
@@ -157,6 +197,8 @@ void build_sh_mats(int p0, int p1, ShMatsHost &h) {
   h.Mdf = alloc(4L * p0 * p1);
   h.Mfinv = alloc(4L * p0 * p1);
   h.MfT = alloc(4L * p0 * p1);
+  h.Md2f = alloc(4L * p0 * p1);
+  fill_fourier_d2(p0, p1, &D[h.Md2f]);
   fill_fourier(p0, p1, &D[h.Mf]);
   fill_fourier_grad(p0, p1, &D[h.Mdf]);
   fill_fourier_inv(p0, p1, &D[h.Mfinv]);
@@ -167,19 +209,23 @@ void build_sh_mats(int p0, int p1, ShMatsHost &h) {
   gauss_legendre(p0 + 1, x0, w0);
   {  // MatLegendre(p0,p1), MatLegendreGrad(p0,p1): LegPoly(p0) at p1 nodes (:678-701, :735-757)
     std::vector<double> alp((long)(p0 + 1) * (p0 + 2) / 2 * (p1 + 1));
-    std::vector<double> dlp(alp.size());
+    std::vector<double> dlp(alp.size()), d2lp(alp.size());
     legpoly(x1.data(), p1 + 1, p0, alp.data());
     legpoly_deriv(x1.data(), p1 + 1, p0, dlp.data());
+    legpoly_d2(x1.data(), p1 + 1, p0, d2lp.data());
     long src = 0;
     h.Ml.resize(p0 + 1);
     h.Mdl.resize(p0 + 1);
+    h.Md2l.resize(p0 + 1);
     for (int m = 0; m <= p0; ++m) {
       long n = (long)(p0 + 1 - m) * (p1 + 1);
       h.Ml[m] = alloc(n);
       h.Mdl[m] = alloc(n);
+      h.Md2l[m] = alloc(n);
       for (long i = 0; i < n; ++i) {
         D[h.Ml[m] + i] = alp[src + i];
         D[h.Mdl[m] + i] = dlp[src + i];
+        D[h.Md2l[m] + i] = d2lp[src + i];
       }
       src += n;
     }

@@ -15,6 +15,9 @@ void gauss_legendre(int n, std::vector<double> &x, std::vector<double> &w);
 void legpoly(const double *X, long N, int degree, double *out);
 // LegPolyDeriv (src/SphericalHarmonics.cc:930-950).
 void legpoly_deriv(const double *X, long N, int degree, double *out);
+// Second latitude derivative by the raising/lowering ladder the reference's
+// d2legTrans matrices come from (matlab/Ynm.m:27-40), LegPoly row order.
+void legpoly_d2(const double *X, long N, int degree, double *out);
 
 inline int sh_ncoef(int p) { return p * (p + 2); }
 inline int sh_ngrid(int p) { return 2 * p * (p + 1); }
@@ -32,7 +35,9 @@ struct ShMatsHost {
   std::vector<double> data;
   long Mf, Mdf, Mfinv;          // MatFourier(p0,p1), MatFourierGrad(p0,p1), MatFourierInv(p0,p1): [2p0][2p1]
   long MfT;                     // MatFourier(p1,p0): [2p1][2p0] (read transposed by SHC2GridTranspose)
+  long Md2f;                    // second longitude derivative of MatFourier: [2p0][2p1]
   std::vector<long> Ml, Mdl;    // MatLegendre(p0,p1)[m], MatLegendreGrad: [(p0+1-m)][(p1+1)], m=0..p0
+  std::vector<long> Md2l;       // second latitude derivative (ladder of d2legTrans), same shape
   std::vector<long> Mlinv;      // MatLegendreInv(p0,p1)[m]: [(p0+1)][(p1+1-m)], m=0..p1
   std::vector<long> MlT;        // MatLegendre(p1,p0)[m]: [(p1+1-m)][(p0+1)], m=0..p1
 };

@@ -22,7 +22,9 @@ namespace v3d {
 
 namespace {
 
-enum PlanKind { kS2GLeg, kS2GLegGrad, kS2GFour, kS2GFourGrad, kG2SFour, kG2SLeg, kTFour, kTLeg };
+// Synthesis stages carry a derivative variant v in {0, 1, 2} (value, first,
+// second derivative) folded into the kind as kind + 16 * v.
+enum PlanKind { kS2GLeg, kS2GFour, kG2SFour, kG2SLeg, kTFour, kTLeg };
 
 struct Plan {
   GemmDesc *d = nullptr;
@@ -32,7 +34,8 @@ struct Plan {
 std::mutex g_mu;
 std::map<std::tuple<int, int, int, int, long>, Plan> g_plans;  // (device, kind, p0, p1, N)
 
-Plan make_plan(int kind, int p0, int p1, long N, const ShMatsHost &h) {
+Plan make_plan(int kindv, int p0, int p1, long N, const ShMatsHost &h) {
+  const int kind = kindv % 16, var = kindv / 16;
   std::vector<GemmDesc> v;
   const int nc0 = sh_ncoef(p0), nc1 = sh_ncoef(p1);
   auto push = [&v](int M, int Nn, int K, long ao, long ars, long acs, long bo, long brs,
@@ -46,18 +49,16 @@ Plan make_plan(int kind, int p0, int p1, long N, const ShMatsHost &h) {
   };
   switch (kind) {
     case kS2GLeg:
-    case kS2GLegGrad:
       for (int j = 0; j < 2 * p0; ++j) {
         const int m = (j + 1) / 2;
-        const long bo = (kind == kS2GLeg) ? h.Ml[m] : h.Mdl[m];
+        const long bo = var == 0 ? h.Ml[m] : (var == 1 ? h.Mdl[m] : h.Md2l[m]);
         push((int)N, p1 + 1, sh_blocklen(p0, j), sh_blockoff(p0, j), nc0, 1, bo, p1 + 1, 1,
              (long)j * N * (p1 + 1), p1 + 1, 1);
       }
       break;
     case kS2GFour:
-    case kS2GFourGrad:
       push((int)(N * (p1 + 1)), 2 * p1, 2 * p0, 0, 1, N * (p1 + 1),
-           kind == kS2GFour ? h.Mf : h.Mdf, 2 * p1, 1, 0, 2 * p1, 1);
+           var == 0 ? h.Mf : (var == 1 ? h.Mdf : h.Md2f), 2 * p1, 1, 0, 2 * p1, 1);
       break;
     case kG2SFour:
       push((int)(N * (p0 + 1)), 2 * p1, 2 * p0, 0, 2 * p0, 1, h.Mfinv, 2 * p1, 1, 0, 1,
@@ -112,11 +113,11 @@ const float *mats_ptr<float>(const ShMatsDev *m) { return m->d32; }
 
 template <typename T>
 int run(int kind, int p0, int p1, long N, const ShMatsDev *m, const T *A, T *C,
-        hipStream_t st) {
+        hipStream_t st, T alpha = T(1)) {
   const Plan *pl = get_plan(kind, p0, p1, N, *m->h);
   if (!pl) return VES3D_MEMORY_ERROR;
   return gemm_grouped<T>(pl->d, pl->n, 1, pl->max_m, pl->max_n, A, mats_ptr<T>(m), C,
-                         T(1), T(0), st);
+                         alpha, T(0), st);
 }
 
 bool valid_order(int p) { return p >= 1 && p <= 255; }
@@ -132,12 +133,25 @@ int sht_shc2grid(const T *S, long N, int p0, int p1, T *X, T *Xt, T *Xp, T *work
   if (!m) return VES3D_MEMORY_ERROR;
   V3D_CHECK(run<T>(kS2GLeg, p0, p1, N, m, S, work, st));
   V3D_CHECK(run<T>(kS2GFour, p0, p1, N, m, work, X, st));
-  if (Xt) V3D_CHECK(run<T>(kS2GFourGrad, p0, p1, N, m, work, Xt, st));
+  if (Xt) V3D_CHECK(run<T>(kS2GFour + 16, p0, p1, N, m, work, Xt, st));
   if (Xp) {
-    V3D_CHECK(run<T>(kS2GLegGrad, p0, p1, N, m, S, work, st));
+    V3D_CHECK(run<T>(kS2GLeg + 16, p0, p1, N, m, S, work, st));
     V3D_CHECK(run<T>(kS2GFour, p0, p1, N, m, work, Xp, st));
   }
   return VES3D_SUCCESS;
+}
+
+template <typename T>
+int sht_synth(const T *S, long N, int p0, int p1, int dleg, int dfour, T *X, T *work,
+              hipStream_t st, T alpha) {
+  if (!valid_order(p0) || !valid_order(p1) || N < 0 || dleg < 0 || dleg > 2 || dfour < 0 ||
+      dfour > 2)
+    return VES3D_INVALID_PARAMETER;
+  if (N == 0) return VES3D_SUCCESS;
+  const ShMatsDev *m = sh_mats(p0, p1);
+  if (!m) return VES3D_MEMORY_ERROR;
+  V3D_CHECK(run<T>(kS2GLeg + 16 * dleg, p0, p1, N, m, S, work, st, alpha));
+  return run<T>(kS2GFour + 16 * dfour, p0, p1, N, m, work, X, st);
 }
 
 template <typename T>
@@ -214,7 +228,8 @@ int sht_shc2pole(const T *S, long N, int p0, T *P, hipStream_t st) {
   template int sht_shc2grid<T>(const T *, long, int, int, T *, T *, T *, T *, hipStream_t); \
   template int sht_grid2shc<T>(const T *, long, int, int, T *, T *, hipStream_t);          \
   template int sht_shc2grid_transpose<T>(const T *, long, int, int, T *, T *, hipStream_t); \
-  template int sht_shc2pole<T>(const T *, long, int, T *, hipStream_t);
+  template int sht_shc2pole<T>(const T *, long, int, T *, hipStream_t);                 \
+  template int sht_synth<T>(const T *, long, int, int, int, int, T *, T *, hipStream_t, T);
 V3D_INST(double)
 V3D_INST(float)
 #undef V3D_INST

@@ -24,6 +24,12 @@ int sht_grid2shc(const T *X, long N, int p0, int p1, T *S, T *work, hipStream_t 
 template <typename T>
 int sht_shc2grid_transpose(const T *X, long N, int p0, int p1, T *S, T *work,
                            hipStream_t st);
+// Synthesis with the latitude (Legendre) stage differentiated dleg times and
+// the longitude (Fourier) stage dfour times, dleg, dfour in {0,1,2}; the
+// Legendre stage is scaled by alpha.  Same workspace as sht_shc2grid.
+template <typename T>
+int sht_synth(const T *S, long N, int p0, int p1, int dleg, int dfour, T *X, T *work,
+              hipStream_t st, T alpha);
 // S [N][p0(p0+2)] -> P [N][2] at x = -1, +1.
 template <typename T>
 int sht_shc2pole(const T *S, long N, int p0, T *P, hipStream_t st);

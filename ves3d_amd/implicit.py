@@ -1,0 +1,122 @@
+"""Python access to the implicit step (ves3d_hip_stepper_*) and the device
+BiCGStab (ves3d_hip_bicgstab)."""
+import ctypes
+
+import torch
+
+from . import call, load, ptr, stream_ptr
+
+
+class StepParams(ctypes.Structure):
+    _fields_ = [("nves", ctypes.c_size_t), ("p", ctypes.c_int), ("p_up", ctypes.c_int),
+                ("filter_freq", ctypes.c_int), ("dt", ctypes.c_double),
+                ("shear_rate", ctypes.c_double), ("bending_modulus", ctypes.c_double),
+                ("tol", ctypes.c_double), ("max_iter", ctypes.c_int),
+                ("precision", ctypes.c_int), ("precond", ctypes.c_int)]
+
+
+class StepStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("matvecs", ctypes.c_int), ("status", ctypes.c_int),
+                ("fell_back", ctypes.c_int), ("relres", ctypes.c_double),
+                ("true_relres", ctypes.c_double), ("ms_setup", ctypes.c_double),
+                ("ms_rhs", ctypes.c_double), ("ms_solve", ctypes.c_double),
+                ("ms_total", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+APPLY_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
+
+
+class Stepper:
+    """One globally implicit interfacial-velocity step per call (velocity
+    form, Galerkin packing, shear flow)."""
+
+    def __init__(self, nves, p, dt=1e-3, shear_rate=0.1, bending_modulus=1e-2, tol=1e-6,
+                 max_iter=200, precision=0, precond=1, p_up=0, filter_freq=0):
+        self.params = StepParams(nves, p, p_up, filter_freq, dt, shear_rate, bending_modulus, tol,
+                                 max_iter, precision, precond)
+        self.nves, self.p = nves, p
+        self.ng = 2 * p * (p + 1)
+        h = ctypes.c_void_p()
+        call("ves3d_hip_stepper_create", ctypes.byref(self.params), ctypes.byref(h))
+        self.h = h
+        n = ctypes.c_size_t()
+        call("ves3d_hip_stepper_size", self.h, ctypes.byref(n))
+        self.n = n.value
+
+    def close(self):
+        if self.h:
+            call("ves3d_hip_stepper_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, t, numel):
+        if t is None:
+            return
+        if not t.is_cuda or t.dtype != torch.float64 or not t.is_contiguous() or t.numel() != numel:
+            raise ValueError("expected a contiguous float64 CUDA tensor of %d elements" % numel)
+
+    def set_state(self, X, U=None, sigma=None):
+        self._chk(X, 3 * self.nves * self.ng)
+        self._chk(U, 3 * self.nves * self.ng)
+        self._chk(sigma, self.nves * self.ng)
+        torch.cuda.synchronize()
+        call("ves3d_hip_stepper_set_state", self.h, ptr(X), ptr(U), ptr(sigma))
+
+    def get_state(self):
+        X = torch.empty((self.nves, 3, self.ng), dtype=torch.float64, device="cuda")
+        U = torch.empty_like(X)
+        s = torch.empty((self.nves, self.ng), dtype=torch.float64, device="cuda")
+        call("ves3d_hip_stepper_get_state", self.h, ptr(X), ptr(U), ptr(s))
+        return X, U, s
+
+    def prepare(self):
+        call("ves3d_hip_stepper_prepare", self.h)
+
+    def rhs(self):
+        b = torch.empty(self.n, dtype=torch.float64, device="cuda")
+        call("ves3d_hip_stepper_rhs", self.h, ptr(b))
+        return b
+
+    def matvec(self, x):
+        self._chk(x, self.n)
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        call("ves3d_hip_stepper_matvec", self.h, ptr(x), ptr(y))
+        return y
+
+    def precond(self, x):
+        self._chk(x, self.n)
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        call("ves3d_hip_stepper_precond", self.h, ptr(x), ptr(y))
+        return y
+
+    def step(self):
+        st = StepStats()
+        torch.cuda.synchronize()
+        call("ves3d_hip_stepper_step", self.h, ctypes.byref(st))
+        return st.as_dict()
+
+
+def bicgstab(apply_a, x, b, tol, max_iter, apply_p=None, stream=None):
+    """Device BiCGStab with Python-side operator callbacks receiving raw
+    device pointers (ctx, x, y).  Returns (iterations, relres, status)."""
+    a_cb = APPLY_FN(apply_a)
+    p_cb = APPLY_FN(apply_p) if apply_p is not None else None
+    it = ctypes.c_int(max_iter)
+    tl = ctypes.c_double(tol)
+    status = ctypes.c_int(0)
+    torch.cuda.synchronize()
+    fa = ctypes.cast(a_cb, ctypes.c_void_p)
+    fp = ctypes.cast(p_cb, ctypes.c_void_p) if p_cb is not None else None
+    call("ves3d_hip_bicgstab", x.numel(), fa, None, fp, None, ptr(x), ptr(b),
+         ctypes.byref(it), ctypes.byref(tl), ctypes.byref(status), stream_ptr(stream))
+    return it.value, tl.value, status.value
