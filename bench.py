@@ -2,14 +2,23 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
-Prints ONE JSON line (rank 0) with the measured metric, a roofline record for
-the dominant kernel (timed with HIP events on the launch stream inside the
-timed region) and a CPU baseline measured on a bounded sample on the host.
+Prints ONE JSON line with the measured metric, a roofline record for the
+dominant kernel (timed with HIP events on its launch stream inside the timed
+region) and a CPU baseline measured on a bounded sample on the host.
 
 Workloads
-  far : the far-field direct Stokes SL+DL sum of 512 unit-sphere vesicles at
-        p=16 (278,528 points; own-vesicle pairs excluded), FP64 and
-        FP32-compensated.
+  step (default): the full implicit time step of 512 perturbed ellipsoids
+        (semi-axes 1, 1, 0.6 +- 0.05, random orientations) at p=16, 20% volume
+        fraction, shear rate 1.0, dt = 1e-3, BiCGStab to relative residual
+        1e-6.  Two forms run back to back from the same state:
+          plain  : FP64 everywhere, diagonal spectral preconditioner;
+          learned: FP32 singular operator + FP32-compensated far field and
+                   the learned SH-space preconditioner, trained beforehand on
+                   (A e, e) pairs recorded during plain solves (untimed).
+        `value` is the learned + mixed form's ms per step; the plain form's is
+        `ms_per_step_plain`.
+  far : the far-field direct Stokes SL+DL sum alone, 512 unit spheres at
+        p=16 (278,528 points; own-vesicle pairs excluded).
 """
 import argparse
 import json
@@ -22,6 +31,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+METRIC = "ms per implicit step @512 ves p=16 (learned-PC+mixed vs plain) at 1e-6"
 PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 vector / matrix
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector (published; counts packed FP32)
 SL_FLOP_PER_PAIR = 28      # src/CPUKernels.cc:1008
@@ -32,6 +42,168 @@ def _cpu_threads():
     return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
 
+def _log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------- step
+def bench_step(args):
+    import torch
+    import ves3d_amd
+    from ves3d_amd import implicit as I
+    from ves3d_amd import workloads as W
+
+    ves3d_amd.init(0)
+    nv, p = args.nves, args.p
+    X0 = W.implicit_step_case(nv=nv, p=p)
+    Xg = torch.from_numpy(X0).cuda()
+    common = dict(dt=1e-3, shear_rate=1.0, bending_modulus=1e-2, tol=1e-6, max_iter=args.max_iter)
+
+    # ---- plain FP64 form
+    plain = I.Stepper(nv, p, precision=0, precond=1, **common)
+    nin = 4 * p * (p + 2)
+    pc = I.LearnedPC(nin, args.width, capacity=args.capacity, seed=7)
+    I.attach(plain, pc, collect=True)       # records training pairs while it runs
+    plain.set_state(Xg)
+    stats_plain = []
+    for k in range(args.warmup + args.steps):
+        s = plain.step()
+        _log("plain step %d: %.1f ms, %d iters, relres %.2e true %.2e" %
+             (k, s["ms_total"], s["iterations"], s["relres"], s["true_relres"]))
+        if k >= args.warmup:
+            stats_plain.append(s)
+    _, U_plain1, _ = plain.get_state()
+
+    # ---- train the preconditioner on the recorded pairs (untimed)
+    t0 = time.perf_counter()
+    loss = pc.train(args.epochs, 512, args.lr)
+    train_s = time.perf_counter() - t0
+    n_samples = pc.samples()
+    _log("trained on %d samples: loss %.3e in %.1f s" % (n_samples, loss, train_s))
+    I.attach(plain, None, collect=False)
+
+    # ---- learned + mixed form from the same initial state
+    mixed = I.Stepper(nv, p, precision=1, precond=2, **common)
+    I.attach(mixed, pc, collect=False)
+    mixed.set_state(Xg)
+    stats_mixed = []
+    for k in range(args.warmup + args.steps):
+        s = mixed.step()
+        _log("learned+mixed step %d: %.1f ms, %d iters, relres %.2e true %.2e fell_back %d" %
+             (k, s["ms_total"], s["iterations"], s["relres"], s["true_relres"], s["fell_back"]))
+        if k >= args.warmup:
+            stats_mixed.append(s)
+
+    # velocity error of the learned + mixed form against the FP64 step, both
+    # taken from the identical initial state (first step)
+    check = I.Stepper(nv, p, precision=1, precond=2, **common)
+    I.attach(check, pc, collect=False)
+    check.set_state(Xg)
+    ref = I.Stepper(nv, p, precision=0, precond=1, **common)
+    ref.set_state(Xg)
+    ref.step()
+    check.step()
+    _, Ur, _ = ref.get_state()
+    _, Uc, _ = check.get_state()
+    vel_err = float(torch.linalg.norm(Uc - Ur) / torch.linalg.norm(Ur))
+    ref.close()
+    check.close()
+
+    ms_mixed = float(np.mean([s["ms_total"] for s in stats_mixed]))
+    ms_plain = float(np.mean([s["ms_total"] for s in stats_plain]))
+    it_mixed = [s["iterations"] for s in stats_mixed]
+    it_plain = [s["iterations"] for s in stats_plain]
+
+    # roofline: far-field SL direct sum (dominant kernel), learned+mixed form
+    ng = 2 * p * (p + 1)
+    pu = 2 * p
+    mu = 2 * pu * (pu + 1) + 2
+    pairs = (nv * ng) * (nv - 1) * mu
+    flop = pairs * SL_FLOP_PER_PAIR
+    far_ms = sum(s["ms_far"] for s in stats_mixed) / max(1, sum(s["far_calls"] for s in stats_mixed))
+    far_ms_plain = sum(s["ms_far"] for s in stats_plain) / max(1, sum(s["far_calls"] for s in stats_plain))
+    achieved = flop / (far_ms * 1e-3) / 1e12
+    roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+            "kernel": "k_far_f32c<SL> (+k_sum_slices)", "kernel_ms": far_ms,
+            "flop_per_launch": flop, "pairs_per_launch": pairs,
+            "fp64_form": {"kernel": "k_far_f64<SL>", "kernel_ms": far_ms_plain,
+                          "achieved": flop / (far_ms_plain * 1e-3) / 1e12,
+                          "peak": PEAK_FP64_TFLOPS}}
+
+    cpu = None if args.no_cpu else cpu_baseline_step(nv, p, X0, np.mean(
+        [s["matvecs"] for s in stats_plain]))
+    plain.close()
+    mixed.close()
+    pc.close()
+    return {
+        "metric": METRIC, "value": ms_mixed, "unit": "ms", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_mixed, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "implicit step, %d perturbed ellipsoids (1,1,0.6+-0.05) p=%d, "
+                               "20%% volume fraction, shear 1.0, dt 1e-3, BiCGStab 1e-6" % (nv, p),
+                   "nves": nv, "p": p, "p_up": 2 * p, "unknowns": nv * nin},
+        "ms_per_step_plain": ms_plain, "speedup_vs_plain": ms_plain / ms_mixed,
+        "iterations_learned": it_mixed, "iterations_plain": it_plain,
+        "true_relres_learned": [s["true_relres"] for s in stats_mixed],
+        "true_relres_plain": [s["true_relres"] for s in stats_plain],
+        "fell_back": [s["fell_back"] for s in stats_mixed],
+        "velocity_rel_err_vs_fp64": vel_err, "pc_train_seconds": train_s,
+        "pc_train_samples": int(n_samples), "pc_train_loss": loss,
+        "stage_ms_plain": {k: float(np.mean([s[k] for s in stats_plain]))
+                           for k in ("ms_setup", "ms_rhs", "ms_solve")},
+        "stage_ms_learned": {k: float(np.mean([s[k] for s in stats_mixed]))
+                             for k in ("ms_setup", "ms_rhs", "ms_solve")},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+
+
+def cpu_baseline_step(nv, p, X, n_matvec):
+    """The FP64 oracle on the host, on a bounded sample, extrapolated to one
+    step of the same workload: per-vesicle singular-operator build (2
+    vesicles timed) x nv, plus n_matvec x (far field: all sources against a
+    sample of 256 targets, scaled to all targets; local matvec work: one
+    oracle matvec on 2 vesicles, scaled by nv/2)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as OL
+    from oracle import sph, ves
+    t0 = time.perf_counter()
+    sph.stokes_singular_integ(X[:2], p, 2 * p, want_dl=False)
+    t_sing = (time.perf_counter() - t0) / 2
+    st = ves.StokesSurface(X[:2], p, 2 * p)  # small: the far geometry of 2 vesicles
+    # far field sample at full size
+    pu = 2 * p
+    mu = 2 * pu * (pu + 1)
+    shc = sph.grid2shc(X.reshape(-1, 2 * p * (p + 1)), p, p)
+    up = sph.shc2grid(shc, p, pu).reshape(nv, 3, mu)
+    src = np.ascontiguousarray(np.transpose(up, (0, 2, 1)).reshape(-1, 3))
+    den = np.ones_like(src) * 1e-3
+    ntr = 256
+    trg = np.ascontiguousarray(np.transpose(X, (0, 2, 1)).reshape(-1, 3)[:: (nv * 2 * p * (p + 1)) // ntr][:ntr])
+    out = np.zeros((ntr, 3))
+    orc = OL.load()
+    t0 = time.perf_counter()
+    orc.orc_far_field(OL.p(src), OL.p(den), None, OL.cs(src.shape[0]), OL.p(trg), OL.p(out), OL.cs(ntr),
+                      OL.cs(0), OL.cs(0))
+    t_far = (time.perf_counter() - t0) * (nv * 2 * p * (p + 1)) / ntr
+    imp = ves.Implicit(X[:2], p, dt=1e-3, shear=1.0)
+    x = np.random.default_rng(0).standard_normal(4 * 2 * p * (p + 2))
+    t0 = time.perf_counter()
+    imp.matvec(x)
+    t_local = (time.perf_counter() - t0) * nv / 2
+    total = t_sing * nv + n_matvec * (t_far + t_local)
+    del st
+    return {"value": total * 1e3, "unit": "ms per implicit step (extrapolated)", "cores": _cpu_threads(),
+            "kind": "port",
+            "sample": "FP64 oracle: singular operator of 2 vesicles (x%d), far field of all %d sources "
+                      "on 256 targets (x%d), one 2-vesicle matvec (x%d), times %.0f matvecs per step "
+                      "(the GPU plain run's count)" % (nv, src.shape[0], (nv * 2 * p * (p + 1)) // ntr,
+                                                       nv // 2, n_matvec),
+            "components_s": {"singular_per_vesicle": t_sing, "far_per_matvec": t_far,
+                             "local_per_matvec": t_local}}
+
+
+# ---------------------------------------------------------------- far only
 def bench_far(args):
     import torch
     import ves3d_amd
@@ -62,7 +234,6 @@ def bench_far(args):
     ms = wall / args.steps * 1e3
     achieved = flop / (kms * 1e-3) / 1e12
     peak = PEAK_FP32_TFLOPS if mixed else PEAK_FP64_TFLOPS
-
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -98,17 +269,24 @@ def bench_far(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="far", choices=["far"])
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="step", choices=["step", "far"])
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32c"])
+    ap.add_argument("--nves", type=int, default=512)
+    ap.add_argument("--p", type=int, default=16)
+    ap.add_argument("--max-iter", type=int, default=300)
+    ap.add_argument("--width", type=int, default=256)
+    ap.add_argument("--capacity", type=int, default=200000)
+    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
               "(the path shards by vesicle, see DESIGN.md)" % args.gpus, file=sys.stderr)
         sys.exit(2)
-    res = bench_far(args)
+    res = bench_step(args) if args.workload == "step" else bench_far(args)
     print(json.dumps(res))
 
 

@@ -166,6 +166,34 @@ int ves3d_hip_shc2pole_f64(const double *S, size_t N, int p0, double *P,
 int ves3d_hip_shc2pole_f32(const float *S, size_t N, int p0, float *P,
                            ves3d_stream_t stream);
 
+/* ---- device SHT in the SHTrans layout (src/SHTrans.cc) -------------------
+ * Coefficients are "order-major": coefficient block j (order (j+1)/2) of all
+ * nfun functions in turn, degrees ascending within a block -- the layout of
+ * the reference's Galerkin vectors (src/SHTrans.cc:58-120).  p(p+2)
+ * coefficients per function.  FP64. */
+
+/* SHTrans::forward: grid [nfun][(p+1) 2p] -> shc (order-major). */
+int ves3d_hip_shtrans_forward(int p, size_t nfun, const double *grid, double *shc,
+                              ves3d_stream_t stream);
+/* SHTrans::backward (du = dv = 0), backward_du (1,0), backward_dv (0,1),
+ * backward_d2u (2,0), backward_d2v (0,2), backward_duv (1,1): u is the
+ * latitude direction (d1legTrans / d2legTrans matrices), v the longitude. */
+int ves3d_hip_shtrans_backward(int p, size_t nfun, const double *shc, int du, int dv,
+                               double *grid, ves3d_stream_t stream);
+/* SHTrans::ScaleFreq: shc_out = scale * shc_in, scale [p(p+2)] given per
+ * coefficient of one function (function-major order, as the reference's
+ * filter and preconditioner arrays). */
+int ves3d_hip_shtrans_scale_freq(int p, size_t nfun, const double *shc_in,
+                                 const double *scale, double *shc_out,
+                                 ves3d_stream_t stream);
+/* SHTrans::lowPassFilter with the box filter of frequency freq (degrees
+ * n <= freq kept, src/SHTrans.cc:23-31): grid -> grid. */
+int ves3d_hip_shtrans_low_pass(int p, size_t nfun, int freq, const double *in,
+                               double *out, ves3d_stream_t stream);
+/* Resample (src/HelperFuns.cc:436-493): grid at order p -> grid at order q. */
+int ves3d_hip_resample(int p, int q, size_t nfun, const double *in, double *out,
+                       ves3d_stream_t stream);
+
 /* ---- rotations and the singular self-interaction ------------------------ */
 
 /* RotateAll (src/SphericalHarmonics.cc:362-449): S [N*dof][p(p+2)] ->
@@ -247,6 +275,8 @@ typedef struct ves3d_step_stats {
   double relres;          /* recursive relative residual at exit */
   double true_relres;     /* ||b - A x|| / ||b|| of the returned solution */
   double ms_setup, ms_rhs, ms_solve, ms_total;
+  double ms_far;          /* summed device time of the far-field direct sums */
+  int far_calls;          /* number of far-field direct sums in the step */
 } ves3d_step_stats;
 
 /* Creates / destroys a stepper (device state for nves vesicles). */
@@ -275,6 +305,41 @@ int ves3d_hip_stepper_precond(void *handle, const double *x, double *y);
  * previous (velocity, tension) as initial guess, true-residual check,
  * X += dt U.  stats_host may be NULL. */
 int ves3d_hip_stepper_step(void *handle, ves3d_step_stats *stats_host);
+
+/* ---- learned SH-space preconditioner ------------------------------------
+ * No counterpart in the reference: a per-vesicle network that fills the
+ * preconditioner slot of ImplicitPrecond (src/InterfacialVelocity.cc:1054-1109).
+ * For the Galerkin vector r, with D the diagonal spectral preconditioner and
+ * z_v = (D r)_v the 4 p(p+2) coefficients of vesicle v, s_v = |z_v|:
+ *     P(r)_v = z_v + s_v g(z_v / s_v),
+ * g: nin -> width -> width -> nin with tanh-GELU, FP32 weights, parameters
+ * laid out [W1 (width x nin), b1, W2 (width x width), b2, W3 (nin x width), b3].
+ * Trained online on (A e, e) pairs recorded during solves. */
+int ves3d_hip_lpc_create(int nin, int width, size_t capacity, unsigned long long seed,
+                         void **handle_host);
+int ves3d_hip_lpc_destroy(void *handle);
+int ves3d_hip_lpc_num_params(void *handle, size_t *n_host);
+/* Number of training samples recorded so far (the buffer keeps the last
+ * `capacity`). */
+int ves3d_hip_lpc_samples(void *handle, size_t *n_host);
+/* Device-to-device copies of the FP32 parameter vector. */
+int ves3d_hip_lpc_set_params(void *handle, const float *params);
+int ves3d_hip_lpc_get_params(void *handle, float *params);
+/* Y [B][nin] = g(Z [B][nin]). */
+int ves3d_hip_lpc_forward(void *handle, const float *Z, size_t B, float *Y,
+                          ves3d_stream_t stream);
+/* loss = sum((g(Z) - T)^2) / (2 B nin) into *loss_host (may be NULL); the
+ * gradient with respect to the parameters into grads (device, may be NULL). */
+int ves3d_hip_lpc_loss_grad(void *handle, const float *Z, const float *T, size_t B,
+                            double *loss_host, float *grads, ves3d_stream_t stream);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with the last gradient. */
+int ves3d_hip_lpc_adam(void *handle, float lr, ves3d_stream_t stream);
+/* epochs passes over the recorded samples in minibatches of `batch`. */
+int ves3d_hip_lpc_train(void *handle, int epochs, int batch, float lr, double *last_loss_host);
+/* Attaches a learned preconditioner to a stepper (precond = 2 uses it once
+ * it has been trained or given parameters); collect != 0 records training
+ * pairs during every solve. */
+int ves3d_hip_stepper_attach_pc(void *stepper, void *pc, int collect);
 
 #ifdef __cplusplus
 }

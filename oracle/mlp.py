@@ -1,0 +1,71 @@
+"""FP64 numpy oracle of the learned SH-space preconditioner network.
+
+TEST INFRASTRUCTURE ONLY: the checker for the GPU forward / backward / Adam
+kernels (ves3d_amd/csrc/learned_pc.hip).  The network has no counterpart in
+the reference; its role is the preconditioner slot of
+ParallelLinSolver::UpdatePrecond / ImplicitPrecond
+(src/InterfacialVelocity.cc:1054-1109).
+
+Network (per vesicle, rows of Z):  H1 = gelu(Z W1^T + b1),
+H2 = gelu(H1 W2^T + b2),  Y = H2 W3^T + b3;  loss = mean((Y - T)^2) / 2
+over all entries.  gelu is the tanh form
+  gelu(x) = x/2 (1 + tanh(sqrt(2/pi)(x + 0.044715 x^3))).
+"""
+import numpy as np
+
+C0 = np.sqrt(2.0 / np.pi)
+C1 = 0.044715
+
+
+def gelu(x):
+    return 0.5 * x * (1.0 + np.tanh(C0 * (x + C1 * x ** 3)))
+
+
+def gelu_grad(x):
+    u = C0 * (x + C1 * x ** 3)
+    t = np.tanh(u)
+    return 0.5 * (1.0 + t) + 0.5 * x * (1.0 - t * t) * C0 * (1.0 + 3.0 * C1 * x * x)
+
+
+def forward(params, Z):
+    W1, b1, W2, b2, W3, b3 = params
+    A1 = Z @ W1.T + b1
+    H1 = gelu(A1)
+    A2 = H1 @ W2.T + b2
+    H2 = gelu(A2)
+    Y = H2 @ W3.T + b3
+    return Y, (Z, A1, H1, A2, H2)
+
+
+def loss_and_grads(params, Z, T):
+    W1, b1, W2, b2, W3, b3 = params
+    Y, (Z, A1, H1, A2, H2) = forward(params, Z)
+    n = Y.size
+    D = Y - T
+    loss = 0.5 * np.sum(D * D) / n
+    dY = D / n
+    gW3 = dY.T @ H2
+    gb3 = dY.sum(axis=0)
+    dH2 = dY @ W3
+    dA2 = dH2 * gelu_grad(A2)
+    gW2 = dA2.T @ H1
+    gb2 = dA2.sum(axis=0)
+    dH1 = dA2 @ W2
+    dA1 = dH1 * gelu_grad(A1)
+    gW1 = dA1.T @ Z
+    gb1 = dA1.sum(axis=0)
+    return loss, (gW1, gb1, gW2, gb2, gW3, gb3)
+
+
+def adam_step(params, grads, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam update (Kingma & Ba 2015, Algorithm 1) on every tensor."""
+    out_p, out_m, out_v = [], [], []
+    for p, g, mm, vv in zip(params, grads, m, v):
+        mm = b1 * mm + (1 - b1) * g
+        vv = b2 * vv + (1 - b2) * g * g
+        mh = mm / (1 - b1 ** t)
+        vh = vv / (1 - b2 ** t)
+        out_p.append(p - lr * mh / (np.sqrt(vh) + eps))
+        out_m.append(mm)
+        out_v.append(vv)
+    return out_p, out_m, out_v

@@ -62,6 +62,11 @@ for _t in ("f64", "f32"):
     _sig("ves3d_hip_shc2pole_" + _t, P, S, c_int, P, P)
     _sig("ves3d_hip_singular_build_" + _t, P, S, c_int, c_int, P, P, P)
     _sig("ves3d_hip_self_apply_" + _t, P, P, P, P, S, c_int, P, P)
+_sig("ves3d_hip_shtrans_forward", c_int, S, P, P, P)
+_sig("ves3d_hip_shtrans_backward", c_int, S, P, c_int, c_int, P, P)
+_sig("ves3d_hip_shtrans_scale_freq", c_int, S, P, P, P, P)
+_sig("ves3d_hip_shtrans_low_pass", c_int, S, c_int, P, P, P)
+_sig("ves3d_hip_resample", c_int, c_int, S, P, P, P)
 _sig("ves3d_hip_rotate_all_f64", P, S, c_int, c_int, P, P)
 _sig("ves3d_hip_rotate_transpose_f64", P, S, c_int, c_int, P, P)
 _sig("ves3d_hip_bicgstab", S, P, P, P, P, P, P, P, P, P, P)
@@ -75,6 +80,17 @@ _sig("ves3d_hip_stepper_rhs", P, P)
 _sig("ves3d_hip_stepper_matvec", P, P, P)
 _sig("ves3d_hip_stepper_precond", P, P, P)
 _sig("ves3d_hip_stepper_step", P, P)
+_sig("ves3d_hip_stepper_attach_pc", P, P, c_int)
+_sig("ves3d_hip_lpc_create", c_int, c_int, S, ctypes.c_ulonglong, P)
+_sig("ves3d_hip_lpc_destroy", P)
+_sig("ves3d_hip_lpc_num_params", P, P)
+_sig("ves3d_hip_lpc_samples", P, P)
+_sig("ves3d_hip_lpc_set_params", P, P)
+_sig("ves3d_hip_lpc_get_params", P, P)
+_sig("ves3d_hip_lpc_forward", P, P, S, P, P)
+_sig("ves3d_hip_lpc_loss_grad", P, P, P, S, P, P, P)
+_sig("ves3d_hip_lpc_adam", P, ctypes.c_float, P)
+_sig("ves3d_hip_lpc_train", P, c_int, c_int, ctypes.c_float, P)
 
 _lib = None
 

@@ -396,6 +396,28 @@ int ForcesG::tensile(const double *sigma, double *Fs, hipStream_t st) {  // :94-
 // ------------------------------------------------------------- StokesG
 StokesG::~StokesG() {
   if (SL32) (void)hipFree(SL32);
+  for (auto e : ev0) (void)hipEventDestroy(e);
+  for (auto e : ev1) (void)hipEventDestroy(e);
+}
+
+void StokesG::timing_begin() {
+  timing = true;
+  nev = 0;
+}
+
+int StokesG::far_time(double *ms, int *calls) {
+  double tot = 0;
+  for (int i = 0; i < nev; ++i) {
+    V3D_HIP(hipEventSynchronize(ev1[i]));
+    float t = 0;
+    V3D_HIP(hipEventElapsedTime(&t, ev0[i], ev1[i]));
+    tot += t;
+  }
+  if (ms) *ms = tot;
+  if (calls) *calls = nev;
+  timing = false;
+  nev = 0;
+  return VES3D_SUCCESS;
 }
 
 int StokesG::init(int p_, int pu_, long nv_, int precision_) {
@@ -456,12 +478,23 @@ int StokesG::apply(const double *f, double *vel, hipStream_t st) {
   V3D_CHECK(sht_shc2grid<double>(F, 3 * nv, p, pu, up, nullptr, nullptr, work, st));
   k_pack_density<<<gblocks(nv * (mu + 2)), 256, 0, st>>>(up, aqw, nv, mu, q);
   V3D_CHECK(launch_status());
+  if (timing) {
+    if (nev == (int)ev0.size()) {
+      hipEvent_t a, b;
+      V3D_HIP(hipEventCreate(&a));
+      V3D_HIP(hipEventCreate(&b));
+      ev0.push_back(a);
+      ev1.push_back(b);
+    }
+    V3D_HIP(hipEventRecord(ev0[nev], st));
+  }
   if (precision == 0)
     V3D_CHECK(ves3d_hip_direct_eval_f64(far_src, q, nullptr, nv * (mu + 2), trg, farv, nv * ng,
                                         mu + 2, ng, st));
   else
     V3D_CHECK(ves3d_hip_direct_eval_f32c(far_src, q, nullptr, nv * (mu + 2), trg, farv, nv * ng,
                                          mu + 2, ng, st));
+  if (timing) V3D_HIP(hipEventRecord(ev1[nev++], st));
   k_add_points<<<gblocks(nv * ng), 256, 0, st>>>(farv, nv, ng, vel);
   return launch_status();
 }

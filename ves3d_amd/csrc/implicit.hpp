@@ -69,8 +69,16 @@ class StokesG {
   int setup(const double *X, hipStream_t st);
   int apply(const double *f, double *vel, hipStream_t st);
 
+  // HIP events around every far-field launch (on the launch stream); the
+  // elapsed times are summed by far_time() after the stream has drained.
+  void timing_begin();
+  int far_time(double *ms, int *calls);
+
   int p = 0, pu = 0, precision = 0;
   long nv = 0;
+  bool timing = false;
+  std::vector<hipEvent_t> ev0, ev1;
+  int nev = 0;
   DBuf SL;            // FP64 operator (precision 0)
   float *SL32 = nullptr;
   DBuf far_src, trg, aqw, F, V, up, q, farv, shc, work, tmpx, tmpt, tmpp, pole;

@@ -1,0 +1,365 @@
+// learned_pc.hip -- forward / backward / Adam kernels of the learned
+// SH-space preconditioner (see learned_pc.hpp) and its hooks into the
+// Galerkin solver vector.
+#include <algorithm>
+#include <cmath>
+#include <random>
+#include <vector>
+
+#include "learned_pc.hpp"
+#include "runtime.hpp"
+
+namespace v3d {
+
+namespace {
+inline unsigned gb(long n) {
+  long b = (n + 255) / 256;
+  if (b > 65536) b = 65536;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+#define GL(i, n) \
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (long)gridDim.x * blockDim.x)
+
+constexpr float kC0 = 0.7978845608028654f;  // sqrt(2/pi)
+constexpr float kC1 = 0.044715f;
+
+__device__ __forceinline__ float gelu_f(float x) {
+  return 0.5f * x * (1.0f + tanhf(kC0 * (x + kC1 * x * x * x)));
+}
+__device__ __forceinline__ float gelu_g(float x) {
+  const float t = tanhf(kC0 * (x + kC1 * x * x * x));
+  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * kC0 * (1.0f + 3.0f * kC1 * x * x);
+}
+
+// splitmix64 -> uniform in [-1, 1)
+__device__ __forceinline__ float hash_uniform(unsigned long long s) {
+  s += 0x9E3779B97F4A7C15ULL;
+  s = (s ^ (s >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  s = (s ^ (s >> 27)) * 0x94D049BB133111EBULL;
+  s ^= s >> 31;
+  return (float)((double)(s >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0);
+}
+
+__global__ void k_init_uniform(float *p, long n, float a, unsigned long long seed) {
+  GL(i, n) p[i] = a * hash_uniform(seed * 0x100000001B3ULL + (unsigned long long)i);
+}
+__global__ void k_bias_gelu(float *A, const float *__restrict__ b, long B, int w, float *H) {
+  GL(i, B * w) {
+    const float a = A[i] + b[i % w];
+    A[i] = a;
+    H[i] = gelu_f(a);
+  }
+}
+__global__ void k_add_bias(float *Y, const float *__restrict__ b, long B, int w) {
+  GL(i, B * w) Y[i] += b[i % w];
+}
+// dY = (Y - T) / n, per-block partial sums of (Y - T)^2 in FP64
+__global__ __launch_bounds__(256) void k_diff_loss(const float *__restrict__ Y,
+                                                   const float *__restrict__ T, long n, float invn,
+                                                   float *dY, double *part) {
+  double acc = 0;
+  GL(i, n) {
+    const float d = Y[i] - T[i];
+    dY[i] = d * invn;
+    acc += (double)d * d;
+  }
+  __shared__ double sh[4];
+  acc = wave_sum(acc);
+  if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+__global__ void k_sum_parts(const double *part, int n, double scale, double *out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0;
+    for (int i = 0; i < n; ++i) s += part[i];
+    out[0] = s * scale;
+  }
+}
+__global__ void k_gelu_back(float *dH, const float *__restrict__ A, long n) {
+  GL(i, n) dH[i] *= gelu_g(A[i]);
+}
+__global__ void k_colsum(const float *__restrict__ D, long B, int w, float *g) {
+  GL(c, (long)w) {
+    float s = 0;
+    for (long r = 0; r < B; ++r) s += D[r * w + c];
+    g[c] = s;
+  }
+}
+// Adam (Kingma & Ba 2015, Algorithm 1)
+__global__ void k_adam(float *P, const float *__restrict__ G, float *M, float *V, long n, float lr,
+                       float c1, float c2) {
+  GL(i, n) {
+    const float g = G[i];
+    const float m = 0.9f * M[i] + 0.1f * g;
+    const float v = 0.999f * V[i] + 0.001f * g * g;
+    M[i] = m;
+    V[i] = v;
+    P[i] -= lr * (m / c1) / (sqrtf(v / c2) + 1e-8f);
+  }
+}
+
+// Row element j of vesicle v inside the order-major Galerkin vector.
+__device__ __forceinline__ long row_index(long v, int j, long nv, int nc, const int2 *cinfo) {
+  long base, f, nf;
+  int c;
+  if (j < 3 * nc) {
+    const int d = j / nc;
+    c = j % nc;
+    f = 3 * v + d;
+    nf = 3 * nv;
+    base = 0;
+  } else {
+    c = j - 3 * nc;
+    f = v;
+    nf = nv;
+    base = 3 * nv * (long)nc;
+  }
+  const int2 bl = cinfo[c];
+  return base + nf * bl.x + f * bl.y + (c - bl.x);
+}
+
+// scale[v] = |row_v(y)|, one workgroup per vesicle
+__global__ __launch_bounds__(256) void k_row_norms(const double *__restrict__ y, long nv, int nc,
+                                                   const int2 *__restrict__ cinfo, double *scale) {
+  const long v = blockIdx.x;
+  double acc = 0;
+  for (int j = threadIdx.x; j < 4 * nc; j += 256) {
+    const double a = y[row_index(v, j, nv, nc, cinfo)];
+    acc += a * a;
+  }
+  __shared__ double sh[4];
+  acc = wave_sum(acc);
+  if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) scale[v] = sqrt(sh[0] + sh[1] + sh[2] + sh[3]) + 1e-300;
+}
+__global__ void k_gather_rows(const double *__restrict__ y, long nv, int nc,
+                              const int2 *__restrict__ cinfo, const double *__restrict__ scale,
+                              float *Z) {
+  const int nin = 4 * nc;
+  GL(i, nv * nin) {
+    const long v = i / nin;
+    const int j = (int)(i % nin);
+    Z[i] = (float)(y[row_index(v, j, nv, nc, cinfo)] / scale[v]);
+  }
+}
+__global__ void k_scatter_add(const float *__restrict__ Y, long nv, int nc,
+                              const int2 *__restrict__ cinfo, const double *__restrict__ scale,
+                              double *y) {
+  const int nin = 4 * nc;
+  GL(i, nv * nin) {
+    const long v = i / nin;
+    const int j = (int)(i % nin);
+    y[row_index(v, j, nv, nc, cinfo)] += scale[v] * (double)Y[i];
+  }
+}
+// samples: Z = Dr_v / s, T = (e_v - Dr_v) / s, written at rows slot0 + v (mod cap)
+__global__ void k_record(const double *__restrict__ Dr, const double *__restrict__ e, long nv,
+                         int nc, const int2 *__restrict__ cinfo, const double *__restrict__ scale,
+                         long slot0, long cap, float *Zs, float *Ts) {
+  const int nin = 4 * nc;
+  GL(i, nv * nin) {
+    const long v = i / nin;
+    const int j = (int)(i % nin);
+    const long o = row_index(v, j, nv, nc, cinfo);
+    const long row = (slot0 + v) % cap;
+    const double s = scale[v];
+    Zs[row * nin + j] = (float)(Dr[o] / s);
+    Ts[row * nin + j] = (float)((e[o] - Dr[o]) / s);
+  }
+}
+}  // namespace
+
+LearnedPC::~LearnedPC() {
+  for (float *p : {P_, G_, M_, V_, Zs_, Ts_, A1_, H1_, A2_, H2_, Y_, dY_, dA2_, dA1_, Zb_, Tb_})
+    if (p) (void)hipFree(p);
+  if (scale_) (void)hipFree(scale_);
+  if (lossbuf_) (void)hipFree(lossbuf_);
+  for (auto &kv : descs_) (void)hipFree(kv.second);
+}
+
+int LearnedPC::init(int nin, int width, long capacity, unsigned long long seed) {
+  nin_ = nin;
+  width_ = width;
+  cap_ = capacity;
+  seed_ = seed;
+  oW1_ = 0;
+  ob1_ = oW1_ + (long)width * nin;
+  oW2_ = ob1_ + width;
+  ob2_ = oW2_ + (long)width * width;
+  oW3_ = ob2_ + width;
+  ob3_ = oW3_ + (long)nin * width;
+  np_ = ob3_ + nin;
+  for (float **p : {&P_, &G_, &M_, &V_})
+    if (hipMalloc(p, sizeof(float) * np_) != hipSuccess) return VES3D_MEMORY_ERROR;
+  V3D_HIP(hipMemset(P_, 0, sizeof(float) * np_));
+  V3D_HIP(hipMemset(M_, 0, sizeof(float) * np_));
+  V3D_HIP(hipMemset(V_, 0, sizeof(float) * np_));
+  // Glorot-uniform hidden layers; the output layer starts small so that
+  // the untrained preconditioner is close to D.
+  const float a1 = std::sqrt(6.0f / (nin + width)), a2 = std::sqrt(6.0f / (2.0f * width));
+  k_init_uniform<<<gb((long)width * nin), 256>>>(P_ + oW1_, (long)width * nin, a1, seed * 3 + 1);
+  k_init_uniform<<<gb((long)width * width), 256>>>(P_ + oW2_, (long)width * width, a2, seed * 3 + 2);
+  k_init_uniform<<<gb((long)nin * width), 256>>>(P_ + oW3_, (long)nin * width, 0.01f * a1, seed * 3 + 3);
+  V3D_CHECK(launch_status());
+  if (cap_ > 0) {
+    if (hipMalloc(&Zs_, sizeof(float) * cap_ * nin) != hipSuccess) return VES3D_MEMORY_ERROR;
+    if (hipMalloc(&Ts_, sizeof(float) * cap_ * nin) != hipSuccess) return VES3D_MEMORY_ERROR;
+  }
+  if (hipMalloc(&lossbuf_, sizeof(double) * 4200) != hipSuccess) return VES3D_MEMORY_ERROR;
+  return hip_status(hipDeviceSynchronize());
+}
+
+int LearnedPC::ensure_batch(long B) {
+  if (B <= bcap_) return VES3D_SUCCESS;
+  for (float *p : {A1_, H1_, A2_, H2_, Y_, dY_, dA2_, dA1_, Zb_, Tb_})
+    if (p) (void)hipFree(p);
+  if (scale_) (void)hipFree(scale_);
+  const long w = width_, n = nin_;
+  float **wb[] = {&A1_, &H1_, &A2_, &H2_, &dA2_, &dA1_};
+  for (float **p : wb)
+    if (hipMalloc(p, sizeof(float) * B * w) != hipSuccess) return VES3D_MEMORY_ERROR;
+  float **nb[] = {&Y_, &dY_, &Zb_, &Tb_};
+  for (float **p : nb)
+    if (hipMalloc(p, sizeof(float) * B * n) != hipSuccess) return VES3D_MEMORY_ERROR;
+  if (hipMalloc(&scale_, sizeof(double) * B) != hipSuccess) return VES3D_MEMORY_ERROR;
+  bcap_ = B;
+  return VES3D_SUCCESS;
+}
+
+// C[M][N] = A B with A(r,k) = A[r ars + k acs], B(k,c) = B[k brs + c bcs]
+int LearnedPC::gemm(int M, int N, int K, const float *A, long ars, long acs, const float *Bm,
+                    long brs, long bcs, float *C, hipStream_t st) {
+  auto key = std::make_tuple(M, N, K, ars, acs, brs, bcs);
+  GemmDesc *d = nullptr;
+  auto it = descs_.find(key);
+  if (it != descs_.end()) {
+    d = it->second;
+  } else {
+    GemmDesc h{};
+    h.M = M;
+    h.N = N;
+    h.K = K;
+    h.a_rs = ars;
+    h.a_cs = acs;
+    h.b_rs = brs;
+    h.b_cs = bcs;
+    h.c_rs = N;
+    h.c_cs = 1;
+    d = upload_descs(&h, 1);
+    if (!d) return VES3D_MEMORY_ERROR;
+    descs_[key] = d;
+  }
+  return gemm_grouped<float>(d, 1, 1, M, N, A, Bm, C, 1.0f, 0.0f, st);
+}
+
+int LearnedPC::forward(const float *Z, long B, float *Y, hipStream_t st) {
+  V3D_CHECK(ensure_batch(B));
+  const int w = width_, n = nin_;
+  V3D_CHECK(gemm((int)B, w, n, Z, n, 1, P_ + oW1_, 1, n, A1_, st));   // Z W1^T
+  k_bias_gelu<<<gb(B * w), 256, 0, st>>>(A1_, P_ + ob1_, B, w, H1_);
+  V3D_CHECK(launch_status());
+  V3D_CHECK(gemm((int)B, w, w, H1_, w, 1, P_ + oW2_, 1, w, A2_, st));  // H1 W2^T
+  k_bias_gelu<<<gb(B * w), 256, 0, st>>>(A2_, P_ + ob2_, B, w, H2_);
+  V3D_CHECK(launch_status());
+  V3D_CHECK(gemm((int)B, n, w, H2_, w, 1, P_ + oW3_, 1, w, Y, st));    // H2 W3^T
+  k_add_bias<<<gb(B * n), 256, 0, st>>>(Y, P_ + ob3_, B, n);
+  return launch_status();
+}
+
+int LearnedPC::loss_grad(const float *Z, const float *T, long B, double *loss_host,
+                         hipStream_t st) {
+  V3D_CHECK(forward(Z, B, Y_, st));
+  const int w = width_, n = nin_;
+  const long tot = B * n;
+  const int nb = (int)std::min<long>(4096, std::max<long>(1, (tot + 255) / 256));
+  k_diff_loss<<<nb, 256, 0, st>>>(Y_, T, tot, 1.0f / (float)tot, dY_, lossbuf_);
+  V3D_CHECK(launch_status());
+  k_sum_parts<<<1, 64, 0, st>>>(lossbuf_, nb, 0.5 / (double)tot, lossbuf_ + 4100);
+  V3D_CHECK(launch_status());
+  // layer 3
+  V3D_CHECK(gemm(n, w, (int)B, dY_, 1, n, H2_, w, 1, G_ + oW3_, st));  // dY^T H2
+  k_colsum<<<gb(n), 256, 0, st>>>(dY_, B, n, G_ + ob3_);
+  V3D_CHECK(launch_status());
+  V3D_CHECK(gemm((int)B, w, n, dY_, n, 1, P_ + oW3_, w, 1, dA2_, st));  // dY W3
+  k_gelu_back<<<gb(B * w), 256, 0, st>>>(dA2_, A2_, B * w);
+  V3D_CHECK(launch_status());
+  // layer 2
+  V3D_CHECK(gemm(w, w, (int)B, dA2_, 1, w, H1_, w, 1, G_ + oW2_, st));
+  k_colsum<<<gb(w), 256, 0, st>>>(dA2_, B, w, G_ + ob2_);
+  V3D_CHECK(launch_status());
+  V3D_CHECK(gemm((int)B, w, w, dA2_, w, 1, P_ + oW2_, w, 1, dA1_, st));
+  k_gelu_back<<<gb(B * w), 256, 0, st>>>(dA1_, A1_, B * w);
+  V3D_CHECK(launch_status());
+  // layer 1
+  V3D_CHECK(gemm(w, n, (int)B, dA1_, 1, w, Z, n, 1, G_ + oW1_, st));
+  k_colsum<<<gb(w), 256, 0, st>>>(dA1_, B, w, G_ + ob1_);
+  V3D_CHECK(launch_status());
+  if (loss_host) {
+    V3D_HIP(hipMemcpyAsync(loss_host, lossbuf_ + 4100, sizeof(double), hipMemcpyDeviceToHost, st));
+    V3D_HIP(hipStreamSynchronize(st));
+  }
+  return VES3D_SUCCESS;
+}
+
+int LearnedPC::adam(float lr, hipStream_t st) {
+  ++t_;
+  const float c1 = 1.0f - std::pow(0.9f, (float)t_), c2 = 1.0f - std::pow(0.999f, (float)t_);
+  k_adam<<<gb(np_), 256, 0, st>>>(P_, G_, M_, V_, np_, lr, c1, c2);
+  return launch_status();
+}
+
+int LearnedPC::train(int epochs, int batch, float lr, double *last_loss, hipStream_t st) {
+  const long have = std::min(count_, cap_);
+  if (have < 1 || batch < 1) return VES3D_INVALID_PARAMETER;
+  const long B = std::min<long>(batch, have);
+  V3D_CHECK(ensure_batch(B));
+  std::mt19937_64 rng(seed_ + 12345);
+  const long steps = std::max<long>(1, epochs * have / B);
+  double loss = 0;
+  for (long s = 0; s < steps; ++s) {
+    const long start = (long)(rng() % (unsigned long long)have);
+    const long first = std::min(B, have - start);
+    V3D_HIP(hipMemcpyAsync(Zb_, Zs_ + start * nin_, sizeof(float) * first * nin_, hipMemcpyDeviceToDevice, st));
+    V3D_HIP(hipMemcpyAsync(Tb_, Ts_ + start * nin_, sizeof(float) * first * nin_, hipMemcpyDeviceToDevice, st));
+    if (first < B) {
+      V3D_HIP(hipMemcpyAsync(Zb_ + first * nin_, Zs_, sizeof(float) * (B - first) * nin_, hipMemcpyDeviceToDevice, st));
+      V3D_HIP(hipMemcpyAsync(Tb_ + first * nin_, Ts_, sizeof(float) * (B - first) * nin_, hipMemcpyDeviceToDevice, st));
+    }
+    V3D_CHECK(loss_grad(Zb_, Tb_, B, s + 1 == steps ? &loss : nullptr, st));
+    V3D_CHECK(adam(lr, st));
+  }
+  if (last_loss) *last_loss = loss;
+  trained_ = true;
+  return hip_status(hipStreamSynchronize(st));
+}
+
+int LearnedPC::apply_inplace(double *y, long nv, int p, const int2 *cinfo, hipStream_t st) {
+  const int nc = p * (p + 2);
+  if (4 * nc != nin_) return VES3D_INVALID_PARAMETER;
+  V3D_CHECK(ensure_batch(nv));
+  k_row_norms<<<(unsigned)nv, 256, 0, st>>>(y, nv, nc, cinfo, scale_);
+  V3D_CHECK(launch_status());
+  k_gather_rows<<<gb(nv * nin_), 256, 0, st>>>(y, nv, nc, cinfo, scale_, Zb_);
+  V3D_CHECK(launch_status());
+  V3D_CHECK(forward(Zb_, nv, Y_, st));
+  k_scatter_add<<<gb(nv * nin_), 256, 0, st>>>(Y_, nv, nc, cinfo, scale_, y);
+  return launch_status();
+}
+
+int LearnedPC::record(const double *Dr, const double *e, long nv, int p, const int2 *cinfo,
+                      hipStream_t st) {
+  const int nc = p * (p + 2);
+  if (4 * nc != nin_ || cap_ < nv) return VES3D_INVALID_PARAMETER;
+  V3D_CHECK(ensure_batch(nv));
+  k_row_norms<<<(unsigned)nv, 256, 0, st>>>(Dr, nv, nc, cinfo, scale_);
+  V3D_CHECK(launch_status());
+  k_record<<<gb(nv * nin_), 256, 0, st>>>(Dr, e, nv, nc, cinfo, scale_, head_, cap_, Zs_, Ts_);
+  V3D_CHECK(launch_status());
+  head_ = (head_ + nv) % cap_;
+  count_ += nv;
+  return VES3D_SUCCESS;
+}
+
+}  // namespace v3d

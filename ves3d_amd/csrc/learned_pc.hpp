@@ -1,0 +1,69 @@
+// learned_pc.hpp -- a per-vesicle network in SH-coefficient space used as a
+// (right) preconditioner of the implicit BiCGStab solve, trained online on
+// (A e, e) pairs the solver produces.  No counterpart in the reference: it
+// occupies the preconditioner slot of ImplicitPrecond
+// (src/InterfacialVelocity.cc:1054-1109).
+//
+// For the Galerkin vector r of a step, with D the diagonal spectral
+// preconditioner and, per vesicle v, z_v = (D r)_v (velocity then tension
+// coefficients, 4 p(p+2) values) and s_v = |z_v|:
+//     P(r)_v = z_v + s_v * g(z_v / s_v),
+// g a 3-layer MLP (nin -> width -> width -> nin, tanh-GELU, FP32 weights).
+// g is trained so that P(A e) ~ e, i.e. on inputs z/s and targets
+// (e_v - z_v)/s.  The solve's recursive and true residuals decide whether
+// the result is used (the stepper falls back to unpreconditioned BiCGStab).
+#pragma once
+#include <map>
+#include <tuple>
+
+#include "common.hpp"
+#include "gemm.hpp"
+
+namespace v3d {
+
+class LearnedPC {
+ public:
+  ~LearnedPC();
+  int init(int nin, int width, long capacity, unsigned long long seed);
+  long num_params() const { return np_; }
+
+  // ---- training interface (all device pointers, FP32 rows [B][nin])
+  int forward(const float *Z, long B, float *Y, hipStream_t st);
+  // loss = sum((Y-T)^2) / (2 B nin); gradients into the internal G buffer
+  int loss_grad(const float *Z, const float *T, long B, double *loss_host, hipStream_t st);
+  int adam(float lr, hipStream_t st);
+  int train(int epochs, int batch, float lr, double *last_loss_host, hipStream_t st);
+
+  // ---- solver interface on the Galerkin (order-major) layout
+  // y holds D r on entry and P(r) on exit.
+  int apply_inplace(double *y, long nv, int p, const int2 *cinfo, hipStream_t st);
+  // records samples from (Dr, e): inputs Dr_v/s_v, targets (e_v - Dr_v)/s_v
+  int record(const double *Dr, const double *e, long nv, int p, const int2 *cinfo,
+             hipStream_t st);
+
+  float *params() const { return P_; }
+  float *grads() const { return G_; }
+  long samples() const { return count_; }
+  bool ready() const { return trained_; }
+  void set_ready(bool r) { trained_ = r; }
+
+ private:
+  int ensure_batch(long B);
+  int gemm(int M, int N, int K, const float *A, long ars, long acs, const float *Bm, long brs,
+           long bcs, float *C, hipStream_t st);
+
+  int nin_ = 0, width_ = 0;
+  long np_ = 0, cap_ = 0, count_ = 0, head_ = 0, bcap_ = 0;
+  long oW1_, ob1_, oW2_, ob2_, oW3_, ob3_;
+  float *P_ = nullptr, *G_ = nullptr, *M_ = nullptr, *V_ = nullptr;
+  float *Zs_ = nullptr, *Ts_ = nullptr;                   // sample ring buffers
+  float *A1_ = nullptr, *H1_ = nullptr, *A2_ = nullptr, *H2_ = nullptr, *Y_ = nullptr;
+  float *dY_ = nullptr, *dA2_ = nullptr, *dA1_ = nullptr, *Zb_ = nullptr, *Tb_ = nullptr;
+  double *scale_ = nullptr, *lossbuf_ = nullptr;
+  int t_ = 0;
+  bool trained_ = false;
+  unsigned long long seed_ = 0;
+  std::map<std::tuple<int, int, int, long, long, long, long>, GemmDesc *> descs_;
+};
+
+}  // namespace v3d

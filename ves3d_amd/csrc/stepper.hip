@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "implicit.hpp"
+#include "learned_pc.hpp"
 #include "runtime.hpp"
 #include "sh_tables.hpp"
 #include "sht.hpp"
@@ -186,9 +187,11 @@ struct Stepper {
   ForcesG forces;
   StokesG stokes;
   DBuf X, U, sig;                  // state
-  DBuf f, f2, Sf, ten, ug, sg, cf, rhs, xsol, yres;
+  DBuf f, f2, Sf, ten, ug, sg, cf, rhs, xsol, yres, tmpD;
   DBuf bprec, tprec;               // diagonal-spectral preconditioner
   int2 *cinfo = nullptr;
+  LearnedPC *lpc = nullptr;        // not owned
+  bool collect = false;            // record (A e, e) pairs for the learned PC
   bool prepared = false;
   int matvecs = 0;
   int mode_precision = 0;
@@ -215,6 +218,7 @@ struct Stepper {
     V3D_CHECK(rhs.alloc(n));
     V3D_CHECK(xsol.alloc(n));
     V3D_CHECK(yres.alloc(n));
+    V3D_CHECK(tmpD.alloc(n));
     V3D_HIP(hipMemset(U.p, 0, sizeof(double) * U.n));
     V3D_HIP(hipMemset(sig.p, 0, sizeof(double) * sig.n));
     std::vector<int2> ci(nc);
@@ -318,15 +322,27 @@ struct Stepper {
   }
 
   static int A_cb(const void *ctx, const double *x, double *y) {
-    return ((Stepper *)ctx)->matvec(x, y);
+    Stepper *s = (Stepper *)ctx;
+    V3D_CHECK(s->matvec(x, y));
+    if (s->collect && s->lpc) {  // training pair: input A x, target x
+      V3D_CHECK(s->precond_diag(y, s->tmpD));
+      V3D_CHECK(s->lpc->record(s->tmpD, x, s->nv, s->prm.p, s->cinfo, s->st));
+    }
+    return VES3D_SUCCESS;
   }
   static int Pdiag_cb(const void *ctx, const double *x, double *y) {
     return ((Stepper *)ctx)->precond_diag(x, y);
+  }
+  static int Plearn_cb(const void *ctx, const double *x, double *y) {
+    Stepper *s = (Stepper *)ctx;
+    V3D_CHECK(s->precond_diag(x, y));
+    return s->lpc->apply_inplace(y, s->nv, s->prm.p, s->cinfo, s->st);
   }
 
   int step(ves3d_step_stats *stats) {
     using clk = std::chrono::steady_clock;
     auto t0 = clk::now();
+    stokes.timing_begin();
     matvecs = 0;
     V3D_CHECK(prepare());
     V3D_HIP(hipStreamSynchronize(st));
@@ -339,17 +355,32 @@ struct Stepper {
     double tol = prm.tol;
     int status = 0;
     apply_fn P = nullptr;
-    if (prm.precond == 1) P = Pdiag_cb;
+    const bool learned = prm.precond == 2 && lpc && lpc->ready();
+    if (prm.precond == 1 || (prm.precond == 2 && !learned)) P = Pdiag_cb;
+    if (learned) P = Plearn_cb;
     V3D_CHECK(bicgstab(n, A_cb, this, P, this, xsol, rhs, &it, &tol, &status, st));
-    bool fell_back = false;
-    auto t3 = clk::now();
     // true residual of the returned solution (src/InterfacialVelocity.cc:181-188)
     double true_rel = 0;
     V3D_CHECK(true_residual(xsol, rhs, &true_rel));
+    bool fell_back = false;
+    if (learned && (status != 100 || !(true_rel <= prm.tol * 1.05))) {
+      // the learned preconditioner did not deliver: unpreconditioned
+      // BiCGStab restarts from the initial guess and its result stands
+      fell_back = true;
+      V3D_CHECK(pack(U, sig, xsol));
+      it = prm.max_iter;
+      tol = prm.tol;
+      V3D_CHECK(bicgstab(n, A_cb, this, nullptr, nullptr, xsol, rhs, &it, &tol, &status, st));
+      V3D_CHECK(true_residual(xsol, rhs, &true_rel));
+    }
+    auto t3 = clk::now();
     V3D_CHECK(unpack(xsol, U, sig));
     V3D_CHECK(dev_axpby(3 * nv * ng, prm.dt, U, 1.0, X, st));  // dx = dt u
     V3D_HIP(hipStreamSynchronize(st));
     auto t4 = clk::now();
+    double far_ms = 0;
+    int far_calls = 0;
+    V3D_CHECK(stokes.far_time(&far_ms, &far_calls));
     prepared = false;
     if (stats) {
       auto ms = [](clk::time_point a, clk::time_point b) {
@@ -365,6 +396,8 @@ struct Stepper {
       stats->ms_rhs = ms(t1, t2);
       stats->ms_solve = ms(t2, t3);
       stats->ms_total = ms(t0, t4);
+      stats->ms_far = far_ms;
+      stats->far_calls = far_calls;
     }
     return VES3D_SUCCESS;
   }
@@ -458,6 +491,87 @@ int ves3d_hip_stepper_precond(void *h, const double *x, double *y) {
 int ves3d_hip_stepper_step(void *h, ves3d_step_stats *stats) {
   if (!h) return VES3D_INVALID_PARAMETER;
   return ((Stepper *)h)->step(stats);
+}
+
+int ves3d_hip_stepper_attach_pc(void *h, void *pc, int collect) {
+  if (!h) return VES3D_INVALID_PARAMETER;
+  Stepper *s = (Stepper *)h;
+  LearnedPC *l = (LearnedPC *)pc;
+  if (l && l->num_params() <= 0) return VES3D_INVALID_PARAMETER;
+  s->lpc = l;
+  s->collect = (collect != 0) && l;
+  return VES3D_SUCCESS;
+}
+
+int ves3d_hip_lpc_create(int nin, int width, size_t capacity, unsigned long long seed,
+                         void **handle) {
+  if (!handle || nin < 1 || width < 1) return VES3D_INVALID_PARAMETER;
+  LearnedPC *l = new LearnedPC();
+  int e = l->init(nin, width, (long)capacity, seed);
+  if (e != VES3D_SUCCESS) {
+    delete l;
+    return e;
+  }
+  *handle = l;
+  return VES3D_SUCCESS;
+}
+
+int ves3d_hip_lpc_destroy(void *h) {
+  delete (LearnedPC *)h;
+  return VES3D_SUCCESS;
+}
+
+int ves3d_hip_lpc_num_params(void *h, size_t *n) {
+  if (!h || !n) return VES3D_INVALID_PARAMETER;
+  *n = (size_t)((LearnedPC *)h)->num_params();
+  return VES3D_SUCCESS;
+}
+
+int ves3d_hip_lpc_samples(void *h, size_t *n) {
+  if (!h || !n) return VES3D_INVALID_PARAMETER;
+  *n = (size_t)((LearnedPC *)h)->samples();
+  return VES3D_SUCCESS;
+}
+
+int ves3d_hip_lpc_set_params(void *h, const float *params) {
+  if (!h || !params) return VES3D_INVALID_PARAMETER;
+  LearnedPC *l = (LearnedPC *)h;
+  V3D_HIP(hipMemcpy(l->params(), params, sizeof(float) * l->num_params(), hipMemcpyDeviceToDevice));
+  l->set_ready(true);
+  return VES3D_SUCCESS;
+}
+
+int ves3d_hip_lpc_get_params(void *h, float *params) {
+  if (!h || !params) return VES3D_INVALID_PARAMETER;
+  LearnedPC *l = (LearnedPC *)h;
+  return hip_status(
+      hipMemcpy(params, l->params(), sizeof(float) * l->num_params(), hipMemcpyDeviceToDevice));
+}
+
+int ves3d_hip_lpc_forward(void *h, const float *Z, size_t B, float *Y, ves3d_stream_t stream) {
+  if (!h || !Z || !Y || B == 0) return VES3D_INVALID_PARAMETER;
+  return ((LearnedPC *)h)->forward(Z, (long)B, Y, (hipStream_t)stream);
+}
+
+int ves3d_hip_lpc_loss_grad(void *h, const float *Z, const float *T, size_t B, double *loss,
+                            float *grads, ves3d_stream_t stream) {
+  if (!h || !Z || !T || B == 0) return VES3D_INVALID_PARAMETER;
+  LearnedPC *l = (LearnedPC *)h;
+  V3D_CHECK(l->loss_grad(Z, T, (long)B, loss, (hipStream_t)stream));
+  if (grads)
+    V3D_HIP(hipMemcpyAsync(grads, l->grads(), sizeof(float) * l->num_params(),
+                           hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return hip_status(hipStreamSynchronize((hipStream_t)stream));
+}
+
+int ves3d_hip_lpc_adam(void *h, float lr, ves3d_stream_t stream) {
+  if (!h) return VES3D_INVALID_PARAMETER;
+  return ((LearnedPC *)h)->adam(lr, (hipStream_t)stream);
+}
+
+int ves3d_hip_lpc_train(void *h, int epochs, int batch, float lr, double *last_loss) {
+  if (!h) return VES3D_INVALID_PARAMETER;
+  return ((LearnedPC *)h)->train(epochs, batch, lr, last_loss, nullptr);
 }
 
 }  // extern "C"

@@ -20,7 +20,8 @@ class StepStats(ctypes.Structure):
                 ("fell_back", ctypes.c_int), ("relres", ctypes.c_double),
                 ("true_relres", ctypes.c_double), ("ms_setup", ctypes.c_double),
                 ("ms_rhs", ctypes.c_double), ("ms_solve", ctypes.c_double),
-                ("ms_total", ctypes.c_double)]
+                ("ms_total", ctypes.c_double), ("ms_far", ctypes.c_double),
+                ("far_calls", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -104,6 +105,84 @@ class Stepper:
         torch.cuda.synchronize()
         call("ves3d_hip_stepper_step", self.h, ctypes.byref(st))
         return st.as_dict()
+
+
+class LearnedPC:
+    """The learned SH-space preconditioner (ves3d_hip_lpc_*)."""
+
+    def __init__(self, nin, width=256, capacity=0, seed=1):
+        h = ctypes.c_void_p()
+        call("ves3d_hip_lpc_create", nin, width, capacity, seed, ctypes.byref(h))
+        self.h = h
+        self.nin, self.width = nin, width
+        n = ctypes.c_size_t()
+        call("ves3d_hip_lpc_num_params", self.h, ctypes.byref(n))
+        self.num_params = n.value
+
+    def close(self):
+        if self.h:
+            call("ves3d_hip_lpc_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def layout(self):
+        """Offsets of W1, b1, W2, b2, W3, b3 in the parameter vector."""
+        w, n = self.width, self.nin
+        sizes = [w * n, w, w * w, w, n * w, n]
+        shapes = [(w, n), (w,), (w, w), (w,), (n, w), (n,)]
+        offs = [0]
+        for s in sizes[:-1]:
+            offs.append(offs[-1] + s)
+        return list(zip(offs, sizes, shapes))
+
+    def get_params(self):
+        out = torch.empty(self.num_params, dtype=torch.float32, device="cuda")
+        call("ves3d_hip_lpc_get_params", self.h, ptr(out))
+        return out
+
+    def set_params(self, p):
+        torch.cuda.synchronize()
+        call("ves3d_hip_lpc_set_params", self.h, ptr(p))
+
+    def samples(self):
+        n = ctypes.c_size_t()
+        call("ves3d_hip_lpc_samples", self.h, ctypes.byref(n))
+        return n.value
+
+    def forward(self, Z):
+        Y = torch.empty_like(Z)
+        torch.cuda.synchronize()
+        call("ves3d_hip_lpc_forward", self.h, ptr(Z), Z.shape[0], ptr(Y), stream_ptr())
+        torch.cuda.synchronize()
+        return Y
+
+    def loss_grad(self, Z, T):
+        g = torch.empty(self.num_params, dtype=torch.float32, device="cuda")
+        loss = ctypes.c_double()
+        torch.cuda.synchronize()
+        call("ves3d_hip_lpc_loss_grad", self.h, ptr(Z), ptr(T), Z.shape[0], ctypes.byref(loss),
+             ptr(g), stream_ptr())
+        return loss.value, g
+
+    def adam(self, lr):
+        call("ves3d_hip_lpc_adam", self.h, ctypes.c_float(lr), stream_ptr())
+        torch.cuda.synchronize()
+
+    def train(self, epochs, batch, lr):
+        loss = ctypes.c_double()
+        torch.cuda.synchronize()
+        call("ves3d_hip_lpc_train", self.h, epochs, batch, ctypes.c_float(lr), ctypes.byref(loss))
+        return loss.value
+
+
+def attach(stepper, pc, collect):
+    call("ves3d_hip_stepper_attach_pc", stepper.h, pc.h if pc is not None else None,
+         1 if collect else 0)
 
 
 def bicgstab(apply_a, x, b, tol, max_iter, apply_p=None, stream=None):

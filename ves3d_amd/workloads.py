@@ -84,26 +84,70 @@ def perturbed_shapes(nves, p, amp=0.05, seed=SEED_BASE, radius=1.0):
 
 def ellipsoid_shapes(nves, p, seed=SEED_BASE, c_mean=0.6, c_jitter=0.05, amp=0.0):
     """Ellipsoids with semi-axes (1, 1, c), c = c_mean +- c_jitter uniform,
-    the short axis along the grid's polar (first) coordinate."""
+    the short axis along the grid's polar (first) coordinate.  Returns the
+    shapes [nves][3][2p(p+1)] and their volumes (4/3 pi c)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     X, _ = sphere_grid(p)
     out = np.empty((nves, 3, X.shape[1]))
+    vol = np.empty(nves)
     for v in range(nves):
         c = c_mean + rng.uniform(-c_jitter, c_jitter)
-        rr = 1.0 + (_radial_perturbation(p, amp, rng) if amp > 0 else 0.0)
+        rr = 1.0 + (_radial_perturbation(p, amp, rng) if amp > 0 else np.zeros(X.shape[1]))
         out[v] = X * rr[None, :]
         out[v, 0] *= c
-    return out
+        vol[v] = 4.0 / 3.0 * np.pi * c
+    return out, vol
 
 
-def suspension(shapes, volfrac, seed=SEED_BASE, gap=0.05):
-    """Places the given vesicle shapes [nv][3][np] at rejection-sampled
-    centres in a cube at volume fraction `volfrac` (bounding radius used
-    for overlap rejection).  Returns positions [nv][3][np]."""
+def random_rotation(rng):
+    """Uniformly distributed rotation matrix (QR of a Gaussian matrix with
+    the sign convention that makes the factorisation unique)."""
+    q, r = np.linalg.qr(rng.standard_normal((3, 3)))
+    q = q * np.sign(np.diag(r))[None, :]
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def suspension(shapes, volfrac, seed=SEED_BASE, gap=0.05, volumes=None, rotate=False):
+    """Places the vesicle shapes [nv][3][np] (centred at 0) at rejection-
+    sampled centres in a cube sized so that the shapes fill `volfrac` of it
+    (volumes: per-shape volumes, default that of the bounding sphere); the
+    bounding spheres plus `gap` may not overlap.  rotate: apply a seeded
+    random rotation to every shape first.  Returns positions [nv][3][np]."""
     nv = shapes.shape[0]
+    rng = np.random.Generator(np.random.PCG64(seed + 77))
+    if rotate:
+        shapes = np.stack([random_rotation(rng) @ s for s in shapes])
     rad = float(np.max(np.sqrt(np.sum(shapes ** 2, axis=1))))
-    c, _ = random_centres(nv, volfrac * rad ** 3, rad, seed, gap)
+    if volumes is None:
+        volumes = np.full(nv, 4.0 / 3.0 * np.pi * rad ** 3)
+    eq = float(np.sum(volumes)) / nv / (4.0 / 3.0 * np.pi * rad ** 3)  # per-shape fill of its ball
+    ball_frac = volfrac / eq
+    if ball_frac <= 0.25:
+        c, _ = random_centres(nv, ball_frac, rad, seed, gap)
+    else:
+        # dense: random sequential addition of the bounding balls jams near
+        # this fraction, so centres go on a jittered cubic lattice instead
+        # (jitter bounded so that bounding balls + gap stay disjoint)
+        k = int(np.ceil(nv ** (1.0 / 3.0)))
+        side = (float(np.sum(volumes)) / volfrac) ** (1.0 / 3.0)
+        h = side / k
+        if h < 2 * rad + gap:
+            raise RuntimeError("volume fraction %g too dense for a lattice of bounding balls" % volfrac)
+        jit = 0.5 * (h - 2 * rad - gap)
+        idx = np.stack(np.meshgrid(np.arange(k), np.arange(k), np.arange(k), indexing="ij"), -1)
+        idx = idx.reshape(-1, 3)[rng.permutation(k ** 3)[:nv]]
+        c = (idx + 0.5) * h + rng.uniform(-jit, jit, size=(nv, 3))
     return shapes + c[:, :, None]
+
+
+def implicit_step_case(nv=512, p=16, volfrac=0.20, seed=SEED_BASE + 3):
+    """The full implicit-step workload: nv perturbed ellipsoids (semi-axes
+    1, 1, 0.6 +- 0.05) with random orientations at the given volume fraction.
+    Returns positions [nv][3][2p(p+1)]."""
+    shapes, vol = ellipsoid_shapes(nv, p, seed=seed)
+    return suspension(shapes, volfrac, seed=seed, gap=0.05, volumes=vol, rotate=True)
 
 
 def stokes_direct_sum_case(nv=512, p=16, volfrac=0.15, seed=SEED_BASE + 1):
