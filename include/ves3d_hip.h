@@ -63,6 +63,65 @@ int ves3d_hip_memcpy(void *dst, const void *src, size_t nbytes, int kind);
 int ves3d_hip_memset(void *ptr, int value, size_t nbytes);
 int ves3d_hip_synchronize(void);
 
+/* ---- Device<> elementwise / reduction / transpose / GEMM members
+ *      (include/Device.h:121-232; semantics of src/DeviceCPU.cc:64-640).
+ * Vector fields are AxisMajor [nv][3][stride], scalar fields [nv][stride].
+ * Work is enqueued on the default stream; the scalar-returning calls
+ * (maxabs, algebraic_dot, isnumeric) wait for their result.  NULL y / w
+ * arguments select the reference's "x only" branches. */
+int ves3d_hip_dot_product_f64(const double *u, const double *v, size_t stride, size_t n_vecs, double *out);
+int ves3d_hip_dot_product_f32(const float *u, const float *v, size_t stride, size_t n_vecs, float *out);
+int ves3d_hip_cross_product_f64(const double *u, const double *v, size_t stride, size_t n_vecs, double *out);
+int ves3d_hip_cross_product_f32(const float *u, const float *v, size_t stride, size_t n_vecs, float *out);
+int ves3d_hip_sqrt_f64(const double *x, size_t length, double *out);
+int ves3d_hip_sqrt_f32(const float *x, size_t length, float *out);
+/* ax: out[n][i] = a[i] x[n][i] (a has `stride` entries) */
+int ves3d_hip_ax_f64(const double *a, const double *x, size_t stride, size_t n_vecs, double *out);
+int ves3d_hip_ax_f32(const float *a, const float *x, size_t stride, size_t n_vecs, float *out);
+int ves3d_hip_xy_f64(const double *x, const double *y, size_t length, double *out);
+int ves3d_hip_xy_f32(const float *x, const float *y, size_t length, float *out);
+/* xyInv: out = x / y (x == NULL: 1 / y) */
+int ves3d_hip_xyinv_f64(const double *x, const double *y, size_t length, double *out);
+int ves3d_hip_xyinv_f32(const float *x, const float *y, size_t length, float *out);
+/* uyInv: vector u divided by scalar y per point */
+int ves3d_hip_uyinv_f64(const double *u, const double *y, size_t stride, size_t n_vecs, double *out);
+int ves3d_hip_uyinv_f32(const float *u, const float *y, size_t stride, size_t n_vecs, float *out);
+/* axpy: out = a x + y (y == NULL: a x) */
+int ves3d_hip_axpy_f64(double a, const double *x, const double *y, size_t length, double *out);
+int ves3d_hip_axpy_f32(float a, const float *x, const float *y, size_t length, float *out);
+/* apx: out[n][i] = a[n] + x[n][i] */
+int ves3d_hip_apx_f64(const double *a, const double *x, size_t stride, size_t n_subs, double *out);
+int ves3d_hip_apx_f32(const float *a, const float *x, size_t stride, size_t n_subs, float *out);
+/* avpw: out = a[vec] v + w (vector fields, w may be NULL) */
+int ves3d_hip_avpw_f64(const double *a, const double *v, const double *w, size_t stride, size_t n_vecs, double *out);
+int ves3d_hip_avpw_f32(const float *a, const float *v, const float *w, size_t stride, size_t n_vecs, float *out);
+/* xvpw: out = x (scalar field) v + w (vector fields, w may be NULL) */
+int ves3d_hip_xvpw_f64(const double *x, const double *v, const double *w, size_t stride, size_t n_vecs, double *out);
+int ves3d_hip_xvpw_f32(const float *x, const float *v, const float *w, size_t stride, size_t n_vecs, float *out);
+/* Reduce: out[ii][kk] = sum_j x[ii][kk][j] w[ii][j] qw[j] (x == NULL: sum w qw) */
+int ves3d_hip_reduce_f64(const double *x, int x_dim, const double *w, const double *quad_w, size_t stride, size_t ns, double *out);
+int ves3d_hip_reduce_f32(const float *x, int x_dim, const float *w, const float *quad_w, size_t stride, size_t ns, float *out);
+int ves3d_hip_maxabs_f64(const double *x, size_t length, double *result_host);
+int ves3d_hip_maxabs_f32(const float *x, size_t length, float *result_host);
+/* AlgebraicDot with FP64 accumulation (the reference's GPU form only reduces
+ * block 0, src/CudaKernels.cu:1125) */
+int ves3d_hip_algebraic_dot_f64(const double *x, const double *y, size_t length, double *result_host);
+int ves3d_hip_algebraic_dot_f32(const float *x, const float *y, size_t length, float *result_host);
+/* isNumeric: 1 if every entry is finite.  (The reference's CPU version
+ * starts from false and ANDs, so it can never fail, src/DeviceCPU.cc:632-638.) */
+int ves3d_hip_isnumeric_f64(const double *x, size_t length, int *result_host);
+int ves3d_hip_isnumeric_f32(const float *x, size_t length, int *result_host);
+/* Transpose: out[w][h] = in[h][w] */
+int ves3d_hip_transpose_f64(const double *in, size_t height, size_t width, double *out);
+int ves3d_hip_transpose_f32(const float *in, size_t height, size_t width, float *out);
+/* gemm: BLAS column-major C = alpha op(A) op(B) + beta C ('N' / 'T'). */
+int ves3d_hip_gemm_f64(char transA, char transB, int m, int n, int k, double alpha,
+                       const double *A, int lda, const double *B, int ldb, double beta,
+                       double *C, int ldc);
+int ves3d_hip_gemm_f32(char transA, char transB, int m, int n, int k, float alpha,
+                       const float *A, int lda, const float *B, int ldb, float beta,
+                       float *C, int ldc);
+
 /* ---- direct Stokes sums ------------------------------------------------ */
 
 /* Device<DT>::DirectStokes (include/Device.h:205-209).  For each vesicle vt

@@ -51,6 +51,25 @@ _sig("ves3d_hip_synchronize")
 for _t in ("f64", "f32"):
     _sig("ves3d_hip_direct_stokes_" + _t, P, P, P, S, S, S, P, S, S, P, P)
     _sig("ves3d_hip_direct_stokes_dl_" + _t, P, P, P, P, S, S, S, P, S, S, P, P)
+for _t, _real in (("f64", c_double), ("f32", ctypes.c_float)):
+    _sig("ves3d_hip_dot_product_" + _t, P, P, S, S, P)
+    _sig("ves3d_hip_cross_product_" + _t, P, P, S, S, P)
+    _sig("ves3d_hip_sqrt_" + _t, P, S, P)
+    _sig("ves3d_hip_ax_" + _t, P, P, S, S, P)
+    _sig("ves3d_hip_xy_" + _t, P, P, S, P)
+    _sig("ves3d_hip_xyinv_" + _t, P, P, S, P)
+    _sig("ves3d_hip_uyinv_" + _t, P, P, S, S, P)
+    _sig("ves3d_hip_axpy_" + _t, _real, P, P, S, P)
+    _sig("ves3d_hip_apx_" + _t, P, P, S, S, P)
+    _sig("ves3d_hip_avpw_" + _t, P, P, P, S, S, P)
+    _sig("ves3d_hip_xvpw_" + _t, P, P, P, S, S, P)
+    _sig("ves3d_hip_reduce_" + _t, P, c_int, P, P, S, S, P)
+    _sig("ves3d_hip_maxabs_" + _t, P, S, P)
+    _sig("ves3d_hip_algebraic_dot_" + _t, P, P, S, P)
+    _sig("ves3d_hip_isnumeric_" + _t, P, S, P)
+    _sig("ves3d_hip_transpose_" + _t, P, S, S, P)
+    _sig("ves3d_hip_gemm_" + _t, ctypes.c_char, ctypes.c_char, c_int, c_int, c_int, _real, P, c_int,
+         P, c_int, _real, P, c_int)
 _sig("ves3d_hip_stokes_all2all_f64", P, P, S, P, P)
 _sig("ves3d_hip_stokes_all2all_f32c", P, P, S, P, P)
 _sig("ves3d_hip_direct_eval_f64", P, P, P, S, P, P, S, S, S, P)

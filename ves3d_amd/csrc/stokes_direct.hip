@@ -123,7 +123,9 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_f64(
         for (int j = 0; j < cnt; ++j) {
           const double dx = tx - sh[j].x, dy = ty - sh[j].y, dz = tz - sh[j].z;
           const double r2 = dx * dx + dy * dy + dz * dz;
-          const double ri = (r2 > eps2) ? rsqrt(r2) : 0.0;
+          // unconditional rsqrt + select: no per-pair branch in the loop
+          const double rq = rsqrt(r2);
+          const double ri = (r2 > eps2) ? rq : 0.0;
           const double ri2 = ri * ri;
           if (SL) {
             const double ip =
@@ -225,7 +227,8 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_f32c(
           for (int j = j0; j < j1; ++j) {
             const float dx = lx - sh[j].x, dy = ly - sh[j].y, dz = lz - sh[j].z;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            const float ri = (r2 > eps2) ? __frsqrt_rn(r2) : 0.0f;
+            const float rq = __frsqrt_rn(r2);
+            const float ri = (r2 > eps2) ? rq : 0.0f;
             const float ri2 = ri * ri;
             if (SL) {
               const float ip = (sh[j].fx * dx + sh[j].fy * dy + sh[j].fz * dz) * ri2;
