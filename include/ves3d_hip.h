@@ -324,6 +324,9 @@ typedef struct ves3d_step_params {
   int max_iter;           /* BiCGStab iteration limit */
   int precision;          /* 0: FP64; 1: FP32 singular operator + FP32-compensated far field */
   int precond;            /* 0: none; 1: diagonal spectral (ConfigurePrecond); 2: learned */
+  int near_off;           /* 1 disables the near-singular correction and the repulsion force */
+  double repul_dist;      /* repulsion length (0: the reference default 5e-2; < 0: none) */
+  double r_near;          /* near-zone radius (0: r_ves 2/sqrt(p_up), src/NearSingular.cc:83) */
 } ves3d_step_params;
 
 typedef struct ves3d_step_stats {
@@ -364,6 +367,37 @@ int ves3d_hip_stepper_precond(void *handle, const double *x, double *y);
  * previous (velocity, tension) as initial guess, true-residual check,
  * X += dt U.  stats_host may be NULL. */
 int ves3d_hip_stepper_step(void *handle, ves3d_step_stats *stats_host);
+
+/* ---- near-singular correction (NearSingular<Real>, include/NearSingular.h:8-30)
+ * For surface targets (trg_is_surf) and a single-layer density, FP64.
+ * Sources: the upsampled surfaces with the two poles first, point-major
+ * [nves][M][3], M = 2 + 2 p_up (p_up + 1) (scoord_far,
+ * src/StokesVelocity.cc:199-218).  Targets: nves blocks of (p+1)2p points,
+ * point-major [nves (p+1)2p][3] (tcoord_repl, :227-239).  A near pair is a
+ * target and a vesicle other than its own with a source point closer than
+ * r_near, paired with the closest such point (src/NearSingular.cc:797-918).
+ * The handle keeps pointers to the coordinates passed to near_set_coord:
+ * they must stay valid until the next near_set_coord. */
+int ves3d_hip_near_create(int p, int p_up, size_t nves, double repul_dist, double r_near,
+                          void **handle_host);
+int ves3d_hip_near_destroy(void *handle);
+/* SetSrcCoord + SetTrgCoord + SetupCoordData (src/NearSingular.cc:40-80,
+ * :80-1120): near-pair search, patch projection, repulsion force.
+ * Synchronises the stream. */
+int ves3d_hip_near_set_coord(void *handle, const double *src, const double *trg,
+                             ves3d_stream_t stream);
+/* Number of near pairs and the near-zone radius of the last set_coord. */
+int ves3d_hip_near_info(void *handle, size_t *npairs_host, double *r_near_host);
+/* ForceRepul (:1122-1126), added to f [nves][3][(p+1)2p] (AxisMajor). */
+int ves3d_hip_near_force_repul(void *handle, double *f, ves3d_stream_t stream);
+/* operator() minus SubtractDirect (:1143-1212, :1375-1551): vel
+ * [nves][3][(p+1)2p] += near-singular velocity - direct sum, for every
+ * near target.  q: SL density times area times quadrature weight,
+ * point-major [nves][M][3] (qforce_single, src/StokesVelocity.cc:308-334);
+ * vup [nves][3][2 p_up (p_up+1)] and vpole [nves][3][2]: the surface
+ * velocity on the upsampled grid and at the poles (S_vel_up, :452-470). */
+int ves3d_hip_near_apply(void *handle, const double *q, const double *vup, const double *vpole,
+                         double *vel, ves3d_stream_t stream);
 
 /* ---- learned SH-space preconditioner ------------------------------------
  * No counterpart in the reference: a per-vesicle network that fills the

@@ -101,7 +101,7 @@ def _sphere(p):
 
 
 def test_closed_forms_on_sphere(orc):
-    p = 8
+    p = 10
     X, qw = _sphere(p)
     npts = X.shape[2]
     f = np.array([0.3, -1.1, 0.7])

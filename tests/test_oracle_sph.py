@@ -147,7 +147,8 @@ def test_singular_closed_forms(p0):
     Fn = sph.grid2shc(X, p0, p0).reshape(-1)
     Vn = sph.shc2grid((Fn @ SL[0]).reshape(3, nc), p0, p0)
     assert np.max(np.abs(V - 2.0 / 3.0 * f)) < 1e-9
-    assert np.max(np.abs(Vd - 0.5 * f)) < 1e-9
+    # outward grid normal (reference orientation): principal value -1/2 f
+    assert np.max(np.abs(Vd + 0.5 * f)) < 1e-9
     assert np.max(np.abs(Vn)) < 1e-9
 
 

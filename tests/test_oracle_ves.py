@@ -68,10 +68,12 @@ def test_shtrans_matrices_match_reference_files(p):
 def test_sphere_geometry(p):
     X, _ = W.sphere_grid(p)
     S = ves.Surface(X[None], p)
-    assert np.allclose(S.H, 1.0, atol=1e-11)
+    # reference orientation: outward x_u x x_v normal, H = -1 on the unit sphere
+    assert np.allclose(S.normal[0], X, atol=1e-11)
+    assert np.allclose(S.H, -1.0, atol=1e-11)
     assert np.allclose(S.K, 1.0, atol=1e-11)
     assert abs(S.area()[0] - 4 * np.pi) < 1e-11
-    assert abs(abs(S.volume()[0]) - 4 * np.pi / 3) < 1e-11
+    assert abs(S.volume()[0] - 4 * np.pi / 3) < 1e-11
 
 
 def test_sphere_forces():
@@ -85,7 +87,7 @@ def test_sphere_forces():
     # tension force of a uniform tension on a sphere: 2 sigma H n
     sig = np.full((1, X.shape[1]), 0.7)
     Fs = F.tensile(sig)
-    assert np.allclose(Fs, 2 * 0.7 * 1.0 * S.normal, atol=1e-10)
+    assert np.allclose(Fs, 2 * 0.7 * -1.0 * S.normal, atol=1e-10)
 
 
 def test_perturbed_geometry_converges():

@@ -100,6 +100,12 @@ _sig("ves3d_hip_stepper_matvec", P, P, P)
 _sig("ves3d_hip_stepper_precond", P, P, P)
 _sig("ves3d_hip_stepper_step", P, P)
 _sig("ves3d_hip_stepper_attach_pc", P, P, c_int)
+_sig("ves3d_hip_near_create", c_int, c_int, S, c_double, c_double, P)
+_sig("ves3d_hip_near_destroy", P)
+_sig("ves3d_hip_near_set_coord", P, P, P, P)
+_sig("ves3d_hip_near_info", P, P, P)
+_sig("ves3d_hip_near_force_repul", P, P, P)
+_sig("ves3d_hip_near_apply", P, P, P, P, P, P)
 _sig("ves3d_hip_lpc_create", c_int, c_int, S, ctypes.c_ulonglong, P)
 _sig("ves3d_hip_lpc_destroy", P)
 _sig("ves3d_hip_lpc_num_params", P, P)

@@ -420,11 +420,13 @@ int StokesG::far_time(double *ms, int *calls) {
   return VES3D_SUCCESS;
 }
 
-int StokesG::init(int p_, int pu_, long nv_, int precision_) {
+int StokesG::init(int p_, int pu_, long nv_, int precision_, int near_, double repul_dist,
+                  double r_near) {
   p = p_;
   pu = pu_;
   nv = nv_;
   precision = precision_;
+  use_near = near_ != 0;
   const long nc = sh_ncoef(p), ng = sh_ngrid(p), mu = sh_ngrid(pu);
   const long nmat = nv * 9 * nc * nc;
   if (precision == 0) {
@@ -440,6 +442,12 @@ int StokesG::init(int p_, int pu_, long nv_, int precision_) {
   for (DBuf *b : {&F, &V, &shc}) V3D_CHECK(b->alloc(3 * nv * nc));
   for (DBuf *b : {&up, &tmpx, &tmpt, &tmpp}) V3D_CHECK(b->alloc(3 * nv * mu));
   V3D_CHECK(pole.alloc(3 * nv * 2));
+  if (use_near) {
+    V3D_CHECK(near.init(p, pu, nv, repul_dist, r_near));
+    V3D_CHECK(frep.alloc(3 * nv * ng));
+    V3D_CHECK(vup.alloc(3 * nv * mu));
+    V3D_CHECK(vpole.alloc(3 * nv * 2));
+  }
   return work.alloc(std::max({shc2grid_work(3 * nv, p, pu), grid2shc_work(3 * nv, p, p),
                               shc2grid_work(3 * nv, p, p)}));
 }
@@ -464,11 +472,18 @@ int StokesG::setup(const double *X, hipStream_t st) {
   V3D_CHECK(sht_shc2grid<double>(shc, 3 * nv, p, p, tmpx, nullptr, nullptr, work, st));
   k_to_points<<<gblocks(nv * ng), 256, 0, st>>>(tmpx, nv, ng, trg);
   V3D_CHECK(launch_status());
+  if (use_near) V3D_CHECK(near.setup(far_src, trg, st));
   return hip_status(hipStreamSynchronize(st));  // dqw is freed on return
 }
 
-int StokesG::apply(const double *f, double *vel, hipStream_t st) {
+int StokesG::apply(const double *f, double *vel, hipStream_t st, bool add_repul) {
   const long ng = sh_ngrid(p), mu = sh_ngrid(pu);
+  const bool near_on = use_near && near.npairs > 0;
+  if (add_repul && near_on && near.repul > 0) {
+    V3D_HIP(hipMemcpyAsync(frep, f, sizeof(double) * 3 * nv * ng, hipMemcpyDeviceToDevice, st));
+    V3D_CHECK(near.add_repulsion(frep, st));
+    f = frep;
+  }
   V3D_CHECK(sht_grid2shc<double>(f, 3 * nv, p, p, F, work, st));
   if (precision == 0)
     V3D_CHECK(self_apply<double>(SL.p, F, nullptr, nullptr, nv, p, V, st));
@@ -496,7 +511,13 @@ int StokesG::apply(const double *f, double *vel, hipStream_t st) {
                                          mu + 2, ng, st));
   if (timing) V3D_HIP(hipEventRecord(ev1[nev++], st));
   k_add_points<<<gblocks(nv * ng), 256, 0, st>>>(farv, nv, ng, vel);
-  return launch_status();
+  V3D_CHECK(launch_status());
+  if (!near_on) return VES3D_SUCCESS;
+  // near-singular correction with the self velocity on the upsampled grid
+  // and at the poles (S_vel_up, src/StokesVelocity.cc:452-470)
+  V3D_CHECK(sht_shc2grid<double>(V, 3 * nv, p, pu, vup, nullptr, nullptr, work, st));
+  V3D_CHECK(sht_shc2pole<double>(V, 3 * nv, p, vpole, st));
+  return near.apply(q, vup, vpole, vel, st);
 }
 
 }  // namespace v3d

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "near.hpp"
 
 namespace v3d {
 
@@ -65,9 +66,15 @@ class ForcesG {
 // 1: FP32 singular operator and FP32-compensated far field.
 class StokesG {
  public:
-  int init(int p, int p_up, long nv, int precision);
+  // near: 1 applies the near-singular correction (src/NearSingular.cc) with
+  // repulsion length repul_dist (<= 0: none) and near-zone radius r_near
+  // (<= 0: the reference's r_ves 2/sqrt(p_up)).
+  int init(int p, int p_up, long nv, int precision, int near = 1, double repul_dist = 5e-2,
+           double r_near = 0.0);
   int setup(const double *X, hipStream_t st);
-  int apply(const double *f, double *vel, hipStream_t st);
+  // add_repul: the density is f + NearSingular::ForceRepul (the right-hand
+  // side of the implicit step, src/InterfacialVelocity.cc:733).
+  int apply(const double *f, double *vel, hipStream_t st, bool add_repul = false);
 
   // HIP events around every far-field launch (on the launch stream); the
   // elapsed times are summed by far_time() after the stream has drained.
@@ -82,6 +89,9 @@ class StokesG {
   DBuf SL;            // FP64 operator (precision 0)
   float *SL32 = nullptr;
   DBuf far_src, trg, aqw, F, V, up, q, farv, shc, work, tmpx, tmpt, tmpp, pole;
+  bool use_near = true;
+  NearG near;
+  DBuf frep, vup, vpole;
   ~StokesG();
 };
 

@@ -211,7 +211,9 @@ struct Stepper {
     nc = sh_ncoef(p);
     n = 4 * nv * nc;
     V3D_CHECK(forces.init(p, prm.p_up, prm.filter_freq, nv, prm.bending_modulus));
-    V3D_CHECK(stokes.init(p, prm.p_up, nv, prm.precision));
+    if (prm.repul_dist == 0.0) prm.repul_dist = 5e-2;  // src/Parameters.cc:46
+    V3D_CHECK(stokes.init(p, prm.p_up, nv, prm.precision, prm.near_off ? 0 : 1, prm.repul_dist,
+                          prm.r_near));
     for (DBuf *b : {&X, &U, &f, &f2, &Sf, &ug}) V3D_CHECK(b->alloc(3 * nv * ng));
     for (DBuf *b : {&sig, &ten, &sg}) V3D_CHECK(b->alloc(nv * ng));
     V3D_CHECK(cf.alloc(3 * nv * nc));
@@ -289,13 +291,14 @@ struct Stepper {
     return pack(ug, ten, y);
   }
 
-  // AssembleRhsVel (src/InterfacialVelocity.cc:715-787), no repulsion term
+  // AssembleRhsVel (src/InterfacialVelocity.cc:715-787): the explicit
+  // traction jump plus the repulsion force (:733)
   int assemble_rhs(double *b) {
     if (!prepared) return VES3D_INVALID_PARAMETER;
     k_shear<<<gb(nv * ng), 256, 0, st>>>(X, nv, ng, prm.shear_rate, ug);
     V3D_CHECK(launch_status());
     V3D_CHECK(forces.bending(f, st));
-    V3D_CHECK(stokes.apply(f, Sf, st));
+    V3D_CHECK(stokes.apply(f, Sf, st, true));
     V3D_CHECK(dev_axpby(3 * nv * ng, 1.0, Sf, 1.0, ug, st));
     V3D_CHECK(forces.S.div(ug, ten, st));
     return pack(ug, ten, b);

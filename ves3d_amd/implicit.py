@@ -12,7 +12,9 @@ class StepParams(ctypes.Structure):
                 ("filter_freq", ctypes.c_int), ("dt", ctypes.c_double),
                 ("shear_rate", ctypes.c_double), ("bending_modulus", ctypes.c_double),
                 ("tol", ctypes.c_double), ("max_iter", ctypes.c_int),
-                ("precision", ctypes.c_int), ("precond", ctypes.c_int)]
+                ("precision", ctypes.c_int), ("precond", ctypes.c_int),
+                ("near_off", ctypes.c_int), ("repul_dist", ctypes.c_double),
+                ("r_near", ctypes.c_double)]
 
 
 class StepStats(ctypes.Structure):
@@ -35,9 +37,10 @@ class Stepper:
     form, Galerkin packing, shear flow)."""
 
     def __init__(self, nves, p, dt=1e-3, shear_rate=0.1, bending_modulus=1e-2, tol=1e-6,
-                 max_iter=200, precision=0, precond=1, p_up=0, filter_freq=0):
+                 max_iter=200, precision=0, precond=1, p_up=0, filter_freq=0, near=True,
+                 repul_dist=0.0, r_near=0.0):
         self.params = StepParams(nves, p, p_up, filter_freq, dt, shear_rate, bending_modulus, tol,
-                                 max_iter, precision, precond)
+                                 max_iter, precision, precond, 0 if near else 1, repul_dist, r_near)
         self.nves, self.p = nves, p
         self.ng = 2 * p * (p + 1)
         h = ctypes.c_void_p()

@@ -19,14 +19,19 @@ def gl_nodes(p):
 
 def sphere_grid(p):
     """Unit-sphere grid [3][2p(p+1)] and the smooth quadrature weight per
-    point (w_i * pi/p, which integrates dS exactly for polynomials)."""
+    point (w_i * pi/p, which integrates dS exactly for polynomials).
+
+    Point (i, j) is (rho cos phi_j, rho sin phi_j, x_i) with x_i the ascending
+    Gauss-Legendre nodes and phi_j = pi j / p: the orientation of the
+    reference's shape files (precomputed/shape_gallery_*.txt), for which the
+    grid's x_u x x_v normal points outward and a unit sphere has H = -1."""
     x, w = gl_nodes(p)
     phi = np.pi * np.arange(2 * p) / p
     s = np.sqrt(1.0 - x * x)
     X = np.empty((3, p + 1, 2 * p))
-    X[0] = x[:, None]
+    X[0] = s[:, None] * np.cos(phi)[None, :]
     X[1] = s[:, None] * np.sin(phi)[None, :]
-    X[2] = s[:, None] * np.cos(phi)[None, :]
+    X[2] = x[:, None]
     qw = np.repeat(w * np.pi / p, 2 * p)
     return X.reshape(3, -1), qw
 
