@@ -132,7 +132,7 @@ def bench_step(args):
                           "peak": PEAK_FP64_TFLOPS}}
 
     cpu = None if args.no_cpu else cpu_baseline_step(nv, p, X0, np.mean(
-        [s["matvecs"] for s in stats_plain]))
+        [s["matvecs"] for s in stats_plain]), stats_plain[-1]["near_pairs"])
     plain.close()
     mixed.close()
     pc.close()
@@ -149,6 +149,7 @@ def bench_step(args):
         "true_relres_plain": [s["true_relres"] for s in stats_plain],
         "fell_back": [s["fell_back"] for s in stats_mixed],
         "velocity_rel_err_vs_fp64": vel_err, "pc_train_seconds": train_s,
+        "near_pairs": [s["near_pairs"] for s in stats_plain], "r_near": stats_plain[-1]["r_near"],
         "pc_train_samples": int(n_samples), "pc_train_loss": loss,
         "stage_ms_plain": {k: float(np.mean([s[k] for s in stats_plain]))
                            for k in ("ms_setup", "ms_rhs", "ms_solve")},
@@ -158,12 +159,36 @@ def bench_step(args):
     }
 
 
-def cpu_baseline_step(nv, p, X, n_matvec):
+def _near_cost_per_pair(p):
+    """Oracle seconds per near-singular pair: setup (search, projection) and
+    one application (interpolation minus direct sum), measured on two
+    perturbed spheres 0.05 apart."""
+    from oracle import near as N
+    from oracle import ves
+    from ves3d_amd import workloads as W
+    X = W.perturbed_shapes(2, p, amp=0.05, seed=11)
+    X[1] += np.array([2.05, 0.0, 0.0])[:, None]
+    st = ves.StokesSurface(X, p, 2 * p, near=False)
+    ng = 2 * p * (p + 1)
+    t0 = time.perf_counter()
+    nr = N.NearSingular(st.scoord_far, st.trg, 2, 2 * p, ng)
+    t_setup = time.perf_counter() - t0
+    st.near = nr
+    _, V, q, _ = st.terms(np.random.default_rng(1).uniform(-1, 1, X.shape))
+    t0 = time.perf_counter()
+    st.near_velocity(q, V)
+    t_apply = time.perf_counter() - t0
+    n = max(1, len(nr.pairs))
+    return t_setup / n, t_apply / n
+
+
+def cpu_baseline_step(nv, p, X, n_matvec, near_pairs):
     """The FP64 oracle on the host, on a bounded sample, extrapolated to one
     step of the same workload: per-vesicle singular-operator build (2
     vesicles timed) x nv, plus n_matvec x (far field: all sources against a
     sample of 256 targets, scaled to all targets; local matvec work: one
-    oracle matvec on 2 vesicles, scaled by nv/2)."""
+    oracle matvec on 2 vesicles, scaled by nv/2), plus the near-singular
+    correction per pair (two close vesicles timed) x the step's pair count."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as OL
     from oracle import sph, ves
@@ -186,21 +211,26 @@ def cpu_baseline_step(nv, p, X, n_matvec):
     orc.orc_far_field(OL.p(src), OL.p(den), None, OL.cs(src.shape[0]), OL.p(trg), OL.p(out), OL.cs(ntr),
                       OL.cs(0), OL.cs(0))
     t_far = (time.perf_counter() - t0) * (nv * 2 * p * (p + 1)) / ntr
-    imp = ves.Implicit(X[:2], p, dt=1e-3, shear=1.0)
+    imp = ves.Implicit(X[:2], p, dt=1e-3, shear=1.0, near=False)
     x = np.random.default_rng(0).standard_normal(4 * 2 * p * (p + 2))
     t0 = time.perf_counter()
     imp.matvec(x)
     t_local = (time.perf_counter() - t0) * nv / 2
-    total = t_sing * nv + n_matvec * (t_far + t_local)
+    near_setup, near_apply = _near_cost_per_pair(p)
+    total = t_sing * nv + n_matvec * (t_far + t_local) + \
+        near_pairs * (near_setup + n_matvec * near_apply)
     del st
-    return {"value": total * 1e3, "unit": "ms per implicit step (extrapolated)", "cores": _cpu_threads(),
-            "kind": "port",
-            "sample": "FP64 oracle: singular operator of 2 vesicles (x%d), far field of all %d sources "
-                      "on 256 targets (x%d), one 2-vesicle matvec (x%d), times %.0f matvecs per step "
-                      "(the GPU plain run's count)" % (nv, src.shape[0], (nv * 2 * p * (p + 1)) // ntr,
-                                                       nv // 2, n_matvec),
+    return {"value": total * 1e3, "unit": "ms per implicit step (extrapolated)",
+            "cores": _cpu_threads(), "kind": "port",
+            "sample": "FP64 oracle (numpy; C far field with OpenMP): singular operator of 2 "
+                      "vesicles (x%d), far field of all %d sources on 256 targets (x%d), one "
+                      "2-vesicle matvec (x%d), times %.0f matvecs per step (the GPU plain run's "
+                      "count), plus near-singular setup + %.0f applications per pair on two close "
+                      "vesicles (x%d pairs)" % (nv, src.shape[0], (nv * 2 * p * (p + 1)) // ntr,
+                                                nv // 2, n_matvec, n_matvec, near_pairs),
             "components_s": {"singular_per_vesicle": t_sing, "far_per_matvec": t_far,
-                             "local_per_matvec": t_local}}
+                             "local_per_matvec": t_local, "near_setup_per_pair": near_setup,
+                             "near_apply_per_pair": near_apply}}
 
 
 # ---------------------------------------------------------------- far only

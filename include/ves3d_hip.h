@@ -122,6 +122,13 @@ int ves3d_hip_gemm_f32(char transA, char transB, int m, int n, int k, float alph
                        const float *A, int lda, const float *B, int ldb, float beta,
                        float *C, int ldc);
 
+/* Selects the kernel behind every grouped GEMM (gemm above, the SH
+ * transforms, rotations and singular-operator build): 0 = VALU FMA tiles,
+ * 1 = MFMA (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).  No counterpart
+ * in the reference (its GEMMs are BLAS / PVFMM calls); the initial value
+ * comes from the environment variable VES3D_GEMM ("mfma" / "valu"). */
+int ves3d_hip_set_gemm_kernel(int kind);
+
 /* ---- direct Stokes sums ------------------------------------------------ */
 
 /* Device<DT>::DirectStokes (include/Device.h:205-209).  For each vesicle vt
@@ -339,6 +346,8 @@ typedef struct ves3d_step_stats {
   double ms_setup, ms_rhs, ms_solve, ms_total;
   double ms_far;          /* summed device time of the far-field direct sums */
   int far_calls;          /* number of far-field direct sums in the step */
+  int near_pairs;         /* near-singular (target, vesicle) pairs of the step */
+  double r_near;          /* near-zone radius of the step */
 } ves3d_step_stats;
 
 /* Creates / destroys a stepper (device state for nves vesicles). */
@@ -367,6 +376,37 @@ int ves3d_hip_stepper_precond(void *handle, const double *x, double *y);
  * previous (velocity, tension) as initial guess, true-residual check,
  * X += dt U.  stats_host may be NULL. */
 int ves3d_hip_stepper_step(void *handle, ves3d_step_stats *stats_host);
+
+/* ---- the old pole-moving route (MovePole, include/MovePole.h:8-30) --------
+ * Grid fields [nsub][(p+1)2p], FP64.  scheme: 0 Direct, 1 ViaSpHarm,
+ * 2 DirectEagerEval (enum SingularStokesRot).  all_rot_mats_host /
+ * sh_rot_mats_host: the reference's precomputed/all_rot_mats_{p}_double.txt
+ * and SpHarmRotMats_p{p}_double.txt contents (host memory, file order), or
+ * NULL to build them from their definition (any p).  Replaces the MovePole
+ * constructor (src/MovePole.cc:1-95). */
+int ves3d_hip_movepole_create(int p, int scheme, const double *all_rot_mats_host,
+                              const double *sh_rot_mats_host, void **handle_host);
+int ves3d_hip_movepole_destroy(void *handle);
+/* MovePole::setOperands (src/MovePole.cc:105-156): num operands, operand v
+ * a device array of nsub_host[v] grid functions; the handle keeps the
+ * pointers (valid until the next call). */
+int ves3d_hip_movepole_set_operands(void *handle, const double *const *arr_host,
+                                    const size_t *nsub_host, int num, ves3d_stream_t stream);
+/* MovePole::operator() (src/MovePole.cc:158-163): results_host[v] (device
+ * arrays shaped like operand v) = operand v with its pole moved to grid
+ * point (trg_i, trg_j). */
+int ves3d_hip_movepole_apply(void *handle, int trg_i, int trg_j, double *const *results_host);
+/* Copies the handle's direct rotation matrices to the host,
+ * [p+1][(p+1)2p][(p+1)2p] row-major (row = output point). */
+int ves3d_hip_movepole_matrices(void *handle, double *all_rot_mats_host);
+/* Device<>::AggregateRotation (src/DeviceCPU.cc:644-681): for jj < nlong and
+ * ii < n_vec, res_host[n_vec jj + ii] = CircShift(mat, jj np) vec_host[ii],
+ * mat the column-major np x np matrix of one latitude (np = nlat nlong).
+ * The reference's wrk / n_stream arguments are internal here.  Synchronous. */
+int ves3d_hip_aggregate_rotation_f64(int sh_order, int nlat, int nlong, int n_vec,
+                                     const int *n_sub_host, const double *mat,
+                                     const double *const *vec_host, double *const *res_host,
+                                     ves3d_stream_t stream);
 
 /* ---- near-singular correction (NearSingular<Real>, include/NearSingular.h:8-30)
  * For surface targets (trg_is_surf) and a single-layer density, FP64.

@@ -80,10 +80,13 @@ def test_reductions_transpose_gemm(hip, prec):
     w = RNG.uniform(0.1, 1, (ns, stride)).astype(ndt)
     qw = RNG.uniform(0.1, 1, stride).astype(ndt)
     out = torch.empty(ns * xd, dtype=tdt, device="cuda")
-    call("ves3d_hip_reduce_" + s, ptr(cu(x, tdt)), xd, ptr(cu(w, tdt)), ptr(cu(qw, tdt)), stride, ns, ptr(out))
+    # keep every device input referenced until the kernels have run: a freed
+    # temporary's block would be reused by the next upload
+    gx, gw, gq = cu(x, tdt), cu(w, tdt), cu(qw, tdt)
+    call("ves3d_hip_reduce_" + s, ptr(gx), xd, ptr(gw), ptr(gq), stride, ns, ptr(out))
     torch.cuda.synchronize()
     close(out, D.reduce(x, xd, w, qw, stride, ns), ndt)
-    call("ves3d_hip_reduce_" + s, None, xd, ptr(cu(w, tdt)), ptr(cu(qw, tdt)), stride, ns, ptr(out))
+    call("ves3d_hip_reduce_" + s, None, xd, ptr(gw), ptr(gq), stride, ns, ptr(out))
     torch.cuda.synchronize()
     close(out[:ns], D.reduce(None, 1, w, qw, stride, ns), ndt)
     ctype = ctypes.c_double if s == "f64" else ctypes.c_float
@@ -91,7 +94,8 @@ def test_reductions_transpose_gemm(hip, prec):
     r = ctype()
     call("ves3d_hip_maxabs_" + s, ptr(cu(big, tdt)), big.size, ctypes.byref(r))
     assert r.value == pytest.approx(D.maxabs(big))
-    call("ves3d_hip_algebraic_dot_" + s, ptr(cu(big, tdt)), ptr(cu(big, tdt)), big.size, ctypes.byref(r))
+    gb = cu(big, tdt)
+    call("ves3d_hip_algebraic_dot_" + s, ptr(gb), ptr(gb), big.size, ctypes.byref(r))
     assert r.value == pytest.approx(float(np.dot(big.astype(np.float64), big)), rel=1e-6)
     ok = ctypes.c_int()
     call("ves3d_hip_isnumeric_" + s, ptr(cu(big, tdt)), big.size, ctypes.byref(ok))
@@ -112,9 +116,9 @@ def test_reductions_transpose_gemm(hip, prec):
         A = RNG.standard_normal(lda * (k if ta == "N" else m)).astype(ndt)
         B = RNG.standard_normal(ldb * (n if tb == "N" else k)).astype(ndt)
         C = RNG.standard_normal(ldc * n).astype(ndt)
-        gC = cu(C, tdt)
-        call("ves3d_hip_gemm_" + s, ta.encode(), tb.encode(), m, n, k, 1.5, ptr(cu(A, tdt)), lda,
-             ptr(cu(B, tdt)), ldb, 0.5, ptr(gC), ldc)
+        gA, gB, gC = cu(A, tdt), cu(B, tdt), cu(C, tdt)
+        call("ves3d_hip_gemm_" + s, ta.encode(), tb.encode(), m, n, k, 1.5, ptr(gA), lda,
+             ptr(gB), ldb, 0.5, ptr(gC), ldc)
         torch.cuda.synchronize()
         close(gC, D.gemm(ta, tb, m, n, k, 1.5, A.astype(np.float64), lda, B.astype(np.float64), ldb, 0.5,
                          C.astype(np.float64), ldc), ndt)

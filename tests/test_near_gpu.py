@@ -63,7 +63,8 @@ def test_near_pairs_repulsion_velocity(hip, p, seed):
         vu, vp = st.self_velocity_up(V)
         want = np.transpose(st.near_velocity(q, V).reshape(nv, ng, 3), (0, 2, 1))
         vel = torch.zeros(nv * 3 * ng, dtype=torch.float64, device="cuda")
-        call("ves3d_hip_near_apply", h, ptr(cu(q)), ptr(cu(vu)), ptr(cu(vp)), ptr(vel), None)
+        gq, gu, gp = cu(q), cu(vu), cu(vp)     # referenced until the kernels ran
+        call("ves3d_hip_near_apply", h, ptr(gq), ptr(gu), ptr(gp), ptr(vel), None)
         torch.cuda.synchronize()
         got = vel.cpu().numpy().reshape(nv, 3, ng)
         assert rel(got, want) < 1e-9
@@ -80,7 +81,8 @@ def test_no_pairs_when_far_apart(hip):
     h = ctypes.c_void_p()
     call("ves3d_hip_near_create", p, 2 * p, 2, 5e-2, 0.0, ctypes.byref(h))
     try:
-        call("ves3d_hip_near_set_coord", h, ptr(cu(st.scoord_far)), ptr(cu(st.trg)), None)
+        src, trg = cu(st.scoord_far), cu(st.trg)
+        call("ves3d_hip_near_set_coord", h, ptr(src), ptr(trg), None)
         n = ctypes.c_size_t()
         call("ves3d_hip_near_info", h, ctypes.byref(n), None)
         assert n.value == 0

@@ -44,8 +44,8 @@ def test_scale_freq_low_pass_resample(hip):
     S = RNG.standard_normal((nf, nc))
     scale = RNG.uniform(0.5, 2, nc)
     out = torch.zeros(nf * nc, dtype=torch.float64, device="cuda")
-    call("ves3d_hip_shtrans_scale_freq", p, nf, ptr(cu(ves.fm_to_om(S, p))), ptr(cu(scale)), ptr(out),
-         stream_ptr())
+    g_in, g_sc = cu(ves.fm_to_om(S, p)), cu(scale)     # referenced until the kernel ran
+    call("ves3d_hip_shtrans_scale_freq", p, nf, ptr(g_in), ptr(g_sc), ptr(out), stream_ptr())
     assert rel(out.cpu().numpy(), ves.fm_to_om(S * scale[None, :], p)) < 1e-14
     X = sph.shc2grid(S, p, p)
     g = torch.zeros(nf * ng, dtype=torch.float64, device="cuda")

@@ -70,6 +70,7 @@ for _t, _real in (("f64", c_double), ("f32", ctypes.c_float)):
     _sig("ves3d_hip_transpose_" + _t, P, S, S, P)
     _sig("ves3d_hip_gemm_" + _t, ctypes.c_char, ctypes.c_char, c_int, c_int, c_int, _real, P, c_int,
          P, c_int, _real, P, c_int)
+_sig("ves3d_hip_set_gemm_kernel", c_int)
 _sig("ves3d_hip_stokes_all2all_f64", P, P, S, P, P)
 _sig("ves3d_hip_stokes_all2all_f32c", P, P, S, P, P)
 _sig("ves3d_hip_direct_eval_f64", P, P, P, S, P, P, S, S, S, P)
@@ -100,6 +101,12 @@ _sig("ves3d_hip_stepper_matvec", P, P, P)
 _sig("ves3d_hip_stepper_precond", P, P, P)
 _sig("ves3d_hip_stepper_step", P, P)
 _sig("ves3d_hip_stepper_attach_pc", P, P, c_int)
+_sig("ves3d_hip_movepole_create", c_int, c_int, P, P, P)
+_sig("ves3d_hip_movepole_destroy", P)
+_sig("ves3d_hip_movepole_set_operands", P, P, P, c_int, P)
+_sig("ves3d_hip_movepole_apply", P, c_int, c_int, P)
+_sig("ves3d_hip_movepole_matrices", P, P)
+_sig("ves3d_hip_aggregate_rotation_f64", c_int, c_int, c_int, c_int, P, P, P, P, P)
 _sig("ves3d_hip_near_create", c_int, c_int, S, c_double, c_double, P)
 _sig("ves3d_hip_near_destroy", P)
 _sig("ves3d_hip_near_set_coord", P, P, P, P)

@@ -4,6 +4,8 @@
 // of 16; each lane owns a 4x4 register tile.  Operand tiles are loaded with
 // the lane-to-element mapping chosen from the operand's unit stride so global
 // reads stay coalesced whether the operand is row- or column-contiguous.
+#include <atomic>
+#include <cstdlib>
 #include <vector>
 
 #include "gemm.hpp"
@@ -96,6 +98,126 @@ __global__ __launch_bounds__(GB_T) void k_gemm_grouped(
   }
 }
 
+// MFMA form of the same tile: the 64x64 output tile is split into 2x2 wave
+// sub-tiles of 32x32, each a 2x2 block of 16x16 MFMA accumulators
+// (v_mfma_{f64_16x16x4f64, f32_16x16x4f32}: lane l holds A(l%16, l/16) and
+// B(l/16, l%16) of a 16x4 / 4x16 step and D rows 4(l/16)..4(l/16)+3 of
+// column l%16).  LDS staging is shared with the VALU kernel.
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 step(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 step(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename T>
+__global__ __launch_bounds__(GB_T) void k_gemm_mfma(const GemmDesc *__restrict__ descs,
+                                                    const T *__restrict__ A,
+                                                    const T *__restrict__ B, T *__restrict__ C,
+                                                    T alpha, T beta) {
+  typedef typename Mfma<T>::v4 v4;
+  __shared__ T As[GB_K][GB_M + 1];
+  __shared__ T Bs[GB_K][GB_N + 1];
+  const GemmDesc d = descs[blockIdx.y];
+  const int tiles_n = (d.N + GB_N - 1) / GB_N;
+  const int tm = blockIdx.x / tiles_n, tn = blockIdx.x % tiles_n;
+  if (tm * GB_M >= d.M || tn * GB_N >= d.N) return;
+  const long b = blockIdx.z;
+  const T *Ab = A + d.a_off + b * d.a_bs;
+  const T *Bb = B + d.b_off + b * d.b_bs;
+  T *Cb = C + d.c_off + b * d.c_bs;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  const int m0 = tm * GB_M, n0 = tn * GB_N;
+  const bool a_kfast = (d.a_cs == 1);
+  const bool b_nfast = (d.b_cs == 1);
+  v4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = v4{T(0), T(0), T(0), T(0)};
+
+  for (int k0 = 0; k0 < d.K; k0 += GB_K) {
+#pragma unroll
+    for (int q = 0; q < (GB_M * GB_K) / GB_T; ++q) {
+      const int e = tid + q * GB_T;
+      int m, k;
+      if (a_kfast) {
+        k = e % GB_K;
+        m = e / GB_K;
+      } else {
+        m = e % GB_M;
+        k = e / GB_M;
+      }
+      const int gm = m0 + m, gk = k0 + k;
+      As[k][m] = (gm < d.M && gk < d.K) ? Ab[gm * d.a_rs + gk * d.a_cs] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < (GB_N * GB_K) / GB_T; ++q) {
+      const int e = tid + q * GB_T;
+      int n, k;
+      if (b_nfast) {
+        n = e % GB_N;
+        k = e / GB_N;
+      } else {
+        k = e % GB_K;
+        n = e / GB_K;
+      }
+      const int gn = n0 + n, gk = k0 + k;
+      Bs[k][n] = (gn < d.N && gk < d.K) ? Bb[gk * d.b_rs + gn * d.b_cs] : T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GB_K; kk += 4) {
+      T a[2], bv[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) a[i] = As[kk + lk][wm + i * 16 + li];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bv[j] = Bs[kk + lk][wn + j * 16 + li];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<T>::step(a[i], bv[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int gn = n0 + wn + j * 16 + li;
+      if (gn >= d.N) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gm = m0 + wm + i * 16 + lk * 4 + r;
+        if (gm >= d.M) continue;
+        T *c = Cb + gm * d.c_rs + gn * d.c_cs;
+        *c = (beta == T(0)) ? alpha * acc[i][j][r] : alpha * acc[i][j][r] + beta * (*c);
+      }
+    }
+}
+
+// Kernel choice: 1 = MFMA, 0 = VALU.  Initialised from VES3D_GEMM
+// ("mfma" / "valu") at load, changed by ves3d_hip_set_gemm_kernel.
+std::atomic<int> g_gemm_kind{[] {
+  const char *e = getenv("VES3D_GEMM");
+  return (e && e[0] == 'm') ? 1 : 0;
+}()};
+
+bool use_mfma() { return g_gemm_kind.load(std::memory_order_relaxed) == 1; }
+
 template <typename T>
 int gemm_grouped(const GemmDesc *d_descs, int ndesc, int batch, int max_m,
                  int max_n, const T *A, const T *B, T *C, T alpha, T beta,
@@ -105,7 +227,10 @@ int gemm_grouped(const GemmDesc *d_descs, int ndesc, int batch, int max_m,
   const long tiles = (long)cdiv(max_m, GB_M) * cdiv(max_n, GB_N);
   if (tiles > 0x7fffffffL) return VES3D_INVALID_PARAMETER;
   dim3 grid((unsigned)tiles, ndesc, batch);
-  k_gemm_grouped<T><<<grid, GB_T, 0, st>>>(d_descs, A, B, C, alpha, beta);
+  if (use_mfma())
+    k_gemm_mfma<T><<<grid, GB_T, 0, st>>>(d_descs, A, B, C, alpha, beta);
+  else
+    k_gemm_grouped<T><<<grid, GB_T, 0, st>>>(d_descs, A, B, C, alpha, beta);
   return launch_status();
 }
 
@@ -125,3 +250,9 @@ GemmDesc *upload_descs(const GemmDesc *h, int n) {
 }
 
 }  // namespace v3d
+
+extern "C" int ves3d_hip_set_gemm_kernel(int kind) {
+  if (kind != 0 && kind != 1) return VES3D_INVALID_PARAMETER;
+  v3d::g_gemm_kind.store(kind, std::memory_order_relaxed);
+  return VES3D_SUCCESS;
+}

@@ -401,6 +401,8 @@ struct Stepper {
       stats->ms_total = ms(t0, t4);
       stats->ms_far = far_ms;
       stats->far_calls = far_calls;
+      stats->near_pairs = stokes.use_near ? (int)stokes.near.npairs : 0;
+      stats->r_near = stokes.use_near ? stokes.near.r_near : 0.0;
     }
     return VES3D_SUCCESS;
   }

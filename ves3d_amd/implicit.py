@@ -23,7 +23,8 @@ class StepStats(ctypes.Structure):
                 ("true_relres", ctypes.c_double), ("ms_setup", ctypes.c_double),
                 ("ms_rhs", ctypes.c_double), ("ms_solve", ctypes.c_double),
                 ("ms_total", ctypes.c_double), ("ms_far", ctypes.c_double),
-                ("far_calls", ctypes.c_int)]
+                ("far_calls", ctypes.c_int), ("near_pairs", ctypes.c_int),
+                ("r_near", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
