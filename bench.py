@@ -57,7 +57,8 @@ def bench_step(args):
     nv, p = args.nves, args.p
     X0 = W.implicit_step_case(nv=nv, p=p)
     Xg = torch.from_numpy(X0).cuda()
-    common = dict(dt=1e-3, shear_rate=1.0, bending_modulus=1e-2, tol=1e-6, max_iter=args.max_iter)
+    common = dict(dt=1e-3, shear_rate=1.0, bending_modulus=1e-2, tol=1e-6, max_iter=args.max_iter,
+                  r_near=args.r_near)
 
     # ---- plain FP64 form
     plain = I.Stepper(nv, p, precision=0, precond=1, **common)
@@ -142,7 +143,8 @@ def bench_step(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": "implicit step, %d perturbed ellipsoids (1,1,0.6+-0.05) p=%d, "
                                "20%% volume fraction, shear 1.0, dt 1e-3, BiCGStab 1e-6" % (nv, p),
-                   "nves": nv, "p": p, "p_up": 2 * p, "unknowns": nv * nin},
+                   "nves": nv, "p": p, "p_up": 2 * p, "unknowns": nv * nin,
+                   "r_near": stats_plain[-1]["r_near"]},
         "ms_per_step_plain": ms_plain, "speedup_vs_plain": ms_plain / ms_mixed,
         "iterations_learned": it_mixed, "iterations_plain": it_plain,
         "true_relres_learned": [s["true_relres"] for s in stats_mixed],
@@ -311,6 +313,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--r-near", type=float, default=0.1,
+                    help="near-singular zone radius (0: the reference's r_ves 2/sqrt(p_up))")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "

@@ -377,6 +377,18 @@ int ves3d_hip_stepper_precond(void *handle, const double *x, double *y);
  * X += dt U.  stats_host may be NULL. */
 int ves3d_hip_stepper_step(void *handle, ves3d_step_stats *stats_host);
 
+/* ---- input formats (host only, no device work) --------------------------
+ * DataIO::ReadDataStl in ASCII (src/DataIO_templates.cc:72-98): every number
+ * of the file, '#' starting a comment to the end of its line.  With
+ * out_host == NULL only counts. */
+int ves3d_read_ascii(const char *path, double *out_host, size_t capacity, size_t *count_host);
+/* InitializeShapes (src/HelperFuns.cc:366-432): X_host [nves][3][(p+1)2p] from
+ * a shape gallery (shapes [3][(p+1)2p] back to back, gallery_len values) and
+ * a geometry specification of 8 values per vesicle: shape index, scale,
+ * centre (3), rotation angles z, y, z (X = Rz Ry Rz (scale x) + centre). */
+int ves3d_initialize_shapes(int p, size_t nves, const double *gallery, size_t gallery_len,
+                            const double *geo_spec, double *X_host);
+
 /* ---- the old pole-moving route (MovePole, include/MovePole.h:8-30) --------
  * Grid fields [nsub][(p+1)2p], FP64.  scheme: 0 Direct, 1 ViaSpHarm,
  * 2 DirectEagerEval (enum SingularStokesRot).  all_rot_mats_host /

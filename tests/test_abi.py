@@ -14,7 +14,7 @@ def _declared():
     for fn in os.listdir(os.path.join(ROOT, "include")):
         if fn.endswith(".h"):
             txt = open(os.path.join(ROOT, "include", fn)).read()
-            names |= set(re.findall(r"\bint\s+(ves3d_hip_\w+)\s*\(", txt))
+            names |= set(re.findall(r"\bint\s+(ves3d_\w+)\s*\(", txt))
     return names
 
 

@@ -101,6 +101,8 @@ _sig("ves3d_hip_stepper_matvec", P, P, P)
 _sig("ves3d_hip_stepper_precond", P, P, P)
 _sig("ves3d_hip_stepper_step", P, P)
 _sig("ves3d_hip_stepper_attach_pc", P, P, c_int)
+_sig("ves3d_read_ascii", ctypes.c_char_p, P, S, P)
+_sig("ves3d_initialize_shapes", c_int, S, P, S, P, P)
 _sig("ves3d_hip_movepole_create", c_int, c_int, P, P, P)
 _sig("ves3d_hip_movepole_destroy", P)
 _sig("ves3d_hip_movepole_set_operands", P, P, P, c_int, P)

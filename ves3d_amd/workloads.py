@@ -2,9 +2,10 @@
 
 Grids follow the reference's SphericalHarmonics convention: latitudes are the
 p+1 Gauss-Legendre nodes x_i in ascending order, longitudes phi_j = pi j / p
-(j < 2p), and the point (i, j) of the unit sphere is
-(x_i, sqrt(1-x_i^2) sin phi_j, sqrt(1-x_i^2) cos phi_j)
-(src/SphericalHarmonics.cc:785-790).  The point index is i*2p + j.
+(j < 2p), point index i*2p + j.  Shapes are embedded as the reference's
+shape files are (precomputed/shape_gallery_*.txt): the point (i, j) of the
+unit sphere is (sqrt(1-x_i^2) cos phi_j, sqrt(1-x_i^2) sin phi_j, x_i), so the
+polar axis is z and the grid's x_u x x_v normal points outward.
 """
 import numpy as np
 
@@ -89,8 +90,8 @@ def perturbed_shapes(nves, p, amp=0.05, seed=SEED_BASE, radius=1.0):
 
 def ellipsoid_shapes(nves, p, seed=SEED_BASE, c_mean=0.6, c_jitter=0.05, amp=0.0):
     """Ellipsoids with semi-axes (1, 1, c), c = c_mean +- c_jitter uniform,
-    the short axis along the grid's polar (first) coordinate.  Returns the
-    shapes [nves][3][2p(p+1)] and their volumes (4/3 pi c)."""
+    the short axis along the grid's polar axis (z).  Returns the shapes
+    [nves][3][2p(p+1)] and their volumes (4/3 pi c)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     X, _ = sphere_grid(p)
     out = np.empty((nves, 3, X.shape[1]))
@@ -99,7 +100,7 @@ def ellipsoid_shapes(nves, p, seed=SEED_BASE, c_mean=0.6, c_jitter=0.05, amp=0.0
         c = c_mean + rng.uniform(-c_jitter, c_jitter)
         rr = 1.0 + (_radial_perturbation(p, amp, rng) if amp > 0 else np.zeros(X.shape[1]))
         out[v] = X * rr[None, :]
-        out[v, 0] *= c
+        out[v, 2] *= c
         vol[v] = 4.0 / 3.0 * np.pi * c
     return out, vol
 
@@ -173,3 +174,34 @@ def stokes_direct_sum_case(nv=512, p=16, volfrac=0.15, seed=SEED_BASE + 1):
     n = np.tile(X.T, (nv, 1))
     dl = np.concatenate([g, n], axis=1)
     return np.ascontiguousarray(pos), np.ascontiguousarray(sl), np.ascontiguousarray(dl), npv
+
+
+def read_ascii(path):
+    """All numbers of an ASCII data file ('#' comments), through the
+    library's DataIO::ReadDataStl equivalent."""
+    import ctypes
+    from . import call
+    n = ctypes.c_size_t()
+    call("ves3d_read_ascii", path.encode(), None, 0, ctypes.byref(n))
+    out = np.zeros(n.value)
+    call("ves3d_read_ascii", path.encode(), out.ctypes.data_as(ctypes.c_void_p), out.size,
+         ctypes.byref(n))
+    return out
+
+
+def suspension_from_files(gallery_path, spec_path, p, nves=None):
+    """Initial positions [nves][3][2p(p+1)] from a shape gallery and a
+    geometry specification file in the reference's formats (Simulation::
+    setup_from_options, src/ves3d_simulation.cc:126-168)."""
+    import ctypes
+    from . import call
+    gallery = read_ascii(gallery_path)
+    spec = read_ascii(spec_path)
+    nv = spec.size // 8 if nves is None else nves
+    if spec.size < 8 * nv:
+        raise ValueError("geometry file holds %d vesicles, %d requested" % (spec.size // 8, nv))
+    X = np.zeros((nv, 3, 2 * p * (p + 1)))
+    spec = np.ascontiguousarray(spec[:8 * nv])
+    call("ves3d_initialize_shapes", p, nv, gallery.ctypes.data_as(ctypes.c_void_p), gallery.size,
+         spec.ctypes.data_as(ctypes.c_void_p), X.ctypes.data_as(ctypes.c_void_p))
+    return X
