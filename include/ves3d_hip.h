@@ -334,6 +334,8 @@ typedef struct ves3d_step_params {
   int near_off;           /* 1 disables the near-singular correction and the repulsion force */
   double repul_dist;      /* repulsion length (0: the reference default 5e-2; < 0: none) */
   double r_near;          /* near-zone radius (0: r_ves 2/sqrt(p_up), src/NearSingular.cc:83) */
+  int post_off;           /* 1 skips the reparametrization and area / volume correction
+                             that follow the position update (src/EvolveSurface.cc:330-335) */
 } ves3d_step_params;
 
 typedef struct ves3d_step_stats {
@@ -348,6 +350,9 @@ typedef struct ves3d_step_stats {
   int far_calls;          /* number of far-field direct sums in the step */
   int near_pairs;         /* near-singular (target, vesicle) pairs of the step */
   double r_near;          /* near-zone radius of the step */
+  int reparam_iters;      /* reparametrization iterations (0 with post_off) */
+  int av_iters;           /* area / volume correction iterations */
+  double ms_post;         /* wall time of the two */
 } ves3d_step_stats;
 
 /* Creates / destroys a stepper (device state for nves vesicles). */
@@ -374,8 +379,22 @@ int ves3d_hip_stepper_matvec(void *handle, const double *x, double *y);
 int ves3d_hip_stepper_precond(void *handle, const double *x, double *y);
 /* One full implicit step: prepare, right-hand side, BiCGStab from the
  * previous (velocity, tension) as initial guess, true-residual check,
- * X += dt U.  stats_host may be NULL. */
+ * X += dt U, then (unless post_off) InterfacialVelocity::reparam and
+ * EvolveSurface::AreaVolumeCorrection against the area and volume recorded
+ * by the last set_state with positions.  stats_host may be NULL. */
 int ves3d_hip_stepper_step(void *handle, ves3d_step_stats *stats_host);
+/* InterfacialVelocity::reparam without upsampling (src/InterfacialVelocity.cc:
+ * 1639-1773) on the current positions, with the reference's default
+ * parameters (PolyKReparam, exponent 4, rep_ts 1e-3, rep_tol 1e-4, at most 10
+ * iterations, src/Parameters.cc:39-45).  Synchronous. */
+int ves3d_hip_stepper_reparam(void *handle, int *iters_host);
+/* EvolveSurface::AreaVolumeCorrection (src/EvolveSurface.cc:352-470): Newton
+ * steps on the upsampled surface along (-2H n, n) until area and volume
+ * match the recorded ones to 1e-12 (at most 10).  Synchronous. */
+int ves3d_hip_stepper_area_volume(void *handle, int *iters_host);
+/* Overrides the recorded area / volume targets (host arrays [nves]). */
+int ves3d_hip_stepper_set_area_volume(void *handle, const double *area_host,
+                                      const double *vol_host);
 
 /* ---- input formats (host only, no device work) --------------------------
  * DataIO::ReadDataStl in ASCII (src/DataIO_templates.cc:72-98): every number

@@ -43,11 +43,13 @@ def test_rhs_matvec_precond_f64(hip, nv, p):
     st.prepare()
     b = st.rhs().cpu().numpy()
     b0 = orc.rhs()
-    assert rel(b, b0) < 1e-9, rel(b, b0)
+    # 2e-8 where the case has near-singular pairs (see test_near_gpu.py)
+    tol = 2e-8 if len(orc.stokes.near.pairs) else 1e-9
+    assert rel(b, b0) < tol, rel(b, b0)
     x = np.random.default_rng(nv).standard_normal(st.n)
     y = st.matvec(cu(x)).cpu().numpy()
     y0 = orc.matvec(x)
-    assert rel(y, y0) < 1e-9, rel(y, y0)
+    assert rel(y, y0) < tol, rel(y, y0)
     z = st.precond(cu(x)).cpu().numpy()
     assert rel(z, orc.precond(x)) < 1e-14
     st.close()
@@ -101,7 +103,7 @@ def test_full_step_f64(hip, precond):
     assert st0 == "success"
     u0, s0 = orc.unpack(x0)
     st = I.Stepper(nv, p, dt=dt, shear_rate=shear, precision=0, precond=precond, tol=1e-6,
-                   max_iter=300)
+                   max_iter=300, post=False)
     st.set_state(cu(X))
     stats = st.step()
     assert stats["status"] == 100 and stats["true_relres"] < 1e-6 * 1.01, stats
@@ -109,4 +111,46 @@ def test_full_step_f64(hip, precond):
     U = U.cpu().numpy()
     assert np.linalg.norm(U - u0) / np.linalg.norm(u0) < 1e-5
     assert np.linalg.norm(Xn.cpu().numpy() - (X + dt * u0)) / np.linalg.norm(X) < 1e-9
+    st.close()
+
+
+def test_reparam_and_area_volume_vs_oracle(hip):
+    """Reparametrization and area / volume correction on the device against
+    the oracle (oracle/ves.py reparam, area_volume_correction): same
+    iteration counts, positions to 1e-9 relative."""
+    nv, p = 3, 8
+    X = _case(nv, p, seed=21, volfrac=0.02)
+    st = I.Stepper(nv, p, precision=0, precond=1)
+    st.set_state(cu(X))
+    it = st.reparam()
+    Xr, it0 = ves.reparam(X, p)
+    Xg, _, _ = st.get_state()
+    assert it == it0
+    assert rel(Xg.cpu().numpy(), Xr) < 1e-9
+    # perturb the recorded targets: the correction restores them
+    a0, v0 = ves.area_volume(Xr, p, 2 * p)
+    st.set_area_volume(a0 * 1.001, v0 * 0.999)
+    it = st.area_volume()
+    Xa, it1 = ves.area_volume_correction(Xr, p, 2 * p, a0 * 1.001, v0 * 0.999)
+    Xg, _, _ = st.get_state()
+    assert it == it1 and it > 0
+    assert rel(Xg.cpu().numpy(), Xa) < 1e-9
+    a1, v1 = ves.area_volume(Xg.cpu().numpy(), p, 2 * p)
+    assert np.max(np.abs(a1 - a0 * 1.001) / a0) < 1e-11
+    assert np.max(np.abs(v1 - v0 * 0.999) / v0) < 1e-11
+    st.close()
+
+
+def test_full_step_with_post(hip):
+    nv, p = 2, 6
+    X = _case(nv, p, seed=4)
+    st = I.Stepper(nv, p, dt=1e-3, shear_rate=1.0, precision=0, precond=1, tol=1e-6, max_iter=300)
+    st.set_state(cu(X))
+    stats = st.step()
+    assert stats["status"] == 100 and stats["reparam_iters"] >= 1, stats
+    Xn, U, _ = st.get_state()
+    Xr, _ = ves.reparam(X + 1e-3 * U.cpu().numpy(), p)
+    a0, v0 = ves.area_volume(X, p, 2 * p)
+    Xa, _ = ves.area_volume_correction(Xr, p, 2 * p, a0, v0)
+    assert rel(Xn.cpu().numpy(), Xa) < 1e-9
     st.close()

@@ -2,10 +2,12 @@
 against the oracle (oracle/near.py) through the C ABI, and of the implicit
 step with near pairs present.
 
-Tolerances: pair set identical; repulsion force and near velocity within
-1e-9 relative (FP64; the patch projection iterates to a 1e-8 tangential
-residual, and the two sides evaluate the same formulas in a different
-association order); matvec / rhs 1e-9 as in test_implicit_gpu.py.
+Tolerances: pair set identical; repulsion force 1e-9 relative; near velocity,
+and the right-hand side / matvec with near pairs, 2e-8 relative: the patch
+projection is a Newton iteration that stops at a 1e-8 tangential residual,
+and the two sides reach that point along paths that differ in rounding (the
+same formulas in a different association order), which moves the projected
+point, and so the interpolated velocity, at the 1e-9..1e-8 level.
 """
 import ctypes
 
@@ -67,7 +69,7 @@ def test_near_pairs_repulsion_velocity(hip, p, seed):
         call("ves3d_hip_near_apply", h, ptr(gq), ptr(gu), ptr(gp), ptr(vel), None)
         torch.cuda.synchronize()
         got = vel.cpu().numpy().reshape(nv, 3, ng)
-        assert rel(got, want) < 1e-9
+        assert rel(got, want) < 2e-8
     finally:
         call("ves3d_hip_near_destroy", h)
 
@@ -102,8 +104,8 @@ def test_stepper_rhs_matvec_with_near(hip, precision):
     st.prepare()
     print("stepper rhs/matvec with near pairs=%d" % len(orc.stokes.near.pairs), flush=True)
     b = st.rhs().cpu().numpy()
-    assert rel(b, orc.rhs()) < 1e-9
+    assert rel(b, orc.rhs()) < 2e-8
     x = np.random.default_rng(3).standard_normal(st.n)
     y = st.matvec(cu(x)).cpu().numpy()
-    assert rel(y, orc.matvec(x)) < 1e-9
+    assert rel(y, orc.matvec(x)) < 2e-8
     st.close()

@@ -163,3 +163,40 @@ def test_implicit_solve_small():
     x, res, it, st = ves.bicgstab(imp.matvec, b, np.zeros_like(b), 1e-6, 200, imp.precond)
     assert st == "success"
     assert np.linalg.norm(imp.matvec(x) - b) / np.linalg.norm(b) < 1e-6
+
+
+def test_reparam_is_tangential_and_lowers_the_energy():
+    """InterfacialVelocity::reparam moves points along the surface: the
+    normal component of the displacement is second order, the surface's area
+    and volume barely change, and the degree-weighted energy decreases."""
+    p = 8
+    X = W.perturbed_shapes(1, p, amp=0.1, seed=12)
+    Xr, it = ves.reparam(X, p)
+    d = Xr - X
+    S = ves.Surface(X, p)
+    dn = np.abs(np.sum(d * S.normal, axis=1))
+    assert it >= 1 and np.max(np.abs(d)) > 1e-4
+    # tangential up to the band-limiting projection that follows each
+    # tangential projection (it re-introduces a normal part)
+    assert np.max(dn) < 0.2 * np.max(np.sqrt(np.sum(d * d, axis=1)))
+    a0, v0 = ves.area_volume(X, p, 2 * p)
+    a1, v1 = ves.area_volume(Xr, p, 2 * p)
+    assert abs(a1[0] - a0[0]) / a0[0] < 1e-3 and abs(v1[0] - v0[0]) / v0[0] < 1e-3
+    w = ves._rep_weights(p, 4)[ves._degree_of_coeff(p)]
+    energy = lambda Y: float(np.sum(sph.grid2shc(Y.reshape(3, -1), p, p) ** 2 * w[None, :]))
+    assert energy(Xr) < energy(X)
+
+
+def test_area_volume_correction_restores_targets():
+    p = 8
+    X = W.perturbed_shapes(2, p, amp=0.1, seed=13)
+    a0, v0 = ves.area_volume(X, p, 2 * p)
+    # the filtered curvature and the resampling make the iteration converge
+    # linearly (about x0.08 per step): from 1e-3 it needs ~9 of its 10 steps
+    Xc, it = ves.area_volume_correction(X * 1.0005, p, 2 * p, a0, v0)
+    a1, v1 = ves.area_volume(Xc, p, 2 * p)
+    assert 1 <= it <= 10
+    assert np.max(np.abs(a1 - a0) / a0) < 1e-11 and np.max(np.abs(v1 - v0) / v0) < 1e-11
+    # already on target: no iteration, no change
+    Xs, it0 = ves.area_volume_correction(X, p, 2 * p, a0, v0)
+    assert it0 == 0 and np.array_equal(Xs, X)

@@ -101,6 +101,9 @@ _sig("ves3d_hip_stepper_matvec", P, P, P)
 _sig("ves3d_hip_stepper_precond", P, P, P)
 _sig("ves3d_hip_stepper_step", P, P)
 _sig("ves3d_hip_stepper_attach_pc", P, P, c_int)
+_sig("ves3d_hip_stepper_reparam", P, P)
+_sig("ves3d_hip_stepper_area_volume", P, P)
+_sig("ves3d_hip_stepper_set_area_volume", P, P, P)
 _sig("ves3d_read_ascii", ctypes.c_char_p, P, S, P)
 _sig("ves3d_initialize_shapes", c_int, S, P, S, P, P)
 _sig("ves3d_hip_movepole_create", c_int, c_int, P, P, P)

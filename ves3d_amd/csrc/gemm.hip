@@ -99,10 +99,12 @@ __global__ __launch_bounds__(GB_T) void k_gemm_grouped(
 }
 
 // MFMA form of the same tile: the 64x64 output tile is split into 2x2 wave
-// sub-tiles of 32x32, each a 2x2 block of 16x16 MFMA accumulators
-// (v_mfma_{f64_16x16x4f64, f32_16x16x4f32}: lane l holds A(l%16, l/16) and
-// B(l/16, l%16) of a 16x4 / 4x16 step and D rows 4(l/16)..4(l/16)+3 of
-// column l%16).  LDS staging is shared with the VALU kernel.
+// sub-tiles of 32x32, each a 2x2 block of 16x16 MFMA accumulators.  For both
+// v_mfma_f64_16x16x4f64 and v_mfma_f32_16x16x4f32 lane l supplies A(l%16, l/16)
+// and B(l/16, l%16) of a 16x4 / 4x16 step and holds 4 results of column
+// l%16; their rows differ: l/16 + 4r (f64) and 4(l/16) + r (f32) for result
+// register r (the accumulator layouts composable_kernel's xdlops_gemm.hpp
+// records as group_size 1 / 4).  LDS staging is shared with the VALU kernel.
 template <typename T>
 struct Mfma;
 template <>
@@ -111,6 +113,7 @@ struct Mfma<double> {
   static __device__ __forceinline__ v4 step(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
 };
 template <>
 struct Mfma<float> {
@@ -118,6 +121,7 @@ struct Mfma<float> {
   static __device__ __forceinline__ v4 step(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
 };
 
 template <typename T>
@@ -201,7 +205,7 @@ __global__ __launch_bounds__(GB_T) void k_gemm_mfma(const GemmDesc *__restrict__
       if (gn >= d.N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gm = m0 + wm + i * 16 + lk * 4 + r;
+        const int gm = m0 + wm + i * 16 + Mfma<T>::row(lk, r);
         if (gm >= d.M) continue;
         T *c = Cb + gm * d.c_rs + gn * d.c_cs;
         *c = (beta == T(0)) ? alpha * acc[i][j][r] : alpha * acc[i][j][r] + beta * (*c);

@@ -52,6 +52,161 @@ __global__ void k_om_to_fm(const double *__restrict__ om, long nf, int nc,
     fm[i] = om[nf * bl.x + f * bl.y + (c - bl.x)];
   }
 }
+// ---- after the solve: reparametrization and area / volume correction
+// S[f][c] *= w[c] (function-major coefficients, w per coefficient).
+__global__ void k_scale_fm(double *S, long nf, int nc, const double *__restrict__ w) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nf * nc;
+       i += (long)gridDim.x * blockDim.x)
+    S[i] *= w[i % nc];
+}
+// u -= (u . n) n on AxisMajor [nv][3][ng] fields.
+__global__ void k_tangent(double *u, const double *__restrict__ n, long nv, long ng) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nv * ng;
+       i += (long)gridDim.x * blockDim.x) {
+    const long v = i / ng, g = i % ng, b = v * 3 * ng + g;
+    const double d = u[b] * n[b] + u[b + ng] * n[b + ng] + u[b + 2 * ng] * n[b + 2 * ng];
+    u[b] -= d * n[b];
+    u[b + ng] -= d * n[b + ng];
+    u[b + 2 * ng] -= d * n[b + 2 * ng];
+  }
+}
+// Per vesicle (one workgroup each): u /= max_g |u_g|, then the degree-weighted
+// coefficient inner products xx, xu, uu of x and u (Xc, Uc: function-major
+// coefficients of the 3 components; the scaling of u enters uu and xu).
+__global__ __launch_bounds__(256) void k_rep_norm_inner(double *u, long ng,
+                                                        const double *__restrict__ Xc,
+                                                        double *Uc, int nc,
+                                                        const double *__restrict__ w,
+                                                        double *out3) {
+  const long v = blockIdx.x;
+  double *uv = u + v * 3 * ng;
+  __shared__ double sh[3][4];
+  __shared__ double smax;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double m = 0;
+  for (long g = threadIdx.x; g < ng; g += 256) {
+    const double a = uv[g], b = uv[ng + g], c = uv[2 * ng + g];
+    m = fmax(m, sqrt(a * a + b * b + c * c));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if (lane == 0) sh[0][wv] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) smax = fmax(fmax(sh[0][0], sh[0][1]), fmax(sh[0][2], sh[0][3]));
+  __syncthreads();
+  const double inv = 1.0 / smax;
+  for (long g = threadIdx.x; g < 3 * ng; g += 256) uv[g] *= inv;
+  const double *xc = Xc + v * 3 * nc;
+  double *uc = Uc + v * 3 * nc;
+  double xx = 0, xu = 0, uu = 0;
+  for (long c = threadIdx.x; c < 3L * nc; c += 256) {
+    const double ww = w[c % nc], a = xc[c], b = uc[c] * inv;
+    uc[c] = b;
+    xx += ww * a * a;
+    xu += ww * a * b;
+    uu += ww * b * b;
+  }
+  __syncthreads();
+  xx = wave_sum(xx);
+  xu = wave_sum(xu);
+  uu = wave_sum(uu);
+  if (lane == 0) {
+    sh[0][wv] = xx;
+    sh[1][wv] = xu;
+    sh[2][wv] = uu;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int k = threadIdx.x;
+    out3[v * 3 + k] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
+  }
+}
+// Per-vesicle step of the reparametrization (src/InterfacialVelocity.cc:
+// 1728-1741): dt = min(ts, -xu/uu), zero (and the vesicle frozen) when below
+// tol or frozen before (E == 0 with have_E).
+__global__ void k_rep_step(const double *__restrict__ in3, long nv, double ts, double tol,
+                           int have_E, double *E, double *dt) {
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+       v += (long)gridDim.x * blockDim.x) {
+    double xx = in3[3 * v], d = fmin(ts, -in3[3 * v + 1] / in3[3 * v + 2]);
+    if (d < tol || (have_E && E[v] == 0.0)) {
+      xx = 0.0;
+      d = 0.0;
+    }
+    dt[v] = d;
+    E[nv + v] = xx;  // candidate energy, committed by the host if the loop goes on
+  }
+}
+// x += dt[v] u.
+__global__ void k_rep_update(double *x, const double *__restrict__ u,
+                             const double *__restrict__ dt, long nv, long n3) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nv * n3;
+       i += (long)gridDim.x * blockDim.x)
+    x[i] += dt[i / n3] * u[i];
+}
+// Per vesicle on the upsampled surface: area, volume and the integrals of
+// (-2H)^2, -2H and 1 against the area element (GaussLegendreIntegrator,
+// src/EvolveSurface.cc:352-440): out4 = [A, V, h2, h1].
+__global__ __launch_bounds__(256) void k_av_integrals(const double *__restrict__ X,
+                                                      const double *__restrict__ nrm,
+                                                      const double *__restrict__ W,
+                                                      const double *__restrict__ H,
+                                                      const double *__restrict__ qw, long ng,
+                                                      int nlon, double *out4) {
+  const long v = blockIdx.x;
+  __shared__ double sh[4][4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double a = 0, vol = 0, h2 = 0, h1 = 0;
+  for (long g = threadIdx.x; g < ng; g += 256) {
+    const long b = v * 3 * ng + g;
+    const double dw = W[v * ng + g] * qw[g / nlon];
+    const double xn = X[b] * nrm[b] + X[b + ng] * nrm[b + ng] + X[b + 2 * ng] * nrm[b + 2 * ng];
+    const double hx = -2.0 * H[v * ng + g];
+    a += dw;
+    vol += xn / 3.0 * dw;
+    h2 += hx * hx * dw;
+    h1 += hx * dw;
+  }
+  a = wave_sum(a);
+  vol = wave_sum(vol);
+  h2 = wave_sum(h2);
+  h1 = wave_sum(h1);
+  if (lane == 0) {
+    sh[0][wv] = a;
+    sh[1][wv] = vol;
+    sh[2][wv] = h2;
+    sh[3][wv] = h1;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int k = threadIdx.x;
+    out4[v * 4 + k] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
+  }
+}
+// Newton update of the area / volume correction: per vesicle
+// [dA/dX dA/dY; dV/dX dV/dY] (dX, dY) = (area err, vol err), then
+// x_up += (dX (-2H) + dY) n.
+__global__ void k_av_update(double *Xu, const double *__restrict__ nrm,
+                            const double *__restrict__ H, const double *__restrict__ in4,
+                            const double *__restrict__ a0, const double *__restrict__ v0,
+                            long nv, long ng) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nv * ng;
+       i += (long)gridDim.x * blockDim.x) {
+    const long v = i / ng, g = i % ng;
+    const double *q = in4 + 4 * v;
+    const double dAdX = q[2], dAdY = q[3], dVdX = q[3], dVdY = q[0];
+    const double ae = a0[v] - q[0], ve = v0[v] - q[1];
+    const double di = 1.0 / (dAdX * dVdY - dAdY * dVdX);
+    const double dX = dVdY * di * ae + (-dAdY * di) * ve;
+    const double dY = (-dVdX * di) * ae + dAdX * di * ve;
+    const double s = dX * (-2.0 * H[v * ng + g]) + dY;
+    const long b = v * 3 * ng + g;
+    Xu[b] += s * nrm[b];
+    Xu[b + ng] += s * nrm[b + ng];
+    Xu[b + 2 * ng] += s * nrm[b + 2 * ng];
+  }
+}
+
 // ScaleFreq on the order-major layout (src/SHTrans.cc:192-223).
 __global__ void k_scale_om(const double *__restrict__ x, long nf, int nc,
                            const int2 *__restrict__ cinfo, const double *__restrict__ scale,
@@ -195,6 +350,11 @@ struct Stepper {
   bool prepared = false;
   int matvecs = 0;
   int mode_precision = 0;
+  // after the solve (reparametrization, area / volume correction)
+  DBuf repw, repwn, ru, rxc, ruc, r3, rdt, rE, a0, v0, av4, qwu;
+  std::vector<double> h_a0, h_v0;
+  bool have_av0 = false;
+  int rep_iters = 0, av_iters = 0;
 
   ~Stepper() {
     if (cinfo) (void)hipFree(cinfo);
@@ -221,6 +381,7 @@ struct Stepper {
     V3D_CHECK(xsol.alloc(n));
     V3D_CHECK(yres.alloc(n));
     V3D_CHECK(tmpD.alloc(n));
+    V3D_CHECK(post_init());
     V3D_HIP(hipMemset(U.p, 0, sizeof(double) * U.n));
     V3D_HIP(hipMemset(sig.p, 0, sizeof(double) * sig.n));
     std::vector<int2> ci(nc);
@@ -342,6 +503,142 @@ struct Stepper {
     return s->lpc->apply_inplace(y, s->nv, s->prm.p, s->cinfo, s->st);
   }
 
+  // ---- reparametrization and area / volume correction ---------------------
+  // Defaults of the reference's Parameters (src/Parameters.cc:39-45):
+  // PolyKReparam with exponent 4, rep_ts = 1e-3, rep_tol = 1e-4 (FP64),
+  // rep_maxit = 10, no upsampling; the area / volume tolerance 1e-12
+  // (include/EvolveSurface.h:105).
+  static constexpr int kRepExp = 4, kRepMaxit = 10;
+  static constexpr double kRepTs = 1e-3, kRepTol = 1e-4, kAvTol = 1e-12;
+
+  int post_init() {
+    const int p = prm.p, pu = prm.p_up;
+    const long mu = sh_ngrid(pu);
+    // A[n] (src/InterfacialVelocity.cc:1553-1566) per coefficient
+    std::vector<double> w(nc), wn(nc);
+    const long ff = kRepExp ? p / 2 : p / 3;
+    for (int j = 0; j < 2 * p; ++j) {
+      const int m = (j + 1) / 2, off = sh_blockoff(p, j), len = sh_blocklen(p, j);
+      for (int k = 0; k < len; ++k) {
+        const int deg = m + k;
+        double a = 1.0 - (kRepExp ? std::pow(deg * 1.0 / ff, kRepExp) : 0.0);
+        a *= deg > ff ? 0.0 : 1.0;
+        w[off + k] = 1.0 - a;
+        wn[off + k] = -(1.0 - a);
+      }
+    }
+    std::vector<double> q;
+    legendre_weights(pu, q);
+    V3D_CHECK(repw.alloc(nc));
+    V3D_CHECK(repwn.alloc(nc));
+    V3D_CHECK(qwu.alloc(pu + 1));
+    V3D_HIP(hipMemcpy(repw.p, w.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
+    V3D_HIP(hipMemcpy(repwn.p, wn.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
+    V3D_HIP(hipMemcpy(qwu.p, q.data(), sizeof(double) * (pu + 1), hipMemcpyHostToDevice));
+    V3D_CHECK(ru.alloc(3 * nv * ng));
+    for (DBuf *b : {&rxc, &ruc}) V3D_CHECK(b->alloc(3 * nv * nc));
+    V3D_CHECK(r3.alloc(3 * nv));
+    V3D_CHECK(rdt.alloc(nv));
+    V3D_CHECK(rE.alloc(2 * nv));
+    V3D_CHECK(a0.alloc(nv));
+    V3D_CHECK(v0.alloc(nv));
+    V3D_CHECK(av4.alloc(4 * nv));
+    (void)mu;
+    return VES3D_SUCCESS;
+  }
+
+  // area / volume of every vesicle on the upsampled surface into av4 (and host)
+  int av_measure(std::vector<double> &h4) {
+    const long mu = sh_ngrid(prm.p_up);
+    V3D_CHECK(forces.resample(X, 3 * nv, prm.p, prm.p_up, forces.xu, st));
+    V3D_CHECK(forces.Su.update(forces.xu, st));
+    k_av_integrals<<<(unsigned)nv, 256, 0, st>>>(forces.xu, forces.Su.nrm, forces.Su.W,
+                                                 forces.Su.H, qwu, mu, 2 * prm.p_up, av4);
+    V3D_CHECK(launch_status());
+    h4.resize(4 * nv);
+    V3D_HIP(hipMemcpyAsync(h4.data(), av4.p, sizeof(double) * 4 * nv, hipMemcpyDeviceToHost, st));
+    return hip_status(hipStreamSynchronize(st));
+  }
+
+  // the reference area / volume (EvolveSurface::Evolve, src/EvolveSurface.cc:119-126)
+  int record_area_volume() {
+    std::vector<double> h4;
+    V3D_CHECK(av_measure(h4));
+    h_a0.resize(nv);
+    h_v0.resize(nv);
+    for (long v = 0; v < nv; ++v) {
+      h_a0[v] = h4[4 * v];
+      h_v0[v] = h4[4 * v + 1];
+    }
+    V3D_HIP(hipMemcpy(a0.p, h_a0.data(), sizeof(double) * nv, hipMemcpyHostToDevice));
+    V3D_HIP(hipMemcpy(v0.p, h_v0.data(), sizeof(double) * nv, hipMemcpyHostToDevice));
+    have_av0 = true;
+    return VES3D_SUCCESS;
+  }
+
+  // InterfacialVelocity::reparam, no upsampling (src/InterfacialVelocity.cc:1639-1773)
+  int reparam() {
+    SurfaceG &S = forces.S;
+    int it = 0;
+    int have_E = 0;
+    std::vector<double> hdt(nv);
+    while (it < kRepMaxit) {
+      V3D_CHECK(S.update(X, st));
+      V3D_CHECK(S.forward(X, 3 * nv, rxc, st));
+      V3D_HIP(hipMemcpyAsync(ruc.p, rxc.p, sizeof(double) * 3 * nv * nc, hipMemcpyDeviceToDevice, st));
+      k_scale_fm<<<gb(3 * nv * nc), 256, 0, st>>>(ruc, 3 * nv, (int)nc, repwn);
+      V3D_CHECK(launch_status());
+      V3D_CHECK(S.backward(ruc, 3 * nv, 0, 0, ru, st));            // sht_filter
+      for (int r = 0; r < 2; ++r) {                                 // tangent + band-limit, twice
+        k_tangent<<<gb(nv * ng), 256, 0, st>>>(ru, S.nrm, nv, ng);
+        V3D_CHECK(launch_status());
+        V3D_CHECK(S.forward(ru, 3 * nv, ruc, st));
+        V3D_CHECK(S.backward(ruc, 3 * nv, 0, 0, ru, st));
+      }
+      k_rep_norm_inner<<<(unsigned)nv, 256, 0, st>>>(ru, ng, rxc, ruc, (int)nc, repw, r3);
+      V3D_CHECK(launch_status());
+      k_rep_step<<<gb(nv), 256, 0, st>>>(r3, nv, kRepTs, kRepTol, have_E, rE, rdt);
+      V3D_CHECK(launch_status());
+      V3D_HIP(hipMemcpyAsync(hdt.data(), rdt.p, sizeof(double) * nv, hipMemcpyDeviceToHost, st));
+      V3D_HIP(hipStreamSynchronize(st));
+      double dt_max = 0;
+      for (double d : hdt) dt_max = std::max(dt_max, d);
+      if (dt_max == 0) break;
+      V3D_HIP(hipMemcpyAsync(rE.p, rE.p + nv, sizeof(double) * nv, hipMemcpyDeviceToDevice, st));
+      have_E = 1;
+      k_rep_update<<<gb(3 * nv * ng), 256, 0, st>>>(X, ru, rdt, nv, 3 * ng);
+      V3D_CHECK(launch_status());
+      ++it;
+    }
+    rep_iters = it;
+    return hip_status(hipStreamSynchronize(st));
+  }
+
+  // EvolveSurface::AreaVolumeCorrection (src/EvolveSurface.cc:352-470)
+  int area_volume_correction() {
+    if (!have_av0) V3D_CHECK(record_area_volume());
+    const long mu = sh_ngrid(prm.p_up);
+    std::vector<double> h4;
+    int it = -1;
+    while (++it < kRepMaxit) {
+      V3D_CHECK(av_measure(h4));
+      double amax = 0, vmax = 0, aerr = 0, verr = 0;
+      for (long v = 0; v < nv; ++v) {
+        amax = std::max(amax, std::fabs(h_a0[v]));
+        vmax = std::max(vmax, std::fabs(h_v0[v]));
+        aerr = std::max(aerr, std::fabs(h_a0[v] - h4[4 * v]));
+        verr = std::max(verr, std::fabs(h_v0[v] - h4[4 * v + 1]));
+      }
+      if (std::max(aerr / amax, verr / vmax) < kAvTol) break;
+      k_av_update<<<gb(nv * mu), 256, 0, st>>>(forces.xu, forces.Su.nrm, forces.Su.H, av4, a0, v0,
+                                               nv, mu);
+      V3D_CHECK(launch_status());
+      V3D_CHECK(forces.resample(forces.xu, 3 * nv, prm.p_up, prm.p, X, st));
+    }
+    av_iters = it;
+    return hip_status(hipStreamSynchronize(st));
+  }
+
   int step(ves3d_step_stats *stats) {
     using clk = std::chrono::steady_clock;
     auto t0 = clk::now();
@@ -380,6 +677,12 @@ struct Stepper {
     V3D_CHECK(unpack(xsol, U, sig));
     V3D_CHECK(dev_axpby(3 * nv * ng, prm.dt, U, 1.0, X, st));  // dx = dt u
     V3D_HIP(hipStreamSynchronize(st));
+    auto t3b = clk::now();
+    rep_iters = av_iters = 0;
+    if (!prm.post_off) {  // EvolveSurface::Evolve after the update (src/EvolveSurface.cc:330-335)
+      V3D_CHECK(reparam());
+      V3D_CHECK(area_volume_correction());
+    }
     auto t4 = clk::now();
     double far_ms = 0;
     int far_calls = 0;
@@ -403,6 +706,9 @@ struct Stepper {
       stats->far_calls = far_calls;
       stats->near_pairs = stokes.use_near ? (int)stokes.near.npairs : 0;
       stats->r_near = stokes.use_near ? stokes.near.r_near : 0.0;
+      stats->reparam_iters = rep_iters;
+      stats->av_iters = av_iters;
+      stats->ms_post = ms(t3b, t4);
     }
     return VES3D_SUCCESS;
   }
@@ -453,6 +759,7 @@ int ves3d_hip_stepper_set_state(void *h, const double *X, const double *U, const
   if (U) V3D_HIP(hipMemcpy(s->U.p, U, sizeof(double) * s->U.n, hipMemcpyDeviceToDevice));
   if (sigma) V3D_HIP(hipMemcpy(s->sig.p, sigma, sizeof(double) * s->sig.n, hipMemcpyDeviceToDevice));
   s->prepared = false;
+  if (X) V3D_CHECK(s->record_area_volume());  // the targets of the area / volume correction
   return VES3D_SUCCESS;
 }
 
@@ -496,6 +803,35 @@ int ves3d_hip_stepper_precond(void *h, const double *x, double *y) {
 int ves3d_hip_stepper_step(void *h, ves3d_step_stats *stats) {
   if (!h) return VES3D_INVALID_PARAMETER;
   return ((Stepper *)h)->step(stats);
+}
+
+int ves3d_hip_stepper_reparam(void *h, int *iters_host) {
+  if (!h) return VES3D_INVALID_PARAMETER;
+  Stepper *s = (Stepper *)h;
+  V3D_CHECK(s->reparam());
+  s->prepared = false;
+  if (iters_host) *iters_host = s->rep_iters;
+  return VES3D_SUCCESS;
+}
+
+int ves3d_hip_stepper_area_volume(void *h, int *iters_host) {
+  if (!h) return VES3D_INVALID_PARAMETER;
+  Stepper *s = (Stepper *)h;
+  V3D_CHECK(s->area_volume_correction());
+  s->prepared = false;
+  if (iters_host) *iters_host = s->av_iters;
+  return VES3D_SUCCESS;
+}
+
+int ves3d_hip_stepper_set_area_volume(void *h, const double *area_host, const double *vol_host) {
+  if (!h || !area_host || !vol_host) return VES3D_INVALID_PARAMETER;
+  Stepper *s = (Stepper *)h;
+  s->h_a0.assign(area_host, area_host + s->nv);
+  s->h_v0.assign(vol_host, vol_host + s->nv);
+  V3D_HIP(hipMemcpy(s->a0.p, area_host, sizeof(double) * s->nv, hipMemcpyHostToDevice));
+  V3D_HIP(hipMemcpy(s->v0.p, vol_host, sizeof(double) * s->nv, hipMemcpyHostToDevice));
+  s->have_av0 = true;
+  return VES3D_SUCCESS;
 }
 
 int ves3d_hip_stepper_attach_pc(void *h, void *pc, int collect) {

@@ -57,6 +57,16 @@ __device__ __forceinline__ void slice_range(long slice, int nsplit, long n_src,
   }
 }
 
+// 1/sqrt(x) for x > 0: the hardware estimate (v_rsq_f64, about 2^-20
+// relative) refined by one third-order step y (1 + e/2 + 3e^2/8),
+// e = 1 - x y^2, which leaves a rounding-level error.  x = 0 gives NaN here
+// (callers select it away); no special-value fix-ups in the pair loop.
+__device__ __forceinline__ double rsqrt_pair(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(y * -x, y, 1.0);
+  return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
 template <bool SL, bool DL>
 __global__ __launch_bounds__(FF_BLOCK) void k_far_f64(
     const double *__restrict__ spos, const double *__restrict__ sld,
@@ -124,7 +134,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_f64(
           const double dx = tx - sh[j].x, dy = ty - sh[j].y, dz = tz - sh[j].z;
           const double r2 = dx * dx + dy * dy + dz * dz;
           // unconditional rsqrt + select: no per-pair branch in the loop
-          const double rq = rsqrt(r2);
+          const double rq = rsqrt_pair(r2);
           const double ri = (r2 > eps2) ? rq : 0.0;
           const double ri2 = ri * ri;
           if (SL) {

@@ -2,6 +2,8 @@
 BiCGStab (ves3d_hip_bicgstab)."""
 import ctypes
 
+import numpy as np
+
 import torch
 
 from . import call, load, ptr, stream_ptr
@@ -14,7 +16,7 @@ class StepParams(ctypes.Structure):
                 ("tol", ctypes.c_double), ("max_iter", ctypes.c_int),
                 ("precision", ctypes.c_int), ("precond", ctypes.c_int),
                 ("near_off", ctypes.c_int), ("repul_dist", ctypes.c_double),
-                ("r_near", ctypes.c_double)]
+                ("r_near", ctypes.c_double), ("post_off", ctypes.c_int)]
 
 
 class StepStats(ctypes.Structure):
@@ -24,7 +26,8 @@ class StepStats(ctypes.Structure):
                 ("ms_rhs", ctypes.c_double), ("ms_solve", ctypes.c_double),
                 ("ms_total", ctypes.c_double), ("ms_far", ctypes.c_double),
                 ("far_calls", ctypes.c_int), ("near_pairs", ctypes.c_int),
-                ("r_near", ctypes.c_double)]
+                ("r_near", ctypes.c_double), ("reparam_iters", ctypes.c_int),
+                ("av_iters", ctypes.c_int), ("ms_post", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -39,9 +42,10 @@ class Stepper:
 
     def __init__(self, nves, p, dt=1e-3, shear_rate=0.1, bending_modulus=1e-2, tol=1e-6,
                  max_iter=200, precision=0, precond=1, p_up=0, filter_freq=0, near=True,
-                 repul_dist=0.0, r_near=0.0):
+                 repul_dist=0.0, r_near=0.0, post=True):
         self.params = StepParams(nves, p, p_up, filter_freq, dt, shear_rate, bending_modulus, tol,
-                                 max_iter, precision, precond, 0 if near else 1, repul_dist, r_near)
+                                 max_iter, precision, precond, 0 if near else 1, repul_dist, r_near,
+                                 0 if post else 1)
         self.nves, self.p = nves, p
         self.ng = 2 * p * (p + 1)
         h = ctypes.c_void_p()
@@ -109,6 +113,27 @@ class Stepper:
         torch.cuda.synchronize()
         call("ves3d_hip_stepper_step", self.h, ctypes.byref(st))
         return st.as_dict()
+
+    def reparam(self):
+        """Reparametrization of the current positions; returns the iterations."""
+        it = ctypes.c_int()
+        torch.cuda.synchronize()
+        call("ves3d_hip_stepper_reparam", self.h, ctypes.byref(it))
+        return it.value
+
+    def area_volume(self):
+        """Area / volume correction of the current positions; returns the
+        iterations."""
+        it = ctypes.c_int()
+        torch.cuda.synchronize()
+        call("ves3d_hip_stepper_area_volume", self.h, ctypes.byref(it))
+        return it.value
+
+    def set_area_volume(self, area, vol):
+        a = np.ascontiguousarray(area, dtype=np.float64)
+        v = np.ascontiguousarray(vol, dtype=np.float64)
+        call("ves3d_hip_stepper_set_area_volume", self.h, a.ctypes.data_as(ctypes.c_void_p),
+             v.ctypes.data_as(ctypes.c_void_p))
 
 
 class LearnedPC:
