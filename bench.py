@@ -12,9 +12,13 @@ Workloads
         fraction, shear rate 1.0, dt = 1e-3, BiCGStab to relative residual
         1e-6.  Two forms run back to back from the same state:
           plain  : FP64 everywhere, diagonal spectral preconditioner;
-          learned: FP32 singular operator + FP32-compensated far field and
-                   the learned SH-space preconditioner, trained beforehand on
-                   (A e, e) pairs recorded during plain solves (untimed).
+          learned: FP32-compensated far field (and, with --mixed-precision 1,
+                   the FP32 singular operator) and the learned SH-space
+                   preconditioner, trained beforehand (untimed) on
+                   (A_loc e, e) pairs recorded during plain solves of separate
+                   256-vesicle training suspensions at 10-25% volume fraction.
+        The velocity error of both forms' first step is measured against an
+        FP64 solve of the same system converged to relative residual 1e-10.
         `value` is the learned + mixed form's ms per step; the plain form's is
         `ms_per_step_plain`.
   far : the far-field direct Stokes SL+DL sum alone, 512 unit spheres at
@@ -60,55 +64,69 @@ def bench_step(args):
     common = dict(dt=1e-3, shear_rate=1.0, bending_modulus=1e-2, tol=1e-6, max_iter=args.max_iter,
                   r_near=args.r_near)
 
-    # ---- plain FP64 form
-    plain = I.Stepper(nv, p, precision=0, precond=1, **common)
+    # ---- train the learned preconditioner (untimed) on pairs that plain
+    # solves of separate training suspensions generate: args.train_nves
+    # vesicles of the same shape family at volume fractions 10% .. 25%
     nin = 4 * p * (p + 2)
     pc = I.LearnedPC(nin, args.width, capacity=args.capacity, seed=7)
-    I.attach(plain, pc, collect=True)       # records training pairs while it runs
-    plain.set_state(Xg)
-    stats_plain = []
-    for k in range(args.warmup + args.steps):
-        s = plain.step()
-        _log("plain step %d: %.1f ms, %d iters, relres %.2e true %.2e" %
-             (k, s["ms_total"], s["iterations"], s["relres"], s["true_relres"]))
-        if k >= args.warmup:
-            stats_plain.append(s)
-    _, U_plain1, _ = plain.get_state()
-
-    # ---- train the preconditioner on the recorded pairs (untimed)
+    t0 = time.perf_counter()
+    fracs = np.linspace(0.10, 0.25, max(1, args.train_steps))
+    trainer = I.Stepper(args.train_nves, p, precision=0, precond=1, **common)
+    I.attach(trainer, pc, collect=True)
+    for k, vf in enumerate(fracs):
+        Xt = W.implicit_step_case(nv=args.train_nves, p=p, volfrac=float(vf), seed=1000 + k)
+        trainer.set_state(torch.from_numpy(Xt).cuda())
+        s = trainer.step()
+        _log("training suspension %d (%d ves, %.0f%%): %d iters" %
+             (k, args.train_nves, 100 * vf, s["iterations"]))
+    trainer.close()
+    collect_s = time.perf_counter() - t0
     t0 = time.perf_counter()
     loss = pc.train(args.epochs, 512, args.lr)
     train_s = time.perf_counter() - t0
     n_samples = pc.samples()
     _log("trained on %d samples: loss %.3e in %.1f s" % (n_samples, loss, train_s))
-    I.attach(plain, None, collect=False)
+
+    # ---- plain FP64 form
+    plain = I.Stepper(nv, p, precision=0, precond=1, **common)
+    plain.set_state(Xg)
+    stats_plain = []
+    U_plain1 = None
+    for k in range(args.warmup + args.steps):
+        s = plain.step()
+        _log("plain step %d: %.1f ms, %d iters, relres %.2e true %.2e" %
+             (k, s["ms_total"], s["iterations"], s["relres"], s["true_relres"]))
+        if k == 0:
+            U_plain1 = plain.get_state()[1].clone()
+        if k >= args.warmup:
+            stats_plain.append(s)
 
     # ---- learned + mixed form from the same initial state
-    mixed = I.Stepper(nv, p, precision=1, precond=2, **common)
+    mixed = I.Stepper(nv, p, precision=args.mixed_precision, precond=2, **common)
     I.attach(mixed, pc, collect=False)
     mixed.set_state(Xg)
     stats_mixed = []
+    U_mixed1 = None
     for k in range(args.warmup + args.steps):
         s = mixed.step()
         _log("learned+mixed step %d: %.1f ms, %d iters, relres %.2e true %.2e fell_back %d" %
              (k, s["ms_total"], s["iterations"], s["relres"], s["true_relres"], s["fell_back"]))
+        if k == 0:
+            U_mixed1 = mixed.get_state()[1].clone()
         if k >= args.warmup:
             stats_mixed.append(s)
 
-    # velocity error of the learned + mixed form against the FP64 step, both
-    # taken from the identical initial state (first step)
-    check = I.Stepper(nv, p, precision=1, precond=2, **common)
-    I.attach(check, pc, collect=False)
-    check.set_state(Xg)
-    ref = I.Stepper(nv, p, precision=0, precond=1, **common)
+    # velocity error of both forms' first step against the FP64 solve of the
+    # same system converged to relative residual args.ref_tol (untimed)
+    ref = I.Stepper(nv, p, precision=0, precond=1, **dict(common, tol=args.ref_tol))
     ref.set_state(Xg)
-    ref.step()
-    check.step()
+    sr = ref.step()
+    _log("FP64 reference solve: %d iters, true relres %.2e" % (sr["iterations"], sr["true_relres"]))
     _, Ur, _ = ref.get_state()
-    _, Uc, _ = check.get_state()
-    vel_err = float(torch.linalg.norm(Uc - Ur) / torch.linalg.norm(Ur))
+    nr = torch.linalg.norm(Ur)
+    vel_err = float(torch.linalg.norm(U_mixed1 - Ur) / nr)
+    vel_err_plain = float(torch.linalg.norm(U_plain1 - Ur) / nr)
     ref.close()
-    check.close()
 
     ms_mixed = float(np.mean([s["ms_total"] for s in stats_mixed]))
     ms_plain = float(np.mean([s["ms_total"] for s in stats_plain]))
@@ -150,7 +168,11 @@ def bench_step(args):
         "true_relres_learned": [s["true_relres"] for s in stats_mixed],
         "true_relres_plain": [s["true_relres"] for s in stats_plain],
         "fell_back": [s["fell_back"] for s in stats_mixed],
-        "velocity_rel_err_vs_fp64": vel_err, "pc_train_seconds": train_s,
+        "velocity_rel_err_vs_fp64": vel_err, "velocity_rel_err_plain_vs_fp64": vel_err_plain,
+        "fp64_reference_relres": sr["true_relres"], "pc_train_seconds": train_s,
+        "pc_collect_seconds": collect_s,
+        "pc_train_suspensions": "%d x %d ves at %s%% volume fraction" % (
+            len(fracs), args.train_nves, "/".join("%.0f" % (100 * f) for f in fracs)),
         "near_pairs": [s["near_pairs"] for s in stats_plain], "r_near": stats_plain[-1]["r_near"],
         "pc_train_samples": int(n_samples), "pc_train_loss": loss,
         "stage_ms_plain": {k: float(np.mean([s[k] for s in stats_plain]))
@@ -310,15 +332,21 @@ def main():
     ap.add_argument("--max-iter", type=int, default=300)
     ap.add_argument("--width", type=int, default=256)
     ap.add_argument("--capacity", type=int, default=200000)
-    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--train-nves", type=int, default=256)
+    ap.add_argument("--train-steps", type=int, default=2)
+    ap.add_argument("--ref-tol", type=float, default=1e-10)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mixed-precision", type=int, default=2, choices=[1, 2],
+                    help="precision of the learned form: 1 FP32 singular operator + FP32-"
+                         "compensated far field, 2 FP64 singular operator + FP32-compensated far field")
     ap.add_argument("--r-near", type=float, default=0.1,
                     help="near-singular zone radius (0: the reference's r_ves 2/sqrt(p_up))")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
-              "(the path shards by vesicle, see DESIGN.md)" % args.gpus, file=sys.stderr)
+        print("bench.py: only the one-GPU form is built; --gpus %d is not supported "
+              "(across GPUs the path would shard by vesicle)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     res = bench_step(args) if args.workload == "step" else bench_far(args)
     print(json.dumps(res))

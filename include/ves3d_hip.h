@@ -329,7 +329,8 @@ typedef struct ves3d_step_params {
   double bending_modulus; /* kappa_b, the same for every vesicle */
   double tol;             /* BiCGStab relative residual tolerance */
   int max_iter;           /* BiCGStab iteration limit */
-  int precision;          /* 0: FP64; 1: FP32 singular operator + FP32-compensated far field */
+  int precision;          /* 0: FP64; 1: FP32 singular operator + FP32-compensated far field;
+                             2: FP64 singular operator + FP32-compensated far field */
   int precond;            /* 0: none; 1: diagonal spectral (ConfigurePrecond); 2: learned */
   int near_off;           /* 1 disables the near-singular correction and the repulsion force */
   double repul_dist;      /* repulsion length (0: the reference default 5e-2; < 0: none) */
@@ -377,6 +378,19 @@ int ves3d_hip_stepper_matvec(void *handle, const double *x, double *y);
 /* ImplicitPrecond with the diagonal spectral preconditioner
  * (src/InterfacialVelocity.cc:1054-1109): y = D x. */
 int ves3d_hip_stepper_precond(void *handle, const double *x, double *y);
+/* No counterpart in the reference (it feeds the learned preconditioner):
+ * ImplicitApply with every vesicle's own operator only, the singular
+ * self-interaction without the far field and the near-singular correction:
+ * y = A_loc x. */
+int ves3d_hip_stepper_matvec_local(void *handle, const double *x, double *y);
+/* The attached, trained learned preconditioner (see ves3d_hip_lpc_create):
+ * y = P(x); the stepper must be prepared. */
+int ves3d_hip_stepper_precond_learned(void *handle, const double *x, double *y);
+/* The vesicles' canonical frames of the last prepare into frames [nv][3][3]
+ * (device, row-major): the orthogonal polar factor of
+ * C = sum_k w_k X(k) s(k)^T over the grid, s the parameter sphere
+ * (sin t cos p, sin t sin p, cos t) and w the Gauss-Legendre weights. */
+int ves3d_hip_stepper_frames(void *handle, double *frames);
 /* One full implicit step: prepare, right-hand side, BiCGStab from the
  * previous (velocity, tension) as initial guess, true-residual check,
  * X += dt U, then (unless post_off) InterfacialVelocity::reparam and
@@ -474,11 +488,21 @@ int ves3d_hip_near_apply(void *handle, const double *q, const double *vup, const
  * No counterpart in the reference: a per-vesicle network that fills the
  * preconditioner slot of ImplicitPrecond (src/InterfacialVelocity.cc:1054-1109).
  * For the Galerkin vector r, with D the diagonal spectral preconditioner and
- * z_v = (D r)_v the 4 p(p+2) coefficients of vesicle v, s_v = |z_v|:
- *     P(r)_v = z_v + s_v g(z_v / s_v),
+ * z_v = (D r)_v the 4 p(p+2) coefficients of vesicle v, s_v = |z_v|, and
+ * F_v its canonical frame (ves3d_hip_stepper_frames) acting on the three
+ * velocity blocks:
+ *     P(r)_v = z_v + s_v K F_v g(F_v^T z_v / s_v),
+ * K zeroes the tension coefficients of degree > filter_freq: Surface::div
+ * is low-pass filtered at that degree (src/Surface.cc:372), so they span the
+ * null space of the implicit operator, and keeping the correction out of them
+ * makes the solve reach the velocity of the diagonal-preconditioned one.
  * g: nin -> width -> width -> nin with tanh-GELU, FP32 weights, parameters
  * laid out [W1 (width x nin), b1, W2 (width x width), b2, W3 (nin x width), b3].
- * Trained online on (A e, e) pairs recorded during solves. */
+ * Rotating a shape rotates its own operator's velocity blocks by the same
+ * matrix, so one network serves vesicles in every orientation.  Trained
+ * online on (A_loc e, e) pairs recorded during solves: e the solver's own
+ * Krylov vectors, A_loc the vesicles' own operators
+ * (ves3d_hip_stepper_matvec_local), mapped into the canonical frames. */
 int ves3d_hip_lpc_create(int nin, int width, size_t capacity, unsigned long long seed,
                          void **handle_host);
 int ves3d_hip_lpc_destroy(void *handle);

@@ -1,0 +1,118 @@
+"""GPU parity of the learned preconditioner's solver interface against the
+oracle (oracle/learned.py), and what the trained network gains.
+
+Tolerances: canonical frames 1e-12 absolute; local operator 1e-9 relative
+(the FP64 matvec's own bar); P(r) 1e-5 relative to max|P(r)| (the network
+runs in FP32).  Gain: after training on solver-generated pairs from one
+suspension, the learned solve of another suspension takes at most 3/4 of
+the diagonal-preconditioned iterations, reaches true relative residual
+1e-6, and its velocity is within 1e-5 of a tightly converged FP64 solve.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import learned as L
+from oracle import ves
+from ves3d_amd import implicit as I
+from ves3d_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def cu(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _ellipsoids(nv, p, vf, seed):
+    return W.implicit_step_case(nv, p, vf, seed=seed)
+
+
+def test_frames_and_local_matvec(hip):
+    nv, p = 4, 8
+    X = _ellipsoids(nv, p, 0.1, seed=31)
+    st = I.Stepper(nv, p, precision=0, precond=1)
+    st.set_state(cu(X))
+    st.prepare()
+    print("frames / local matvec nv=%d p=%d" % (nv, p), flush=True)
+    F = st.frames().cpu().numpy()
+    assert np.max(np.abs(F - L.frames(X, p))) < 1e-12
+    orc = ves.Implicit(X, p, dt=1e-3, shear=1.0, near=False)
+    x = np.random.default_rng(2).standard_normal(st.n)
+    xg = cu(x)
+    y = st.matvec_local(xg).cpu().numpy()
+    assert rel(y, orc.matvec(x, local=True)) < 1e-9
+    st.close()
+
+
+def test_precond_learned_vs_oracle(hip):
+    nv, p = 3, 6
+    X = _ellipsoids(nv, p, 0.1, seed=32)
+    nin = 4 * p * (p + 2)
+    pc = I.LearnedPC(nin, 64, capacity=0, seed=5)
+    rng = np.random.default_rng(6)
+    params = [rng.standard_normal((64, nin)) * 0.1, rng.standard_normal(64) * 0.1,
+              rng.standard_normal((64, 64)) * 0.1, rng.standard_normal(64) * 0.1,
+              rng.standard_normal((nin, 64)) * 0.1, rng.standard_normal(nin) * 0.1]
+    flat = np.concatenate([q.reshape(-1) for q in params]).astype(np.float32)
+    pc.set_params(torch.from_numpy(flat).cuda())
+    p32 = [q.astype(np.float32).astype(np.float64) for q in params]
+    st = I.Stepper(nv, p, dt=1e-3, shear_rate=1.0, precision=0, precond=2)
+    I.attach(st, pc, collect=False)
+    st.set_state(cu(X))
+    st.prepare()
+    print("learned precond nv=%d p=%d" % (nv, p), flush=True)
+    orc = ves.Implicit(X, p, dt=1e-3, shear=1.0, near=False)
+    x = rng.standard_normal(st.n)
+    xg = cu(x)
+    y = st.precond_learned(xg).cpu().numpy()
+    want = L.precond(orc, p32, x, L.frames(X, p))
+    assert rel(y, want) < 1e-5
+    # the gauge: tension coefficients above the divergence filter keep D r
+    keep = np.tile(L.tension_gauge(p, 2 * p // 3), (nv, 1))
+    d = L.rows(orc, orc.precond(x))
+    assert np.max(np.abs(L.rows(orc, y)[~keep] - d[~keep])) < 1e-13 * np.abs(d).max()
+    st.close()
+    pc.close()
+
+
+def test_trained_network_gain(hip):
+    """Trained on pairs from one suspension, evaluated on another."""
+    nv, p = 8, 8
+    common = dict(dt=1e-3, shear_rate=1.0, tol=1e-6, max_iter=300, post=False)
+    nin = 4 * p * (p + 2)
+    pc = I.LearnedPC(nin, 256, capacity=8192, seed=7)
+    train = I.Stepper(nv, p, precision=0, precond=1, **common)
+    I.attach(train, pc, collect=True)
+    for seed in (41, 42, 43):
+        train.set_state(cu(_ellipsoids(nv, p, 0.2, seed)))
+        train.step()
+    print("training on %d samples" % pc.samples(), flush=True)
+    loss = pc.train(300, 256, 1e-3)
+    print("trained: loss %.3e" % loss, flush=True)
+    train.close()
+    X = _ellipsoids(nv, p, 0.2, seed=44)
+    its = {}
+    U = {}
+    for name, kind, tol in (("ref", 1, 1e-11), ("diag", 1, 1e-6), ("learned", 2, 1e-6)):
+        st = I.Stepper(nv, p, precision=0, precond=kind, **dict(common, tol=tol))
+        if kind == 2:
+            I.attach(st, pc, collect=False)
+        st.set_state(cu(X))
+        s = st.step()
+        print(name, s["iterations"], s["true_relres"], s["fell_back"], flush=True)
+        assert s["status"] == 100 and s["true_relres"] < tol * 1.05
+        its[name] = s["iterations"]
+        U[name] = st.get_state()[1].cpu().numpy()
+        if kind == 2:
+            assert s["fell_back"] == 0
+        st.close()
+    assert its["learned"] <= 0.75 * its["diag"], its
+    for name in ("diag", "learned"):
+        e = np.linalg.norm(U[name] - U["ref"]) / np.linalg.norm(U["ref"])
+        assert e < 1e-5, (name, e)
+    pc.close()

@@ -120,7 +120,8 @@ def test_singular_build_f32(hip):
 @pytest.mark.parametrize("p0", [12, 16])
 def test_singular_closed_forms_gpu(hip, p0):
     """Size-independent property at the workload's order: on unit spheres
-    SL[c] = (2/3) c, DL[c] = c/2, SL[n] = 0 (p1 = 2 p0)."""
+    SL[c] = (2/3) c, DL[c] = -c/2 (the grid's outward normal, the
+    reference orientation), SL[n] = 0 (p1 = 2 p0)."""
     nv = 2
     Xs, _ = W.sphere_grid(p0)
     X = np.stack([Xs, Xs * 1.0])
@@ -140,7 +141,7 @@ def test_singular_closed_forms_gpu(hip, p0):
         Vd = O.shc2grid((F @ DLv).reshape(3, nc), p0, p0)
         Vn = O.shc2grid((Fn @ SLv).reshape(3, nc), p0, p0)
         assert np.max(np.abs(V - 2.0 / 3.0 * f)) < 1e-9
-        assert np.max(np.abs(Vd - 0.5 * f)) < 1e-9
+        assert np.max(np.abs(Vd + 0.5 * f)) < 1e-9
         assert np.max(np.abs(Vn)) < 1e-9
 
 

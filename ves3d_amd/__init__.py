@@ -99,6 +99,9 @@ _sig("ves3d_hip_stepper_prepare", P)
 _sig("ves3d_hip_stepper_rhs", P, P)
 _sig("ves3d_hip_stepper_matvec", P, P, P)
 _sig("ves3d_hip_stepper_precond", P, P, P)
+_sig("ves3d_hip_stepper_matvec_local", P, P, P)
+_sig("ves3d_hip_stepper_precond_learned", P, P, P)
+_sig("ves3d_hip_stepper_frames", P, P)
 _sig("ves3d_hip_stepper_step", P, P)
 _sig("ves3d_hip_stepper_attach_pc", P, P, c_int)
 _sig("ves3d_hip_stepper_reparam", P, P)

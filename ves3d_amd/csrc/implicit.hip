@@ -429,7 +429,7 @@ int StokesG::init(int p_, int pu_, long nv_, int precision_, int near_, double r
   use_near = near_ != 0;
   const long nc = sh_ncoef(p), ng = sh_ngrid(p), mu = sh_ngrid(pu);
   const long nmat = nv * 9 * nc * nc;
-  if (precision == 0) {
+  if (precision != 1) {
     V3D_CHECK(SL.alloc(nmat));
   } else {
     if (hipMalloc(&SL32, sizeof(float) * nmat) != hipSuccess) return VES3D_MEMORY_ERROR;
@@ -454,7 +454,7 @@ int StokesG::init(int p_, int pu_, long nv_, int precision_, int near_, double r
 
 int StokesG::setup(const double *X, hipStream_t st) {
   const long ng = sh_ngrid(p);
-  if (precision == 0)
+  if (precision != 1)
     V3D_CHECK(singular_build<double>(X, nv, p, pu, SL.p, (double *)nullptr, st));
   else
     V3D_CHECK(singular_build<float>(X, nv, p, pu, SL32, (float *)nullptr, st));
@@ -476,7 +476,7 @@ int StokesG::setup(const double *X, hipStream_t st) {
   return hip_status(hipStreamSynchronize(st));  // dqw is freed on return
 }
 
-int StokesG::apply(const double *f, double *vel, hipStream_t st, bool add_repul) {
+int StokesG::apply(const double *f, double *vel, hipStream_t st, bool add_repul, bool local) {
   const long ng = sh_ngrid(p), mu = sh_ngrid(pu);
   const bool near_on = use_near && near.npairs > 0;
   if (add_repul && near_on && near.repul > 0) {
@@ -485,11 +485,12 @@ int StokesG::apply(const double *f, double *vel, hipStream_t st, bool add_repul)
     f = frep;
   }
   V3D_CHECK(sht_grid2shc<double>(f, 3 * nv, p, p, F, work, st));
-  if (precision == 0)
+  if (precision != 1)
     V3D_CHECK(self_apply<double>(SL.p, F, nullptr, nullptr, nv, p, V, st));
   else
     V3D_CHECK(self_apply<float>(SL32, F, nullptr, nullptr, nv, p, V, st));
   V3D_CHECK(sht_shc2grid<double>(V, 3 * nv, p, p, vel, nullptr, nullptr, work, st));
+  if (local) return VES3D_SUCCESS;
   V3D_CHECK(sht_shc2grid<double>(F, 3 * nv, p, pu, up, nullptr, nullptr, work, st));
   k_pack_density<<<gblocks(nv * (mu + 2)), 256, 0, st>>>(up, aqw, nv, mu, q);
   V3D_CHECK(launch_status());

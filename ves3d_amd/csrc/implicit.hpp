@@ -63,7 +63,8 @@ class ForcesG {
 // StokesVelocity<Real>::operator() for surface targets, single layer, no
 // viscosity contrast: self (singular operator) + far field (direct sum over
 // the other vesicles' upsampled sources).  precision 0: FP64 everywhere;
-// 1: FP32 singular operator and FP32-compensated far field.
+// 1: FP32 singular operator and FP32-compensated far field; 2: FP64
+// singular operator and FP32-compensated far field.
 class StokesG {
  public:
   // near: 1 applies the near-singular correction (src/NearSingular.cc) with
@@ -73,8 +74,11 @@ class StokesG {
            double r_near = 0.0);
   int setup(const double *X, hipStream_t st);
   // add_repul: the density is f + NearSingular::ForceRepul (the right-hand
-  // side of the implicit step, src/InterfacialVelocity.cc:733).
-  int apply(const double *f, double *vel, hipStream_t st, bool add_repul = false);
+  // side of the implicit step, src/InterfacialVelocity.cc:733).  local: the
+  // self-interaction (singular operator) alone, without the far field and
+  // the near-singular correction.
+  int apply(const double *f, double *vel, hipStream_t st, bool add_repul = false,
+            bool local = false);
 
   // HIP events around every far-field launch (on the launch stream); the
   // elapsed times are summed by far_time() after the stream has drained.

@@ -134,39 +134,78 @@ __global__ __launch_bounds__(256) void k_row_norms(const double *__restrict__ y,
   __syncthreads();
   if (threadIdx.x == 0) scale[v] = sqrt(sh[0] + sh[1] + sh[2] + sh[3]) + 1e-300;
 }
+// Frames F_v (row-major 3x3 per vesicle, columns the vesicle's canonical
+// axes; nullptr: identity) rotate the three velocity blocks of a vesicle's
+// row into its canonical frame, z_d = sum_e F[e][d] x_e, and back,
+// x_e = sum_d F[e][d] z_d.  The tension block is a scalar and not rotated.
+__device__ __forceinline__ double to_canonical(const double *__restrict__ y, long v, int j, long nv,
+                                               int nc, const int2 *__restrict__ cinfo,
+                                               const double *__restrict__ frames) {
+  if (j >= 3 * nc || frames == nullptr) return y[row_index(v, j, nv, nc, cinfo)];
+  const int d = j / nc, c = j % nc;
+  const double *F = frames + 9 * v;
+  double z = 0;
+#pragma unroll
+  for (int e = 0; e < 3; ++e) z += F[3 * e + d] * y[row_index(v, e * nc + c, nv, nc, cinfo)];
+  return z;
+}
+
 __global__ void k_gather_rows(const double *__restrict__ y, long nv, int nc,
                               const int2 *__restrict__ cinfo, const double *__restrict__ scale,
-                              float *Z) {
+                              const double *__restrict__ frames, float *Z) {
   const int nin = 4 * nc;
   GL(i, nv * nin) {
     const long v = i / nin;
     const int j = (int)(i % nin);
-    Z[i] = (float)(y[row_index(v, j, nv, nc, cinfo)] / scale[v]);
+    Z[i] = (float)(to_canonical(y, v, j, nv, nc, cinfo, frames) / scale[v]);
   }
 }
-__global__ void k_scatter_add(const float *__restrict__ Y, long nv, int nc,
+// The correction skips the tension coefficients of degree > tcut: the
+// surface divergence is low-pass filtered at that degree (src/Surface.cc:372),
+// so those coefficients span the null space of the implicit operator, and
+// the solution the diagonal-preconditioned solve reaches has them at zero.
+__global__ void k_scatter_add(const float *__restrict__ Y, long nv, int nc, int p, int tcut,
                               const int2 *__restrict__ cinfo, const double *__restrict__ scale,
-                              double *y) {
+                              const double *__restrict__ frames, double *y) {
   const int nin = 4 * nc;
   GL(i, nv * nin) {
     const long v = i / nin;
     const int j = (int)(i % nin);
-    y[row_index(v, j, nv, nc, cinfo)] += scale[v] * (double)Y[i];
+    double add;
+    if (j >= 3 * nc) {
+      const int c = j - 3 * nc;
+      const int2 bl = cinfo[c];
+      const int degree = (p + 1 - bl.y) + (c - bl.x);  // order m = p + 1 - block length
+      if (degree > tcut) continue;
+      add = (double)Y[i];
+    } else if (frames == nullptr) {
+      add = (double)Y[i];
+    } else {
+      const int e = j / nc, c = j % nc;
+      const double *F = frames + 9 * v;
+      const float *Yv = Y + v * nin;
+      add = F[3 * e] * (double)Yv[c] + F[3 * e + 1] * (double)Yv[nc + c] +
+            F[3 * e + 2] * (double)Yv[2 * nc + c];
+    }
+    y[row_index(v, j, nv, nc, cinfo)] += scale[v] * add;
   }
 }
-// samples: Z = Dr_v / s, T = (e_v - Dr_v) / s, written at rows slot0 + v (mod cap)
+// samples in the canonical frames: Z = F^T Dr_v / s, T = F^T (e_v - Dr_v) / s,
+// written at rows slot0 + v (mod cap)
 __global__ void k_record(const double *__restrict__ Dr, const double *__restrict__ e, long nv,
                          int nc, const int2 *__restrict__ cinfo, const double *__restrict__ scale,
-                         long slot0, long cap, float *Zs, float *Ts) {
+                         const double *__restrict__ frames, long slot0, long cap, float *Zs,
+                         float *Ts) {
   const int nin = 4 * nc;
   GL(i, nv * nin) {
     const long v = i / nin;
     const int j = (int)(i % nin);
-    const long o = row_index(v, j, nv, nc, cinfo);
     const long row = (slot0 + v) % cap;
     const double s = scale[v];
-    Zs[row * nin + j] = (float)(Dr[o] / s);
-    Ts[row * nin + j] = (float)((e[o] - Dr[o]) / s);
+    const double z = to_canonical(Dr, v, j, nv, nc, cinfo, frames);
+    const double t = to_canonical(e, v, j, nv, nc, cinfo, frames);
+    Zs[row * nin + j] = (float)(z / s);
+    Ts[row * nin + j] = (float)((t - z) / s);
   }
 }
 }  // namespace
@@ -335,27 +374,30 @@ int LearnedPC::train(int epochs, int batch, float lr, double *last_loss, hipStre
   return hip_status(hipStreamSynchronize(st));
 }
 
-int LearnedPC::apply_inplace(double *y, long nv, int p, const int2 *cinfo, hipStream_t st) {
+int LearnedPC::apply_inplace(double *y, long nv, int p, int tcut, const int2 *cinfo,
+                             const double *frames, hipStream_t st) {
   const int nc = p * (p + 2);
   if (4 * nc != nin_) return VES3D_INVALID_PARAMETER;
   V3D_CHECK(ensure_batch(nv));
   k_row_norms<<<(unsigned)nv, 256, 0, st>>>(y, nv, nc, cinfo, scale_);
   V3D_CHECK(launch_status());
-  k_gather_rows<<<gb(nv * nin_), 256, 0, st>>>(y, nv, nc, cinfo, scale_, Zb_);
+  k_gather_rows<<<gb(nv * nin_), 256, 0, st>>>(y, nv, nc, cinfo, scale_, frames, Zb_);
   V3D_CHECK(launch_status());
   V3D_CHECK(forward(Zb_, nv, Y_, st));
-  k_scatter_add<<<gb(nv * nin_), 256, 0, st>>>(Y_, nv, nc, cinfo, scale_, y);
+  k_scatter_add<<<gb(nv * nin_), 256, 0, st>>>(Y_, nv, nc, p, tcut, cinfo, scale_, frames,
+                                                y);
   return launch_status();
 }
 
 int LearnedPC::record(const double *Dr, const double *e, long nv, int p, const int2 *cinfo,
-                      hipStream_t st) {
+                      const double *frames, hipStream_t st) {
   const int nc = p * (p + 2);
   if (4 * nc != nin_ || cap_ < nv) return VES3D_INVALID_PARAMETER;
   V3D_CHECK(ensure_batch(nv));
   k_row_norms<<<(unsigned)nv, 256, 0, st>>>(Dr, nv, nc, cinfo, scale_);
   V3D_CHECK(launch_status());
-  k_record<<<gb(nv * nin_), 256, 0, st>>>(Dr, e, nv, nc, cinfo, scale_, head_, cap_, Zs_, Ts_);
+  k_record<<<gb(nv * nin_), 256, 0, st>>>(Dr, e, nv, nc, cinfo, scale_, frames, head_, cap_,
+                                           Zs_, Ts_);
   V3D_CHECK(launch_status());
   head_ = (head_ + nv) % cap_;
   count_ += nv;

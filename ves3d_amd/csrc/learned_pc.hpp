@@ -35,11 +35,16 @@ class LearnedPC {
   int train(int epochs, int batch, float lr, double *last_loss_host, hipStream_t st);
 
   // ---- solver interface on the Galerkin (order-major) layout
-  // y holds D r on entry and P(r) on exit.
-  int apply_inplace(double *y, long nv, int p, const int2 *cinfo, hipStream_t st);
-  // records samples from (Dr, e): inputs Dr_v/s_v, targets (e_v - Dr_v)/s_v
+  // y holds D r on entry and P(r) on exit; the correction leaves the tension
+  // coefficients of degree > tcut (the divergence filter) untouched.  frames:
+  // per-vesicle 3x3 canonical frames (row-major, columns the axes) in which
+  // the network sees and returns the velocity blocks; nullptr: identity.
+  int apply_inplace(double *y, long nv, int p, int tcut, const int2 *cinfo, const double *frames,
+                    hipStream_t st);
+  // records samples from (Dr, e) in the canonical frames: inputs F^T Dr_v/s_v,
+  // targets F^T (e_v - Dr_v)/s_v
   int record(const double *Dr, const double *e, long nv, int p, const int2 *cinfo,
-             hipStream_t st);
+             const double *frames, hipStream_t st);
 
   float *params() const { return P_; }
   float *grads() const { return G_; }

@@ -30,6 +30,87 @@ inline unsigned gb(long n) {
   return (unsigned)(b < 1 ? 1 : b);
 }
 
+// Canonical frame of every vesicle for the learned preconditioner: the
+// orthogonal polar factor Q of C = int X (x) s dA_param, s = (sin t cos p,
+// sin t sin p, cos t) the parameter sphere (its degree-1 harmonics) and
+// fgrid[4][ng] = (s_x, s_y, s_z, Gauss-Legendre weight) per grid point.  A
+// rotated copy R X of a shape has C' = R C and frame R Q, so F^T x is the
+// same for both.  One workgroup per vesicle; C is reduced in LDS and thread 0
+// runs the scaled Newton iteration Q <- (g Q + Q^-T / g) / 2 (Higham 1986,
+// g = sqrt(|Q^-1|_F / |Q|_F)), which converges quadratically for the
+// nonsingular C of a closed surface; a singular C gives the identity.
+__device__ inline double inv3(const double *a, double *r) {  // r = a^-1, returns det
+  r[0] = a[4] * a[8] - a[5] * a[7];
+  r[1] = a[2] * a[7] - a[1] * a[8];
+  r[2] = a[1] * a[5] - a[2] * a[4];
+  r[3] = a[5] * a[6] - a[3] * a[8];
+  r[4] = a[0] * a[8] - a[2] * a[6];
+  r[5] = a[2] * a[3] - a[0] * a[5];
+  r[6] = a[3] * a[7] - a[4] * a[6];
+  r[7] = a[1] * a[6] - a[0] * a[7];
+  r[8] = a[0] * a[4] - a[1] * a[3];
+  const double det = a[0] * r[0] + a[1] * r[3] + a[2] * r[6];
+  const double id = det != 0.0 ? 1.0 / det : 0.0;
+  for (int k = 0; k < 9; ++k) r[k] *= id;
+  return det;
+}
+__global__ __launch_bounds__(256) void k_frames(const double *__restrict__ X, long ng,
+                                                const double *__restrict__ fgrid,
+                                                double *__restrict__ frames) {
+  const long v = blockIdx.x;
+  const double *Xv = X + v * 3 * ng;
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (long j = threadIdx.x; j < ng; j += 256) {
+    const double w = fgrid[3 * ng + j];
+    const double s0 = fgrid[j] * w, s1 = fgrid[ng + j] * w, s2 = fgrid[2 * ng + j] * w;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double x = Xv[i * ng + j];
+      acc[3 * i] += x * s0;
+      acc[3 * i + 1] += x * s1;
+      acc[3 * i + 2] += x * s2;
+    }
+  }
+  __shared__ double red[4][9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const double t = wave_sum(acc[k]);
+    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64][k] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double q[9], qi[9];
+  double nq = 0;
+  for (int k = 0; k < 9; ++k) {
+    q[k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+    nq += q[k] * q[k];
+  }
+  bool ok = nq > 0;
+  for (int it = 0; ok && it < 100; ++it) {
+    const double det = inv3(q, qi);
+    if (!(fabs(det) > 1e-12 * nq * sqrt(nq))) {
+      ok = false;
+      break;
+    }
+    double a = 0, b = 0;
+    for (int k = 0; k < 9; ++k) {
+      a += q[k] * q[k];
+      b += qi[k] * qi[k];
+    }
+    const double g = sqrt(sqrt(b / a));
+    double diff = 0;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        const double nw = 0.5 * (g * q[3 * r + c] + qi[3 * c + r] / g);
+        diff += (nw - q[3 * r + c]) * (nw - q[3 * r + c]);
+        q[3 * r + c] = nw;
+      }
+    nq = 3.0;
+    if (diff < 1e-30) break;
+  }
+  for (int k = 0; k < 9; ++k) frames[9 * v + k] = ok ? q[k] : (k % 4 == 0 ? 1.0 : 0.0);
+}
+
 // Function-major coefficients [nf][nc] <-> order-major Galerkin layout.
 // cinfo[c] = (block offset, block length) of coefficient c.
 __global__ void k_fm_to_om(const double *__restrict__ fm, long nf, int nc,
@@ -342,11 +423,12 @@ struct Stepper {
   ForcesG forces;
   StokesG stokes;
   DBuf X, U, sig;                  // state
-  DBuf f, f2, Sf, ten, ug, sg, cf, rhs, xsol, yres, tmpD;
+  DBuf f, f2, Sf, ten, ug, sg, cf, rhs, xsol, yres, tmpD, yloc;
   DBuf bprec, tprec;               // diagonal-spectral preconditioner
+  DBuf fgrid, frames;              // canonical frames of the vesicles (learned PC)
   int2 *cinfo = nullptr;
   LearnedPC *lpc = nullptr;        // not owned
-  bool collect = false;            // record (A e, e) pairs for the learned PC
+  bool collect = false;            // record (A_loc e, e) pairs for the learned PC
   bool prepared = false;
   int matvecs = 0;
   int mode_precision = 0;
@@ -381,6 +463,22 @@ struct Stepper {
     V3D_CHECK(xsol.alloc(n));
     V3D_CHECK(yres.alloc(n));
     V3D_CHECK(tmpD.alloc(n));
+    V3D_CHECK(frames.alloc(9 * nv));
+    {
+      std::vector<double> gx, gw, fg(4 * ng);
+      gauss_legendre(p + 1, gx, gw);
+      for (int i = 0; i <= p; ++i)
+        for (int j = 0; j < 2 * p; ++j) {
+          const double s = std::sqrt(1.0 - gx[i] * gx[i]), ph = M_PI * j / p;
+          const long k = (long)i * 2 * p + j;
+          fg[k] = s * std::cos(ph);
+          fg[ng + k] = s * std::sin(ph);
+          fg[2 * ng + k] = gx[i];
+          fg[3 * ng + k] = gw[i];
+        }
+      V3D_CHECK(fgrid.alloc(4 * ng));
+      V3D_HIP(hipMemcpy(fgrid.p, fg.data(), sizeof(double) * 4 * ng, hipMemcpyHostToDevice));
+    }
     V3D_CHECK(post_init());
     V3D_HIP(hipMemset(U.p, 0, sizeof(double) * U.n));
     V3D_HIP(hipMemset(sig.p, 0, sizeof(double) * sig.n));
@@ -432,20 +530,24 @@ struct Stepper {
   int prepare() {
     V3D_CHECK(forces.update(X, st));
     V3D_CHECK(stokes.setup(X, st));
+    k_frames<<<(unsigned)nv, 256, 0, st>>>(X, ng, fgrid, frames);
+    V3D_CHECK(launch_status());
     prepared = true;
     return VES3D_SUCCESS;
   }
 
-  // ImplicitMatvecPhysical (src/InterfacialVelocity.cc:921-979), velocity form
-  int matvec(const double *x, double *y) {
+  // ImplicitMatvecPhysical (src/InterfacialVelocity.cc:921-979), velocity form;
+  // local: every vesicle's own operator (self-interaction only), which gives
+  // the learned preconditioner its training pairs
+  int matvec(const double *x, double *y, bool local = false) {
     if (!prepared) return VES3D_INVALID_PARAMETER;
-    ++matvecs;
+    if (!local) ++matvecs;
     V3D_CHECK(unpack(x, ug, sg));
     V3D_CHECK(dev_axpby(3 * nv * ng, prm.dt, ug, 0.0, f2, st));  // Du = dt u
     V3D_CHECK(forces.linear_bending(f2, f, st));
     V3D_CHECK(forces.tensile(sg, f2, st));
     V3D_CHECK(dev_axpby(3 * nv * ng, 1.0, f2, 1.0, f, st));
-    V3D_CHECK(stokes.apply(f, Sf, st));
+    V3D_CHECK(stokes.apply(f, Sf, st, false, local));
     V3D_CHECK(forces.S.div(Sf, ten, st));
     V3D_CHECK(dev_axpby(nv * ng, 0.0, ten, -1.0, ten, st));      // ten = -div
     V3D_CHECK(dev_axpby(3 * nv * ng, -1.0, Sf, 1.0, ug, st));    // vox = u - Sf
@@ -488,9 +590,14 @@ struct Stepper {
   static int A_cb(const void *ctx, const double *x, double *y) {
     Stepper *s = (Stepper *)ctx;
     V3D_CHECK(s->matvec(x, y));
-    if (s->collect && s->lpc) {  // training pair: input A x, target x
-      V3D_CHECK(s->precond_diag(y, s->tmpD));
-      V3D_CHECK(s->lpc->record(s->tmpD, x, s->nv, s->prm.p, s->cinfo, s->st));
+    if (s->collect && s->lpc) {
+      // training pair from the vesicles' own operators: input D A_loc x,
+      // target x (the far-field coupling would enter a per-vesicle
+      // regression as noise correlated with its input)
+      if (!s->yloc.p) V3D_CHECK(s->yloc.alloc(s->n));
+      V3D_CHECK(s->matvec(x, s->yloc, true));
+      V3D_CHECK(s->precond_diag(s->yloc, s->tmpD));
+      V3D_CHECK(s->lpc->record(s->tmpD, x, s->nv, s->prm.p, s->cinfo, s->frames, s->st));
     }
     return VES3D_SUCCESS;
   }
@@ -500,7 +607,8 @@ struct Stepper {
   static int Plearn_cb(const void *ctx, const double *x, double *y) {
     Stepper *s = (Stepper *)ctx;
     V3D_CHECK(s->precond_diag(x, y));
-    return s->lpc->apply_inplace(y, s->nv, s->prm.p, s->cinfo, s->st);
+    return s->lpc->apply_inplace(y, s->nv, s->prm.p, s->prm.filter_freq, s->cinfo, s->frames,
+                                 s->st);
   }
 
   // ---- reparametrization and area / volume correction ---------------------
@@ -790,6 +898,30 @@ int ves3d_hip_stepper_matvec(void *h, const double *x, double *y) {
   if (!h || !x || !y) return VES3D_INVALID_PARAMETER;
   Stepper *s = (Stepper *)h;
   V3D_CHECK(s->matvec(x, y));
+  return hip_status(hipStreamSynchronize(s->st));
+}
+
+int ves3d_hip_stepper_matvec_local(void *h, const double *x, double *y) {
+  if (!h || !x || !y) return VES3D_INVALID_PARAMETER;
+  Stepper *s = (Stepper *)h;
+  V3D_CHECK(s->matvec(x, y, true));
+  return hip_status(hipStreamSynchronize(s->st));
+}
+
+int ves3d_hip_stepper_precond_learned(void *h, const double *x, double *y) {
+  if (!h || !x || !y) return VES3D_INVALID_PARAMETER;
+  Stepper *s = (Stepper *)h;
+  if (!s->prepared || !s->lpc || !s->lpc->ready()) return VES3D_INVALID_PARAMETER;
+  V3D_CHECK(Stepper::Plearn_cb(s, x, y));
+  return hip_status(hipStreamSynchronize(s->st));
+}
+
+int ves3d_hip_stepper_frames(void *h, double *frames) {
+  if (!h || !frames) return VES3D_INVALID_PARAMETER;
+  Stepper *s = (Stepper *)h;
+  if (!s->prepared) return VES3D_INVALID_PARAMETER;
+  V3D_HIP(hipMemcpyAsync(frames, s->frames.p, sizeof(double) * 9 * s->nv, hipMemcpyDeviceToDevice,
+                         s->st));
   return hip_status(hipStreamSynchronize(s->st));
 }
 

@@ -108,6 +108,27 @@ class Stepper:
         call("ves3d_hip_stepper_precond", self.h, ptr(x), ptr(y))
         return y
 
+    def matvec_local(self, x):
+        """Every vesicle's own operator (no far field, no near correction)."""
+        self._chk(x, self.n)
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        call("ves3d_hip_stepper_matvec_local", self.h, ptr(x), ptr(y))
+        return y
+
+    def precond_learned(self, x):
+        self._chk(x, self.n)
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        call("ves3d_hip_stepper_precond_learned", self.h, ptr(x), ptr(y))
+        return y
+
+    def frames(self):
+        """Canonical frames [nv][3][3] of the last prepare."""
+        F = torch.empty((self.nves, 3, 3), dtype=torch.float64, device="cuda")
+        call("ves3d_hip_stepper_frames", self.h, ptr(F))
+        return F
+
     def step(self):
         st = StepStats()
         torch.cuda.synchronize()
