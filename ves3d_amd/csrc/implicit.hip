@@ -504,12 +504,8 @@ int StokesG::apply(const double *f, double *vel, hipStream_t st, bool add_repul,
     }
     V3D_HIP(hipEventRecord(ev0[nev], st));
   }
-  if (precision == 0)
-    V3D_CHECK(ves3d_hip_direct_eval_f64(far_src, q, nullptr, nv * (mu + 2), trg, farv, nv * ng,
-                                        mu + 2, ng, st));
-  else
-    V3D_CHECK(ves3d_hip_direct_eval_f32c(far_src, q, nullptr, nv * (mu + 2), trg, farv, nv * ng,
-                                         mu + 2, ng, st));
+  V3D_CHECK(far_field_between_groups(precision != 0, far_src, q, nv * (mu + 2), trg, farv, nv * ng,
+                                     mu + 2, ng, st));
   if (timing) V3D_HIP(hipEventRecord(ev1[nev++], st));
   k_add_points<<<gblocks(nv * ng), 256, 0, st>>>(farv, nv, ng, vel);
   V3D_CHECK(launch_status());

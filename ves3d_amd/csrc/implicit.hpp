@@ -12,6 +12,15 @@
 
 namespace v3d {
 
+// Single-layer far field between groups (vesicles): targets of group g see
+// the sources of every other group, in FP64 or FP32 with compensated
+// accumulation (f32c).  Distinct vesicles' surfaces are disjoint, so the
+// pair loop runs without the coincident-point guard of the direct-evaluation
+// entry points (stokes_direct.hip).
+int far_field_between_groups(bool f32c, const double *spos, const double *sld, size_t n_src,
+                             const double *tpos, double *tvel, size_t n_trg, size_t spg,
+                             size_t tpg, hipStream_t st);
+
 // Device buffer owned by the step objects.
 struct DBuf {
   double *p = nullptr;

@@ -67,7 +67,10 @@ __device__ __forceinline__ double rsqrt_pair(double x) {
   return fma(y * e, fma(e, 0.375, 0.5), y);
 }
 
-template <bool SL, bool DL>
+// GUARD: drop pairs with r^2 <= eps2.  Without it a coincident pair gives
+// NaN; the far field between vesicles runs without it (distinct vesicles'
+// surfaces are disjoint, so every pair is at a positive distance).
+template <bool SL, bool DL, bool GUARD>
 __global__ __launch_bounds__(FF_BLOCK) void k_far_f64(
     const double *__restrict__ spos, const double *__restrict__ sld,
     const double *__restrict__ dld, long n_src, const double *__restrict__ tpos,
@@ -135,7 +138,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_f64(
           const double r2 = dx * dx + dy * dy + dz * dz;
           // unconditional rsqrt + select: no per-pair branch in the loop
           const double rq = rsqrt_pair(r2);
-          const double ri = (r2 > eps2) ? rq : 0.0;
+          const double ri = GUARD ? ((r2 > eps2) ? rq : 0.0) : rq;
           const double ri2 = ri * ri;
           if (SL) {
             const double ip =
@@ -164,7 +167,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_f64(
   }
 }
 
-template <bool SL, bool DL>
+template <bool SL, bool DL, bool GUARD>
 __global__ __launch_bounds__(FF_BLOCK) void k_far_f32c(
     const double *__restrict__ spos, const double *__restrict__ sld,
     const double *__restrict__ dld, long n_src, const double *__restrict__ tpos,
@@ -238,7 +241,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_f32c(
             const float dx = lx - sh[j].x, dy = ly - sh[j].y, dz = lz - sh[j].z;
             const float r2 = dx * dx + dy * dy + dz * dz;
             const float rq = __frsqrt_rn(r2);
-            const float ri = (r2 > eps2) ? rq : 0.0f;
+            const float ri = GUARD ? ((r2 > eps2) ? rq : 0.0f) : rq;
             const float ri2 = ri * ri;
             if (SL) {
               const float ip = (sh[j].fx * dx + sh[j].fy * dy + sh[j].fz * dz) * ri2;
@@ -304,15 +307,22 @@ static void launch_far(dim3 grid, hipStream_t st, const double *spos,
                        const double *sld, const double *dld, long n_src,
                        const double *tpos, long n_trg, long sp, long tp,
                        long n_groups, int ns, double eps2, double csl,
-                       double cdl, double *part) {
-  if (F32)
-    k_far_f32c<SL, DL><<<grid, FF_BLOCK, 0, st>>>(spos, sld, dld, n_src, tpos,
-                                                  n_trg, sp, tp, n_groups, ns,
-                                                  (float)eps2, csl, cdl, part);
-  else
-    k_far_f64<SL, DL><<<grid, FF_BLOCK, 0, st>>>(spos, sld, dld, n_src, tpos,
-                                                 n_trg, sp, tp, n_groups, ns,
-                                                 eps2, csl, cdl, part);
+                       double cdl, double *part, bool guard) {
+  if (F32) {
+    if (guard)
+      k_far_f32c<SL, DL, true><<<grid, FF_BLOCK, 0, st>>>(spos, sld, dld, n_src, tpos, n_trg, sp, tp,
+                                                          n_groups, ns, (float)eps2, csl, cdl, part);
+    else
+      k_far_f32c<SL, DL, false><<<grid, FF_BLOCK, 0, st>>>(spos, sld, dld, n_src, tpos, n_trg, sp, tp,
+                                                           n_groups, ns, (float)eps2, csl, cdl, part);
+  } else {
+    if (guard)
+      k_far_f64<SL, DL, true><<<grid, FF_BLOCK, 0, st>>>(spos, sld, dld, n_src, tpos, n_trg, sp, tp,
+                                                         n_groups, ns, eps2, csl, cdl, part);
+    else
+      k_far_f64<SL, DL, false><<<grid, FF_BLOCK, 0, st>>>(spos, sld, dld, n_src, tpos, n_trg, sp, tp,
+                                                          n_groups, ns, eps2, csl, cdl, part);
+  }
 }
 
 // Shared driver of the all-pairs and far-field entry points.
@@ -320,7 +330,7 @@ template <bool F32>
 static int far_eval(const double *spos, const double *sld, const double *dld,
                     size_t n_src, const double *tpos, double *tvel, size_t n_trg,
                     size_t spg, size_t tpg, double eps2, double csl, double cdl,
-                    hipStream_t st) {
+                    hipStream_t st, bool guard = true) {
   if (n_trg == 0) return VES3D_SUCCESS;
   if (!tpos || !tvel) return VES3D_INVALID_PARAMETER;
   if (n_src == 0 || (!sld && !dld)) {
@@ -342,11 +352,11 @@ static int far_eval(const double *spos, const double *sld, const double *dld,
   const dim3 grid((unsigned)(tblocks * ns));
   const long sp = grouped ? (long)spg : 0, tp = grouped ? (long)tpg : 0;
   if (sld && dld)
-    launch_far<F32, true, true>(grid, st, spos, sld, dld, n_src, tpos, n_trg, sp, tp, n_groups, ns, eps2, csl, cdl, part);
+    launch_far<F32, true, true>(grid, st, spos, sld, dld, n_src, tpos, n_trg, sp, tp, n_groups, ns, eps2, csl, cdl, part, guard);
   else if (sld)
-    launch_far<F32, true, false>(grid, st, spos, sld, dld, n_src, tpos, n_trg, sp, tp, n_groups, ns, eps2, csl, cdl, part);
+    launch_far<F32, true, false>(grid, st, spos, sld, dld, n_src, tpos, n_trg, sp, tp, n_groups, ns, eps2, csl, cdl, part, guard);
   else
-    launch_far<F32, false, true>(grid, st, spos, sld, dld, n_src, tpos, n_trg, sp, tp, n_groups, ns, eps2, csl, cdl, part);
+    launch_far<F32, false, true>(grid, st, spos, sld, dld, n_src, tpos, n_trg, sp, tp, n_groups, ns, eps2, csl, cdl, part, guard);
   V3D_CHECK(launch_status());
   const long n = (long)n_trg * 3;
   k_sum_slices<<<cdiv(n, 256), 256, 0, st>>>(part, ns, n, tvel);
@@ -448,6 +458,15 @@ static int vesicle_stokes(const Real *src, const Real *nrm, const Real *den,
   k_vesicle_stokes<Real, DLK><<<grid, VS_BLOCK, 0, st>>>(
       src, nrm, den, qw, (long)ss, (long)ts, (long)head, (long)tail, trg, pot, scal);
   return launch_status();
+}
+
+int far_field_between_groups(bool f32c, const double *spos, const double *sld, size_t n_src,
+                             const double *tpos, double *tvel, size_t n_trg, size_t spg,
+                             size_t tpg, hipStream_t st) {
+  if (spg == 0 || tpg == 0) return VES3D_INVALID_PARAMETER;
+  const double csl = 1.0 / (8.0 * kPi), cdl = 3.0 / (4.0 * kPi);
+  if (f32c) return far_eval<true>(spos, sld, nullptr, n_src, tpos, tvel, n_trg, spg, tpg, 0.0, csl, cdl, st, false);
+  return far_eval<false>(spos, sld, nullptr, n_src, tpos, tvel, n_trg, spg, tpg, 0.0, csl, cdl, st, false);
 }
 
 }  // namespace v3d

@@ -136,14 +136,30 @@ def suspension(shapes, volfrac, seed=SEED_BASE, gap=0.05, volumes=None, rotate=F
         # dense: random sequential addition of the bounding balls jams near
         # this fraction, so centres go on a jittered cubic lattice instead
         # (jitter bounded so that bounding balls + gap stay disjoint)
+        # (a box of a x b x c sites, cubic when that fits, else the smallest
+        # site count >= nv, which packs denser when nv is not a cube)
         k = int(np.ceil(nv ** (1.0 / 3.0)))
-        side = (float(np.sum(volumes)) / volfrac) ** (1.0 / 3.0)
-        h = side / k
+        total = float(np.sum(volumes)) / volfrac
+        dims = (k, k, k)
+        h = total ** (1.0 / 3.0) / k
+        if h < 2 * rad + gap:
+            best = None
+            for a in range(1, k + 1):
+                for b in range(a, k + 1):
+                    c = -(-nv // (a * b))
+                    if c < b:
+                        continue
+                    key = (a * b * c, c - a)
+                    if best is None or key < best[0]:
+                        best = (key, (c, b, a))
+            dims = best[1]
+            h = (total / (dims[0] * dims[1] * dims[2])) ** (1.0 / 3.0)
         if h < 2 * rad + gap:
             raise RuntimeError("volume fraction %g too dense for a lattice of bounding balls" % volfrac)
         jit = 0.5 * (h - 2 * rad - gap)
-        idx = np.stack(np.meshgrid(np.arange(k), np.arange(k), np.arange(k), indexing="ij"), -1)
-        idx = idx.reshape(-1, 3)[rng.permutation(k ** 3)[:nv]]
+        idx = np.stack(np.meshgrid(np.arange(dims[0]), np.arange(dims[1]), np.arange(dims[2]),
+                                   indexing="ij"), -1)
+        idx = idx.reshape(-1, 3)[rng.permutation(dims[0] * dims[1] * dims[2])[:nv]]
         c = (idx + 0.5) * h + rng.uniform(-jit, jit, size=(nv, 3))
     return shapes + c[:, :, None]
 
