@@ -58,6 +58,7 @@ def bench_step(args):
     from ves3d_amd import workloads as W
 
     ves3d_amd.init(0)
+    ves3d_amd.call("ves3d_hip_set_far_kernel", 1 if args.far_kernel == "scalar" else 0)
     nv, p = args.nves, args.p
     X0 = W.implicit_step_case(nv=nv, p=p)
     Xg = torch.from_numpy(X0).cuda()
@@ -144,9 +145,11 @@ def bench_step(args):
     achieved = flop / (far_ms * 1e-3) / 1e12
     roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-            "kernel": "k_far_f32c<SL> (+k_sum_slices)", "kernel_ms": far_ms,
+            "kernel": ("k_far_groups_f32c" if args.far_kernel == "scalar" else "k_far_f32c<SL>")
+            + " (+k_pack_far32, k_sum_slices)", "kernel_ms": far_ms,
             "flop_per_launch": flop, "pairs_per_launch": pairs,
-            "fp64_form": {"kernel": "k_far_f64<SL>", "kernel_ms": far_ms_plain,
+            "fp64_form": {"kernel": "k_far_groups_f64" if args.far_kernel == "scalar" else "k_far_f64<SL>",
+                          "kernel_ms": far_ms_plain,
                           "achieved": flop / (far_ms_plain * 1e-3) / 1e12,
                           "peak": PEAK_FP64_TFLOPS}}
 
@@ -341,6 +344,8 @@ def main():
     ap.add_argument("--mixed-precision", type=int, default=2, choices=[1, 2],
                     help="precision of the learned form: 1 FP32 singular operator + FP32-"
                          "compensated far field, 2 FP64 singular operator + FP32-compensated far field")
+    ap.add_argument("--far-kernel", default="scalar", choices=["scalar", "lds"],
+                    help="far field between vesicles: sources through the scalar cache or LDS tiles")
     ap.add_argument("--r-near", type=float, default=0.1,
                     help="near-singular zone radius (0: the reference's r_ves 2/sqrt(p_up))")
     args = ap.parse_args()

@@ -129,6 +129,13 @@ int ves3d_hip_gemm_f32(char transA, char transB, int m, int n, int k, float alph
  * comes from the environment variable VES3D_GEMM ("mfma" / "valu"). */
 int ves3d_hip_set_gemm_kernel(int kind);
 
+/* Selects the kernel of the far field between vesicles inside the implicit
+ * step (StokesVelocity's PVFMMEval call, src/StokesVelocity.cc:481-485):
+ * 0 = sources staged through LDS tiles, 1 = sources read through the scalar
+ * cache (the default).  Both give the same sums up to rounding order.  No
+ * counterpart in the reference. */
+int ves3d_hip_set_far_kernel(int kind);
+
 /* ---- direct Stokes sums ------------------------------------------------ */
 
 /* Device<DT>::DirectStokes (include/Device.h:205-209).  For each vesicle vt

@@ -5,7 +5,9 @@ preconditioner, the device BiCGStab, and one full step.
 Tolerances (relative, max-norm unless stated):
   FP64 rhs / matvec / precond   1e-9
   FP64 step solution            1e-5 (both solves stop at relres 1e-6)
-  mixed-precision matvec        2e-5 against the FP64 oracle
+  mixed-precision matvec        2e-5 against the FP64 oracle (precision 1:
+                                FP32 singular operator + FP32c far field;
+                                2: FP64 singular operator + FP32c far field)
 """
 import ctypes
 
@@ -33,8 +35,18 @@ def _case(nv, p, seed, volfrac=0.05):
     return W.suspension(shapes, volfrac, seed=seed)
 
 
+@pytest.fixture(params=[1, 0], ids=["far-scalar", "far-lds"])
+def far_kernel(request, hip):
+    """Both kernels of the far field between vesicles (scalar-cache source
+    records, LDS-staged source tiles)."""
+    from ves3d_amd import call
+    call("ves3d_hip_set_far_kernel", request.param)
+    yield request.param
+    call("ves3d_hip_set_far_kernel", 1)
+
+
 @pytest.mark.parametrize("nv,p", [(2, 6), (3, 8)])
-def test_rhs_matvec_precond_f64(hip, nv, p):
+def test_rhs_matvec_precond_f64(hip, far_kernel, nv, p):
     X = _case(nv, p, seed=10 + nv)
     dt, shear = 1e-3, 0.3
     orc = ves.Implicit(X, p, dt=dt, shear=shear)
@@ -55,11 +67,12 @@ def test_rhs_matvec_precond_f64(hip, nv, p):
     st.close()
 
 
-def test_matvec_mixed_precision(hip):
+@pytest.mark.parametrize("precision", [1, 2])
+def test_matvec_mixed_precision(hip, far_kernel, precision):
     nv, p = 3, 8
     X = _case(nv, p, seed=7)
     orc = ves.Implicit(X, p, dt=1e-3, shear=0.3)
-    st = I.Stepper(nv, p, dt=1e-3, shear_rate=0.3, precision=1, precond=1)
+    st = I.Stepper(nv, p, dt=1e-3, shear_rate=0.3, precision=precision, precond=1)
     st.set_state(cu(X))
     st.prepare()
     x = np.random.default_rng(1).standard_normal(st.n)

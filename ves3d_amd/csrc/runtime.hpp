@@ -18,6 +18,7 @@ enum WorkspaceSlot {
   kWsSing3,
   kWsSing4,
   kWsSolver,
+  kWsFarRecords,
   kWsCount
 };
 

@@ -22,6 +22,7 @@
 //     DirectStokesDoubleLayer sums (src/CPUKernels.cc:8-118, :1059-1136) on
 //     AxisMajor [ves][xyz][stride] data; one wave per 64 targets of one
 //     vesicle, sources of that vesicle staged through LDS.
+#include <atomic>
 #include "common.hpp"
 #include "runtime.hpp"
 
@@ -364,6 +365,154 @@ static int far_eval(const double *spos, const double *sld, const double *dld,
 }
 
 // ---------------------------------------------------------------------------
+// Far field between groups with the sources read through the scalar cache.
+// Every lane of a wave uses the same source at the same time, so the source
+// records are loaded with uniform addresses (s_load into SGPRs, which the
+// VALU reads as operands): no LDS staging, no barriers, and the per-pair
+// instruction stream is the arithmetic alone.  Records are packed once per
+// call; the FP32 form stores each source relative to its group's first point
+// (the group anchor, FP64), and the lane shifts its target by the same anchor
+// in FP64 before rounding, as the LDS form does per tile.
+struct __align__(16) FarRec64 {
+  double x, y, z, fx, fy, fz, _p0, _p1;
+};
+struct __align__(16) FarRec32 {
+  float x, y, z, fx, fy, fz, _p0, _p1;
+};
+
+__global__ void k_pack_far64(const double *__restrict__ spos, const double *__restrict__ sld,
+                             long n, FarRec64 *__restrict__ rec) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  FarRec64 r;
+  r.x = spos[3 * i];
+  r.y = spos[3 * i + 1];
+  r.z = spos[3 * i + 2];
+  r.fx = sld[3 * i];
+  r.fy = sld[3 * i + 1];
+  r.fz = sld[3 * i + 2];
+  r._p0 = r._p1 = 0.0;
+  rec[i] = r;
+}
+__global__ void k_pack_far32(const double *__restrict__ spos, const double *__restrict__ sld,
+                             long n, long spg, FarRec32 *__restrict__ rec) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long a = (i / spg) * spg;  // group anchor: its first source
+  FarRec32 r;
+  r.x = (float)(spos[3 * i] - spos[3 * a]);
+  r.y = (float)(spos[3 * i + 1] - spos[3 * a + 1]);
+  r.z = (float)(spos[3 * i + 2] - spos[3 * a + 2]);
+  r.fx = (float)sld[3 * i];
+  r.fy = (float)sld[3 * i + 1];
+  r.fz = (float)sld[3 * i + 2];
+  r._p0 = r._p1 = 0.0f;
+  rec[i] = r;
+}
+
+__global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f64(
+    const FarRec64 *__restrict__ rec, long spg, long n_groups, const double *__restrict__ tpos,
+    long n_trg, long tpg, int nsplit, double csl, double *__restrict__ part) {
+  const long slice = blockIdx.x % nsplit;
+  const long tb = blockIdx.x / nsplit;
+  const long t = tb * FF_BLOCK + threadIdx.x;
+  const bool tvalid = t < n_trg;
+  double tx = 0, ty = 0, tz = 0;
+  if (tvalid) {
+    tx = tpos[3 * t];
+    ty = tpos[3 * t + 1];
+    tz = tpos[3 * t + 2];
+  }
+  const long my_g = tvalid ? t / tpg : -1;
+  const long tfirst = tb * FF_BLOCK;
+  const long tlast = min(tfirst + FF_BLOCK, n_trg) - 1;
+  double Sx = 0, Sy = 0, Sz = 0;
+  long g0, g1;
+  slice_range(slice, nsplit, 0, spg, n_groups, g0, g1);
+  for (long g = g0; g < g1; ++g) {
+    if (tfirst / tpg == g && tlast / tpg == g) continue;  // block-uniform skip
+    if (!tvalid || my_g == g) continue;
+    const FarRec64 *__restrict__ r = rec + g * spg;
+#pragma unroll 4
+    for (long j = 0; j < spg; ++j) {
+      const FarRec64 s = r[j];
+      const double dx = tx - s.x, dy = ty - s.y, dz = tz - s.z;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double ri = rsqrt_pair(r2);
+      const double ri2 = ri * ri;
+      const double ip = (s.fx * dx + s.fy * dy + s.fz * dz) * ri2;
+      Sx += ri * (s.fx + dx * ip);
+      Sy += ri * (s.fy + dy * ip);
+      Sz += ri * (s.fz + dz * ip);
+    }
+  }
+  if (tvalid) {
+    double *o = part + (slice * n_trg + t) * 3;
+    o[0] = csl * Sx;
+    o[1] = csl * Sy;
+    o[2] = csl * Sz;
+  }
+}
+
+__global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c(
+    const FarRec32 *__restrict__ rec, const double *__restrict__ spos, long spg, long n_groups,
+    const double *__restrict__ tpos, long n_trg, long tpg, int nsplit, double csl,
+    double *__restrict__ part) {
+  const long slice = blockIdx.x % nsplit;
+  const long tb = blockIdx.x / nsplit;
+  const long t = tb * FF_BLOCK + threadIdx.x;
+  const bool tvalid = t < n_trg;
+  double tx = 0, ty = 0, tz = 0;
+  if (tvalid) {
+    tx = tpos[3 * t];
+    ty = tpos[3 * t + 1];
+    tz = tpos[3 * t + 2];
+  }
+  const long my_g = tvalid ? t / tpg : -1;
+  const long tfirst = tb * FF_BLOCK;
+  const long tlast = min(tfirst + FF_BLOCK, n_trg) - 1;
+  float Sh[3] = {0, 0, 0}, Sl[3] = {0, 0, 0};
+  long g0, g1;
+  slice_range(slice, nsplit, 0, spg, n_groups, g0, g1);
+  for (long g = g0; g < g1; ++g) {
+    if (tfirst / tpg == g && tlast / tpg == g) continue;  // block-uniform skip
+    if (!tvalid || my_g == g) continue;
+    const double *a = spos + 3 * g * spg;
+    const float lx = (float)(tx - a[0]), ly = (float)(ty - a[1]), lz = (float)(tz - a[2]);
+    const FarRec32 *__restrict__ r = rec + g * spg;
+    for (long j0 = 0; j0 < spg; j0 += FF_SUB) {
+      const long j1 = min(spg, j0 + FF_SUB);
+      float sx = 0, sy = 0, sz = 0;
+#pragma unroll 4
+      for (long j = j0; j < j1; ++j) {
+        const FarRec32 s = r[j];
+        const float dx = lx - s.x, dy = ly - s.y, dz = lz - s.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float ri = __frsqrt_rn(r2);
+        const float ri2 = ri * ri;
+        const float ip = (s.fx * dx + s.fy * dy + s.fz * dz) * ri2;
+        sx += ri * (s.fx + dx * ip);
+        sy += ri * (s.fy + dy * ip);
+        sz += ri * (s.fz + dz * ip);
+      }
+      float s_, e_;
+      two_sum(Sh[0], sx, s_, e_); Sh[0] = s_; Sl[0] += e_;
+      two_sum(Sh[1], sy, s_, e_); Sh[1] = s_; Sl[1] += e_;
+      two_sum(Sh[2], sz, s_, e_); Sh[2] = s_; Sl[2] += e_;
+    }
+  }
+  if (tvalid) {
+    double *o = part + (slice * n_trg + t) * 3;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = csl * ((double)Sh[c] + (double)Sl[c]);
+  }
+}
+
+// 0: LDS-staged source tiles (k_far_*), 1: scalar-cache source records
+// (k_far_groups_*); selects the far field between groups.
+static std::atomic<int> g_far_kind{1};
+
+// ---------------------------------------------------------------------------
 // Per-vesicle Device<>::DirectStokes / DirectStokesDoubleLayer.
 constexpr int VS_BLOCK = 64;
 constexpr int VS_TILE = 256;
@@ -465,8 +614,37 @@ int far_field_between_groups(bool f32c, const double *spos, const double *sld, s
                              size_t tpg, hipStream_t st) {
   if (spg == 0 || tpg == 0) return VES3D_INVALID_PARAMETER;
   const double csl = 1.0 / (8.0 * kPi), cdl = 3.0 / (4.0 * kPi);
-  if (f32c) return far_eval<true>(spos, sld, nullptr, n_src, tpos, tvel, n_trg, spg, tpg, 0.0, csl, cdl, st, false);
-  return far_eval<false>(spos, sld, nullptr, n_src, tpos, tvel, n_trg, spg, tpg, 0.0, csl, cdl, st, false);
+  if (g_far_kind.load() == 0) {
+    if (f32c)
+      return far_eval<true>(spos, sld, nullptr, n_src, tpos, tvel, n_trg, spg, tpg, 0.0, csl, cdl, st, false);
+    return far_eval<false>(spos, sld, nullptr, n_src, tpos, tvel, n_trg, spg, tpg, 0.0, csl, cdl, st, false);
+  }
+  if (n_trg == 0) return VES3D_SUCCESS;
+  if (!spos || !sld || !tpos || !tvel) return VES3D_INVALID_PARAMETER;
+  if (n_src % spg || n_trg % tpg || n_src / spg != n_trg / tpg) return VES3D_INVALID_PARAMETER;
+  const long n_groups = (long)(n_src / spg);
+  const int ns = choose_nsplit((long)n_trg, (long)n_src, n_groups);
+  double *part = (double *)workspace(kWsFarPartial, (size_t)ns * n_trg * 3 * sizeof(double));
+  void *rec = workspace(kWsFarRecords, n_src * (f32c ? sizeof(FarRec32) : sizeof(FarRec64)));
+  if (!part || !rec) return VES3D_MEMORY_ERROR;
+  const long tblocks = ((long)n_trg + FF_BLOCK - 1) / FF_BLOCK;
+  const dim3 grid((unsigned)(tblocks * ns));
+  if (f32c) {
+    k_pack_far32<<<cdiv((long)n_src, 256), 256, 0, st>>>(spos, sld, (long)n_src, (long)spg,
+                                                          (FarRec32 *)rec);
+    V3D_CHECK(launch_status());
+    k_far_groups_f32c<<<grid, FF_BLOCK, 0, st>>>((const FarRec32 *)rec, spos, (long)spg, n_groups,
+                                                 tpos, (long)n_trg, (long)tpg, ns, csl, part);
+  } else {
+    k_pack_far64<<<cdiv((long)n_src, 256), 256, 0, st>>>(spos, sld, (long)n_src, (FarRec64 *)rec);
+    V3D_CHECK(launch_status());
+    k_far_groups_f64<<<grid, FF_BLOCK, 0, st>>>((const FarRec64 *)rec, (long)spg, n_groups, tpos,
+                                                (long)n_trg, (long)tpg, ns, csl, part);
+  }
+  V3D_CHECK(launch_status());
+  const long n = (long)n_trg * 3;
+  k_sum_slices<<<cdiv(n, 256), 256, 0, st>>>(part, ns, n, tvel);
+  return launch_status();
 }
 
 }  // namespace v3d
@@ -521,6 +699,12 @@ int ves3d_hip_stokes_all2all_f32c(const double *src, const double *den,
                                   size_t np, double *pot, ves3d_stream_t stream) {
   return far_eval<true>(src, den, nullptr, np, src, pot, np, 0, 0, 0.0,
                         1.0 / kPi / 8.0, 0.0, (hipStream_t)stream);
+}
+
+int ves3d_hip_set_far_kernel(int kind) {
+  if (kind != 0 && kind != 1) return VES3D_INVALID_PARAMETER;
+  g_far_kind.store(kind);
+  return VES3D_SUCCESS;
 }
 
 static const double kEvalEps2 = 128.0 * 2.220446049250313e-16;
