@@ -23,6 +23,8 @@ Workloads
         `ms_per_step_plain`.
   far : the far-field direct Stokes SL+DL sum alone, 512 unit spheres at
         p=16 (278,528 points; own-vesicle pairs excluded).
+  matvec: K Galerkin matvecs of the step workload (--dtype f32c: the mixed
+        form) after a 512 MiB device copy that calibrates HBM counters.
 """
 import argparse
 import json
@@ -261,6 +263,54 @@ def cpu_baseline_step(nv, p, X, n_matvec, near_pairs):
 
 
 # ---------------------------------------------------------------- far only
+def bench_matvec(args):
+    """One prepared stepper of the step workload and K Galerkin matvecs (each
+    one far-field launch between vesicles plus the local operator), after a
+    device copy of known size whose HBM counters calibrate the far kernel's
+    (FETCH_SIZE / WRITE_SIZE under rocprofv3 --pmc)."""
+    import torch
+    import ves3d_amd
+    from ves3d_amd import implicit as I
+    from ves3d_amd import workloads as W
+
+    ves3d_amd.init(0)
+    ves3d_amd.call("ves3d_hip_set_far_kernel", 1 if args.far_kernel == "scalar" else 0)
+    nv, p = args.nves, args.p
+    prec = args.mixed_precision if args.dtype == "f32c" else 0
+    st = I.Stepper(nv, p, dt=1e-3, shear_rate=1.0, precision=prec, precond=1, r_near=args.r_near)
+    st.set_state(torch.from_numpy(W.implicit_step_case(nv=nv, p=p)).cuda())
+    st.prepare()
+    x = torch.from_numpy(np.random.default_rng(3).standard_normal(st.n)).cuda()
+    cal_bytes = 1 << 29                          # 512 MiB read + 512 MiB written
+    a = torch.ones(cal_bytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    _log("calibration copy: %d bytes read, %d bytes written" % (cal_bytes, cal_bytes))
+    for _ in range(args.warmup):
+        st.matvec(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st.matvec(x)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / max(1, args.steps) * 1e3
+    ng = 2 * p * (p + 1)
+    mu = 2 * (2 * p) * (2 * p + 1) + 2
+    pairs = (nv * ng) * (nv - 1) * mu
+    st.close()
+    return {
+        "metric": "ms per Galerkin matvec @512 ves p=16", "value": ms, "unit": "ms", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if prec else "f64",
+        "data": "synthetic",
+        "config": {"workload": "implicit-step matvec, 512 perturbed ellipsoids p=16, 20% volume "
+                               "fraction", "precision": prec, "far_kernel": args.far_kernel,
+                   "far_pairs": pairs},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def bench_far(args):
     import torch
     import ves3d_amd
@@ -328,7 +378,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="step", choices=["step", "far"])
+    ap.add_argument("--workload", default="step", choices=["step", "far", "matvec"])
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32c"])
     ap.add_argument("--nves", type=int, default=512)
     ap.add_argument("--p", type=int, default=16)
@@ -353,7 +403,7 @@ def main():
         print("bench.py: only the one-GPU form is built; --gpus %d is not supported "
               "(across GPUs the path would shard by vesicle)" % args.gpus, file=sys.stderr)
         sys.exit(2)
-    res = bench_step(args) if args.workload == "step" else bench_far(args)
+    res = {"step": bench_step, "far": bench_far, "matvec": bench_matvec}[args.workload](args)
     print(json.dumps(res))
 
 

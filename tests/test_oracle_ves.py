@@ -200,3 +200,17 @@ def test_area_volume_correction_restores_targets():
     # already on target: no iteration, no change
     Xs, it0 = ves.area_volume_correction(X, p, 2 * p, a0, v0)
     assert it0 == 0 and np.array_equal(Xs, X)
+
+
+def test_bending_precond_singular_degree():
+    """dt n^3 = 1 (n = 10 at dt = 1e-3): the reference's 1/|1 - dt n^3| is
+    +inf in IEEE arithmetic and falls back to 1.0
+    (src/InterfacialVelocity.cc:680-682); neighbouring degrees keep the
+    formula, and degrees whose value exceeds 1e3 in magnitude also give 1."""
+    p = 12
+    bp = ves.bending_precond(p, 1e-3)
+    deg = ves._degree_of_coeff(p)
+    assert np.all(bp[deg == 10] == 1.0)
+    assert np.allclose(bp[deg == 9], 1.0 / abs(1.0 - 1e-3 * 729))
+    assert np.allclose(bp[deg == 12], 1.0 / abs(1.0 - 1e-3 * 1728))
+    assert np.all(bp[deg == 0] == 1.0)
