@@ -350,7 +350,8 @@ typedef struct ves3d_step_stats {
   int iterations;         /* BiCGStab iterations */
   int matvecs;            /* operator applications (incl. residual checks) */
   int status;             /* BiCGSReturn code */
-  int fell_back;          /* 1 if the learned preconditioner was abandoned */
+  int fell_back;          /* 1 if the learned solve failed or missed the tolerance
+                             and was repeated with the diagonal preconditioner */
   double relres;          /* recursive relative residual at exit */
   double true_relres;     /* ||b - A x|| / ||b|| of the returned solution */
   double ms_setup, ms_rhs, ms_solve, ms_total;
@@ -529,8 +530,14 @@ int ves3d_hip_lpc_loss_grad(void *handle, const float *Z, const float *T, size_t
                             double *loss_host, float *grads, ves3d_stream_t stream);
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with the last gradient. */
 int ves3d_hip_lpc_adam(void *handle, float lr, ves3d_stream_t stream);
-/* epochs passes over the recorded samples in minibatches of `batch`. */
+/* epochs passes over the recorded samples in shuffled minibatches of
+ * `batch`, Adam with a cosine decay of lr; keeps the parameters with the
+ * lowest loss over all samples (evaluated after every epoch) and returns
+ * that loss. */
 int ves3d_hip_lpc_train(void *handle, int epochs, int batch, float lr, double *last_loss_host);
+/* Loss over all recorded samples and the mean square of their targets (their
+ * ratio is the network's relative fit). */
+int ves3d_hip_lpc_dataset_loss(void *handle, double *loss_host, double *target_ms_host);
 /* Attaches a learned preconditioner to a stepper (precond = 2 uses it once
  * it has been trained or given parameters); collect != 0 records training
  * pairs during every solve. */

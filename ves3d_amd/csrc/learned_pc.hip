@@ -87,6 +87,50 @@ __global__ void k_colsum(const float *__restrict__ D, long B, int w, float *g) {
   }
 }
 // Adam (Kingma & Ba 2015, Algorithm 1)
+// minibatch rows Zb[i] = Zs[perm[i]], Tb[i] = Ts[perm[i]]
+__global__ void k_gather_batch(const float *__restrict__ Zs, const float *__restrict__ Ts,
+                               const int *__restrict__ perm, long B, int nin, float *Zb, float *Tb) {
+  GL(i, B * nin) {
+    const long r = i / nin;
+    const long src = (long)perm[r] * nin + (i % nin);
+    Zb[i] = Zs[src];
+    Tb[i] = Ts[src];
+  }
+}
+// partial sums of (Y - T)^2 and T^2 per block into part[2 * block + {0, 1}]
+__global__ __launch_bounds__(256) void k_sq_sums(const float *__restrict__ Y,
+                                                 const float *__restrict__ T, long n,
+                                                 double *part) {
+  double a = 0, b = 0;
+  GL(i, n) {
+    const double d = (double)Y[i] - (double)T[i];
+    a += d * d;
+    b += (double)T[i] * (double)T[i];
+  }
+  __shared__ double sh[2][4];
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if (threadIdx.x % 64 == 0) {
+    sh[0][threadIdx.x / 64] = a;
+    sh[1][threadIdx.x / 64] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
+    part[2 * blockIdx.x + 1] = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+  }
+}
+// acc[0..1] += sums of the nb partial pairs (one thread, fixed order)
+__global__ void k_acc_pairs(const double *part, int nb, double *acc) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double a = 0, b = 0;
+  for (int k = 0; k < nb; ++k) {
+    a += part[2 * k];
+    b += part[2 * k + 1];
+  }
+  acc[0] += a;
+  acc[1] += b;
+}
 __global__ void k_adam(float *P, const float *__restrict__ G, float *M, float *V, long n, float lr,
                        float c1, float c2) {
   GL(i, n) {
@@ -215,6 +259,9 @@ LearnedPC::~LearnedPC() {
     if (p) (void)hipFree(p);
   if (scale_) (void)hipFree(scale_);
   if (lossbuf_) (void)hipFree(lossbuf_);
+  if (evalbuf_) (void)hipFree(evalbuf_);
+  if (perm_) (void)hipFree(perm_);
+  if (Pbest_) (void)hipFree(Pbest_);
   for (auto &kv : descs_) (void)hipFree(kv.second);
 }
 
@@ -247,6 +294,8 @@ int LearnedPC::init(int nin, int width, long capacity, unsigned long long seed) 
     if (hipMalloc(&Ts_, sizeof(float) * cap_ * nin) != hipSuccess) return VES3D_MEMORY_ERROR;
   }
   if (hipMalloc(&lossbuf_, sizeof(double) * 4200) != hipSuccess) return VES3D_MEMORY_ERROR;
+  if (hipMalloc(&evalbuf_, sizeof(double) * (2 + 2 * kEvalParts)) != hipSuccess)
+    return VES3D_MEMORY_ERROR;
   return hip_status(hipDeviceSynchronize());
 }
 
@@ -349,27 +398,70 @@ int LearnedPC::adam(float lr, hipStream_t st) {
   return launch_status();
 }
 
+int LearnedPC::dataset_loss(long have, double *loss_host, double *target_ms_host,
+                             hipStream_t st) {
+  const long B = std::min<long>(std::max<long>(bcap_, 1), have);
+  V3D_CHECK(ensure_batch(B));
+  double *acc = evalbuf_, *part = evalbuf_ + 2;
+  V3D_HIP(hipMemsetAsync(acc, 0, 2 * sizeof(double), st));
+  for (long r0 = 0; r0 < have; r0 += B) {
+    const long cnt = std::min(B, have - r0);
+    V3D_CHECK(forward(Zs_ + r0 * nin_, cnt, Y_, st));
+    const long tot = cnt * nin_;
+    const int nb = (int)std::min<long>(kEvalParts, std::max<long>(1, (tot + 255) / 256));
+    k_sq_sums<<<nb, 256, 0, st>>>(Y_, Ts_ + r0 * nin_, tot, part);
+    V3D_CHECK(launch_status());
+    k_acc_pairs<<<1, 64, 0, st>>>(part, nb, acc);
+    V3D_CHECK(launch_status());
+  }
+  double h[2];
+  V3D_HIP(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, st));
+  V3D_HIP(hipStreamSynchronize(st));
+  const double n = (double)have * nin_;
+  if (loss_host) *loss_host = 0.5 * h[0] / n;
+  if (target_ms_host) *target_ms_host = h[1] / n;
+  return VES3D_SUCCESS;
+}
+
 int LearnedPC::train(int epochs, int batch, float lr, double *last_loss, hipStream_t st) {
   const long have = std::min(count_, cap_);
-  if (have < 1 || batch < 1) return VES3D_INVALID_PARAMETER;
+  if (have < 1 || batch < 1 || epochs < 1) return VES3D_INVALID_PARAMETER;
   const long B = std::min<long>(batch, have);
   V3D_CHECK(ensure_batch(B));
-  std::mt19937_64 rng(seed_ + 12345);
-  const long steps = std::max<long>(1, epochs * have / B);
-  double loss = 0;
-  for (long s = 0; s < steps; ++s) {
-    const long start = (long)(rng() % (unsigned long long)have);
-    const long first = std::min(B, have - start);
-    V3D_HIP(hipMemcpyAsync(Zb_, Zs_ + start * nin_, sizeof(float) * first * nin_, hipMemcpyDeviceToDevice, st));
-    V3D_HIP(hipMemcpyAsync(Tb_, Ts_ + start * nin_, sizeof(float) * first * nin_, hipMemcpyDeviceToDevice, st));
-    if (first < B) {
-      V3D_HIP(hipMemcpyAsync(Zb_ + first * nin_, Zs_, sizeof(float) * (B - first) * nin_, hipMemcpyDeviceToDevice, st));
-      V3D_HIP(hipMemcpyAsync(Tb_ + first * nin_, Ts_, sizeof(float) * (B - first) * nin_, hipMemcpyDeviceToDevice, st));
+  if (!perm_ && hipMalloc(&perm_, sizeof(int) * cap_) != hipSuccess) return VES3D_MEMORY_ERROR;
+  if (!Pbest_ && hipMalloc(&Pbest_, sizeof(float) * np_) != hipSuccess) return VES3D_MEMORY_ERROR;
+  std::mt19937_64 rng(seed_ + 12345 + (unsigned long long)t_);
+  std::vector<int> perm(have);
+  for (long i = 0; i < have; ++i) perm[i] = (int)i;
+  const long per_epoch = std::max<long>(1, have / B);
+  const long total = epochs * per_epoch;
+  // Adam with shuffled minibatches and a cosine learning-rate decay; the
+  // parameters with the lowest loss over all recorded samples (evaluated
+  // after every epoch) are kept
+  double best = 0;
+  V3D_CHECK(dataset_loss(have, &best, nullptr, st));
+  V3D_HIP(hipMemcpyAsync(Pbest_, P_, sizeof(float) * np_, hipMemcpyDeviceToDevice, st));
+  long k = 0;
+  for (int ep = 0; ep < epochs; ++ep) {
+    std::shuffle(perm.begin(), perm.end(), rng);
+    V3D_HIP(hipStreamSynchronize(st));
+    V3D_HIP(hipMemcpy(perm_, perm.data(), sizeof(int) * have, hipMemcpyHostToDevice));
+    for (long b = 0; b < per_epoch; ++b, ++k) {
+      k_gather_batch<<<gb(B * nin_), 256, 0, st>>>(Zs_, Ts_, perm_ + b * B, B, nin_, Zb_, Tb_);
+      V3D_CHECK(launch_status());
+      V3D_CHECK(loss_grad(Zb_, Tb_, B, nullptr, st));
+      const double c = 0.5 * (1.0 + std::cos(M_PI * (double)k / (double)total));
+      V3D_CHECK(adam((float)(lr * c), st));
     }
-    V3D_CHECK(loss_grad(Zb_, Tb_, B, s + 1 == steps ? &loss : nullptr, st));
-    V3D_CHECK(adam(lr, st));
+    double l = 0;
+    V3D_CHECK(dataset_loss(have, &l, nullptr, st));
+    if (l < best) {
+      best = l;
+      V3D_HIP(hipMemcpyAsync(Pbest_, P_, sizeof(float) * np_, hipMemcpyDeviceToDevice, st));
+    }
   }
-  if (last_loss) *last_loss = loss;
+  V3D_HIP(hipMemcpyAsync(P_, Pbest_, sizeof(float) * np_, hipMemcpyDeviceToDevice, st));
+  if (last_loss) *last_loss = best;
   trained_ = true;
   return hip_status(hipStreamSynchronize(st));
 }

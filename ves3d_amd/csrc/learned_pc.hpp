@@ -32,7 +32,13 @@ class LearnedPC {
   // loss = sum((Y-T)^2) / (2 B nin); gradients into the internal G buffer
   int loss_grad(const float *Z, const float *T, long B, double *loss_host, hipStream_t st);
   int adam(float lr, hipStream_t st);
+  // Adam over shuffled minibatches with a cosine learning-rate decay; keeps
+  // the parameters with the lowest loss over all samples; *last_loss_host is
+  // that loss
   int train(int epochs, int batch, float lr, double *last_loss_host, hipStream_t st);
+  // loss over all recorded samples and the mean square of their targets
+  int dataset_loss(long have, double *loss_host, double *target_ms_host, hipStream_t st);
+  long stored() const { return count_ < cap_ ? count_ : cap_; }
 
   // ---- solver interface on the Galerkin (order-major) layout
   // y holds D r on entry and P(r) on exit; the correction leaves the tension
@@ -64,7 +70,10 @@ class LearnedPC {
   float *Zs_ = nullptr, *Ts_ = nullptr;                   // sample ring buffers
   float *A1_ = nullptr, *H1_ = nullptr, *A2_ = nullptr, *H2_ = nullptr, *Y_ = nullptr;
   float *dY_ = nullptr, *dA2_ = nullptr, *dA1_ = nullptr, *Zb_ = nullptr, *Tb_ = nullptr;
-  double *scale_ = nullptr, *lossbuf_ = nullptr;
+  double *scale_ = nullptr, *lossbuf_ = nullptr, *evalbuf_ = nullptr;
+  int *perm_ = nullptr;
+  float *Pbest_ = nullptr;
+  static constexpr int kEvalParts = 1024;
   int t_ = 0;
   bool trained_ = false;
   unsigned long long seed_ = 0;

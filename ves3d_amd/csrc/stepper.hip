@@ -772,13 +772,14 @@ struct Stepper {
     V3D_CHECK(true_residual(xsol, rhs, &true_rel));
     bool fell_back = false;
     if (learned && (status != 100 || !(true_rel <= prm.tol * 1.05))) {
-      // the learned preconditioner did not deliver: unpreconditioned
-      // BiCGStab restarts from the initial guess and its result stands
+      // the learned preconditioner did not deliver: BiCGStab restarts from
+      // the initial guess with the diagonal spectral preconditioner
+      // (ImplicitPrecond) and its result stands
       fell_back = true;
       V3D_CHECK(pack(U, sig, xsol));
       it = prm.max_iter;
       tol = prm.tol;
-      V3D_CHECK(bicgstab(n, A_cb, this, nullptr, nullptr, xsol, rhs, &it, &tol, &status, st));
+      V3D_CHECK(bicgstab(n, A_cb, this, Pdiag_cb, this, xsol, rhs, &it, &tol, &status, st));
       V3D_CHECK(true_residual(xsol, rhs, &true_rel));
     }
     auto t3 = clk::now();
@@ -1045,6 +1046,13 @@ int ves3d_hip_lpc_adam(void *h, float lr, ves3d_stream_t stream) {
 int ves3d_hip_lpc_train(void *h, int epochs, int batch, float lr, double *last_loss) {
   if (!h) return VES3D_INVALID_PARAMETER;
   return ((LearnedPC *)h)->train(epochs, batch, lr, last_loss, nullptr);
+}
+
+int ves3d_hip_lpc_dataset_loss(void *h, double *loss, double *target_ms) {
+  if (!h) return VES3D_INVALID_PARAMETER;
+  LearnedPC *l = (LearnedPC *)h;
+  if (l->stored() < 1) return VES3D_INVALID_PARAMETER;
+  return l->dataset_loss(l->stored(), loss, target_ms, nullptr);
 }
 
 }  // extern "C"

@@ -229,6 +229,13 @@ class LearnedPC:
         call("ves3d_hip_lpc_train", self.h, epochs, batch, ctypes.c_float(lr), ctypes.byref(loss))
         return loss.value
 
+    def dataset_loss(self):
+        """(loss over all recorded samples, mean square of their targets)."""
+        loss, tms = ctypes.c_double(), ctypes.c_double()
+        torch.cuda.synchronize()
+        call("ves3d_hip_lpc_dataset_loss", self.h, ctypes.byref(loss), ctypes.byref(tms))
+        return loss.value, tms.value
+
 
 def attach(stepper, pc, collect):
     call("ves3d_hip_stepper_attach_pc", stepper.h, pc.h if pc is not None else None,
