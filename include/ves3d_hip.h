@@ -499,7 +499,9 @@ int ves3d_hip_near_apply(void *handle, const double *q, const double *vup, const
  * z_v = (D r)_v the 4 p(p+2) coefficients of vesicle v, s_v = |z_v|, and
  * F_v its canonical frame (ves3d_hip_stepper_frames) acting on the three
  * velocity blocks:
- *     P(r)_v = z_v + s_v K F_v g(F_v^T z_v / s_v),
+ *     P(r)_v = z_v + s_v K F_v g_o(F_v^T z_v / s_v),
+ * g_o(z) = (g(z) - g(-z)) / 2 the odd part of the network, so that
+ * P(a r) = a P(r) for every real a,
  * K zeroes the tension coefficients of degree > filter_freq: Surface::div
  * is low-pass filtered at that degree (src/Surface.cc:372), so they span the
  * null space of the implicit operator, and keeping the correction out of them

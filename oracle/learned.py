@@ -13,8 +13,9 @@ ImplicitPrecond (src/InterfacialVelocity.cc:1054-1109).
   rows(imp, x)        the per-vesicle rows [nv][4 nc] of a Galerkin vector
                       (three velocity blocks, then tension; function-major
                       coefficient order inside each block)
-  precond(imp, ...)   P(r)_v = z_v + s_v K F_v g(F_v^T z_v / s_v), z = D r,
-                      s_v = |z_v|, K zeroing the tension coefficients of
+  precond(imp, ...)   P(r)_v = z_v + s_v K F_v g_o(F_v^T z_v / s_v), z = D r,
+                      s_v = |z_v|, g_o(z) = (g(z) - g(-z)) / 2 the odd part
+                      of the network, K zeroing the tension coefficients of
                       degree > the divergence filter (src/Surface.cc:372)
   samples(imp, ...)   the (input, target) rows recorded from (A_loc e, e)
 """
@@ -77,7 +78,7 @@ def precond(imp, params, x, F, tcut=None):
     z = rows(imp, imp.precond(x))
     s = np.sqrt(np.sum(z * z, axis=1)) + 1e-300
     zc = rotate_rows(z, F, nc, True) / s[:, None]
-    g = mlp.forward(params, zc)[0]
+    g = 0.5 * (mlp.forward(params, zc)[0] - mlp.forward(params, -zc)[0])
     g = rotate_rows(g, F, nc, False) * s[:, None]
     g[:, ~tension_gauge(p, tcut)] = 0.0
     return unrows(imp, z + g)

@@ -87,14 +87,27 @@ __global__ void k_colsum(const float *__restrict__ D, long B, int w, float *g) {
   }
 }
 // Adam (Kingma & Ba 2015, Algorithm 1)
-// minibatch rows Zb[i] = Zs[perm[i]], Tb[i] = Ts[perm[i]]
+// out = -in
+__global__ void k_negate(const float *__restrict__ in, long n, float *out) {
+  GL(i, n) out[i] = -in[i];
+}
+// Y = (Y - Yneg) / 2: the odd part g(z) - g(-z) of the network, so that the
+// preconditioner satisfies P(-r) = -P(r) (with the row scaling, P(a r) =
+// a P(r) for every real a)
+__global__ void k_odd_part(float *Y, const float *__restrict__ Yneg, long n) {
+  GL(i, n) Y[i] = 0.5f * (Y[i] - Yneg[i]);
+}
+// minibatch rows Zb[i] = sgn Zs[perm[i]], Tb[i] = sgn Ts[perm[i]], with
+// sgn = -1 on odd batch rows: the samples enter with both signs, as the
+// target map is odd
 __global__ void k_gather_batch(const float *__restrict__ Zs, const float *__restrict__ Ts,
                                const int *__restrict__ perm, long B, int nin, float *Zb, float *Tb) {
   GL(i, B * nin) {
     const long r = i / nin;
     const long src = (long)perm[r] * nin + (i % nin);
-    Zb[i] = Zs[src];
-    Tb[i] = Ts[src];
+    const float sg = (r & 1) ? -1.0f : 1.0f;
+    Zb[i] = sg * Zs[src];
+    Tb[i] = sg * Ts[src];
   }
 }
 // partial sums of (Y - T)^2 and T^2 per block into part[2 * block + {0, 1}]
@@ -406,7 +419,7 @@ int LearnedPC::dataset_loss(long have, double *loss_host, double *target_ms_host
   V3D_HIP(hipMemsetAsync(acc, 0, 2 * sizeof(double), st));
   for (long r0 = 0; r0 < have; r0 += B) {
     const long cnt = std::min(B, have - r0);
-    V3D_CHECK(forward(Zs_ + r0 * nin_, cnt, Y_, st));
+    V3D_CHECK(odd_forward(Zs_ + r0 * nin_, cnt, Y_, st));
     const long tot = cnt * nin_;
     const int nb = (int)std::min<long>(kEvalParts, std::max<long>(1, (tot + 255) / 256));
     k_sq_sums<<<nb, 256, 0, st>>>(Y_, Ts_ + r0 * nin_, tot, part);
@@ -475,9 +488,20 @@ int LearnedPC::apply_inplace(double *y, long nv, int p, int tcut, const int2 *ci
   V3D_CHECK(launch_status());
   k_gather_rows<<<gb(nv * nin_), 256, 0, st>>>(y, nv, nc, cinfo, scale_, frames, Zb_);
   V3D_CHECK(launch_status());
-  V3D_CHECK(forward(Zb_, nv, Y_, st));
+  V3D_CHECK(odd_forward(Zb_, nv, Y_, st));
   k_scatter_add<<<gb(nv * nin_), 256, 0, st>>>(Y_, nv, nc, p, tcut, cinfo, scale_, frames,
                                                 y);
+  return launch_status();
+}
+
+int LearnedPC::odd_forward(const float *Z, long B, float *Y, hipStream_t st) {
+  V3D_CHECK(ensure_batch(B));
+  const long n = B * nin_;
+  V3D_CHECK(forward(Z, B, Y, st));
+  k_negate<<<gb(n), 256, 0, st>>>(Z, n, Tb_);
+  V3D_CHECK(launch_status());
+  V3D_CHECK(forward(Tb_, B, dY_, st));
+  k_odd_part<<<gb(n), 256, 0, st>>>(Y, dY_, n);
   return launch_status();
 }
 

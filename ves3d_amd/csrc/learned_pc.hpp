@@ -36,7 +36,10 @@ class LearnedPC {
   // the parameters with the lowest loss over all samples; *last_loss_host is
   // that loss
   int train(int epochs, int batch, float lr, double *last_loss_host, hipStream_t st);
-  // loss over all recorded samples and the mean square of their targets
+  // Y = (g(Z) - g(-Z)) / 2, the odd part of the network (what the solver applies)
+  int odd_forward(const float *Z, long B, float *Y, hipStream_t st);
+  // loss of the odd part over all recorded samples and the mean square of
+  // their targets
   int dataset_loss(long have, double *loss_host, double *target_ms_host, hipStream_t st);
   long stored() const { return count_ < cap_ ? count_ : cap_; }
 

@@ -437,6 +437,7 @@ struct Stepper {
   std::vector<double> h_a0, h_v0;
   bool have_av0 = false;
   int rep_iters = 0, av_iters = 0;
+  int last_iters = 0;              // iterations of the last converged solve
 
   ~Stepper() {
     if (cinfo) (void)hipFree(cinfo);
@@ -766,6 +767,12 @@ struct Stepper {
     const bool learned = prm.precond == 2 && lpc && lpc->ready();
     if (prm.precond == 1 || (prm.precond == 2 && !learned)) P = Pdiag_cb;
     if (learned) P = Plearn_cb;
+    // a learned solve gets an iteration budget tied to the last solve's count
+    // (a third of max_iter before there is one), so that a preconditioner
+    // that does not deliver costs little before the fallback below
+    if (learned)
+      it = last_iters > 0 ? std::min(prm.max_iter, 2 * last_iters + 10)
+                          : std::max(10, prm.max_iter / 3);
     V3D_CHECK(bicgstab(n, A_cb, this, P, this, xsol, rhs, &it, &tol, &status, st));
     // true residual of the returned solution (src/InterfacialVelocity.cc:181-188)
     double true_rel = 0;
@@ -782,6 +789,7 @@ struct Stepper {
       V3D_CHECK(bicgstab(n, A_cb, this, Pdiag_cb, this, xsol, rhs, &it, &tol, &status, st));
       V3D_CHECK(true_residual(xsol, rhs, &true_rel));
     }
+    if (status == 100) last_iters = it;
     auto t3 = clk::now();
     V3D_CHECK(unpack(xsol, U, sig));
     V3D_CHECK(dev_axpby(3 * nv * ng, prm.dt, U, 1.0, X, st));  // dx = dt u
