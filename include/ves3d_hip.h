@@ -532,13 +532,13 @@ int ves3d_hip_lpc_loss_grad(void *handle, const float *Z, const float *T, size_t
                             double *loss_host, float *grads, ves3d_stream_t stream);
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with the last gradient. */
 int ves3d_hip_lpc_adam(void *handle, float lr, ves3d_stream_t stream);
-/* epochs passes over the recorded samples in shuffled minibatches of
- * `batch`, Adam with a cosine decay of lr; keeps the parameters with the
- * lowest loss over all samples (evaluated after every epoch) and returns
- * that loss. */
+/* epochs passes over nine tenths of the recorded samples in shuffled
+ * minibatches of `batch` (each sample enters with both signs), Adam with a
+ * cosine decay of lr; keeps the parameters with the lowest loss on the
+ * held-out tenth (evaluated after every epoch) and returns that loss. */
 int ves3d_hip_lpc_train(void *handle, int epochs, int batch, float lr, double *last_loss_host);
-/* Loss over all recorded samples and the mean square of their targets (their
- * ratio is the network's relative fit). */
+/* Loss of the odd part of the network over all recorded samples and the mean
+ * square of their targets (their ratio is the network's relative fit). */
 int ves3d_hip_lpc_dataset_loss(void *handle, double *loss_host, double *target_ms_host);
 /* Attaches a learned preconditioner to a stepper (precond = 2 uses it once
  * it has been trained or given parameters); collect != 0 records training

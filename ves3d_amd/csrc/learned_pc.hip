@@ -268,12 +268,14 @@ __global__ void k_record(const double *__restrict__ Dr, const double *__restrict
 }  // namespace
 
 LearnedPC::~LearnedPC() {
-  for (float *p : {P_, G_, M_, V_, Zs_, Ts_, A1_, H1_, A2_, H2_, Y_, dY_, dA2_, dA1_, Zb_, Tb_})
+  for (float *p : {P_, G_, M_, V_, Zs_, Ts_, A1_, H1_, A2_, H2_, Y_, dY_, dA2_, dA1_, Zb_, Tb_,
+                   Zneg_, Yneg_})
     if (p) (void)hipFree(p);
   if (scale_) (void)hipFree(scale_);
   if (lossbuf_) (void)hipFree(lossbuf_);
   if (evalbuf_) (void)hipFree(evalbuf_);
   if (perm_) (void)hipFree(perm_);
+  if (vidx_) (void)hipFree(vidx_);
   if (Pbest_) (void)hipFree(Pbest_);
   for (auto &kv : descs_) (void)hipFree(kv.second);
 }
@@ -314,14 +316,14 @@ int LearnedPC::init(int nin, int width, long capacity, unsigned long long seed) 
 
 int LearnedPC::ensure_batch(long B) {
   if (B <= bcap_) return VES3D_SUCCESS;
-  for (float *p : {A1_, H1_, A2_, H2_, Y_, dY_, dA2_, dA1_, Zb_, Tb_})
+  for (float *p : {A1_, H1_, A2_, H2_, Y_, dY_, dA2_, dA1_, Zb_, Tb_, Zneg_, Yneg_})
     if (p) (void)hipFree(p);
   if (scale_) (void)hipFree(scale_);
   const long w = width_, n = nin_;
   float **wb[] = {&A1_, &H1_, &A2_, &H2_, &dA2_, &dA1_};
   for (float **p : wb)
     if (hipMalloc(p, sizeof(float) * B * w) != hipSuccess) return VES3D_MEMORY_ERROR;
-  float **nb[] = {&Y_, &dY_, &Zb_, &Tb_};
+  float **nb[] = {&Y_, &dY_, &Zb_, &Tb_, &Zneg_, &Yneg_};
   for (float **p : nb)
     if (hipMalloc(p, sizeof(float) * B * n) != hipSuccess) return VES3D_MEMORY_ERROR;
   if (hipMalloc(&scale_, sizeof(double) * B) != hipSuccess) return VES3D_MEMORY_ERROR;
@@ -411,18 +413,20 @@ int LearnedPC::adam(float lr, hipStream_t st) {
   return launch_status();
 }
 
-int LearnedPC::dataset_loss(long have, double *loss_host, double *target_ms_host,
-                             hipStream_t st) {
-  const long B = std::min<long>(std::max<long>(bcap_, 1), have);
+int LearnedPC::subset_loss(const int *idx, long m, double *loss_host, double *target_ms_host,
+                            hipStream_t st) {
+  const long B = std::min<long>(std::max<long>(bcap_, 1), m);
   V3D_CHECK(ensure_batch(B));
   double *acc = evalbuf_, *part = evalbuf_ + 2;
   V3D_HIP(hipMemsetAsync(acc, 0, 2 * sizeof(double), st));
-  for (long r0 = 0; r0 < have; r0 += B) {
-    const long cnt = std::min(B, have - r0);
-    V3D_CHECK(odd_forward(Zs_ + r0 * nin_, cnt, Y_, st));
+  for (long r0 = 0; r0 < m; r0 += B) {
+    const long cnt = std::min(B, m - r0);
+    k_gather_batch<<<gb(cnt * nin_), 256, 0, st>>>(Zs_, Ts_, idx + r0, cnt, nin_, Zb_, Tb_);
+    V3D_CHECK(launch_status());
+    V3D_CHECK(odd_forward(Zb_, cnt, Y_, st));
     const long tot = cnt * nin_;
     const int nb = (int)std::min<long>(kEvalParts, std::max<long>(1, (tot + 255) / 256));
-    k_sq_sums<<<nb, 256, 0, st>>>(Y_, Ts_ + r0 * nin_, tot, part);
+    k_sq_sums<<<nb, 256, 0, st>>>(Y_, Tb_, tot, part);
     V3D_CHECK(launch_status());
     k_acc_pairs<<<1, 64, 0, st>>>(part, nb, acc);
     V3D_CHECK(launch_status());
@@ -430,35 +434,52 @@ int LearnedPC::dataset_loss(long have, double *loss_host, double *target_ms_host
   double h[2];
   V3D_HIP(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, st));
   V3D_HIP(hipStreamSynchronize(st));
-  const double n = (double)have * nin_;
+  const double n = (double)m * nin_;
   if (loss_host) *loss_host = 0.5 * h[0] / n;
   if (target_ms_host) *target_ms_host = h[1] / n;
   return VES3D_SUCCESS;
 }
 
+int LearnedPC::dataset_loss(long have, double *loss_host, double *target_ms_host,
+                            hipStream_t st) {
+  if (!perm_ && hipMalloc(&perm_, sizeof(int) * cap_) != hipSuccess) return VES3D_MEMORY_ERROR;
+  std::vector<int> all(have);
+  for (long i = 0; i < have; ++i) all[i] = (int)i;
+  V3D_HIP(hipStreamSynchronize(st));
+  V3D_HIP(hipMemcpy(perm_, all.data(), sizeof(int) * have, hipMemcpyHostToDevice));
+  return subset_loss(perm_, have, loss_host, target_ms_host, st);
+}
+
 int LearnedPC::train(int epochs, int batch, float lr, double *last_loss, hipStream_t st) {
   const long have = std::min(count_, cap_);
-  if (have < 1 || batch < 1 || epochs < 1) return VES3D_INVALID_PARAMETER;
-  const long B = std::min<long>(batch, have);
+  if (have < 2 || batch < 1 || epochs < 1) return VES3D_INVALID_PARAMETER;
+  // a fixed tenth of the samples is held out; the kept parameters are those
+  // with the lowest held-out loss (evaluated after every epoch)
+  const long nval = std::max<long>(1, have / 10), ntr = have - nval;
+  const long B = std::min<long>(batch, ntr);
   V3D_CHECK(ensure_batch(B));
   if (!perm_ && hipMalloc(&perm_, sizeof(int) * cap_) != hipSuccess) return VES3D_MEMORY_ERROR;
+  if (!vidx_ && hipMalloc(&vidx_, sizeof(int) * cap_) != hipSuccess) return VES3D_MEMORY_ERROR;
   if (!Pbest_ && hipMalloc(&Pbest_, sizeof(float) * np_) != hipSuccess) return VES3D_MEMORY_ERROR;
+  std::mt19937_64 split(seed_ + 777);
+  std::vector<int> all(have);
+  for (long i = 0; i < have; ++i) all[i] = (int)i;
+  std::shuffle(all.begin(), all.end(), split);
+  std::vector<int> tr(all.begin() + nval, all.end());
+  V3D_HIP(hipStreamSynchronize(st));
+  V3D_HIP(hipMemcpy(vidx_, all.data(), sizeof(int) * nval, hipMemcpyHostToDevice));
   std::mt19937_64 rng(seed_ + 12345 + (unsigned long long)t_);
-  std::vector<int> perm(have);
-  for (long i = 0; i < have; ++i) perm[i] = (int)i;
-  const long per_epoch = std::max<long>(1, have / B);
+  const long per_epoch = std::max<long>(1, ntr / B);
   const long total = epochs * per_epoch;
-  // Adam with shuffled minibatches and a cosine learning-rate decay; the
-  // parameters with the lowest loss over all recorded samples (evaluated
-  // after every epoch) are kept
+  // Adam over shuffled minibatches with a cosine learning-rate decay
   double best = 0;
-  V3D_CHECK(dataset_loss(have, &best, nullptr, st));
+  V3D_CHECK(subset_loss(vidx_, nval, &best, nullptr, st));
   V3D_HIP(hipMemcpyAsync(Pbest_, P_, sizeof(float) * np_, hipMemcpyDeviceToDevice, st));
   long k = 0;
   for (int ep = 0; ep < epochs; ++ep) {
-    std::shuffle(perm.begin(), perm.end(), rng);
+    std::shuffle(tr.begin(), tr.end(), rng);
     V3D_HIP(hipStreamSynchronize(st));
-    V3D_HIP(hipMemcpy(perm_, perm.data(), sizeof(int) * have, hipMemcpyHostToDevice));
+    V3D_HIP(hipMemcpy(perm_, tr.data(), sizeof(int) * ntr, hipMemcpyHostToDevice));
     for (long b = 0; b < per_epoch; ++b, ++k) {
       k_gather_batch<<<gb(B * nin_), 256, 0, st>>>(Zs_, Ts_, perm_ + b * B, B, nin_, Zb_, Tb_);
       V3D_CHECK(launch_status());
@@ -467,9 +488,10 @@ int LearnedPC::train(int epochs, int batch, float lr, double *last_loss, hipStre
       V3D_CHECK(adam((float)(lr * c), st));
     }
     double l = 0;
-    V3D_CHECK(dataset_loss(have, &l, nullptr, st));
+    V3D_CHECK(subset_loss(vidx_, nval, &l, nullptr, st));
     if (l < best) {
       best = l;
+      best_epoch_ = ep + 1;
       V3D_HIP(hipMemcpyAsync(Pbest_, P_, sizeof(float) * np_, hipMemcpyDeviceToDevice, st));
     }
   }
@@ -498,10 +520,10 @@ int LearnedPC::odd_forward(const float *Z, long B, float *Y, hipStream_t st) {
   V3D_CHECK(ensure_batch(B));
   const long n = B * nin_;
   V3D_CHECK(forward(Z, B, Y, st));
-  k_negate<<<gb(n), 256, 0, st>>>(Z, n, Tb_);
+  k_negate<<<gb(n), 256, 0, st>>>(Z, n, Zneg_);
   V3D_CHECK(launch_status());
-  V3D_CHECK(forward(Tb_, B, dY_, st));
-  k_odd_part<<<gb(n), 256, 0, st>>>(Y, dY_, n);
+  V3D_CHECK(forward(Zneg_, B, Yneg_, st));
+  k_odd_part<<<gb(n), 256, 0, st>>>(Y, Yneg_, n);
   return launch_status();
 }
 

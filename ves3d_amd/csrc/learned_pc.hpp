@@ -32,15 +32,19 @@ class LearnedPC {
   // loss = sum((Y-T)^2) / (2 B nin); gradients into the internal G buffer
   int loss_grad(const float *Z, const float *T, long B, double *loss_host, hipStream_t st);
   int adam(float lr, hipStream_t st);
-  // Adam over shuffled minibatches with a cosine learning-rate decay; keeps
-  // the parameters with the lowest loss over all samples; *last_loss_host is
-  // that loss
+  // Adam over shuffled minibatches with a cosine learning-rate decay on nine
+  // tenths of the samples; keeps the parameters with the lowest loss on the
+  // held-out tenth; *last_loss_host is that loss
   int train(int epochs, int batch, float lr, double *last_loss_host, hipStream_t st);
   // Y = (g(Z) - g(-Z)) / 2, the odd part of the network (what the solver applies)
   int odd_forward(const float *Z, long B, float *Y, hipStream_t st);
   // loss of the odd part over all recorded samples and the mean square of
   // their targets
   int dataset_loss(long have, double *loss_host, double *target_ms_host, hipStream_t st);
+  // the same over the samples idx[0..m) (device indices)
+  int subset_loss(const int *idx, long m, double *loss_host, double *target_ms_host,
+                  hipStream_t st);
+  int best_epoch() const { return best_epoch_; }
   long stored() const { return count_ < cap_ ? count_ : cap_; }
 
   // ---- solver interface on the Galerkin (order-major) layout
@@ -74,7 +78,9 @@ class LearnedPC {
   float *A1_ = nullptr, *H1_ = nullptr, *A2_ = nullptr, *H2_ = nullptr, *Y_ = nullptr;
   float *dY_ = nullptr, *dA2_ = nullptr, *dA1_ = nullptr, *Zb_ = nullptr, *Tb_ = nullptr;
   double *scale_ = nullptr, *lossbuf_ = nullptr, *evalbuf_ = nullptr;
-  int *perm_ = nullptr;
+  int *perm_ = nullptr, *vidx_ = nullptr;
+  float *Zneg_ = nullptr, *Yneg_ = nullptr;
+  int best_epoch_ = 0;
   float *Pbest_ = nullptr;
   static constexpr int kEvalParts = 1024;
   int t_ = 0;
