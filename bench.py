@@ -385,7 +385,7 @@ def main():
     ap.add_argument("--max-iter", type=int, default=300)
     ap.add_argument("--width", type=int, default=256)
     ap.add_argument("--capacity", type=int, default=200000)
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--train-nves", type=int, default=256)
     ap.add_argument("--train-steps", type=int, default=2)
     ap.add_argument("--ref-tol", type=float, default=1e-10)
