@@ -40,7 +40,13 @@ sys.path.insert(0, ROOT)
 METRIC = "ms per implicit step @512 ves p=16 (learned-PC+mixed vs plain) at 1e-6"
 PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 vector / matrix
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector (published; counts packed FP32)
+PEAK_FP32_NONPACKED_TFLOPS = 78.6   # the same without packed FP32 (disabled in this build)
 SL_FLOP_PER_PAIR = 28      # src/CPUKernels.cc:1008
+# HBM bytes per launch of k_far_groups_f32c at 512 vesicles, p = 16, from two
+# rocprofv3 --pmc runs (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a
+# 512 MiB device copy gives the two counters in the same runs)
+FAR32_TRAFFIC_BYTES = 2.0 * 344539.1 * 1024 + 52224.0 * 1024
+FAR32_TRAFFIC_SOURCE = "profiles/k_far_groups_f32c_hbm_counters_512ves_p16.md"
 DL_FLOP_PER_PAIR = 31      # src/PVFMMInterface.cc:365 (no Newton steps)
 
 
@@ -145,8 +151,13 @@ def bench_step(args):
     far_ms = sum(s["ms_far"] for s in stats_mixed) / max(1, sum(s["far_calls"] for s in stats_mixed))
     far_ms_plain = sum(s["ms_far"] for s in stats_plain) / max(1, sum(s["far_calls"] for s in stats_plain))
     achieved = flop / (far_ms * 1e-3) / 1e12
+    scalar = args.far_kernel == "scalar" and args.nves == 512 and args.p == 16
     roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": FAR32_TRAFFIC_BYTES if scalar else None,
+            "traffic_source": FAR32_TRAFFIC_SOURCE if scalar else None,
+            "peak_nonpacked": PEAK_FP32_NONPACKED_TFLOPS,
+            "frac_nonpacked": achieved / PEAK_FP32_NONPACKED_TFLOPS,
             "kernel": ("k_far_groups_f32c" if args.far_kernel == "scalar" else "k_far_f32c<SL>")
             + " (+k_pack_far32, k_sum_slices)", "kernel_ms": far_ms,
             "flop_per_launch": flop, "pairs_per_launch": pairs,
