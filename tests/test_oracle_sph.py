@@ -10,7 +10,8 @@
   rotation     : per-degree blocks preserve the weighted energy
                  (Mr D Mr^T = D, D = 2 for m = 0 else 1).
   singular op  : unit-sphere closed forms  SL[c] = (2/3) c,  SL[n] = 0,
-                 DL[c] = c/2 (this grid's normal orientation), and spectral
+                 DL[c] = -c/2 (the grid's outward normal, the reference's
+                 orientation), and spectral
                  convergence on a perturbed shape.
 """
 import os
