@@ -47,6 +47,8 @@ SL_FLOP_PER_PAIR = 28      # src/CPUKernels.cc:1008
 # 512 MiB device copy gives the two counters in the same runs)
 FAR32_TRAFFIC_BYTES = 2.0 * 344539.1 * 1024 + 52224.0 * 1024
 FAR32_TRAFFIC_SOURCE = "profiles/k_far_groups_f32c_hbm_counters_512ves_p16.md"
+FAR_KINDS = {"lds": 0, "scalar": 1, "scalar2": 2}
+FAR32_NAMES = {"lds": "k_far_f32c<SL>", "scalar": "k_far_groups_f32c", "scalar2": "k_far_groups_f32c2"}
 DL_FLOP_PER_PAIR = 31      # src/PVFMMInterface.cc:365 (no Newton steps)
 
 
@@ -66,7 +68,7 @@ def bench_step(args):
     from ves3d_amd import workloads as W
 
     ves3d_amd.init(0)
-    ves3d_amd.call("ves3d_hip_set_far_kernel", 1 if args.far_kernel == "scalar" else 0)
+    ves3d_amd.call("ves3d_hip_set_far_kernel", FAR_KINDS[args.far_kernel])
     nv, p = args.nves, args.p
     X0 = W.implicit_step_case(nv=nv, p=p)
     Xg = torch.from_numpy(X0).cuda()
@@ -158,10 +160,10 @@ def bench_step(args):
             "traffic_source": FAR32_TRAFFIC_SOURCE if scalar else None,
             "peak_nonpacked": PEAK_FP32_NONPACKED_TFLOPS,
             "frac_nonpacked": achieved / PEAK_FP32_NONPACKED_TFLOPS,
-            "kernel": ("k_far_groups_f32c" if args.far_kernel == "scalar" else "k_far_f32c<SL>")
-            + " (+k_pack_far32, k_sum_slices)", "kernel_ms": far_ms,
+            "kernel": FAR32_NAMES[args.far_kernel] + " (+k_pack_far32, k_sum_slices)",
+            "kernel_ms": far_ms,
             "flop_per_launch": flop, "pairs_per_launch": pairs,
-            "fp64_form": {"kernel": "k_far_groups_f64" if args.far_kernel == "scalar" else "k_far_f64<SL>",
+            "fp64_form": {"kernel": "k_far_f64<SL>" if args.far_kernel == "lds" else "k_far_groups_f64",
                           "kernel_ms": far_ms_plain,
                           "achieved": flop / (far_ms_plain * 1e-3) / 1e12,
                           "peak": PEAK_FP64_TFLOPS}}
@@ -285,7 +287,7 @@ def bench_matvec(args):
     from ves3d_amd import workloads as W
 
     ves3d_amd.init(0)
-    ves3d_amd.call("ves3d_hip_set_far_kernel", 1 if args.far_kernel == "scalar" else 0)
+    ves3d_amd.call("ves3d_hip_set_far_kernel", FAR_KINDS[args.far_kernel])
     nv, p = args.nves, args.p
     prec = args.mixed_precision if args.dtype == "f32c" else 0
     st = I.Stepper(nv, p, dt=1e-3, shear_rate=1.0, precision=prec, precond=1, r_near=args.r_near)
@@ -405,8 +407,9 @@ def main():
     ap.add_argument("--mixed-precision", type=int, default=2, choices=[1, 2],
                     help="precision of the learned form: 1 FP32 singular operator + FP32-"
                          "compensated far field, 2 FP64 singular operator + FP32-compensated far field")
-    ap.add_argument("--far-kernel", default="scalar", choices=["scalar", "lds"],
-                    help="far field between vesicles: sources through the scalar cache or LDS tiles")
+    ap.add_argument("--far-kernel", default="scalar", choices=sorted(FAR_KINDS),
+                    help="far field between vesicles: sources through the scalar cache (scalar2: "
+                         "two targets per lane in FP32) or LDS tiles")
     ap.add_argument("--r-near", type=float, default=0.1,
                     help="near-singular zone radius (0: the reference's r_ves 2/sqrt(p_up))")
     args = ap.parse_args()

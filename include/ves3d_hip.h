@@ -132,8 +132,9 @@ int ves3d_hip_set_gemm_kernel(int kind);
 /* Selects the kernel of the far field between vesicles inside the implicit
  * step (StokesVelocity's PVFMMEval call, src/StokesVelocity.cc:481-485):
  * 0 = sources staged through LDS tiles, 1 = sources read through the scalar
- * cache (the default).  Both give the same sums up to rounding order.  No
- * counterpart in the reference. */
+ * cache (the default), 2 = as 1 with two targets per lane in the FP32 form.
+ * All give the same sums up to rounding order.  No counterpart in the
+ * reference. */
 int ves3d_hip_set_far_kernel(int kind);
 
 /* ---- direct Stokes sums ------------------------------------------------ */

@@ -35,10 +35,11 @@ def _case(nv, p, seed, volfrac=0.05):
     return W.suspension(shapes, volfrac, seed=seed)
 
 
-@pytest.fixture(params=[1, 0], ids=["far-scalar", "far-lds"])
+@pytest.fixture(params=[1, 0, 2], ids=["far-scalar", "far-lds", "far-scalar2"])
 def far_kernel(request, hip):
-    """Both kernels of the far field between vesicles (scalar-cache source
-    records, LDS-staged source tiles)."""
+    """The kernels of the far field between vesicles (scalar-cache source
+    records; LDS-staged source tiles; scalar-cache records with two targets
+    per lane in the FP32 form)."""
     from ves3d_amd import call
     call("ves3d_hip_set_far_kernel", request.param)
     yield request.param

@@ -508,8 +508,95 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c(
   }
 }
 
+// Two targets per lane (t and t + FF_BLOCK): each source record loaded
+// through the scalar cache serves two pairs.  A target must not see its own
+// group's sources; instead of a lane-divergent skip (which would move the
+// source addressing out of SGPRs), its r^2 carries a bias of 1e30 for that
+// group, folded into the fma chain at no cost: each such source then adds
+// about 1e-15 |f| (including the coincident one, without a NaN), some 1e-14
+// of the sum and far below FP32 resolution.
+struct Acc32 {
+  float h[3], l[3];
+};
+__device__ __forceinline__ void acc_merge(Acc32 &a, float x, float y, float z) {
+  float s_, e_;
+  two_sum(a.h[0], x, s_, e_); a.h[0] = s_; a.l[0] += e_;
+  two_sum(a.h[1], y, s_, e_); a.h[1] = s_; a.l[1] += e_;
+  two_sum(a.h[2], z, s_, e_); a.h[2] = s_; a.l[2] += e_;
+}
+constexpr float kOwnGroupBias = 1e30f;
+
+__global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c2(
+    const FarRec32 *__restrict__ rec, const double *__restrict__ spos, long spg, long n_groups,
+    const double *__restrict__ tpos, long n_trg, long tpg, int nsplit, double csl,
+    double *__restrict__ part) {
+  const long slice = blockIdx.x % nsplit;
+  const long tb = blockIdx.x / nsplit;
+  const long ta = tb * (2 * FF_BLOCK) + threadIdx.x, tbb = ta + FF_BLOCK;
+  const bool va = ta < n_trg, vb = tbb < n_trg;
+  const long ca = va ? ta : n_trg - 1, cb = vb ? tbb : n_trg - 1;
+  const double Ax = tpos[3 * ca], Ay = tpos[3 * ca + 1], Az = tpos[3 * ca + 2];
+  const double Bx = tpos[3 * cb], By = tpos[3 * cb + 1], Bz = tpos[3 * cb + 2];
+  const long ga = ca / tpg, gbb = cb / tpg;
+  const long tfirst = tb * (2 * FF_BLOCK);
+  const long tlast = min(tfirst + 2 * FF_BLOCK, n_trg) - 1;
+  Acc32 A = {{0, 0, 0}, {0, 0, 0}}, B = {{0, 0, 0}, {0, 0, 0}};
+  long g0, g1;
+  slice_range(slice, nsplit, 0, spg, n_groups, g0, g1);
+  for (long g = g0; g < g1; ++g) {
+    if (tfirst / tpg == g && tlast / tpg == g) continue;  // block-uniform skip
+    const double *a = spos + 3 * g * spg;
+    const float ax = (float)(Ax - a[0]), ay = (float)(Ay - a[1]), az = (float)(Az - a[2]);
+    const float bx = (float)(Bx - a[0]), by = (float)(By - a[1]), bz = (float)(Bz - a[2]);
+    const FarRec32 *__restrict__ r = rec + g * spg;
+    const float bias_a = ga == g ? kOwnGroupBias : 0.0f;
+    const float bias_b = gbb == g ? kOwnGroupBias : 0.0f;
+    for (long j0 = 0; j0 < spg; j0 += FF_SUB) {
+      const long j1 = min(spg, j0 + FF_SUB);
+      float ux = 0, uy = 0, uz = 0, vx = 0, vy = 0, vz = 0;
+#pragma unroll 4
+      for (long j = j0; j < j1; ++j) {
+        const FarRec32 s = r[j];
+        {
+          const float dx = ax - s.x, dy = ay - s.y, dz = az - s.z;
+          const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, bias_a)));
+          const float ri = __frsqrt_rn(r2);
+          const float ri2 = ri * ri;
+          const float ip = (s.fx * dx + s.fy * dy + s.fz * dz) * ri2;
+          ux += ri * (s.fx + dx * ip);
+          uy += ri * (s.fy + dy * ip);
+          uz += ri * (s.fz + dz * ip);
+        }
+        {
+          const float dx = bx - s.x, dy = by - s.y, dz = bz - s.z;
+          const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, bias_b)));
+          const float ri = __frsqrt_rn(r2);
+          const float ri2 = ri * ri;
+          const float ip = (s.fx * dx + s.fy * dy + s.fz * dz) * ri2;
+          vx += ri * (s.fx + dx * ip);
+          vy += ri * (s.fy + dy * ip);
+          vz += ri * (s.fz + dz * ip);
+        }
+      }
+      acc_merge(A, ux, uy, uz);
+      acc_merge(B, vx, vy, vz);
+    }
+  }
+  if (va) {
+    double *o = part + (slice * n_trg + ta) * 3;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = csl * ((double)A.h[c] + (double)A.l[c]);
+  }
+  if (vb) {
+    double *o = part + (slice * n_trg + tbb) * 3;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = csl * ((double)B.h[c] + (double)B.l[c]);
+  }
+}
+
 // 0: LDS-staged source tiles (k_far_*), 1: scalar-cache source records
-// (k_far_groups_*); selects the far field between groups.
+// (k_far_groups_*), 2: as 1 with two targets per lane in the FP32 form;
+// selects the far field between groups.
 static std::atomic<int> g_far_kind{1};
 
 // ---------------------------------------------------------------------------
@@ -633,8 +720,16 @@ int far_field_between_groups(bool f32c, const double *spos, const double *sld, s
     k_pack_far32<<<cdiv((long)n_src, 256), 256, 0, st>>>(spos, sld, (long)n_src, (long)spg,
                                                           (FarRec32 *)rec);
     V3D_CHECK(launch_status());
-    k_far_groups_f32c<<<grid, FF_BLOCK, 0, st>>>((const FarRec32 *)rec, spos, (long)spg, n_groups,
-                                                 tpos, (long)n_trg, (long)tpg, ns, csl, part);
+    if (g_far_kind.load() == 2) {
+      const long tb2 = ((long)n_trg + 2 * FF_BLOCK - 1) / (2 * FF_BLOCK);
+      k_far_groups_f32c2<<<dim3((unsigned)(tb2 * ns)), FF_BLOCK, 0, st>>>(
+          (const FarRec32 *)rec, spos, (long)spg, n_groups, tpos, (long)n_trg, (long)tpg, ns, csl,
+          part);
+    } else {
+      k_far_groups_f32c<<<grid, FF_BLOCK, 0, st>>>((const FarRec32 *)rec, spos, (long)spg,
+                                                   n_groups, tpos, (long)n_trg, (long)tpg, ns, csl,
+                                                   part);
+    }
   } else {
     k_pack_far64<<<cdiv((long)n_src, 256), 256, 0, st>>>(spos, sld, (long)n_src, (FarRec64 *)rec);
     V3D_CHECK(launch_status());
@@ -702,7 +797,7 @@ int ves3d_hip_stokes_all2all_f32c(const double *src, const double *den,
 }
 
 int ves3d_hip_set_far_kernel(int kind) {
-  if (kind != 0 && kind != 1) return VES3D_INVALID_PARAMETER;
+  if (kind < 0 || kind > 2) return VES3D_INVALID_PARAMETER;
   g_far_kind.store(kind);
   return VES3D_SUCCESS;
 }
