@@ -40,7 +40,9 @@ enum {
   VES3D_INVALID_PARAMETER = 10,
   VES3D_INVALID_DEVICE = 20,
   VES3D_ACCURACY_ERROR = 30,
-  VES3D_DIVERGENCE_ERROR = 31
+  VES3D_DIVERGENCE_ERROR = 31,
+  VES3D_IO_ERROR = 50,
+  VES3D_IO_BAD_STREAM = 51
 };
 
 /* ---- device and memory (Device<DT> ctor, Malloc/Free/Memcpy/Memset:
@@ -424,6 +426,50 @@ int ves3d_hip_stepper_set_area_volume(void *handle, const double *area_host,
  * of the file, '#' starting a comment to the end of its line.  With
  * out_host == NULL only counts. */
 int ves3d_read_ascii(const char *path, double *out_host, size_t capacity, size_t *count_host);
+/* ---- checkpoints (host only) ---------------------------------------------
+ * The reference's ASCII checkpoint: Monitor writes Parameters::pack then
+ * EvolveSurface::pack (src/Monitor.cc:58-78), Simulation reads them back to
+ * restart (src/ves3d_simulation.cc:119-123, :237-249).  The parameters block
+ * (Parameters::pack, src/Parameters.cc:424-470) as a struct; string-valued
+ * parameters (file names) are written empty and skipped on reading. */
+typedef struct {
+  int version;                 /* "version:" of the blocks; >= 593 is the current layout */
+  int n_surfs, sh_order, filter_freq, upsample_freq;
+  double bending_modulus, viscosity_contrast, time_horizon, ts, time_tol;
+  int time_iter_max, time_adaptive, solve_for_velocity, pseudospectral;
+  char scheme[32], time_precond[32], bg_flow[32], singular_stokes[32], rep_type[32];
+  int rep_maxit, rep_upsample, rep_filter_freq;
+  double rep_ts, rep_tol, rep_exponent, repul_dist, bg_flow_param, periodic_length;
+  int interaction_upsample, checkpoint;
+  double checkpoint_stride, error_factor;
+  int num_threads;
+  double excess_density, gravity_field[3];
+  /* the SURFACE block (Surface::pack, src/Surface.cc:609-627) */
+  int sht_resample_order, diff_filter_freq, reparam_filter_freq;
+} ves3d_checkpoint_params;
+/* Parameters<T>::init defaults (src/Parameters.cc:23-58) for sh_order and
+ * nves, with the globally implicit velocity scheme and the diagonal spectral
+ * preconditioner. */
+int ves3d_checkpoint_defaults(int sh_order, size_t nves, ves3d_checkpoint_params *prm);
+/* Writes a checkpoint: positions X [nves][3][(p+1)2p] (host, AxisMajor) and
+ * the vesicle properties [nves] (VesicleProperties::pack,
+ * src/VesicleProps.cc:110-122).  Numbers in %.16e (Monitor's stream
+ * settings), which round-trips every double. */
+int ves3d_checkpoint_write(const char *path, const ves3d_checkpoint_params *prm, size_t nves,
+                           const double *X, const double *bending_modulus,
+                           const double *viscosity_contrast, const double *excess_density);
+/* Reads a checkpoint written by the reference or by ves3d_checkpoint_write,
+ * accepting the layouts of the reference's unpack methods (versions <= 590,
+ * < 592 and < 593 differ).  X may be NULL (count only); property arrays may
+ * be NULL; versions <= 590 store no properties and the parameters' values
+ * are returned (VesicleProperties::setFromParams).  VES3D_IO_ERROR: the file
+ * cannot be opened; VES3D_IO_BAD_STREAM: a key or block is not where the
+ * format puts it. */
+int ves3d_checkpoint_read(const char *path, ves3d_checkpoint_params *prm, size_t *nves_host,
+                          double *X, size_t x_capacity, double *bending_modulus,
+                          double *viscosity_contrast, double *excess_density,
+                          size_t props_capacity);
+
 /* InitializeShapes (src/HelperFuns.cc:366-432): X_host [nves][3][(p+1)2p] from
  * a shape gallery (shapes [3][(p+1)2p] back to back, gallery_len values) and
  * a geometry specification of 8 values per vesicle: shape index, scale,

@@ -21,7 +21,7 @@ c_double = ctypes.c_double
 ERRORS = {
     0: "Success", 1: "UnknownError", 2: "ArithmeticError", 5: "MemoryError",
     7: "NotImplementedError", 10: "InvalidParameterError", 20: "InvalidDeviceError",
-    30: "AccuracyError", 31: "DivergenceError",
+    30: "AccuracyError", 31: "DivergenceError", 50: "IOError", 51: "IOBadStream",
 }
 
 
@@ -110,6 +110,9 @@ _sig("ves3d_hip_stepper_area_volume", P, P)
 _sig("ves3d_hip_stepper_set_area_volume", P, P, P)
 _sig("ves3d_read_ascii", ctypes.c_char_p, P, S, P)
 _sig("ves3d_initialize_shapes", c_int, S, P, S, P, P)
+_sig("ves3d_checkpoint_defaults", c_int, S, P)
+_sig("ves3d_checkpoint_write", ctypes.c_char_p, P, S, P, P, P, P)
+_sig("ves3d_checkpoint_read", ctypes.c_char_p, P, P, P, S, P, P, P, S)
 _sig("ves3d_hip_movepole_create", c_int, c_int, P, P, P)
 _sig("ves3d_hip_movepole_destroy", P)
 _sig("ves3d_hip_movepole_set_operands", P, P, P, c_int, P)
