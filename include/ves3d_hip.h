@@ -389,6 +389,12 @@ int ves3d_hip_stepper_matvec(void *handle, const double *x, double *y);
 /* ImplicitPrecond with the diagonal spectral preconditioner
  * (src/InterfacialVelocity.cc:1054-1109): y = D x. */
 int ves3d_hip_stepper_precond(void *handle, const double *x, double *y);
+/* The relative residual |b - A x| / |b| of the current state: the step's
+ * right-hand side b and operator A at the positions, x the packed velocity
+ * and tension (the true-residual check of src/InterfacialVelocity.cc:181-188
+ * for a given solution).  With a precision-0 stepper it judges a solution
+ * of any form against the FP64 operator. */
+int ves3d_hip_stepper_residual(void *handle, double *relres_host);
 /* No counterpart in the reference (it feeds the learned preconditioner):
  * ImplicitApply with every vesicle's own operator only, the singular
  * self-interaction without the far field and the near-singular correction:

@@ -121,11 +121,17 @@ def test_full_step_f64(hip, precond):
     st.set_state(cu(X))
     stats = st.step()
     assert stats["status"] == 100 and stats["true_relres"] < 1e-6 * 1.01, stats
-    Xn, U, sig = st.get_state()
-    U = U.cpu().numpy()
+    Xn, Ug, sig = st.get_state()
+    U = Ug.cpu().numpy()
     assert np.linalg.norm(U - u0) / np.linalg.norm(u0) < 1e-5
     assert np.linalg.norm(Xn.cpu().numpy() - (X + dt * u0)) / np.linalg.norm(X) < 1e-9
     st.close()
+    # the residual entry point reproduces the step's true-residual check
+    chk = I.Stepper(nv, p, dt=dt, shear_rate=shear, precision=0, precond=precond, post=False)
+    chk.set_state(cu(X), Ug, sig)
+    r = chk.residual()
+    assert abs(r - stats["true_relres"]) < 1e-6 * stats["true_relres"] + 1e-15, (r, stats)
+    chk.close()
 
 
 def test_reparam_and_area_volume_vs_oracle(hip):

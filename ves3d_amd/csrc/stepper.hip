@@ -910,6 +910,16 @@ int ves3d_hip_stepper_matvec(void *h, const double *x, double *y) {
   return hip_status(hipStreamSynchronize(s->st));
 }
 
+int ves3d_hip_stepper_residual(void *h, double *relres_host) {
+  if (!h || !relres_host) return VES3D_INVALID_PARAMETER;
+  Stepper *s = (Stepper *)h;
+  if (!s->prepared) V3D_CHECK(s->prepare());
+  V3D_CHECK(s->assemble_rhs(s->rhs));
+  V3D_CHECK(s->pack(s->U, s->sig, s->xsol));
+  V3D_CHECK(s->true_residual(s->xsol, s->rhs, relres_host));
+  return hip_status(hipStreamSynchronize(s->st));
+}
+
 int ves3d_hip_stepper_matvec_local(void *h, const double *x, double *y) {
   if (!h || !x || !y) return VES3D_INVALID_PARAMETER;
   Stepper *s = (Stepper *)h;

@@ -108,6 +108,13 @@ class Stepper:
         call("ves3d_hip_stepper_precond", self.h, ptr(x), ptr(y))
         return y
 
+    def residual(self):
+        """|b - A x| / |b| of the current (positions, velocity, tension)."""
+        r = ctypes.c_double()
+        torch.cuda.synchronize()
+        call("ves3d_hip_stepper_residual", self.h, ctypes.byref(r))
+        return r.value
+
     def matvec_local(self, x):
         """Every vesicle's own operator (no far field, no near correction)."""
         self._chk(x, self.n)
