@@ -174,3 +174,30 @@ def test_full_step_with_post(hip):
     Xa, _ = ves.area_volume_correction(Xr, p, 2 * p, a0, v0)
     assert rel(Xn.cpu().numpy(), Xa) < 1e-9
     st.close()
+
+
+def test_reference_cpu_case_p12(hip):
+    """The plain method's CPU-runnable case (2 perturbed spheres at p = 12,
+    centres (+-1.25, 0, 0), kb = 0.01, shear 0.1, dt = 1e-3, BiCGStab to
+    1e-8) against the oracle's solve of the same step: velocity and tension
+    to 1e-6 relative (both solves stop at relres 1e-8 from a zero guess with
+    the diagonal preconditioner, so they share the tension gauge)."""
+    p = 12
+    X = W.reference_cpu_case(p)
+    orc = ves.Implicit(X, p, dt=1e-3, kb=1e-2, shear=0.1)
+    b0 = orc.rhs()
+    x0, res0, it0, st0 = ves.bicgstab(orc.matvec, b0, np.zeros_like(b0), 1e-8, 300, orc.precond)
+    assert st0 == "success"
+    u0, s0 = orc.unpack(x0)
+    st = I.Stepper(2, p, dt=1e-3, shear_rate=0.1, bending_modulus=1e-2, precision=0, precond=1,
+                   tol=1e-8, max_iter=300, post=False)
+    st.set_state(cu(X))
+    print("reference CPU case p=12: oracle %d iterations" % it0, flush=True)
+    stats = st.step()
+    assert stats["status"] == 100 and stats["true_relres"] < 1e-8 * 1.05, stats
+    Xn, U, sig = st.get_state()
+    U, sig = U.cpu().numpy(), sig.cpu().numpy()
+    assert np.linalg.norm(U - u0) / np.linalg.norm(u0) < 1e-6
+    assert np.linalg.norm(sig - s0) / np.linalg.norm(s0) < 1e-6
+    assert np.max(np.abs(Xn.cpu().numpy() - (X + 1e-3 * U))) < 1e-14
+    st.close()

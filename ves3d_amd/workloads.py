@@ -164,6 +164,17 @@ def suspension(shapes, volfrac, seed=SEED_BASE, gap=0.05, volumes=None, rotate=F
     return shapes + c[:, :, None]
 
 
+def reference_cpu_case(p=12, seed=SEED_BASE):
+    """The plain method's CPU-runnable case: 2 unit spheres with seeded radial
+    perturbations (coefficients N(0, (0.05/(l+1)^2)^2), l = 1..p), centred at
+    (-1.25, 0, 0) and (1.25, 0, 0).  Returns positions [2][3][2p(p+1)]; the
+    step is bending modulus 0.01, shear u = (0.1 z, 0, 0), dt = 1e-3,
+    BiCGStab to 1e-8."""
+    shapes = perturbed_shapes(2, p, amp=0.05, seed=seed)
+    c = np.array([[-1.25, 0.0, 0.0], [1.25, 0.0, 0.0]])
+    return shapes + c[:, :, None]
+
+
 def implicit_step_case(nv=512, p=16, volfrac=0.20, seed=SEED_BASE + 3):
     """The full implicit-step workload: nv perturbed ellipsoids (semi-axes
     1, 1, 0.6 +- 0.05) with random orientations at the given volume fraction.
