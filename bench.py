@@ -166,6 +166,8 @@ def bench_step(args):
             "kernel": FAR32_NAMES[args.far_kernel] + " (+k_pack_far32, k_sum_slices)",
             "kernel_ms": far_ms,
             "flop_per_launch": flop, "pairs_per_launch": pairs,
+            "two_density_pass_ms": (sum(s["ms_far2"] for s in stats_mixed)
+                                    / max(1, sum(s["far2_calls"] for s in stats_mixed))),
             "fp64_form": {"kernel": "k_far_f64<SL>" if args.far_kernel == "lds" else "k_far_groups_f64",
                           "kernel_ms": far_ms_plain,
                           "achieved": flop / (far_ms_plain * 1e-3) / 1e12,

@@ -365,6 +365,9 @@ typedef struct ves3d_step_stats {
   int reparam_iters;      /* reparametrization iterations (0 with post_off) */
   int av_iters;           /* area / volume correction iterations */
   double ms_post;         /* wall time of the two */
+  double ms_far2;         /* summed device time of the two-density far-field passes
+                             (the right-hand side together with the first matvec) */
+  int far2_calls;         /* number of two-density passes */
 } ves3d_step_stats;
 
 /* Creates / destroys a stepper (device state for nves vesicles). */
@@ -386,6 +389,10 @@ int ves3d_hip_stepper_prepare(void *handle);
 int ves3d_hip_stepper_rhs(void *handle, double *b);
 /* ImplicitApply (src/InterfacialVelocity.cc:981-1052): y = A x. */
 int ves3d_hip_stepper_matvec(void *handle, const double *x, double *y);
+/* AssembleRhsVel into b and ImplicitApply y = A x together, the two Stokes
+ * evaluations sharing one far-field pass (what each step does for its right-
+ * hand side and the solver's first matvec). */
+int ves3d_hip_stepper_rhs_matvec(void *handle, const double *x, double *b, double *y);
 /* ImplicitPrecond with the diagonal spectral preconditioner
  * (src/InterfacialVelocity.cc:1054-1109): y = D x. */
 int ves3d_hip_stepper_precond(void *handle, const double *x, double *y);

@@ -201,3 +201,23 @@ def test_reference_cpu_case_p12(hip):
     assert np.linalg.norm(sig - s0) / np.linalg.norm(s0) < 1e-6
     assert np.max(np.abs(Xn.cpu().numpy() - (X + 1e-3 * U))) < 1e-14
     st.close()
+
+
+@pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 1e-6)])
+def test_rhs_matvec_one_pass(hip, far_kernel, precision, tol):
+    """The right-hand side and a matvec from one two-density far-field pass
+    equal the separate evaluations (to rounding; the FP32 form to its own
+    accuracy), with near pairs present."""
+    from test_near_gpu import _close_triple
+    p = 8
+    X = _close_triple(p, 9)
+    st = I.Stepper(3, p, dt=1e-3, shear_rate=0.3, precision=precision, precond=1)
+    st.set_state(cu(X))
+    st.prepare()
+    x = cu(np.random.default_rng(5).standard_normal(st.n))
+    b1, y1 = st.rhs_matvec(x)
+    b0 = st.rhs()
+    y0 = st.matvec(x)
+    assert rel(b1.cpu().numpy(), b0.cpu().numpy()) < tol
+    assert rel(y1.cpu().numpy(), y0.cpu().numpy()) < tol
+    st.close()

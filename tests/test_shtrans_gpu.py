@@ -21,7 +21,7 @@ def rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
-@pytest.mark.parametrize("p,nf", [(6, 5), (12, 3), (16, 7)])
+@pytest.mark.parametrize("p,nf", [(6, 5), (12, 3), (16, 7), (24, 3)])
 def test_forward_backward(hip, p, nf):
     ng, nc = 2 * p * (p + 1), p * (p + 2)
     S = RNG.standard_normal((nf, nc))

@@ -117,7 +117,7 @@ def test_singular_build_f32(hip):
     assert rel(gdl.double().cpu().numpy(), DL) < 1e-5
 
 
-@pytest.mark.parametrize("p0", [12, 16])
+@pytest.mark.parametrize("p0", [12, 16, 24])
 def test_singular_closed_forms_gpu(hip, p0):
     """Size-independent property at the workload's order: on unit spheres
     SL[c] = (2/3) c, DL[c] = -c/2 (the grid's outward normal, the

@@ -405,16 +405,20 @@ void StokesG::timing_begin() {
   nev = 0;
 }
 
-int StokesG::far_time(double *ms, int *calls) {
-  double tot = 0;
+int StokesG::far_time(double *ms, int *calls, double *ms2, int *calls2) {
+  double tot[3] = {0, 0, 0};
+  int cnt[3] = {0, 0, 0};
   for (int i = 0; i < nev; ++i) {
     V3D_HIP(hipEventSynchronize(ev1[i]));
     float t = 0;
     V3D_HIP(hipEventElapsedTime(&t, ev0[i], ev1[i]));
-    tot += t;
+    tot[evkind[i]] += t;
+    ++cnt[evkind[i]];
   }
-  if (ms) *ms = tot;
-  if (calls) *calls = nev;
+  if (ms) *ms = tot[1];
+  if (calls) *calls = cnt[1];
+  if (ms2) *ms2 = tot[2];
+  if (calls2) *calls2 = cnt[2];
   timing = false;
   nev = 0;
   return VES3D_SUCCESS;
@@ -476,6 +480,49 @@ int StokesG::setup(const double *X, hipStream_t st) {
   return hip_status(hipStreamSynchronize(st));  // dqw is freed on return
 }
 
+int StokesG::self_and_density(const double *f, double *Fc, double *Vc, double *vel, double *upc,
+                              double *qc, bool pack, hipStream_t st) {
+  const long mu = sh_ngrid(pu);
+  V3D_CHECK(sht_grid2shc<double>(f, 3 * nv, p, p, Fc, work, st));
+  if (precision != 1)
+    V3D_CHECK(self_apply<double>(SL.p, Fc, nullptr, nullptr, nv, p, Vc, st));
+  else
+    V3D_CHECK(self_apply<float>(SL32, Fc, nullptr, nullptr, nv, p, Vc, st));
+  V3D_CHECK(sht_shc2grid<double>(Vc, 3 * nv, p, p, vel, nullptr, nullptr, work, st));
+  if (!pack) return VES3D_SUCCESS;
+  V3D_CHECK(sht_shc2grid<double>(Fc, 3 * nv, p, pu, upc, nullptr, nullptr, work, st));
+  k_pack_density<<<gblocks(nv * (mu + 2)), 256, 0, st>>>(upc, aqw, nv, mu, qc);
+  return launch_status();
+}
+
+int StokesG::near_correct(const double *qc, const double *Vc, double *vel, hipStream_t st) {
+  // near-singular correction with the self velocity on the upsampled grid
+  // and at the poles (S_vel_up, src/StokesVelocity.cc:452-470)
+  V3D_CHECK(sht_shc2grid<double>(Vc, 3 * nv, p, pu, vup, nullptr, nullptr, work, st));
+  V3D_CHECK(sht_shc2pole<double>(Vc, 3 * nv, p, vpole, st));
+  return near.apply(qc, vup, vpole, vel, st);
+}
+
+int StokesG::event_begin(int kind, hipStream_t st) {
+  if (!timing) return VES3D_SUCCESS;
+  if (nev == (int)ev0.size()) {
+    hipEvent_t a, b;
+    V3D_HIP(hipEventCreate(&a));
+    V3D_HIP(hipEventCreate(&b));
+    ev0.push_back(a);
+    ev1.push_back(b);
+    evkind.push_back(0);
+  }
+  evkind[nev] = kind;
+  V3D_HIP(hipEventRecord(ev0[nev], st));
+  return VES3D_SUCCESS;
+}
+int StokesG::event_end(hipStream_t st) {
+  if (!timing) return VES3D_SUCCESS;
+  V3D_HIP(hipEventRecord(ev1[nev++], st));
+  return VES3D_SUCCESS;
+}
+
 int StokesG::apply(const double *f, double *vel, hipStream_t st, bool add_repul, bool local) {
   const long ng = sh_ngrid(p), mu = sh_ngrid(pu);
   const bool near_on = use_near && near.npairs > 0;
@@ -484,37 +531,46 @@ int StokesG::apply(const double *f, double *vel, hipStream_t st, bool add_repul,
     V3D_CHECK(near.add_repulsion(frep, st));
     f = frep;
   }
-  V3D_CHECK(sht_grid2shc<double>(f, 3 * nv, p, p, F, work, st));
-  if (precision != 1)
-    V3D_CHECK(self_apply<double>(SL.p, F, nullptr, nullptr, nv, p, V, st));
-  else
-    V3D_CHECK(self_apply<float>(SL32, F, nullptr, nullptr, nv, p, V, st));
-  V3D_CHECK(sht_shc2grid<double>(V, 3 * nv, p, p, vel, nullptr, nullptr, work, st));
+  V3D_CHECK(self_and_density(f, F, V, vel, up, q, !local, st));
   if (local) return VES3D_SUCCESS;
-  V3D_CHECK(sht_shc2grid<double>(F, 3 * nv, p, pu, up, nullptr, nullptr, work, st));
-  k_pack_density<<<gblocks(nv * (mu + 2)), 256, 0, st>>>(up, aqw, nv, mu, q);
-  V3D_CHECK(launch_status());
-  if (timing) {
-    if (nev == (int)ev0.size()) {
-      hipEvent_t a, b;
-      V3D_HIP(hipEventCreate(&a));
-      V3D_HIP(hipEventCreate(&b));
-      ev0.push_back(a);
-      ev1.push_back(b);
-    }
-    V3D_HIP(hipEventRecord(ev0[nev], st));
-  }
+  V3D_CHECK(event_begin(1, st));
   V3D_CHECK(far_field_between_groups(precision != 0, far_src, q, nv * (mu + 2), trg, farv, nv * ng,
                                      mu + 2, ng, st));
-  if (timing) V3D_HIP(hipEventRecord(ev1[nev++], st));
+  V3D_CHECK(event_end(st));
   k_add_points<<<gblocks(nv * ng), 256, 0, st>>>(farv, nv, ng, vel);
   V3D_CHECK(launch_status());
   if (!near_on) return VES3D_SUCCESS;
-  // near-singular correction with the self velocity on the upsampled grid
-  // and at the poles (S_vel_up, src/StokesVelocity.cc:452-470)
-  V3D_CHECK(sht_shc2grid<double>(V, 3 * nv, p, pu, vup, nullptr, nullptr, work, st));
-  V3D_CHECK(sht_shc2pole<double>(V, 3 * nv, p, vpole, st));
-  return near.apply(q, vup, vpole, vel, st);
+  return near_correct(q, V, vel, st);
+}
+
+int StokesG::apply2(const double *f1, double *vel1, bool add_repul1, const double *f2,
+                    double *vel2, hipStream_t st) {
+  const long ng = sh_ngrid(p), mu = sh_ngrid(pu);
+  const bool near_on = use_near && near.npairs > 0;
+  if (!F2.p) {
+    for (DBuf *b : {&F2, &V2}) V3D_CHECK(b->alloc(3 * nv * sh_ncoef(p)));
+    V3D_CHECK(up2.alloc(3 * nv * mu));
+    V3D_CHECK(q2.alloc(nv * (mu + 2) * 3));
+    V3D_CHECK(farv2.alloc(nv * ng * 3));
+  }
+  if (add_repul1 && near_on && near.repul > 0) {
+    V3D_HIP(hipMemcpyAsync(frep, f1, sizeof(double) * 3 * nv * ng, hipMemcpyDeviceToDevice, st));
+    V3D_CHECK(near.add_repulsion(frep, st));
+    f1 = frep;
+  }
+  V3D_CHECK(self_and_density(f1, F, V, vel1, up, q, true, st));
+  V3D_CHECK(self_and_density(f2, F2, V2, vel2, up2, q2, true, st));
+  V3D_CHECK(event_begin(2, st));
+  V3D_CHECK(far_field_between_groups2(precision != 0, far_src, q, q2, nv * (mu + 2), trg, farv,
+                                      farv2, nv * ng, mu + 2, ng, st));
+  V3D_CHECK(event_end(st));
+  k_add_points<<<gblocks(nv * ng), 256, 0, st>>>(farv, nv, ng, vel1);
+  V3D_CHECK(launch_status());
+  k_add_points<<<gblocks(nv * ng), 256, 0, st>>>(farv2, nv, ng, vel2);
+  V3D_CHECK(launch_status());
+  if (!near_on) return VES3D_SUCCESS;
+  V3D_CHECK(near_correct(q, V, vel1, st));
+  return near_correct(q2, V2, vel2, st);
 }
 
 }  // namespace v3d

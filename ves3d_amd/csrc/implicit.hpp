@@ -21,6 +21,12 @@ int far_field_between_groups(bool f32c, const double *spos, const double *sld, s
                              const double *tpos, double *tvel, size_t n_trg, size_t spg,
                              size_t tpg, hipStream_t st);
 
+// The same with two densities on the same sources and targets, in one pass
+// (the pair geometry is computed once and serves both sums).
+int far_field_between_groups2(bool f32c, const double *spos, const double *sld1,
+                              const double *sld2, size_t n_src, const double *tpos, double *tvel1,
+                              double *tvel2, size_t n_trg, size_t spg, size_t tpg, hipStream_t st);
+
 // Device buffer owned by the step objects.
 struct DBuf {
   double *p = nullptr;
@@ -88,16 +94,23 @@ class StokesG {
   // the near-singular correction.
   int apply(const double *f, double *vel, hipStream_t st, bool add_repul = false,
             bool local = false);
+  // apply for two densities at once (one far-field pass over both): the
+  // right-hand side's (f1, with the repulsion when add_repul1) and the
+  // solver's first matvec's (f2)
+  int apply2(const double *f1, double *vel1, bool add_repul1, const double *f2, double *vel2,
+             hipStream_t st);
 
   // HIP events around every far-field launch (on the launch stream); the
-  // elapsed times are summed by far_time() after the stream has drained.
+  // elapsed times are summed by far_time() after the stream has drained:
+  // one-density launches into (ms, calls), two-density ones into (ms2, calls2)
   void timing_begin();
-  int far_time(double *ms, int *calls);
+  int far_time(double *ms, int *calls, double *ms2 = nullptr, int *calls2 = nullptr);
 
   int p = 0, pu = 0, precision = 0;
   long nv = 0;
   bool timing = false;
   std::vector<hipEvent_t> ev0, ev1;
+  std::vector<int> evkind;
   int nev = 0;
   DBuf SL;            // FP64 operator (precision 0)
   float *SL32 = nullptr;
@@ -105,7 +118,17 @@ class StokesG {
   bool use_near = true;
   NearG near;
   DBuf frep, vup, vpole;
+  DBuf F2, V2, up2, q2, farv2;  // second density of apply2
   ~StokesG();
+
+ private:
+  // density SHT, self apply (vel = S_self f on the p grid) and, with pack,
+  // the upsampled far-field density qc
+  int self_and_density(const double *f, double *Fc, double *Vc, double *vel, double *upc,
+                       double *qc, bool pack, hipStream_t st);
+  int near_correct(const double *qc, const double *Vc, double *vel, hipStream_t st);
+  int event_begin(int kind, hipStream_t st);
+  int event_end(hipStream_t st);
 };
 
 }  // namespace v3d

@@ -319,14 +319,18 @@ typedef int (*apply_fn)(const void *ctx, const double *x, double *y);
 // BiCGStab (src/BiCGStab.cc:45-138), right-preconditioned, FP64 dots.
 // Returns Error_t; *status gets the BiCGSReturn code (include/BiCGStab.h:20-24).
 int bicgstab(long n, apply_fn A, const void *actx, apply_fn P, const void *pctx, double *x,
-             const double *b, int *max_iter, double *tol, int *status, hipStream_t st) {
+             const double *b, int *max_iter, double *tol, int *status, hipStream_t st,
+             const double *Ax0 = nullptr) {
   DBuf r, rt, p, v, s, t, ph, sh;
   for (DBuf *d : {&r, &rt, &p, &v, &s, &t, &ph, &sh}) V3D_CHECK(d->alloc(n));
   int err = 0;
   double normb = dev_norm(n, b, st, &err);
   V3D_CHECK(err);
   if (normb == 0.0) normb = 1.0;
-  V3D_CHECK(A(actx, x, r));
+  if (Ax0)  // A x computed by the caller (together with b)
+    V3D_CHECK(dev_copy(n, Ax0, r, st));
+  else
+    V3D_CHECK(A(actx, x, r));
   V3D_CHECK(dev_axpby(n, 1.0, b, -1.0, r, st));  // r = b - Ax
   V3D_CHECK(dev_copy(n, r, rt, st));
   double resid = dev_norm(n, r, st, &err) / normb;
@@ -424,6 +428,7 @@ struct Stepper {
   StokesG stokes;
   DBuf X, U, sig;                  // state
   DBuf f, f2, Sf, ten, ug, sg, cf, rhs, xsol, yres, tmpD, yloc;
+  DBuf ub, fb, Sfb, Ax0;           // right-hand side fused with the first matvec
   DBuf bprec, tprec;               // diagonal-spectral preconditioner
   DBuf fgrid, frames;              // canonical frames of the vesicles (learned PC)
   int2 *cinfo = nullptr;
@@ -566,6 +571,35 @@ struct Stepper {
     V3D_CHECK(dev_axpby(3 * nv * ng, 1.0, Sf, 1.0, ug, st));
     V3D_CHECK(forces.S.div(ug, ten, st));
     return pack(ug, ten, b);
+  }
+
+  // AssembleRhsVel into b and the matvec y = A x together: the two Stokes
+  // evaluations share one far-field pass (two densities, same sources)
+  int rhs_and_matvec(double *b, const double *x, double *y) {
+    if (!prepared) return VES3D_INVALID_PARAMETER;
+    if (!ub.p)
+      for (DBuf *d : {&ub, &fb, &Sfb}) V3D_CHECK(d->alloc(3 * nv * ng));
+    ++matvecs;
+    // density of A x (as in matvec)
+    V3D_CHECK(unpack(x, ug, sg));
+    V3D_CHECK(dev_axpby(3 * nv * ng, prm.dt, ug, 0.0, f2, st));
+    V3D_CHECK(forces.linear_bending(f2, f, st));
+    V3D_CHECK(forces.tensile(sg, f2, st));
+    V3D_CHECK(dev_axpby(3 * nv * ng, 1.0, f2, 1.0, f, st));
+    // density of the right-hand side (as in assemble_rhs)
+    k_shear<<<gb(nv * ng), 256, 0, st>>>(X, nv, ng, prm.shear_rate, ub);
+    V3D_CHECK(launch_status());
+    V3D_CHECK(forces.bending(fb, st));
+    V3D_CHECK(stokes.apply2(fb, Sfb, true, f, Sf, st));
+    // right-hand side
+    V3D_CHECK(dev_axpby(3 * nv * ng, 1.0, Sfb, 1.0, ub, st));
+    V3D_CHECK(forces.S.div(ub, ten, st));
+    V3D_CHECK(pack(ub, ten, b));
+    // matvec
+    V3D_CHECK(forces.S.div(Sf, ten, st));
+    V3D_CHECK(dev_axpby(nv * ng, 0.0, ten, -1.0, ten, st));      // ten = -div
+    V3D_CHECK(dev_axpby(3 * nv * ng, -1.0, Sf, 1.0, ug, st));    // vox = u - Sf
+    return pack(ug, ten, y);
   }
 
   // ImplicitPrecond (src/InterfacialVelocity.cc:1054-1109), Galerkin form
@@ -756,8 +790,10 @@ struct Stepper {
     V3D_CHECK(prepare());
     V3D_HIP(hipStreamSynchronize(st));
     auto t1 = clk::now();
-    V3D_CHECK(assemble_rhs(rhs));
     V3D_CHECK(pack(U, sig, xsol));  // AssembleInitial (src/InterfacialVelocity.cc:873-919)
+    // the right-hand side and the solver's first matvec A x0 in one pass
+    if (!Ax0.p) V3D_CHECK(Ax0.alloc(n));
+    V3D_CHECK(rhs_and_matvec(rhs, xsol, Ax0));
     V3D_HIP(hipStreamSynchronize(st));
     auto t2 = clk::now();
     int it = prm.max_iter;
@@ -773,7 +809,7 @@ struct Stepper {
     if (learned)
       it = last_iters > 0 ? std::min(prm.max_iter, 2 * last_iters + 10)
                           : std::max(10, prm.max_iter / 3);
-    V3D_CHECK(bicgstab(n, A_cb, this, P, this, xsol, rhs, &it, &tol, &status, st));
+    V3D_CHECK(bicgstab(n, A_cb, this, P, this, xsol, rhs, &it, &tol, &status, st, Ax0));
     // true residual of the returned solution (src/InterfacialVelocity.cc:181-188)
     double true_rel = 0;
     V3D_CHECK(true_residual(xsol, rhs, &true_rel));
@@ -786,7 +822,7 @@ struct Stepper {
       V3D_CHECK(pack(U, sig, xsol));
       it = prm.max_iter;
       tol = prm.tol;
-      V3D_CHECK(bicgstab(n, A_cb, this, Pdiag_cb, this, xsol, rhs, &it, &tol, &status, st));
+      V3D_CHECK(bicgstab(n, A_cb, this, Pdiag_cb, this, xsol, rhs, &it, &tol, &status, st, Ax0));
       V3D_CHECK(true_residual(xsol, rhs, &true_rel));
     }
     if (status == 100) last_iters = it;
@@ -801,9 +837,9 @@ struct Stepper {
       V3D_CHECK(area_volume_correction());
     }
     auto t4 = clk::now();
-    double far_ms = 0;
-    int far_calls = 0;
-    V3D_CHECK(stokes.far_time(&far_ms, &far_calls));
+    double far_ms = 0, far2_ms = 0;
+    int far_calls = 0, far2_calls = 0;
+    V3D_CHECK(stokes.far_time(&far_ms, &far_calls, &far2_ms, &far2_calls));
     prepared = false;
     if (stats) {
       auto ms = [](clk::time_point a, clk::time_point b) {
@@ -826,6 +862,8 @@ struct Stepper {
       stats->reparam_iters = rep_iters;
       stats->av_iters = av_iters;
       stats->ms_post = ms(t3b, t4);
+      stats->ms_far2 = far2_ms;
+      stats->far2_calls = far2_calls;
     }
     return VES3D_SUCCESS;
   }
@@ -907,6 +945,13 @@ int ves3d_hip_stepper_matvec(void *h, const double *x, double *y) {
   if (!h || !x || !y) return VES3D_INVALID_PARAMETER;
   Stepper *s = (Stepper *)h;
   V3D_CHECK(s->matvec(x, y));
+  return hip_status(hipStreamSynchronize(s->st));
+}
+
+int ves3d_hip_stepper_rhs_matvec(void *h, const double *x, double *b, double *y) {
+  if (!h || !x || !b || !y) return VES3D_INVALID_PARAMETER;
+  Stepper *s = (Stepper *)h;
+  V3D_CHECK(s->rhs_and_matvec(b, x, y));
   return hip_status(hipStreamSynchronize(s->st));
 }
 

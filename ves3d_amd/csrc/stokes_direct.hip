@@ -594,6 +594,165 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c2(
   }
 }
 
+// Two densities on the same sources and targets in one pass (the step's
+// right-hand side and the solver's first matvec): the pair geometry (d, r^2,
+// 1/r, 1/r^2) is computed once and serves both Stokeslet sums.
+struct __align__(16) FarRec32x2 {
+  float x, y, z, ax, ay, az, bx, by, bz, _p0, _p1, _p2;
+};
+struct __align__(16) FarRec64x2 {
+  double x, y, z, ax, ay, az, bx, by, bz, _p0, _p1, _p2;
+};
+
+__global__ void k_pack_far32x2(const double *__restrict__ spos, const double *__restrict__ s1,
+                               const double *__restrict__ s2, long n, long spg,
+                               FarRec32x2 *__restrict__ rec) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long a = (i / spg) * spg;
+  FarRec32x2 r;
+  r.x = (float)(spos[3 * i] - spos[3 * a]);
+  r.y = (float)(spos[3 * i + 1] - spos[3 * a + 1]);
+  r.z = (float)(spos[3 * i + 2] - spos[3 * a + 2]);
+  r.ax = (float)s1[3 * i];
+  r.ay = (float)s1[3 * i + 1];
+  r.az = (float)s1[3 * i + 2];
+  r.bx = (float)s2[3 * i];
+  r.by = (float)s2[3 * i + 1];
+  r.bz = (float)s2[3 * i + 2];
+  r._p0 = r._p1 = r._p2 = 0.0f;
+  rec[i] = r;
+}
+__global__ void k_pack_far64x2(const double *__restrict__ spos, const double *__restrict__ s1,
+                               const double *__restrict__ s2, long n,
+                               FarRec64x2 *__restrict__ rec) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  FarRec64x2 r;
+  r.x = spos[3 * i];
+  r.y = spos[3 * i + 1];
+  r.z = spos[3 * i + 2];
+  r.ax = s1[3 * i];
+  r.ay = s1[3 * i + 1];
+  r.az = s1[3 * i + 2];
+  r.bx = s2[3 * i];
+  r.by = s2[3 * i + 1];
+  r.bz = s2[3 * i + 2];
+  r._p0 = r._p1 = r._p2 = 0.0;
+  rec[i] = r;
+}
+
+__global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c_x2(
+    const FarRec32x2 *__restrict__ rec, const double *__restrict__ spos, long spg, long n_groups,
+    const double *__restrict__ tpos, long n_trg, long tpg, int nsplit, double csl,
+    double *__restrict__ part1, double *__restrict__ part2) {
+  const long slice = blockIdx.x % nsplit;
+  const long tb = blockIdx.x / nsplit;
+  const long t = tb * FF_BLOCK + threadIdx.x;
+  const bool tvalid = t < n_trg;
+  double tx = 0, ty = 0, tz = 0;
+  if (tvalid) {
+    tx = tpos[3 * t];
+    ty = tpos[3 * t + 1];
+    tz = tpos[3 * t + 2];
+  }
+  const long my_g = tvalid ? t / tpg : -1;
+  const long tfirst = tb * FF_BLOCK;
+  const long tlast = min(tfirst + FF_BLOCK, n_trg) - 1;
+  Acc32 A = {{0, 0, 0}, {0, 0, 0}}, B = {{0, 0, 0}, {0, 0, 0}};
+  long g0, g1;
+  slice_range(slice, nsplit, 0, spg, n_groups, g0, g1);
+  for (long g = g0; g < g1; ++g) {
+    if (tfirst / tpg == g && tlast / tpg == g) continue;  // block-uniform skip
+    if (!tvalid || my_g == g) continue;
+    const double *a = spos + 3 * g * spg;
+    const float lx = (float)(tx - a[0]), ly = (float)(ty - a[1]), lz = (float)(tz - a[2]);
+    const FarRec32x2 *__restrict__ r = rec + g * spg;
+    for (long j0 = 0; j0 < spg; j0 += FF_SUB) {
+      const long j1 = min(spg, j0 + FF_SUB);
+      float ux = 0, uy = 0, uz = 0, vx = 0, vy = 0, vz = 0;
+#pragma unroll 4
+      for (long j = j0; j < j1; ++j) {
+        const FarRec32x2 s = r[j];
+        const float dx = lx - s.x, dy = ly - s.y, dz = lz - s.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float ri = __frsqrt_rn(r2);
+        const float ri2 = ri * ri;
+        const float ip = (s.ax * dx + s.ay * dy + s.az * dz) * ri2;
+        ux += ri * (s.ax + dx * ip);
+        uy += ri * (s.ay + dy * ip);
+        uz += ri * (s.az + dz * ip);
+        const float iq = (s.bx * dx + s.by * dy + s.bz * dz) * ri2;
+        vx += ri * (s.bx + dx * iq);
+        vy += ri * (s.by + dy * iq);
+        vz += ri * (s.bz + dz * iq);
+      }
+      acc_merge(A, ux, uy, uz);
+      acc_merge(B, vx, vy, vz);
+    }
+  }
+  if (tvalid) {
+    double *o1 = part1 + (slice * n_trg + t) * 3, *o2 = part2 + (slice * n_trg + t) * 3;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      o1[c] = csl * ((double)A.h[c] + (double)A.l[c]);
+      o2[c] = csl * ((double)B.h[c] + (double)B.l[c]);
+    }
+  }
+}
+
+__global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f64_x2(
+    const FarRec64x2 *__restrict__ rec, long spg, long n_groups, const double *__restrict__ tpos,
+    long n_trg, long tpg, int nsplit, double csl, double *__restrict__ part1,
+    double *__restrict__ part2) {
+  const long slice = blockIdx.x % nsplit;
+  const long tb = blockIdx.x / nsplit;
+  const long t = tb * FF_BLOCK + threadIdx.x;
+  const bool tvalid = t < n_trg;
+  double tx = 0, ty = 0, tz = 0;
+  if (tvalid) {
+    tx = tpos[3 * t];
+    ty = tpos[3 * t + 1];
+    tz = tpos[3 * t + 2];
+  }
+  const long my_g = tvalid ? t / tpg : -1;
+  const long tfirst = tb * FF_BLOCK;
+  const long tlast = min(tfirst + FF_BLOCK, n_trg) - 1;
+  double Sx = 0, Sy = 0, Sz = 0, Tx = 0, Ty = 0, Tz = 0;
+  long g0, g1;
+  slice_range(slice, nsplit, 0, spg, n_groups, g0, g1);
+  for (long g = g0; g < g1; ++g) {
+    if (tfirst / tpg == g && tlast / tpg == g) continue;  // block-uniform skip
+    if (!tvalid || my_g == g) continue;
+    const FarRec64x2 *__restrict__ r = rec + g * spg;
+#pragma unroll 2
+    for (long j = 0; j < spg; ++j) {
+      const FarRec64x2 s = r[j];
+      const double dx = tx - s.x, dy = ty - s.y, dz = tz - s.z;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double ri = rsqrt_pair(r2);
+      const double ri2 = ri * ri;
+      const double ip = (s.ax * dx + s.ay * dy + s.az * dz) * ri2;
+      Sx += ri * (s.ax + dx * ip);
+      Sy += ri * (s.ay + dy * ip);
+      Sz += ri * (s.az + dz * ip);
+      const double iq = (s.bx * dx + s.by * dy + s.bz * dz) * ri2;
+      Tx += ri * (s.bx + dx * iq);
+      Ty += ri * (s.by + dy * iq);
+      Tz += ri * (s.bz + dz * iq);
+    }
+  }
+  if (tvalid) {
+    double *o1 = part1 + (slice * n_trg + t) * 3, *o2 = part2 + (slice * n_trg + t) * 3;
+    o1[0] = csl * Sx;
+    o1[1] = csl * Sy;
+    o1[2] = csl * Sz;
+    o2[0] = csl * Tx;
+    o2[1] = csl * Ty;
+    o2[2] = csl * Tz;
+  }
+}
+
 // 0: LDS-staged source tiles (k_far_*), 1: scalar-cache source records
 // (k_far_groups_*), 2: as 1 with two targets per lane in the FP32 form;
 // selects the far field between groups.
@@ -739,6 +898,50 @@ int far_field_between_groups(bool f32c, const double *spos, const double *sld, s
   V3D_CHECK(launch_status());
   const long n = (long)n_trg * 3;
   k_sum_slices<<<cdiv(n, 256), 256, 0, st>>>(part, ns, n, tvel);
+  return launch_status();
+}
+
+int far_field_between_groups2(bool f32c, const double *spos, const double *sld1,
+                              const double *sld2, size_t n_src, const double *tpos, double *tvel1,
+                              double *tvel2, size_t n_trg, size_t spg, size_t tpg,
+                              hipStream_t st) {
+  if (spg == 0 || tpg == 0) return VES3D_INVALID_PARAMETER;
+  if (g_far_kind.load() != 1) {  // the other kernels evaluate one density at a time
+    V3D_CHECK(far_field_between_groups(f32c, spos, sld1, n_src, tpos, tvel1, n_trg, spg, tpg, st));
+    return far_field_between_groups(f32c, spos, sld2, n_src, tpos, tvel2, n_trg, spg, tpg, st);
+  }
+  if (n_trg == 0) return VES3D_SUCCESS;
+  if (!spos || !sld1 || !sld2 || !tpos || !tvel1 || !tvel2) return VES3D_INVALID_PARAMETER;
+  if (n_src % spg || n_trg % tpg || n_src / spg != n_trg / tpg) return VES3D_INVALID_PARAMETER;
+  const double csl = 1.0 / (8.0 * kPi);
+  const long n_groups = (long)(n_src / spg);
+  const int ns = choose_nsplit((long)n_trg, (long)n_src, n_groups);
+  double *part = (double *)workspace(kWsFarPartial, (size_t)2 * ns * n_trg * 3 * sizeof(double));
+  void *rec = workspace(kWsFarRecords, n_src * (f32c ? sizeof(FarRec32x2) : sizeof(FarRec64x2)));
+  if (!part || !rec) return VES3D_MEMORY_ERROR;
+  double *part2 = part + (size_t)ns * n_trg * 3;
+  const long tblocks = ((long)n_trg + FF_BLOCK - 1) / FF_BLOCK;
+  const dim3 grid((unsigned)(tblocks * ns));
+  if (f32c) {
+    k_pack_far32x2<<<cdiv((long)n_src, 256), 256, 0, st>>>(spos, sld1, sld2, (long)n_src,
+                                                            (long)spg, (FarRec32x2 *)rec);
+    V3D_CHECK(launch_status());
+    k_far_groups_f32c_x2<<<grid, FF_BLOCK, 0, st>>>((const FarRec32x2 *)rec, spos, (long)spg,
+                                                    n_groups, tpos, (long)n_trg, (long)tpg, ns,
+                                                    csl, part, part2);
+  } else {
+    k_pack_far64x2<<<cdiv((long)n_src, 256), 256, 0, st>>>(spos, sld1, sld2, (long)n_src,
+                                                            (FarRec64x2 *)rec);
+    V3D_CHECK(launch_status());
+    k_far_groups_f64_x2<<<grid, FF_BLOCK, 0, st>>>((const FarRec64x2 *)rec, (long)spg, n_groups,
+                                                   tpos, (long)n_trg, (long)tpg, ns, csl, part,
+                                                   part2);
+  }
+  V3D_CHECK(launch_status());
+  const long n = (long)n_trg * 3;
+  k_sum_slices<<<cdiv(n, 256), 256, 0, st>>>(part, ns, n, tvel1);
+  V3D_CHECK(launch_status());
+  k_sum_slices<<<cdiv(n, 256), 256, 0, st>>>(part2, ns, n, tvel2);
   return launch_status();
 }
 

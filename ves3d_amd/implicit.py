@@ -27,7 +27,8 @@ class StepStats(ctypes.Structure):
                 ("ms_total", ctypes.c_double), ("ms_far", ctypes.c_double),
                 ("far_calls", ctypes.c_int), ("near_pairs", ctypes.c_int),
                 ("r_near", ctypes.c_double), ("reparam_iters", ctypes.c_int),
-                ("av_iters", ctypes.c_int), ("ms_post", ctypes.c_double)]
+                ("av_iters", ctypes.c_int), ("ms_post", ctypes.c_double),
+                ("ms_far2", ctypes.c_double), ("far2_calls", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -107,6 +108,15 @@ class Stepper:
         torch.cuda.synchronize()
         call("ves3d_hip_stepper_precond", self.h, ptr(x), ptr(y))
         return y
+
+    def rhs_matvec(self, x):
+        """(b, A x) with one far-field pass for both."""
+        self._chk(x, self.n)
+        b = torch.empty_like(x)
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        call("ves3d_hip_stepper_rhs_matvec", self.h, ptr(x), ptr(b), ptr(y))
+        return b, y
 
     def residual(self):
         """|b - A x| / |b| of the current (positions, velocity, tension)."""
