@@ -179,9 +179,13 @@ def test_full_step_with_post(hip):
 def test_reference_cpu_case_p12(hip):
     """The plain method's CPU-runnable case (2 perturbed spheres at p = 12,
     centres (+-1.25, 0, 0), kb = 0.01, shear 0.1, dt = 1e-3, BiCGStab to
-    1e-8) against the oracle's solve of the same step: velocity and tension
-    to 1e-6 relative (both solves stop at relres 1e-8 from a zero guess with
-    the diagonal preconditioner, so they share the tension gauge)."""
+    1e-8) against the oracle's solve of the same step: velocity to 1e-6
+    relative; tension to 1e-6 without each vesicle's mean and 1e-5 with it.
+    Both solves stop at relres 1e-8 from a zero guess with the diagonal
+    preconditioner, so they share the tension gauge; the uniform tension of
+    each vesicle is weakly determined (the oracle's own 1e-8 and 1e-12 solves
+    differ by 1.3e-6 in tension, almost all of it a per-vesicle constant,
+    and by 2e-8 in velocity)."""
     p = 12
     X = W.reference_cpu_case(p)
     orc = ves.Implicit(X, p, dt=1e-3, kb=1e-2, shear=0.1)
@@ -198,7 +202,9 @@ def test_reference_cpu_case_p12(hip):
     Xn, U, sig = st.get_state()
     U, sig = U.cpu().numpy(), sig.cpu().numpy()
     assert np.linalg.norm(U - u0) / np.linalg.norm(u0) < 1e-6
-    assert np.linalg.norm(sig - s0) / np.linalg.norm(s0) < 1e-6
+    assert np.linalg.norm(sig - s0) / np.linalg.norm(s0) < 1e-5
+    d = (sig - sig.mean(axis=1, keepdims=True)) - (s0 - s0.mean(axis=1, keepdims=True))
+    assert np.linalg.norm(d) / np.linalg.norm(s0) < 1e-6
     assert np.max(np.abs(Xn.cpu().numpy() - (X + 1e-3 * U))) < 1e-14
     st.close()
 
