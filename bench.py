@@ -410,7 +410,7 @@ def main():
     ap.add_argument("--ref-tol", type=float, default=1e-10)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mixed-precision", type=int, default=1, choices=[1, 2],
+    ap.add_argument("--mixed-precision", type=int, default=2, choices=[1, 2],
                     help="precision of the learned form: 1 FP32 singular operator + FP32-"
                          "compensated far field, 2 FP64 singular operator + FP32-compensated far field")
     ap.add_argument("--far-kernel", default="scalar", choices=sorted(FAR_KINDS),
