@@ -125,10 +125,11 @@ int ves3d_hip_gemm_f32(char transA, char transB, int m, int n, int k, float alph
                        float *C, int ldc);
 
 /* Selects the kernel behind every grouped GEMM (gemm above, the SH
- * transforms, rotations and singular-operator build): 0 = VALU FMA tiles,
- * 1 = MFMA (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).  No counterpart
- * in the reference (its GEMMs are BLAS / PVFMM calls); the initial value
- * comes from the environment variable VES3D_GEMM ("mfma" / "valu"). */
+ * transforms, rotations, singular-operator build and the learned
+ * preconditioner's network): 0 = VALU FMA tiles, 1 = MFMA
+ * (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32, the default).  No
+ * counterpart in the reference (its GEMMs are BLAS / PVFMM calls); the
+ * environment variable VES3D_GEMM=valu sets the initial value to 0. */
 int ves3d_hip_set_gemm_kernel(int kind);
 
 /* Selects the kernel of the far field between vesicles inside the implicit

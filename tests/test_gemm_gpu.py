@@ -24,11 +24,12 @@ def cu(a, dt=torch.float64):
     return torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dt)
 
 
-@pytest.fixture
-def mfma(hip):
+@pytest.fixture(params=[1, 0], ids=["mfma", "valu"])
+def mfma(request, hip):
+    """Both kernels behind the grouped GEMM; the default (MFMA) is restored."""
+    call("ves3d_hip_set_gemm_kernel", request.param)
+    yield request.param
     call("ves3d_hip_set_gemm_kernel", 1)
-    yield
-    call("ves3d_hip_set_gemm_kernel", 0)
 
 
 @pytest.mark.parametrize("m,n,k", [(1, 1, 1), (16, 16, 4), (13, 9, 17), (64, 64, 16), (70, 130, 33),
@@ -76,7 +77,7 @@ def test_transforms_mfma_vs_valu(hip, p0, p1):
             torch.cuda.synchronize()
             outs.append((g.cpu().numpy(), c.cpu().numpy()))
         finally:
-            call("ves3d_hip_set_gemm_kernel", 0)
+            call("ves3d_hip_set_gemm_kernel", 1)
     for a, b in zip(outs[0], outs[1]):
         assert np.max(np.abs(a - b)) <= 1e-13 * max(1.0, np.max(np.abs(a)))
     assert np.max(np.abs(outs[1][0].reshape(nf, -1) - sph.shc2grid(Sh, p0, p1))) < 1e-12 * \

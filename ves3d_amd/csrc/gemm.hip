@@ -213,11 +213,12 @@ __global__ __launch_bounds__(GB_T) void k_gemm_mfma(const GemmDesc *__restrict__
     }
 }
 
-// Kernel choice: 1 = MFMA, 0 = VALU.  Initialised from VES3D_GEMM
-// ("mfma" / "valu") at load, changed by ves3d_hip_set_gemm_kernel.
+// Kernel choice: 1 = MFMA (the default), 0 = VALU.  Initialised from
+// VES3D_GEMM ("valu" selects the VALU tiles) at load, changed by
+// ves3d_hip_set_gemm_kernel.
 std::atomic<int> g_gemm_kind{[] {
   const char *e = getenv("VES3D_GEMM");
-  return (e && e[0] == 'm') ? 1 : 0;
+  return (e && e[0] == 'v') ? 0 : 1;
 }()};
 
 bool use_mfma() { return g_gemm_kind.load(std::memory_order_relaxed) == 1; }
