@@ -25,6 +25,8 @@ Workloads
         p=16 (278,528 points; own-vesicle pairs excluded).
   matvec: K Galerkin matvecs of the step workload (--dtype f32c: the mixed
         form) after a 512 MiB device copy that calibrates HBM counters.
+  probe: the VALU rate the GPU sustains (independent FMA chains, FP32
+        non-packed and FP64), the attainable ceiling of the far-field kernels.
 """
 import argparse
 import json
@@ -330,6 +332,21 @@ def bench_matvec(args):
     }
 
 
+def bench_probe(args):
+    """The VALU rate this GPU sustains (independent FMA chains, FP32 non-packed
+    and FP64): the attainable ceiling beside the published peaks."""
+    import ctypes
+    import ves3d_amd
+    ves3d_amd.init(0)
+    r32, r64 = ctypes.c_double(), ctypes.c_double()
+    ves3d_amd.call("ves3d_hip_valu_probe", 0, ctypes.byref(r32))
+    ves3d_amd.call("ves3d_hip_valu_probe", 1, ctypes.byref(r64))
+    return {"metric": "sustained lane FMA/s (FP32 non-packed, FP64)", "value": r32.value,
+            "unit": "lane FMA/s", "n_gpus": 1, "fp64_lane_fma_per_s": r64.value,
+            "fp32_tflops": 2 * r32.value / 1e12, "fp64_tflops": 2 * r64.value / 1e12,
+            "higher_is_better": True, "data": "synthetic", "config": {"workload": "probe"}}
+
+
 def bench_far(args):
     import torch
     import ves3d_amd
@@ -397,7 +414,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="step", choices=["step", "far", "matvec"])
+    ap.add_argument("--workload", default="step", choices=["step", "far", "matvec", "probe"])
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32c"])
     ap.add_argument("--nves", type=int, default=512)
     ap.add_argument("--p", type=int, default=16)
@@ -423,7 +440,8 @@ def main():
         print("bench.py: only the one-GPU form is built; --gpus %d is not supported "
               "(across GPUs the path would shard by vesicle)" % args.gpus, file=sys.stderr)
         sys.exit(2)
-    res = {"step": bench_step, "far": bench_far, "matvec": bench_matvec}[args.workload](args)
+    res = {"step": bench_step, "far": bench_far, "matvec": bench_matvec,
+           "probe": bench_probe}[args.workload](args)
     print(json.dumps(res))
 
 

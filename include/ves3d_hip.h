@@ -132,6 +132,12 @@ int ves3d_hip_gemm_f32(char transA, char transB, int m, int n, int k, float alph
  * environment variable VES3D_GEMM=valu sets the initial value to 0. */
 int ves3d_hip_set_gemm_kernel(int kind);
 
+/* The sustained VALU rate of this GPU: lane fused multiply-adds per second
+ * of independent FMA chains, FP32 non-packed (fp64 = 0) or FP64 (fp64 = 1),
+ * the reference for the far-field kernels' rooflines.  Synchronous.  No
+ * counterpart in the reference. */
+int ves3d_hip_valu_probe(int fp64, double *lane_fma_per_s);
+
 /* Selects the kernel of the far field between vesicles inside the implicit
  * step (StokesVelocity's PVFMMEval call, src/StokesVelocity.cc:481-485):
  * 0 = sources staged through LDS tiles, 1 = sources read through the scalar
