@@ -175,6 +175,14 @@ def bench_step(args):
                           "achieved": flop / (far_ms_plain * 1e-3) / 1e12,
                           "peak": PEAK_FP64_TFLOPS}}
 
+    # the sustained FP32 VALU rate of this GPU, measured in the same run
+    # (independent FMA chains): the attainable ceiling beside the published peak
+    import ctypes
+    r32 = ctypes.c_double()
+    ves3d_amd.call("ves3d_hip_valu_probe", 0, ctypes.byref(r32))
+    roof["probe_tflops_fp32"] = 2 * r32.value / 1e12
+    roof["frac_of_probe"] = achieved / roof["probe_tflops_fp32"]
+
     cpu = None if args.no_cpu else cpu_baseline_step(nv, p, X0, np.mean(
         [s["matvecs"] for s in stats_plain]), stats_plain[-1]["near_pairs"])
     plain.close()
