@@ -587,7 +587,8 @@ int ves3d_hip_lpc_num_params(void *handle, size_t *n_host);
 /* Number of training samples recorded so far (the buffer keeps the last
  * `capacity`). */
 int ves3d_hip_lpc_samples(void *handle, size_t *n_host);
-/* Device-to-device copies of the FP32 parameter vector. */
+/* Device-to-device copies of the FP32 parameter vector; setting parameters
+ * also zeroes the Adam moments and step count. */
 int ves3d_hip_lpc_set_params(void *handle, const float *params);
 int ves3d_hip_lpc_get_params(void *handle, float *params);
 /* Y [B][nin] = g(Z [B][nin]). */
@@ -597,6 +598,8 @@ int ves3d_hip_lpc_forward(void *handle, const float *Z, size_t B, float *Y,
  * gradient with respect to the parameters into grads (device, may be NULL). */
 int ves3d_hip_lpc_loss_grad(void *handle, const float *Z, const float *T, size_t B,
                             double *loss_host, float *grads, ves3d_stream_t stream);
+/* Decoupled weight decay of the Adam steps (AdamW; default 0, plain Adam). */
+int ves3d_hip_lpc_set_weight_decay(void *handle, float wd);
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with the last gradient. */
 int ves3d_hip_lpc_adam(void *handle, float lr, ves3d_stream_t stream);
 /* epochs passes over nine tenths of the recorded samples in shuffled

@@ -57,15 +57,16 @@ def loss_and_grads(params, Z, T):
     return loss, (gW1, gb1, gW2, gb2, gW3, gb3)
 
 
-def adam_step(params, grads, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
-    """One Adam update (Kingma & Ba 2015, Algorithm 1) on every tensor."""
+def adam_step(params, grads, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
+    """One Adam update (Kingma & Ba 2015, Algorithm 1) on every tensor, with
+    decoupled weight decay wd (Loshchilov & Hutter 2019)."""
     out_p, out_m, out_v = [], [], []
     for p, g, mm, vv in zip(params, grads, m, v):
         mm = b1 * mm + (1 - b1) * g
         vv = b2 * vv + (1 - b2) * g * g
         mh = mm / (1 - b1 ** t)
         vh = vv / (1 - b2 ** t)
-        out_p.append(p - lr * mh / (np.sqrt(vh) + eps))
+        out_p.append(p - lr * (mh / (np.sqrt(vh) + eps) + wd * p))
         out_m.append(mm)
         out_v.append(vv)
     return out_p, out_m, out_v

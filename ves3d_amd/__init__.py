@@ -138,6 +138,7 @@ _sig("ves3d_hip_lpc_forward", P, P, S, P, P)
 _sig("ves3d_hip_lpc_loss_grad", P, P, P, S, P, P, P)
 _sig("ves3d_hip_lpc_adam", P, ctypes.c_float, P)
 _sig("ves3d_hip_lpc_train", P, c_int, c_int, ctypes.c_float, P)
+_sig("ves3d_hip_lpc_set_weight_decay", P, ctypes.c_float)
 _sig("ves3d_hip_lpc_dataset_loss", P, P, P)
 
 _lib = None

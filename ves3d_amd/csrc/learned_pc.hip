@@ -144,15 +144,17 @@ __global__ void k_acc_pairs(const double *part, int nb, double *acc) {
   acc[0] += a;
   acc[1] += b;
 }
+// Adam (Kingma & Ba 2015, Algorithm 1) with decoupled weight decay wd
+// (Loshchilov & Hutter 2019): P -= lr (m_hat / (sqrt(v_hat) + eps) + wd P)
 __global__ void k_adam(float *P, const float *__restrict__ G, float *M, float *V, long n, float lr,
-                       float c1, float c2) {
+                       float c1, float c2, float wd) {
   GL(i, n) {
     const float g = G[i];
     const float m = 0.9f * M[i] + 0.1f * g;
     const float v = 0.999f * V[i] + 0.001f * g * g;
     M[i] = m;
     V[i] = v;
-    P[i] -= lr * (m / c1) / (sqrtf(v / c2) + 1e-8f);
+    P[i] -= lr * ((m / c1) / (sqrtf(v / c2) + 1e-8f) + wd * P[i]);
   }
 }
 
@@ -406,10 +408,17 @@ int LearnedPC::loss_grad(const float *Z, const float *T, long B, double *loss_ho
   return VES3D_SUCCESS;
 }
 
+int LearnedPC::reset_optimizer() {
+  V3D_HIP(hipMemset(M_, 0, sizeof(float) * np_));
+  V3D_HIP(hipMemset(V_, 0, sizeof(float) * np_));
+  t_ = 0;
+  return VES3D_SUCCESS;
+}
+
 int LearnedPC::adam(float lr, hipStream_t st) {
   ++t_;
   const float c1 = 1.0f - std::pow(0.9f, (float)t_), c2 = 1.0f - std::pow(0.999f, (float)t_);
-  k_adam<<<gb(np_), 256, 0, st>>>(P_, G_, M_, V_, np_, lr, c1, c2);
+  k_adam<<<gb(np_), 256, 0, st>>>(P_, G_, M_, V_, np_, lr, c1, c2, wd_);
   return launch_status();
 }
 

@@ -63,6 +63,10 @@ class LearnedPC {
   float *grads() const { return G_; }
   long samples() const { return count_; }
   bool ready() const { return trained_; }
+  // decoupled weight decay of the Adam steps (0: plain Adam)
+  void set_weight_decay(float wd) { wd_ = wd; }
+  // zeroes the Adam moments and step count (new parameters start afresh)
+  int reset_optimizer();
   void set_ready(bool r) { trained_ = r; }
 
  private:
@@ -81,6 +85,7 @@ class LearnedPC {
   int *perm_ = nullptr, *vidx_ = nullptr;
   float *Zneg_ = nullptr, *Yneg_ = nullptr;
   int best_epoch_ = 0;
+  float wd_ = 0.0f;
   float *Pbest_ = nullptr;
   static constexpr int kEvalParts = 1024;
   int t_ = 0;

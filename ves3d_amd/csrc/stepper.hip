@@ -1074,6 +1074,7 @@ int ves3d_hip_lpc_set_params(void *h, const float *params) {
   if (!h || !params) return VES3D_INVALID_PARAMETER;
   LearnedPC *l = (LearnedPC *)h;
   V3D_HIP(hipMemcpy(l->params(), params, sizeof(float) * l->num_params(), hipMemcpyDeviceToDevice));
+  V3D_CHECK(l->reset_optimizer());
   l->set_ready(true);
   return VES3D_SUCCESS;
 }
@@ -1099,6 +1100,12 @@ int ves3d_hip_lpc_loss_grad(void *h, const float *Z, const float *T, size_t B, d
     V3D_HIP(hipMemcpyAsync(grads, l->grads(), sizeof(float) * l->num_params(),
                            hipMemcpyDeviceToDevice, (hipStream_t)stream));
   return hip_status(hipStreamSynchronize((hipStream_t)stream));
+}
+
+int ves3d_hip_lpc_set_weight_decay(void *h, float wd) {
+  if (!h || !(wd >= 0.0f)) return VES3D_INVALID_PARAMETER;
+  ((LearnedPC *)h)->set_weight_decay(wd);
+  return VES3D_SUCCESS;
 }
 
 int ves3d_hip_lpc_adam(void *h, float lr, ves3d_stream_t stream) {

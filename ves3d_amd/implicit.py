@@ -246,6 +246,9 @@ class LearnedPC:
         call("ves3d_hip_lpc_train", self.h, epochs, batch, ctypes.c_float(lr), ctypes.byref(loss))
         return loss.value
 
+    def set_weight_decay(self, wd):
+        call("ves3d_hip_lpc_set_weight_decay", self.h, ctypes.c_float(wd))
+
     def dataset_loss(self):
         """(loss over all recorded samples, mean square of their targets)."""
         loss, tms = ctypes.c_double(), ctypes.c_double()
