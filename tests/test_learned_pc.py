@@ -77,4 +77,14 @@ def test_gpu_network_vs_oracle(hip):
     P1, _, _ = mlp.adam_step(P32, G0, zero, zero, 1, 1e-3)
     exp = np.concatenate([p.reshape(-1) for p in P1])
     assert np.max(np.abs(newp - exp)) / np.max(np.abs(exp)) < 1e-5
+    # with decoupled weight decay, from the same parameters (set_params also
+    # resets the optimizer state)
+    pc.set_params(torch.from_numpy(flat).cuda())
+    pc.set_weight_decay(0.1)
+    pc.loss_grad(torch.from_numpy(Z).cuda(), torch.from_numpy(T).cuda())
+    pc.adam(1e-3)
+    newp = pc.get_params().cpu().numpy()
+    P1, _, _ = mlp.adam_step(P32, G0, zero, zero, 1, 1e-3, wd=0.1)
+    exp = np.concatenate([p.reshape(-1) for p in P1])
+    assert np.max(np.abs(newp - exp)) / np.max(np.abs(exp)) < 1e-5
     pc.close()
