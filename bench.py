@@ -27,6 +27,10 @@ Workloads
         form) after a 512 MiB device copy that calibrates HBM counters.
   probe: the VALU rate the GPU sustains (independent FMA chains, FP32
         non-packed and FP64), the attainable ceiling of the far-field kernels.
+  stages: per-stage rooflines outside the far field: the batched singular
+        build (SL + DL, 2048 vesicles at p=16 and 512 at p=24, FP64 and FP32
+        transforms, with the FP32 error per output), the self-apply GEMV and
+        the SH transforms of the 512-vesicle step.
 """
 import argparse
 import json
@@ -340,6 +344,96 @@ def bench_matvec(args):
     }
 
 
+# singular build, SL + DL, per vesicle: SURVEY-table accounting of the
+# reference's dense-DFT algorithm (RotateAll, upsampling SHT, pole kernel,
+# transpose SHT, RotateTranspose, final Grid2SHC)
+SING_GFLOP_PER_VES = {16: 1.772, 24: 12.74}
+PEAK_HBM_TBS = 8.0
+
+
+def _timed(fn, reps=3):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    best = 1e30
+    for _ in range(reps):
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    return best
+
+
+def bench_stages(args):
+    """Per-stage rooflines outside the far field: the batched singular build
+    (SL + DL; 2048 vesicles at p = 16 and 512 at p = 24, FP64 and FP32
+    transforms, with the FP32 form's error per output), the self-apply GEMV
+    and the SH transforms of the 512-vesicle step."""
+    import torch
+    import ves3d_amd
+    from ves3d_amd import sph as S
+    from ves3d_amd import workloads as W
+    ves3d_amd.init(0)
+    out = {}
+    rng = np.random.default_rng(11)
+    for nv, p in ((2048, 16), (512, 24)):
+        X = torch.from_numpy(W.perturbed_shapes(nv, p, amp=0.1, seed=21)).cuda()
+        n = 3 * S.ncoef(p)
+        F = torch.from_numpy(rng.uniform(-1, 1, (nv, n))).cuda()   # traction coefficients
+        res = {}
+        mats = {}
+        for dt, name in ((torch.float64, "f64"), (torch.float32, "f32")):
+            SL = torch.empty((nv, n, n), dtype=dt, device="cuda")
+            DL = torch.empty((nv, n, n), dtype=dt, device="cuda")
+            ms = _timed(lambda: S.singular_build(X, nv, p, 2 * p, SL, DL), reps=2)
+            gf = SING_GFLOP_PER_VES[p] * nv
+            peak = PEAK_FP64_TFLOPS if dt == torch.float64 else PEAK_FP32_TFLOPS
+            res[name] = {"ms": ms, "tflops": gf / ms, "frac_of_peak": gf / ms / peak, "peak": peak}
+            V = torch.empty((nv, n), dtype=torch.float64, device="cuda")
+            Vd = torch.empty_like(V)
+            S.self_apply(SL, F, None, None, nv, p, V)
+            S.self_apply(None, None, DL, F, nv, p, Vd)
+            torch.cuda.synchronize()
+            mats[name] = (SL, DL, V, Vd)
+            _log("singular build %d ves p=%d %s: %.1f ms, %.1f TFLOP/s" % (nv, p, name, ms, gf / ms))
+        SLa, DLa, Va, Vda = mats["f64"]
+        SLb, DLb, Vb, Vdb = mats["f32"]
+        rel = lambda a, b: float((a.double() - b).abs().max() / b.abs().max())
+        res["f32_error_vs_f64"] = {"SL_matrix": rel(SLb, SLa), "DL_matrix": rel(DLb, DLa),
+                                   "SL_velocity": rel(Vb, Va), "DL_velocity": rel(Vdb, Vda)}
+        out["singular_build_%dves_p%d" % (nv, p)] = res
+        if nv == 2048:
+            # the self-apply GEMV of 512 vesicles at p = 16 in FP64 (HBM-bound)
+            SL5 = SLa[:512].contiguous()
+            F5, V5 = F[:512].contiguous(), torch.empty((512, n), dtype=torch.float64, device="cuda")
+            ms = _timed(lambda: S.self_apply(SL5, F5, None, None, 512, p, V5), reps=5)
+            byts = SL5.numel() * 8 + 2 * F5.numel() * 8
+            out["self_apply_512ves_p16_f64"] = {"ms": ms, "TBps": byts / ms / 1e9,
+                                                "frac_of_hbm": byts / ms / 1e9 / PEAK_HBM_TBS,
+                                                "bytes": byts}
+            del SL5
+        del mats, SLa, DLa, SLb, DLb
+        torch.cuda.empty_cache()
+    # SH transforms of the step: 3 x 512 functions, p = 16 grid -> coefficients,
+    # coefficients -> p = 32 grid (the far-field density upsampling)
+    p, N = 16, 3 * 512
+    g = torch.from_numpy(rng.standard_normal((N, S.ngrid(p)))).cuda()
+    c = torch.empty((N, S.ncoef(p)), dtype=torch.float64, device="cuda")
+    u = torch.empty((N, S.ngrid(2 * p)), dtype=torch.float64, device="cuda")
+    ms = _timed(lambda: S.grid2shc(g, N, p, p, c), reps=5)
+    byts = (g.numel() + c.numel()) * 8
+    out["grid2shc_1536f_p16"] = {"ms": ms, "TBps": byts / ms / 1e9, "bytes": byts}
+    ms = _timed(lambda: S.shc2grid(c, N, p, 2 * p, u), reps=5)
+    byts = (c.numel() + u.numel()) * 8
+    out["shc2grid_1536f_p16_to_p32"] = {"ms": ms, "TBps": byts / ms / 1e9, "bytes": byts}
+    return {"metric": "per-stage rooflines (singular build, self apply, SH transforms)",
+            "value": out["singular_build_2048ves_p16"]["f64"]["ms"], "unit": "ms",
+            "n_gpus": 1, "higher_is_better": False, "data": "synthetic",
+            "config": {"workload": "stages"}, "stages": out}
+
+
 def bench_probe(args):
     """The VALU rate this GPU sustains (independent FMA chains, FP32 non-packed
     and FP64): the attainable ceiling beside the published peaks."""
@@ -422,7 +516,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="step", choices=["step", "far", "matvec", "probe"])
+    ap.add_argument("--workload", default="step",
+                    choices=["step", "far", "matvec", "probe", "stages"])
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32c"])
     ap.add_argument("--nves", type=int, default=512)
     ap.add_argument("--p", type=int, default=16)
@@ -449,7 +544,7 @@ def main():
               "(across GPUs the path would shard by vesicle)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     res = {"step": bench_step, "far": bench_far, "matvec": bench_matvec,
-           "probe": bench_probe}[args.workload](args)
+           "probe": bench_probe, "stages": bench_stages}[args.workload](args)
     print(json.dumps(res))
 
 
