@@ -31,6 +31,8 @@ Workloads
         build (SL + DL, 2048 vesicles at p=16 and 512 at p=24, FP64 and FP32
         transforms, with the FP32 error per output), the self-apply GEMV and
         the SH transforms of the 512-vesicle step.
+  build: one singular build (SL + DL) of --nves vesicles at --p, FP64 and
+        FP32 transforms (a short program for hardware-counter runs).
 """
 import argparse
 import json
@@ -434,6 +436,32 @@ def bench_stages(args):
             "config": {"workload": "stages"}, "stages": out}
 
 
+def bench_build(args):
+    """One singular-operator build (SL + DL) of args.nves vesicles at args.p in
+    FP64 and in FP32 transforms: a short program for hardware-counter runs
+    (MFMA busy cycles and MFMA flops of the build's GEMMs)."""
+    import torch
+    import ves3d_amd
+    from ves3d_amd import sph as S
+    from ves3d_amd import workloads as W
+    ves3d_amd.init(0)
+    nv, p = args.nves, args.p
+    X = torch.from_numpy(W.perturbed_shapes(nv, p, amp=0.1, seed=21)).cuda()
+    n = 3 * S.ncoef(p)
+    out = {}
+    for dt, name in ((torch.float64, "f64"), (torch.float32, "f32")):
+        SL = torch.empty((nv, n, n), dtype=dt, device="cuda")
+        DL = torch.empty((nv, n, n), dtype=dt, device="cuda")
+        ms = _timed(lambda: S.singular_build(X, nv, p, 2 * p, SL, DL), reps=1)
+        gf = SING_GFLOP_PER_VES.get(p, float("nan")) * nv
+        out[name] = {"ms": ms, "tflops_algorithmic": gf / ms}
+        del SL, DL
+        torch.cuda.empty_cache()
+    return {"metric": "singular build (SL + DL) ms", "value": out["f64"]["ms"], "unit": "ms",
+            "n_gpus": 1, "higher_is_better": False, "data": "synthetic",
+            "config": {"workload": "singular build", "nves": nv, "p": p}, "build": out}
+
+
 def bench_probe(args):
     """The VALU rate this GPU sustains (independent FMA chains, FP32 non-packed
     and FP64): the attainable ceiling beside the published peaks."""
@@ -517,7 +545,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="step",
-                    choices=["step", "far", "matvec", "probe", "stages"])
+                    choices=["step", "far", "matvec", "probe", "stages", "build"])
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32c"])
     ap.add_argument("--nves", type=int, default=512)
     ap.add_argument("--p", type=int, default=16)
@@ -544,7 +572,7 @@ def main():
               "(across GPUs the path would shard by vesicle)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     res = {"step": bench_step, "far": bench_far, "matvec": bench_matvec,
-           "probe": bench_probe, "stages": bench_stages}[args.workload](args)
+           "probe": bench_probe, "stages": bench_stages, "build": bench_build}[args.workload](args)
     print(json.dumps(res))
 
 
