@@ -132,6 +132,13 @@ int ves3d_hip_gemm_f32(char transA, char transB, int m, int n, int k, float alph
  * environment variable VES3D_GEMM=valu sets the initial value to 0. */
 int ves3d_hip_set_gemm_kernel(int kind);
 
+/* Output tile of the MFMA grouped GEMM: -1 (the default) picks per
+ * descriptor list the shape that minimises padded work plus operand
+ * traffic; 0..8 force one of the shapes (rows x columns) 64x64, 64x16,
+ * 128x16, 256x16, 64x32, 128x32, 64x48, 128x48, 128x64.  For tests and
+ * tuning; no counterpart in the reference. */
+int ves3d_hip_set_gemm_tile(int tile);
+
 /* The sustained VALU rate of this GPU: lane fused multiply-adds per second
  * of independent FMA chains, FP32 non-packed (fp64 = 0) or FP64 (fp64 = 1),
  * the reference for the far-field kernels' rooflines.  Synchronous.  No

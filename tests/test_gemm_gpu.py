@@ -24,12 +24,20 @@ def cu(a, dt=torch.float64):
     return torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dt)
 
 
-@pytest.fixture(params=[1, 0], ids=["mfma", "valu"])
+GEMM_KINDS = [(1, -1), (0, -1)] + [(1, t) for t in range(9)]
+
+
+@pytest.fixture(params=GEMM_KINDS,
+                ids=["mfma", "valu"] + ["mfma-tile%d" % t for t in range(9)])
 def mfma(request, hip):
-    """Both kernels behind the grouped GEMM; the default (MFMA) is restored."""
-    call("ves3d_hip_set_gemm_kernel", request.param)
-    yield request.param
+    """Both kernels behind the grouped GEMM and every MFMA tile shape; the
+    defaults (MFMA, automatic tile) are restored."""
+    kind, tile = request.param
+    call("ves3d_hip_set_gemm_kernel", kind)
+    call("ves3d_hip_set_gemm_tile", tile)
+    yield kind
     call("ves3d_hip_set_gemm_kernel", 1)
+    call("ves3d_hip_set_gemm_tile", -1)
 
 
 @pytest.mark.parametrize("m,n,k", [(1, 1, 1), (16, 16, 4), (13, 9, 17), (64, 64, 16), (70, 130, 33),
@@ -67,8 +75,9 @@ def test_transforms_mfma_vs_valu(hip, p0, p1):
     Sh = RNG.standard_normal((nf, p0 * (p0 + 2)))
     outs = []
     gS = cu(Sh)
-    for kind in (0, 1):
+    for kind, tile in ((0, -1), (1, -1), (1, 0), (1, 3), (1, 7)):
         call("ves3d_hip_set_gemm_kernel", kind)
+        call("ves3d_hip_set_gemm_tile", tile)
         try:
             g = torch.empty(nf * 2 * p1 * (p1 + 1), dtype=torch.float64, device="cuda")
             c = torch.empty(nf * p0 * (p0 + 2), dtype=torch.float64, device="cuda")
@@ -78,7 +87,37 @@ def test_transforms_mfma_vs_valu(hip, p0, p1):
             outs.append((g.cpu().numpy(), c.cpu().numpy()))
         finally:
             call("ves3d_hip_set_gemm_kernel", 1)
-    for a, b in zip(outs[0], outs[1]):
-        assert np.max(np.abs(a - b)) <= 1e-13 * max(1.0, np.max(np.abs(a)))
+            call("ves3d_hip_set_gemm_tile", -1)
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.max(np.abs(a - b)) <= 1e-13 * max(1.0, np.max(np.abs(a)))
     assert np.max(np.abs(outs[1][0].reshape(nf, -1) - sph.shc2grid(Sh, p0, p1))) < 1e-12 * \
         max(1.0, np.max(np.abs(outs[1][0])))
+
+
+def test_singular_build_tiles(hip):
+    """The singular-operator build (grouped rotation and transform GEMMs of
+    every shape) with the automatic MFMA tiles against the 64x64 tile and
+    the VALU kernel: FP64 to 1e-12 of the largest entry, FP32 to 1e-5."""
+    from ves3d_amd import workloads as W
+    nv, p = 3, 8
+    X = torch.from_numpy(W.perturbed_shapes(nv, p, amp=0.1, seed=5)).cuda()
+    n = 3 * p * (p + 2)
+    for dt, tol in ((torch.float64, 1e-12), (torch.float32, 1e-5)):
+        res = []
+        for kind, tile in ((1, -1), (1, 0), (1, 5), (0, -1)):
+            call("ves3d_hip_set_gemm_kernel", kind)
+            call("ves3d_hip_set_gemm_tile", tile)
+            try:
+                SL = torch.empty((nv, n, n), dtype=dt, device="cuda")
+                DL = torch.empty((nv, n, n), dtype=dt, device="cuda")
+                print("singular build %s kind %d tile %d" % (dt, kind, tile), flush=True)
+                S.singular_build(X, nv, p, 2 * p, SL, DL)
+                torch.cuda.synchronize()
+                res.append((SL.double().cpu().numpy(), DL.double().cpu().numpy()))
+            finally:
+                call("ves3d_hip_set_gemm_kernel", 1)
+                call("ves3d_hip_set_gemm_tile", -1)
+        for o in res[1:]:
+            for a, b in zip(res[0], o):
+                assert np.max(np.abs(a - b)) <= tol * np.max(np.abs(a))

@@ -199,7 +199,11 @@ int blas_gemm(char ta, char tb, int m, int n, int k, T alpha, const T *A, int ld
       g_gemm[key] = d;
     }
   }
-  return gemm_grouped<T>(d, 1, 1, m, n, A, B, C, alpha, beta, nullptr);
+  GemmDesc shape{};
+  shape.M = m;
+  shape.N = n;
+  shape.K = k;
+  return gemm_grouped<T>(d, 1, 1, m, n, A, B, C, alpha, beta, nullptr, gemm_pick_tile(&shape, 1));
 }
 
 }  // namespace

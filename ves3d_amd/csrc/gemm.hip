@@ -98,13 +98,12 @@ __global__ __launch_bounds__(GB_T) void k_gemm_grouped(
   }
 }
 
-// MFMA form of the same tile: the 64x64 output tile is split into 2x2 wave
-// sub-tiles of 32x32, each a 2x2 block of 16x16 MFMA accumulators.  For both
-// v_mfma_f64_16x16x4f64 and v_mfma_f32_16x16x4f32 lane l supplies A(l%16, l/16)
-// and B(l/16, l%16) of a 16x4 / 4x16 step and holds 4 results of column
-// l%16; their rows differ: l/16 + 4r (f64) and 4(l/16) + r (f32) for result
-// register r (the accumulator layouts composable_kernel's xdlops_gemm.hpp
-// records as group_size 1 / 4).  LDS staging is shared with the VALU kernel.
+// MFMA operand layout.  For both v_mfma_f64_16x16x4f64 and
+// v_mfma_f32_16x16x4f32 lane l supplies A(l%16, l/16) and B(l/16, l%16) of a
+// 16x4 / 4x16 step and holds 4 results of column l%16; their rows differ:
+// l/16 + 4r (f64) and 4(l/16) + r (f32) for result register r (the
+// accumulator layouts composable_kernel's xdlops_gemm.hpp records as
+// group_size 1 / 4).
 template <typename T>
 struct Mfma;
 template <>
@@ -124,18 +123,27 @@ struct Mfma<float> {
   static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
 };
 
-template <typename T>
+// MFMA output tile of BM = 64 NTM rows by BN = 16 NTN columns: wave w owns
+// rows [16 NTM w, 16 NTM (w + 1)) and all BN columns, as NTM x NTN MFMA
+// accumulators.  Operands are staged through LDS in 16-deep K slabs (the
+// same coalesced lane mapping as the VALU kernel), the next slab's loads in
+// flight while the current slab's MFMAs run.  The K loop runs the MFMA steps
+// only up to the descriptor's K (rounded up to 4), and a wave whose rows all
+// lie past M skips its steps.
+template <typename T, int NTM, int NTN>
 __global__ __launch_bounds__(GB_T) void k_gemm_mfma(const GemmDesc *__restrict__ descs,
                                                     const T *__restrict__ A,
                                                     const T *__restrict__ B, T *__restrict__ C,
                                                     T alpha, T beta) {
   typedef typename Mfma<T>::v4 v4;
-  __shared__ T As[GB_K][GB_M + 1];
-  __shared__ T Bs[GB_K][GB_N + 1];
+  constexpr int BM = 64 * NTM, BN = 16 * NTN;
+  static_assert((BM * GB_K) % GB_T == 0 && (BN * GB_K) % GB_T == 0, "tile loads");
+  __shared__ T As[GB_K][BM + 1];
+  __shared__ T Bs[GB_K][BN + 1];
   const GemmDesc d = descs[blockIdx.y];
-  const int tiles_n = (d.N + GB_N - 1) / GB_N;
+  const int tiles_n = (d.N + BN - 1) / BN;
   const int tm = blockIdx.x / tiles_n, tn = blockIdx.x % tiles_n;
-  if (tm * GB_M >= d.M || tn * GB_N >= d.N) return;
+  if (tm * BM >= d.M || tn * BN >= d.N) return;
   const long b = blockIdx.z;
   const T *Ab = A + d.a_off + b * d.a_bs;
   const T *Bb = B + d.b_off + b * d.b_bs;
@@ -143,65 +151,98 @@ __global__ __launch_bounds__(GB_T) void k_gemm_mfma(const GemmDesc *__restrict__
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
-  const int m0 = tm * GB_M, n0 = tn * GB_N;
+  const int wm = w * 16 * NTM;
+  const int m0 = tm * BM, n0 = tn * BN;
   const bool a_kfast = (d.a_cs == 1);
   const bool b_nfast = (d.b_cs == 1);
-  v4 acc[2][2];
+  const bool wave_live = m0 + wm < d.M;
+  v4 acc[NTM][NTN];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < NTM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = v4{T(0), T(0), T(0), T(0)};
+    for (int j = 0; j < NTN; ++j) acc[i][j] = v4{T(0), T(0), T(0), T(0)};
 
+  // The next slab's operands are loaded into registers while the current
+  // slab's MFMAs run.
+  constexpr int QA = (BM * GB_K) / GB_T, QB = (BN * GB_K) / GB_T;
+  T ra[QA], rb[QB];
+  auto a_idx = [&](int q, int &m, int &k) {
+    const int e = tid + q * GB_T;
+    if (a_kfast) {
+      k = e % GB_K;
+      m = e / GB_K;
+    } else {
+      m = e % BM;
+      k = e / BM;
+    }
+  };
+  auto b_idx = [&](int q, int &n, int &k) {
+    const int e = tid + q * GB_T;
+    if (b_nfast) {
+      n = e % BN;
+      k = e / BN;
+    } else {
+      k = e % GB_K;
+      n = e / GB_K;
+    }
+  };
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < QA; ++q) {
+      int m, k;
+      a_idx(q, m, k);
+      const int gm = m0 + m, gk = k0 + k;
+      ra[q] = (gm < d.M && gk < d.K) ? Ab[gm * d.a_rs + gk * d.a_cs] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+      int n, k;
+      b_idx(q, n, k);
+      const int gn = n0 + n, gk = k0 + k;
+      rb[q] = (gn < d.N && gk < d.K) ? Bb[gk * d.b_rs + gn * d.b_cs] : T(0);
+    }
+  };
+  gload(0);
   for (int k0 = 0; k0 < d.K; k0 += GB_K) {
 #pragma unroll
-    for (int q = 0; q < (GB_M * GB_K) / GB_T; ++q) {
-      const int e = tid + q * GB_T;
+    for (int q = 0; q < QA; ++q) {
       int m, k;
-      if (a_kfast) {
-        k = e % GB_K;
-        m = e / GB_K;
-      } else {
-        m = e % GB_M;
-        k = e / GB_M;
-      }
-      const int gm = m0 + m, gk = k0 + k;
-      As[k][m] = (gm < d.M && gk < d.K) ? Ab[gm * d.a_rs + gk * d.a_cs] : T(0);
+      a_idx(q, m, k);
+      As[k][m] = ra[q];
     }
 #pragma unroll
-    for (int q = 0; q < (GB_N * GB_K) / GB_T; ++q) {
-      const int e = tid + q * GB_T;
+    for (int q = 0; q < QB; ++q) {
       int n, k;
-      if (b_nfast) {
-        n = e % GB_N;
-        k = e / GB_N;
-      } else {
-        k = e % GB_K;
-        n = e / GB_K;
-      }
-      const int gn = n0 + n, gk = k0 + k;
-      Bs[k][n] = (gn < d.N && gk < d.K) ? Bb[gk * d.b_rs + gn * d.b_cs] : T(0);
+      b_idx(q, n, k);
+      Bs[k][n] = rb[q];
     }
     __syncthreads();
+    if (k0 + GB_K < d.K) gload(k0 + GB_K);
+    const int kmax = d.K - k0;
+    if (wave_live) {
 #pragma unroll
-    for (int kk = 0; kk < GB_K; kk += 4) {
-      T a[2], bv[2];
+      for (int kk = 0; kk < GB_K; kk += 4) {
+        if (kk < kmax) {
+          T a[NTM], bv[NTN];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) a[i] = As[kk + lk][wm + i * 16 + li];
+          for (int i = 0; i < NTM; ++i) a[i] = As[kk + lk][wm + i * 16 + li];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) bv[j] = Bs[kk + lk][wn + j * 16 + li];
+          for (int j = 0; j < NTN; ++j) bv[j] = Bs[kk + lk][j * 16 + li];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+          for (int i = 0; i < NTM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<T>::step(a[i], bv[j], acc[i][j]);
+            for (int j = 0; j < NTN; ++j) acc[i][j] = Mfma<T>::step(a[i], bv[j], acc[i][j]);
+        }
+      }
     }
     __syncthreads();
   }
+  if (!wave_live) return;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < NTM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int gn = n0 + wn + j * 16 + li;
+    for (int j = 0; j < NTN; ++j) {
+      const int gn = n0 + j * 16 + li;
       if (gn >= d.N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -211,6 +252,41 @@ __global__ __launch_bounds__(GB_T) void k_gemm_mfma(const GemmDesc *__restrict__
         *c = (beta == T(0)) ? alpha * acc[i][j][r] : alpha * acc[i][j][r] + beta * (*c);
       }
     }
+}
+
+// MFMA tile shapes (NTM, NTN); index 0 is the 64x64 tile.
+constexpr int kTiles[][2] = {{1, 4}, {1, 1}, {2, 1}, {4, 1}, {1, 2},
+                             {2, 2}, {1, 3}, {2, 3}, {2, 4}};
+constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
+
+int gemm_pick_tile(const GemmDesc *h, int n) {
+  // Cost of a list in multiply-add equivalents: the padded multiply-adds
+  // (rows in whole 16-row wave blocks, K rounded up to 4), the A operand read
+  // once per column tile and C written once, at 39 multiply-adds per element
+  // (the MFMA rate over the HBM rate, the same in FP64 and FP32), and a fixed
+  // 16384 per tile for the prologue and the barriers.
+  int best = 0;
+  double best_cost = -1;
+  for (int t = 0; t < kNumTiles; ++t) {
+    const int BM = 64 * kTiles[t][0], BN = 16 * kTiles[t][1], WR = 16 * kTiles[t][0];
+    double cost = 0;
+    for (int i = 0; i < n; ++i) {
+      const GemmDesc &d = h[i];
+      if (d.M <= 0 || d.N <= 0 || d.K <= 0) continue;
+      const long full = d.M / BM, rem = d.M % BM;
+      const long rows = full * BM + (rem ? cdiv(rem, WR) * WR : 0);
+      const long cols = cdiv(d.N, BN) * (long)BN;
+      const long tiles = (full + (rem ? 1 : 0)) * cdiv(d.N, BN);
+      cost += (double)rows * cols * (4 * cdiv(d.K, 4)) +
+              39.0 * ((double)d.M * d.K * cdiv(d.N, BN) + (double)d.M * d.N) +
+              16384.0 * tiles;
+    }
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = t;
+    }
+  }
+  return best;
 }
 
 // Kernel choice: 1 = MFMA (the default), 0 = VALU.  Initialised from
@@ -223,26 +299,57 @@ std::atomic<int> g_gemm_kind{[] {
 
 bool use_mfma() { return g_gemm_kind.load(std::memory_order_relaxed) == 1; }
 
-template <typename T>
-int gemm_grouped(const GemmDesc *d_descs, int ndesc, int batch, int max_m,
-                 int max_n, const T *A, const T *B, T *C, T alpha, T beta,
-                 hipStream_t st) {
-  if (ndesc == 0 || batch == 0 || max_m == 0 || max_n == 0) return VES3D_SUCCESS;
-  if (ndesc > 65535 || batch > 65535) return VES3D_INVALID_PARAMETER;
-  const long tiles = (long)cdiv(max_m, GB_M) * cdiv(max_n, GB_N);
+// Forced MFMA tile (-1: the caller's choice), set by ves3d_hip_set_gemm_tile.
+std::atomic<int> g_gemm_tile{-1};
+
+template <typename T, int NTM, int NTN>
+int launch_mfma(const GemmDesc *d_descs, int ndesc, int batch, int max_m, int max_n,
+                const T *A, const T *B, T *C, T alpha, T beta, hipStream_t st) {
+  const long tiles = (long)cdiv(max_m, 64 * NTM) * cdiv(max_n, 16 * NTN);
   if (tiles > 0x7fffffffL) return VES3D_INVALID_PARAMETER;
   dim3 grid((unsigned)tiles, ndesc, batch);
-  if (use_mfma())
-    k_gemm_mfma<T><<<grid, GB_T, 0, st>>>(d_descs, A, B, C, alpha, beta);
-  else
-    k_gemm_grouped<T><<<grid, GB_T, 0, st>>>(d_descs, A, B, C, alpha, beta);
+  k_gemm_mfma<T, NTM, NTN><<<grid, GB_T, 0, st>>>(d_descs, A, B, C, alpha, beta);
   return launch_status();
 }
 
+template <typename T>
+int gemm_grouped(const GemmDesc *d_descs, int ndesc, int batch, int max_m,
+                 int max_n, const T *A, const T *B, T *C, T alpha, T beta,
+                 hipStream_t st, int tile) {
+  if (ndesc == 0 || batch == 0 || max_m == 0 || max_n == 0) return VES3D_SUCCESS;
+  if (ndesc > 65535 || batch > 65535) return VES3D_INVALID_PARAMETER;
+  if (tile < -1 || tile >= kNumTiles) return VES3D_INVALID_PARAMETER;
+  const int forced = g_gemm_tile.load(std::memory_order_relaxed);
+  if (forced >= 0) tile = forced;
+  if (!use_mfma()) {
+    const long tiles = (long)cdiv(max_m, GB_M) * cdiv(max_n, GB_N);
+    if (tiles > 0x7fffffffL) return VES3D_INVALID_PARAMETER;
+    dim3 grid((unsigned)tiles, ndesc, batch);
+    k_gemm_grouped<T><<<grid, GB_T, 0, st>>>(d_descs, A, B, C, alpha, beta);
+    return launch_status();
+  }
+#define V3D_TILE(I, NTM, NTN) \
+  case I:                    \
+    return launch_mfma<T, NTM, NTN>(d_descs, ndesc, batch, max_m, max_n, A, B, C, alpha, beta, st);
+  switch (tile < 0 ? 0 : tile) {
+    V3D_TILE(0, 1, 4)
+    V3D_TILE(1, 1, 1)
+    V3D_TILE(2, 2, 1)
+    V3D_TILE(3, 4, 1)
+    V3D_TILE(4, 1, 2)
+    V3D_TILE(5, 2, 2)
+    V3D_TILE(6, 1, 3)
+    V3D_TILE(7, 2, 3)
+    V3D_TILE(8, 2, 4)
+  }
+#undef V3D_TILE
+  return VES3D_INVALID_PARAMETER;
+}
+
 template int gemm_grouped<double>(const GemmDesc *, int, int, int, int, const double *,
-                                  const double *, double *, double, double, hipStream_t);
+                                  const double *, double *, double, double, hipStream_t, int);
 template int gemm_grouped<float>(const GemmDesc *, int, int, int, int, const float *,
-                                 const float *, float *, float, float, hipStream_t);
+                                 const float *, float *, float, float, hipStream_t, int);
 
 GemmDesc *upload_descs(const GemmDesc *h, int n) {
   GemmDesc *d = nullptr;
@@ -255,6 +362,12 @@ GemmDesc *upload_descs(const GemmDesc *h, int n) {
 }
 
 }  // namespace v3d
+
+extern "C" int ves3d_hip_set_gemm_tile(int tile) {
+  if (tile < -1 || tile >= v3d::kNumTiles) return VES3D_INVALID_PARAMETER;
+  v3d::g_gemm_tile.store(tile, std::memory_order_relaxed);
+  return VES3D_SUCCESS;
+}
 
 extern "C" int ves3d_hip_set_gemm_kernel(int kind) {
   if (kind != 0 && kind != 1) return VES3D_INVALID_PARAMETER;

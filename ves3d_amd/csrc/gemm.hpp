@@ -23,11 +23,18 @@ struct GemmDesc {
 };
 
 // Launches the descriptor list `d_descs` (device memory, `ndesc` entries) over
-// `batch` batches.  `max_m`, `max_n` bound the descriptors' M and N.
+// `batch` batches.  `max_m`, `max_n` bound the descriptors' M and N.  `tile`
+// selects the MFMA output tile (gemm_pick_tile); -1 is the 64x64 tile.
 template <typename T>
 int gemm_grouped(const GemmDesc *d_descs, int ndesc, int batch, int max_m,
                  int max_n, const T *A, const T *B, T *C, T alpha, T beta,
-                 hipStream_t st);
+                 hipStream_t st, int tile = -1);
+
+// MFMA output tile for a descriptor list: the shape (64..256 rows x 16..64
+// columns) that minimises the padded multiply-add count plus a per-tile
+// overhead over the list.  The lists of the SH / rotation stages are skinny
+// (N and K of 1..64, M up to 1e7), where one 64x64 tile pads most of its work.
+int gemm_pick_tile(const GemmDesc *h, int n);
 
 // Uploads a host descriptor list to a new device array.
 GemmDesc *upload_descs(const GemmDesc *h, int n);

@@ -356,7 +356,11 @@ int LearnedPC::gemm(int M, int N, int K, const float *A, long ars, long acs, con
     if (!d) return VES3D_MEMORY_ERROR;
     descs_[key] = d;
   }
-  return gemm_grouped<float>(d, 1, 1, M, N, A, Bm, C, 1.0f, 0.0f, st);
+  GemmDesc shape{};
+  shape.M = M;
+  shape.N = N;
+  shape.K = K;
+  return gemm_grouped<float>(d, 1, 1, M, N, A, Bm, C, 1.0f, 0.0f, st, gemm_pick_tile(&shape, 1));
 }
 
 int LearnedPC::forward(const float *Z, long B, float *Y, hipStream_t st) {

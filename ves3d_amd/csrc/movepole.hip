@@ -259,7 +259,8 @@ int rot_gemm(const double *X, long nrows, int np, const double *R, bool rm, doub
   h.c_cs = 1;
   const GemmDesc *d = cached_descs(std::vector<GemmDesc>{h});
   if (!d) return VES3D_MEMORY_ERROR;
-  return gemm_grouped<double>(d, 1, 1, h.M, h.N, X, R, out, 1.0, 0.0, st);
+  return gemm_grouped<double>(d, 1, 1, h.M, h.N, X, R, out, 1.0, 0.0, st,
+                              gemm_pick_tile(&h, 1));
 }
 
 }  // namespace
@@ -421,7 +422,7 @@ struct MovePoleG {
       const GemmDesc *d = cached_descs(h);
       if (!d) return VES3D_MEMORY_ERROR;
       V3D_CHECK(gemm_grouped<double>(d, p + 1, 1, 2 * p, (int)ns, Rsh.p, work.p, work2.p, 1.0,
-                                     0.0, st));
+                                     0.0, st, gemm_pick_tile(h.data(), p + 1)));
       k_same_order<<<nb(cnt), 256, 0, st>>>(work2.p, p, ns, dof, 0, work.p);
       V3D_CHECK(launch_status());
       V3D_CHECK(ves3d_hip_shtrans_backward(p, ns, work.p, 0, 0, res[v], st));

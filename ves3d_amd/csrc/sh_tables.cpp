@@ -460,8 +460,16 @@ const RotMatsDev *rot_mats(int p) {
       else partner[c] = sn ? c - len : c + len;
     }
   }
+  std::vector<double> phase(2L * p * (p + 1));
+  for (int j = 0; j < p; ++j)
+    for (int k = 0; k <= p; ++k) {
+      const double a = (double)k * j * kPI / p;
+      phase[2 * (j * (p + 1) + k)] = std::cos(a);
+      phase[2 * (j * (p + 1) + k) + 1] = std::sin(a);
+    }
   auto d = std::make_unique<RotMatsDev>();
   d->h = h;
+  d->phase = upload(phase);
   d->d64 = upload(h->data);
   d->d32 = upload(to_f32(h->data));
   d->perm = upload(h->perm);
@@ -469,7 +477,8 @@ const RotMatsDev *rot_mats(int p) {
   d->partner = upload(partner);
   d->order = upload(order);
   d->is_sin = upload(is_sin);
-  if (!d->d64 || !d->d32 || !d->perm || !d->iperm || !d->partner || !d->order || !d->is_sin)
+  if (!d->d64 || !d->d32 || !d->perm || !d->iperm || !d->partner || !d->order || !d->is_sin ||
+      !d->phase)
     return nullptr;
   RotMatsDev *r = d.get();
   g_rdev[key] = std::move(d);

@@ -80,6 +80,7 @@ struct RotMatsDev {
   int *partner = nullptr;  // coefficient index -> cos/sin partner coefficient index (or itself)
   int *order = nullptr;    // coefficient index -> order m
   int *is_sin = nullptr;   // coefficient index -> 1 if sine coefficient
+  double *phase = nullptr; // [p][p+1][2]: cos, sin of k j pi / p (longitude j, order k)
 };
 const RotMatsDev *rot_mats(int p);
 

@@ -28,7 +28,7 @@ enum PlanKind { kS2GLeg, kS2GFour, kG2SFour, kG2SLeg, kTFour, kTLeg };
 
 struct Plan {
   GemmDesc *d = nullptr;
-  int n = 0, max_m = 0, max_n = 0;
+  int n = 0, max_m = 0, max_n = 0, tile = 0;
 };
 
 std::mutex g_mu;
@@ -88,6 +88,7 @@ Plan make_plan(int kindv, int p0, int p1, long N, const ShMatsHost &h) {
     p.max_m = std::max(p.max_m, d.M);
     p.max_n = std::max(p.max_n, d.N);
   }
+  p.tile = gemm_pick_tile(v.data(), p.n);
   p.d = upload_descs(v.data(), p.n);
   return p;
 }
@@ -117,7 +118,7 @@ int run(int kind, int p0, int p1, long N, const ShMatsDev *m, const T *A, T *C,
   const Plan *pl = get_plan(kind, p0, p1, N, *m->h);
   if (!pl) return VES3D_MEMORY_ERROR;
   return gemm_grouped<T>(pl->d, pl->n, 1, pl->max_m, pl->max_n, A, mats_ptr<T>(m), C,
-                         alpha, T(0), st);
+                         alpha, T(0), st, pl->tile);
 }
 
 bool valid_order(int p) { return p >= 1 && p <= 255; }

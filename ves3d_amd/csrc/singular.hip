@@ -30,12 +30,16 @@ namespace v3d {
 namespace {
 
 std::mutex g_mu;
-std::map<std::tuple<int, int, int, int, long>, std::pair<GemmDesc *, int>> g_rplans;
+struct RotPlan {
+  GemmDesc *d;
+  int n, tile;
+};
+std::map<std::tuple<int, int, int, int, long>, RotPlan> g_rplans;
 
 // Grouped GEMM plan of the per-degree rotation products.
 // transpose=false: RotateAll, rows (j, d) of vesicle v, output slab t.
 // transpose=true : RotateTranspose, rows (j, d) of (v, t), Mr read transposed.
-const std::pair<GemmDesc *, int> *rot_plan(int p, int dof, bool transpose,
+const RotPlan *rot_plan(int p, int dof, bool transpose,
                                            const RotMatsHost &h) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;
@@ -76,7 +80,7 @@ const std::pair<GemmDesc *, int> *rot_plan(int p, int dof, bool transpose,
     }
   GemmDesc *dd = upload_descs(v.data(), (int)v.size());
   if (!dd) return nullptr;
-  return &(g_rplans[key] = std::make_pair(dd, (int)v.size()));
+  return &(g_rplans[key] = RotPlan{dd, (int)v.size(), gemm_pick_tile(v.data(), (int)v.size())});
 }
 
 template <typename T>
@@ -86,25 +90,30 @@ const double *rot_ptr<double>(const RotMatsDev *r) { return r->d64; }
 template <>
 const float *rot_ptr<float>(const RotMatsDev *r) { return r->d32; }
 
+// The four coefficient permutation kernels below run one wavefront per row
+// of Ncoef coefficients (four rows per 256-thread block): the row's indices
+// are decomposed once per wavefront, lanes stride over the row, and the
+// permuted reads stay inside one row (L1-resident).
+
 // B0p[v][j][d][pos] = phase_j(S0[v][d])[perm[pos]] with exp(-i k j pi / p)
-// (src/SphericalHarmonics.cc:398-418).
+// (src/SphericalHarmonics.cc:398-418); phase[j][k] = (cos, sin)(k j pi / p).
+// Rows: (v, j, d).
 template <typename T>
-__global__ void k_rot_phase_gather(const T *__restrict__ S0, long Nv, int p, int dof,
-                                   int ncoef, const int *__restrict__ perm,
-                                   const int *__restrict__ partner,
-                                   const int *__restrict__ order,
-                                   const int *__restrict__ is_sin, T *__restrict__ out) {
-  const long total = Nv * p * dof * (long)ncoef;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (long)gridDim.x * blockDim.x) {
-    const int pos = (int)(i % ncoef);
-    long r = i / ncoef;
-    const int d = (int)(r % dof);
-    r /= dof;
-    const int j = (int)(r % p);
-    const long v = r / p;
+__global__ __launch_bounds__(256) void k_rot_phase_gather(
+    const T *__restrict__ S0, long rows, int p, int dof, int ncoef,
+    const int *__restrict__ perm, const int *__restrict__ partner,
+    const int *__restrict__ order, const int *__restrict__ is_sin,
+    const double *__restrict__ phase, T *__restrict__ out) {
+  const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= rows) return;
+  const int d = (int)(row % dof);
+  const long r = row / dof;
+  const int j = (int)(r % p);
+  const long v = r / p;
+  const T *s = S0 + (v * dof + d) * ncoef;
+  T *o = out + row * ncoef;
+  for (int pos = threadIdx.x & 63; pos < ncoef; pos += 64) {
     const int c = perm[pos];
-    const T *s = S0 + (v * dof + d) * ncoef;
     const int k = order[c];
     T val;
     if (k == 0 || k == p) {
@@ -112,54 +121,52 @@ __global__ void k_rot_phase_gather(const T *__restrict__ S0, long Nv, int p, int
     } else {
       const bool sn = is_sin[c];
       const T x0 = s[sn ? partner[c] : c], x1 = s[sn ? c : partner[c]];
-      double sv, cv;
-      sincos((double)k * j * kPi / p, &sv, &cv);
+      const double cv = phase[2 * (j * (p + 1) + k)], sv = phase[2 * (j * (p + 1) + k) + 1];
       const T r0 = (T)cv, r1 = (T)(-sv);
       val = sn ? x0 * r1 + x1 * r0 : x0 * r0 - x1 * r1;
     }
-    out[i] = val;
+    o[pos] = val;
   }
 }
 
 // out[row][c] = in[row][iperm[c]]  (degree-major -> coefficient order)
 template <typename T>
-__global__ void k_scatter_coef(const T *__restrict__ in, long rows, int ncoef,
-                               const int *__restrict__ iperm, T *__restrict__ out) {
-  const long total = rows * ncoef;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (long)gridDim.x * blockDim.x) {
-    const int c = (int)(i % ncoef);
-    out[i] = in[(i - c) + iperm[c]];
-  }
+__global__ __launch_bounds__(256) void k_scatter_coef(const T *__restrict__ in, long rows,
+                                                      int ncoef, const int *__restrict__ iperm,
+                                                      T *__restrict__ out) {
+  const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= rows) return;
+  const T *s = in + row * ncoef;
+  T *o = out + row * ncoef;
+  for (int c = threadIdx.x & 63; c < ncoef; c += 64) o[c] = s[iperm[c]];
 }
 
 // out[row][pos] = in[row][perm[pos]]  (coefficient order -> degree-major)
 template <typename T>
-__global__ void k_gather_coef(const T *__restrict__ in, long rows, int ncoef,
-                              const int *__restrict__ perm, T *__restrict__ out) {
-  const long total = rows * ncoef;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (long)gridDim.x * blockDim.x) {
-    const int pos = (int)(i % ncoef);
-    out[i] = in[(i - pos) + perm[pos]];
-  }
+__global__ __launch_bounds__(256) void k_gather_coef(const T *__restrict__ in, long rows,
+                                                     int ncoef, const int *__restrict__ perm,
+                                                     T *__restrict__ out) {
+  const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= rows) return;
+  const T *s = in + row * ncoef;
+  T *o = out + row * ncoef;
+  for (int pos = threadIdx.x & 63; pos < ncoef; pos += 64) o[pos] = s[perm[pos]];
 }
 
 // out[v][t][j][d][c] = phase_j^+(in)[c] reading the degree-major input
-// (src/SphericalHarmonics.cc:511-537).
+// (src/SphericalHarmonics.cc:511-537).  Rows: (v, t, j, d).
 template <typename T>
-__global__ void k_rot_scatter_phase(const T *__restrict__ in, long Nv, int p, int dof,
-                                    int ncoef, const int *__restrict__ iperm,
-                                    const int *__restrict__ partner,
-                                    const int *__restrict__ order,
-                                    const int *__restrict__ is_sin, T *__restrict__ out) {
-  const long total = Nv * (p + 1) * (long)p * dof * ncoef;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (long)gridDim.x * blockDim.x) {
-    const int c = (int)(i % ncoef);
-    const long row = i / ncoef;
-    const int j = (int)((row / dof) % p);
-    const T *s = in + row * ncoef;
+__global__ __launch_bounds__(256) void k_rot_scatter_phase(
+    const T *__restrict__ in, long rows, int p, int dof, int ncoef,
+    const int *__restrict__ iperm, const int *__restrict__ partner,
+    const int *__restrict__ order, const int *__restrict__ is_sin,
+    const double *__restrict__ phase, T *__restrict__ out) {
+  const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= rows) return;
+  const int j = (int)((row / dof) % p);
+  const T *s = in + row * ncoef;
+  T *o = out + row * ncoef;
+  for (int c = threadIdx.x & 63; c < ncoef; c += 64) {
     const int k = order[c];
     T val;
     if (k == 0 || k == p) {
@@ -167,14 +174,16 @@ __global__ void k_rot_scatter_phase(const T *__restrict__ in, long Nv, int p, in
     } else {
       const bool sn = is_sin[c];
       const T x0 = s[iperm[sn ? partner[c] : c]], x1 = s[iperm[sn ? c : partner[c]]];
-      double sv, cv;
-      sincos((double)k * j * kPi / p, &sv, &cv);
+      const double cv = phase[2 * (j * (p + 1) + k)], sv = phase[2 * (j * (p + 1) + k) + 1];
       const T r0 = (T)cv, r1 = (T)sv;
       val = sn ? x0 * r1 + x1 * r0 : x0 * r0 - x1 * r1;
     }
-    out[i] = val;
+    o[c] = val;
   }
 }
+
+// Blocks of four rows.
+inline unsigned row_blocks(long rows) { return (unsigned)((rows + 3) / 4); }
 
 inline unsigned grid_for(long n) {
   long b = (n + 255) / 256;
@@ -195,14 +204,15 @@ int rotate_all(const T *S, long Nv, int p, int dof, T *out, T *work, hipStream_t
   const int nc = sh_ncoef(p);
   T *B0p = work;
   T *S1p = work + Nv * p * dof * (long)nc;
-  const long n0 = Nv * p * dof * (long)nc;
-  k_rot_phase_gather<T><<<grid_for(n0), 256, 0, st>>>(S, Nv, p, dof, nc, r->perm, r->partner,
-                                                      r->order, r->is_sin, B0p);
-  V3D_CHECK(launch_status());
-  V3D_CHECK(gemm_grouped<T>(pl->first, pl->second, (int)Nv, p * dof, 2 * p, B0p,
-                            rot_ptr<T>(r), S1p, T(1), T(0), st));
+  const long rows0 = Nv * p * (long)dof;
   const long rows = Nv * (p + 1) * (long)p * dof;
-  k_scatter_coef<T><<<grid_for(rows * nc), 256, 0, st>>>(S1p, rows, nc, r->iperm, out);
+  if (rows / 4 >= 0x7fffffffL) return VES3D_INVALID_PARAMETER;
+  k_rot_phase_gather<T><<<row_blocks(rows0), 256, 0, st>>>(
+      S, rows0, p, dof, nc, r->perm, r->partner, r->order, r->is_sin, r->phase, B0p);
+  V3D_CHECK(launch_status());
+  V3D_CHECK(gemm_grouped<T>(pl->d, pl->n, (int)Nv, p * dof, 2 * p, B0p,
+                            rot_ptr<T>(r), S1p, T(1), T(0), st, pl->tile));
+  k_scatter_coef<T><<<row_blocks(rows), 256, 0, st>>>(S1p, rows, nc, r->iperm, out);
   return launch_status();
 }
 
@@ -215,14 +225,15 @@ int rotate_transpose(const T *in, long Nv, int p, int dof, T *out, T *work, hipS
   if (!pl) return VES3D_MEMORY_ERROR;
   const int nc = sh_ncoef(p);
   const long rows = Nv * (p + 1) * (long)p * dof;
+  if (rows / 4 >= 0x7fffffffL) return VES3D_INVALID_PARAMETER;
   T *Ap = work, *Bp = work + rows * nc;
-  k_gather_coef<T><<<grid_for(rows * nc), 256, 0, st>>>(in, rows, nc, r->perm, Ap);
+  k_gather_coef<T><<<row_blocks(rows), 256, 0, st>>>(in, rows, nc, r->perm, Ap);
   V3D_CHECK(launch_status());
-  V3D_CHECK(gemm_grouped<T>(pl->first, pl->second, (int)Nv, p * dof, 2 * p, Ap,
-                            rot_ptr<T>(r), Bp, T(1), T(0), st));
-  k_rot_scatter_phase<T><<<grid_for(rows * nc), 256, 0, st>>>(Bp, Nv, p, dof, nc, r->iperm,
-                                                              r->partner, r->order,
-                                                              r->is_sin, out);
+  V3D_CHECK(gemm_grouped<T>(pl->d, pl->n, (int)Nv, p * dof, 2 * p, Ap,
+                            rot_ptr<T>(r), Bp, T(1), T(0), st, pl->tile));
+  k_rot_scatter_phase<T><<<row_blocks(rows), 256, 0, st>>>(Bp, rows, p, dof, nc, r->iperm,
+                                                           r->partner, r->order, r->is_sin,
+                                                           r->phase, out);
   return launch_status();
 }
 
@@ -277,32 +288,54 @@ __global__ void k_pole_kernel(const T *__restrict__ X, const T *__restrict__ Xt,
 
 // M1arr[v][a*Ncoef+k][b][lat][lon] from K2[v][j1][j2][t2*6+c][k]
 // (src/SphericalHarmonics.cc:1090-1131): t2=0 pole -> grid (j1, j2),
-// t2=1 pole -> grid (p0-j1, p0+j2); c = sym(a, b).
+// t2=1 pole -> grid (p0-j1, p0+j2); c = sym(a, b).  For fixed (v, a, b) this
+// is a transpose between k (contiguous in K2) and the grid point g
+// (contiguous in M1arr): a block moves a 32 (k) x 64 (g) tile through LDS so
+// that both the reads and the writes are whole contiguous runs.
+constexpr int kRaK = 32, kRaG = 64;
 template <typename T>
-__global__ void k_singular_rearrange(const T *__restrict__ K2, long Nv, int p0,
-                                     T *__restrict__ M1) {
+__global__ __launch_bounds__(256) void k_singular_rearrange(const T *__restrict__ K2, long Nv,
+                                                            int p0, T *__restrict__ M1) {
+  __shared__ T tile[kRaG][kRaK + 1];
   const int Ncoef = p0 * (p0 + 2), Ngrid = 2 * p0 * (p0 + 1);
-  const long total = Nv * 3L * Ncoef * 3L * Ngrid;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (long)gridDim.x * blockDim.x) {
-    const int g = (int)(i % Ngrid);
-    long r = i / Ngrid;
-    const int b = (int)(r % 3);
-    r /= 3;
-    const int k = (int)(r % Ncoef);
-    r /= Ncoef;
-    const int a = (int)(r % 3);
-    const long v = r / 3;
-    const int lat = g / (2 * p0), lon = g % (2 * p0);
-    int j1, j2, t2;
-    if (lon < p0) {
-      j1 = lat; j2 = lon; t2 = 0;
-    } else {
-      j1 = p0 - lat; j2 = lon - p0; t2 = 1;
+  const int tk = (Ncoef + kRaK - 1) / kRaK, tg = (Ngrid + kRaG - 1) / kRaG;
+  long bid = blockIdx.x;
+  const int gt = (int)(bid % tg);
+  bid /= tg;
+  const int kt = (int)(bid % tk);
+  bid /= tk;
+  const int ab = (int)(bid % 9);
+  const long v = bid / 9;
+  if (v >= Nv) return;
+  const int a = ab / 3, b = ab % 3;
+  const int lo = a < b ? a : b, hi = a < b ? b : a;
+  const int c = lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
+  const int k0 = kt * kRaK, g0 = gt * kRaG;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < (kRaK * kRaG) / 256; ++q) {
+    const int kk = t % kRaK, gg = t / kRaK + q * (256 / kRaK);
+    const int g = g0 + gg, k = k0 + kk;
+    T val = T(0);
+    if (g < Ngrid && k < Ncoef) {
+      const int lat = g / (2 * p0), lon = g % (2 * p0);
+      int j1, j2, t2;
+      if (lon < p0) {
+        j1 = lat; j2 = lon; t2 = 0;
+      } else {
+        j1 = p0 - lat; j2 = lon - p0; t2 = 1;
+      }
+      val = K2[((((v * (p0 + 1) + j1) * (long)p0 + j2) * 12 + t2 * 6 + c) * (long)Ncoef) + k];
     }
-    const int lo = a < b ? a : b, hi = a < b ? b : a;
-    const int c = lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
-    M1[i] = K2[((((v * (p0 + 1) + j1) * (long)p0 + j2) * 12 + t2 * 6 + c) * (long)Ncoef) + k];
+    tile[gg][kk] = val;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < (kRaK * kRaG) / 256; ++q) {
+    const int gg = t % kRaG, kk = t / kRaG + q * (256 / kRaG);
+    const int g = g0 + gg, k = k0 + kk;
+    if (g < Ngrid && k < Ncoef)
+      M1[(((v * 3 + a) * (long)Ncoef + k) * 3 + b) * Ngrid + g] = tile[gg][kk];
   }
 }
 
@@ -398,7 +431,9 @@ int singular_build(const double *X0, long Nv, int p0, int p1, T *SL, T *DL, hipS
       V3D_CHECK(launch_status());
       V3D_CHECK(sht_shc2grid_transpose<T>(K0, rot * 12, p1, p0, K1, w, st));
       V3D_CHECK(rotate_transpose<T>(K1, cv, p0, 12, K2, w, st));
-      k_singular_rearrange<T><<<grid_for(cv * 9 * Nc * Ng), 256, 0, st>>>(K2, cv, p0, M1a);
+      const long ra_blocks = cv * 9L * cdiv(Nc, kRaK) * cdiv(Ng, kRaG);
+      if (ra_blocks > 0x7fffffffL) return VES3D_INVALID_PARAMETER;
+      k_singular_rearrange<T><<<(unsigned)ra_blocks, 256, 0, st>>>(K2, cv, p0, M1a);
       V3D_CHECK(launch_status());
       V3D_CHECK(sht_grid2shc<T>(M1a, cv * 9 * Nc, p0, p0, dst + v0 * 9 * Nc * Nc, w, st));
     }
