@@ -69,6 +69,35 @@ def test_blas_gemm_mfma(mfma, m, n, k, ta, tb):
                                                     constant_values=1.0)) < tol
 
 
+@pytest.mark.parametrize("m,n,k", [(64, 32, 64), (128, 48, 32), (256, 16, 16), (96, 20, 36)])
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T")])
+def test_blas_gemm_vector_loads(mfma, m, n, k, ta, tb):
+    """Leading dimensions equal to the contiguous extents (multiples of 4), so
+    the A operand qualifies for 16-byte vector loads in both precisions."""
+    for s, tdt, ndt, tol in (("f64", torch.float64, np.float64, 1e-13),
+                             ("f32", torch.float32, np.float32, 3e-6)):
+        lda = m if ta == "N" else k
+        ldb = k if tb == "N" else n
+        ldc = m
+        A = RNG.standard_normal(lda * (k if ta == "N" else m)).astype(ndt)
+        B = RNG.standard_normal(ldb * (n if tb == "N" else k)).astype(ndt)
+        C = RNG.standard_normal(ldc * n).astype(ndt)
+        gA, gB, gC = cu(A, tdt), cu(B, tdt), cu(C, tdt)
+        print("gemm vec %s %s%s m=%d n=%d k=%d" % (s, ta, tb, m, n, k), flush=True)
+        call("ves3d_hip_gemm_" + s, ta.encode(), tb.encode(), m, n, k, 0.75, ptr(gA), lda, ptr(gB),
+             ldb, 0.25, ptr(gC), ldc)
+        torch.cuda.synchronize()
+        Am = A.astype(np.float64).reshape(-1, lda).T
+        Bm = B.astype(np.float64).reshape(-1, ldb).T
+        opA = Am[:k, :m].T if ta == "T" else Am[:m, :k]
+        opB = Bm[:n, :k].T if tb == "T" else Bm[:k, :n]
+        Cm = C.astype(np.float64).reshape(-1, ldc).T
+        want = 0.75 * (opA @ opB) + 0.25 * Cm
+        got = gC.cpu().numpy().astype(np.float64).reshape(-1, ldc).T
+        scale = 0.75 * (np.abs(opA) @ np.abs(opB)) + 0.25 * np.abs(Cm) + 1.0
+        assert np.max(np.abs(got - want) / scale) < tol
+
+
 @pytest.mark.parametrize("p0,p1", [(6, 12), (16, 32)])
 def test_transforms_mfma_vs_valu(hip, p0, p1):
     nf = 37

@@ -174,6 +174,16 @@ int blas_gemm(char ta, char tb, int m, int n, int k, T alpha, const T *A, int ld
   const bool bt = (tb == 'T' || tb == 't' || tb == 'C' || tb == 'c');
   const long ars = at ? lda : 1, acs = at ? 1 : lda;
   const long brs = bt ? ldb : 1, bcs = bt ? 1 : ldb;
+  GemmDesc h{};
+  h.M = m;
+  h.N = n;
+  h.K = k;
+  h.a_rs = ars;
+  h.a_cs = acs;
+  h.b_rs = brs;
+  h.b_cs = bcs;
+  h.c_rs = 1;
+  h.c_cs = ldc;
   GemmDesc *d = nullptr;
   {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -184,26 +194,12 @@ int blas_gemm(char ta, char tb, int m, int n, int k, T alpha, const T *A, int ld
     if (it != g_gemm.end()) {
       d = it->second;
     } else {
-      GemmDesc h{};
-      h.M = m;
-      h.N = n;
-      h.K = k;
-      h.a_rs = ars;
-      h.a_cs = acs;
-      h.b_rs = brs;
-      h.b_cs = bcs;
-      h.c_rs = 1;
-      h.c_cs = ldc;
       d = upload_descs(&h, 1);
       if (!d) return VES3D_MEMORY_ERROR;
       g_gemm[key] = d;
     }
   }
-  GemmDesc shape{};
-  shape.M = m;
-  shape.N = n;
-  shape.K = k;
-  return gemm_grouped<T>(d, 1, 1, m, n, A, B, C, alpha, beta, nullptr, gemm_pick_tile(&shape, 1));
+  return gemm_grouped<T>(d, 1, 1, m, n, A, B, C, alpha, beta, nullptr, gemm_pick_tile(&h, 1));
 }
 
 }  // namespace

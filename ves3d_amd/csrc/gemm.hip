@@ -5,6 +5,7 @@
 // the lane-to-element mapping chosen from the operand's unit stride so global
 // reads stay coalesced whether the operand is row- or column-contiguous.
 #include <atomic>
+#include <cstdint>
 #include <cstdlib>
 #include <vector>
 
@@ -130,13 +131,21 @@ struct Mfma<float> {
 // flight while the current slab's MFMAs run.  The K loop runs the MFMA steps
 // only up to the descriptor's K (rounded up to 4), and a wave whose rows all
 // lie past M skips its steps.
-template <typename T, int NTM, int NTN>
+//
+// AV: the A operand is read as 16-byte vectors along its contiguous direction
+// (k for a row-contiguous A, m for a column-contiguous one).  The host sets it
+// only when every descriptor keeps those vectors aligned and whole (offsets,
+// strides and the contiguous extent multiples of the vector length, base
+// pointer 16-byte aligned): the Fourier stages' operands.
+template <typename T, int NTM, int NTN, bool AV>
 __global__ __launch_bounds__(GB_T) void k_gemm_mfma(const GemmDesc *__restrict__ descs,
                                                     const T *__restrict__ A,
                                                     const T *__restrict__ B, T *__restrict__ C,
                                                     T alpha, T beta) {
   typedef typename Mfma<T>::v4 v4;
   constexpr int BM = 64 * NTM, BN = 16 * NTN;
+  constexpr int VEC = AV ? 16 / (int)sizeof(T) : 1;
+  typedef T vec_t __attribute__((ext_vector_type(VEC)));
   static_assert((BM * GB_K) % GB_T == 0 && (BN * GB_K) % GB_T == 0, "tile loads");
   __shared__ T As[GB_K][BM + 1];
   __shared__ T Bs[GB_K][BN + 1];
@@ -186,13 +195,39 @@ __global__ __launch_bounds__(GB_T) void k_gemm_mfma(const GemmDesc *__restrict__
       n = e / GB_K;
     }
   };
+  // vector q of the A slab: first element (m, k), vector along k (row-
+  // contiguous A) or along m
+  auto a_vidx = [&](int q, int &m, int &k) {
+    const int e = tid + q * GB_T;
+    if (a_kfast) {
+      k = (e % (GB_K / VEC)) * VEC;
+      m = e / (GB_K / VEC);
+    } else {
+      m = (e % (BM / VEC)) * VEC;
+      k = e / (BM / VEC);
+    }
+  };
   auto gload = [&](int k0) {
+    if constexpr (AV) {
 #pragma unroll
-    for (int q = 0; q < QA; ++q) {
-      int m, k;
-      a_idx(q, m, k);
-      const int gm = m0 + m, gk = k0 + k;
-      ra[q] = (gm < d.M && gk < d.K) ? Ab[gm * d.a_rs + gk * d.a_cs] : T(0);
+      for (int q = 0; q < QA / VEC; ++q) {
+        int m, k;
+        a_vidx(q, m, k);
+        const int gm = m0 + m, gk = k0 + k;
+        vec_t v = (gm < d.M && gk < d.K)
+                      ? *(const vec_t *)(Ab + gm * d.a_rs + gk * d.a_cs)
+                      : vec_t(T(0));
+#pragma unroll
+        for (int u = 0; u < VEC; ++u) ra[q * VEC + u] = v[u];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < QA; ++q) {
+        int m, k;
+        a_idx(q, m, k);
+        const int gm = m0 + m, gk = k0 + k;
+        ra[q] = (gm < d.M && gk < d.K) ? Ab[gm * d.a_rs + gk * d.a_cs] : T(0);
+      }
     }
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
@@ -204,11 +239,24 @@ __global__ __launch_bounds__(GB_T) void k_gemm_mfma(const GemmDesc *__restrict__
   };
   gload(0);
   for (int k0 = 0; k0 < d.K; k0 += GB_K) {
+    if constexpr (AV) {
 #pragma unroll
-    for (int q = 0; q < QA; ++q) {
-      int m, k;
-      a_idx(q, m, k);
-      As[k][m] = ra[q];
+      for (int q = 0; q < QA / VEC; ++q) {
+        int m, k;
+        a_vidx(q, m, k);
+#pragma unroll
+        for (int u = 0; u < VEC; ++u) {
+          if (a_kfast) As[k + u][m] = ra[q * VEC + u];
+          else As[k][m + u] = ra[q * VEC + u];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < QA; ++q) {
+        int m, k;
+        a_idx(q, m, k);
+        As[k][m] = ra[q];
+      }
     }
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
@@ -237,6 +285,34 @@ __global__ __launch_bounds__(GB_T) void k_gemm_mfma(const GemmDesc *__restrict__
     }
     __syncthreads();
   }
+  if (d.c_rs == 1 && d.c_cs != 1) {
+    // Column-contiguous C (the Fourier stages write their output transposed):
+    // the accumulators hold 16 columns x 4 rows per store instruction, 16
+    // short runs.  Each 16-column slice of the tile goes through LDS instead
+    // and leaves as whole columns, consecutive lanes on consecutive rows.
+#pragma unroll
+    for (int j = 0; j < NTN; ++j) {
+      __syncthreads();
+      if (wave_live) {
+#pragma unroll
+        for (int i = 0; i < NTM; ++i)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) As[li][wm + i * 16 + Mfma<T>::row(lk, r)] = acc[i][j][r];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < (16 * BM) / GB_T; ++q) {
+        const int e = tid + q * GB_T;
+        const int rr = e % BM, cc = e / BM;
+        const int gm = m0 + rr, gn = n0 + j * 16 + cc;
+        if (gm < d.M && gn < d.N) {
+          T *c = Cb + gm + gn * d.c_cs;
+          *c = (beta == T(0)) ? alpha * As[cc][rr] : alpha * As[cc][rr] + beta * (*c);
+        }
+      }
+    }
+    return;
+  }
   if (!wave_live) return;
 #pragma unroll
   for (int i = 0; i < NTM; ++i)
@@ -254,10 +330,13 @@ __global__ __launch_bounds__(GB_T) void k_gemm_mfma(const GemmDesc *__restrict__
     }
 }
 
-// MFMA tile shapes (NTM, NTN); index 0 is the 64x64 tile.
+// MFMA tile shapes (NTM, NTN); index 0 is the 64x64 tile.  A tile choice is
+// the shape index plus flags saying the list's A operand allows 16-byte
+// vector loads for 8-byte and for 4-byte elements.
 constexpr int kTiles[][2] = {{1, 4}, {1, 1}, {2, 1}, {4, 1}, {1, 2},
                              {2, 2}, {1, 3}, {2, 3}, {2, 4}};
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
+constexpr int kTileShapeMask = 15, kTileVec8 = 16, kTileVec4 = 32;
 
 int gemm_pick_tile(const GemmDesc *h, int n) {
   // Cost of a list in multiply-add equivalents: the padded multiply-adds
@@ -286,7 +365,22 @@ int gemm_pick_tile(const GemmDesc *h, int n) {
       best = t;
     }
   }
-  return best;
+  // whole aligned 16-byte A vectors for 8-byte (2 elements) / 4-byte (4) types
+  auto vec_ok = [&](long v) {
+    for (int i = 0; i < n; ++i) {
+      const GemmDesc &d = h[i];
+      if (d.a_off % v || d.a_bs % v) return false;
+      if (d.a_cs == 1) {
+        if (d.a_rs % v || d.K % v) return false;
+      } else if (d.a_rs == 1) {
+        if (d.a_cs % v || d.M % v) return false;
+      } else {
+        return false;
+      }
+    }
+    return n > 0;
+  };
+  return best | (vec_ok(2) ? kTileVec8 : 0) | (vec_ok(4) ? kTileVec4 : 0);
 }
 
 // Kernel choice: 1 = MFMA (the default), 0 = VALU.  Initialised from
@@ -303,12 +397,15 @@ bool use_mfma() { return g_gemm_kind.load(std::memory_order_relaxed) == 1; }
 std::atomic<int> g_gemm_tile{-1};
 
 template <typename T, int NTM, int NTN>
-int launch_mfma(const GemmDesc *d_descs, int ndesc, int batch, int max_m, int max_n,
+int launch_mfma(bool av, const GemmDesc *d_descs, int ndesc, int batch, int max_m, int max_n,
                 const T *A, const T *B, T *C, T alpha, T beta, hipStream_t st) {
   const long tiles = (long)cdiv(max_m, 64 * NTM) * cdiv(max_n, 16 * NTN);
   if (tiles > 0x7fffffffL) return VES3D_INVALID_PARAMETER;
   dim3 grid((unsigned)tiles, ndesc, batch);
-  k_gemm_mfma<T, NTM, NTN><<<grid, GB_T, 0, st>>>(d_descs, A, B, C, alpha, beta);
+  if (av)
+    k_gemm_mfma<T, NTM, NTN, true><<<grid, GB_T, 0, st>>>(d_descs, A, B, C, alpha, beta);
+  else
+    k_gemm_mfma<T, NTM, NTN, false><<<grid, GB_T, 0, st>>>(d_descs, A, B, C, alpha, beta);
   return launch_status();
 }
 
@@ -318,9 +415,13 @@ int gemm_grouped(const GemmDesc *d_descs, int ndesc, int batch, int max_m,
                  hipStream_t st, int tile) {
   if (ndesc == 0 || batch == 0 || max_m == 0 || max_n == 0) return VES3D_SUCCESS;
   if (ndesc > 65535 || batch > 65535) return VES3D_INVALID_PARAMETER;
-  if (tile < -1 || tile >= kNumTiles) return VES3D_INVALID_PARAMETER;
+  if (tile < -1 || (tile >= 0 && (tile & kTileShapeMask) >= kNumTiles))
+    return VES3D_INVALID_PARAMETER;
+  const bool av = tile >= 0 && (tile & (sizeof(T) == 8 ? kTileVec8 : kTileVec4)) &&
+                  ((uintptr_t)A & 15) == 0;
+  int shape = tile < 0 ? 0 : (tile & kTileShapeMask);
   const int forced = g_gemm_tile.load(std::memory_order_relaxed);
-  if (forced >= 0) tile = forced;
+  if (forced >= 0) shape = forced;
   if (!use_mfma()) {
     const long tiles = (long)cdiv(max_m, GB_M) * cdiv(max_n, GB_N);
     if (tiles > 0x7fffffffL) return VES3D_INVALID_PARAMETER;
@@ -328,10 +429,11 @@ int gemm_grouped(const GemmDesc *d_descs, int ndesc, int batch, int max_m,
     k_gemm_grouped<T><<<grid, GB_T, 0, st>>>(d_descs, A, B, C, alpha, beta);
     return launch_status();
   }
-#define V3D_TILE(I, NTM, NTN) \
-  case I:                    \
-    return launch_mfma<T, NTM, NTN>(d_descs, ndesc, batch, max_m, max_n, A, B, C, alpha, beta, st);
-  switch (tile < 0 ? 0 : tile) {
+#define V3D_TILE(I, NTM, NTN)                                                                  \
+  case I:                                                                                      \
+    return launch_mfma<T, NTM, NTN>(av, d_descs, ndesc, batch, max_m, max_n, A, B, C, alpha, \
+                                    beta, st);
+  switch (shape) {
     V3D_TILE(0, 1, 4)
     V3D_TILE(1, 1, 1)
     V3D_TILE(2, 2, 1)

@@ -31,9 +31,11 @@ int gemm_grouped(const GemmDesc *d_descs, int ndesc, int batch, int max_m,
                  hipStream_t st, int tile = -1);
 
 // MFMA output tile for a descriptor list: the shape (64..256 rows x 16..64
-// columns) that minimises the padded multiply-add count plus a per-tile
-// overhead over the list.  The lists of the SH / rotation stages are skinny
-// (N and K of 1..64, M up to 1e7), where one 64x64 tile pads most of its work.
+// columns) that minimises the padded multiply-add count plus operand traffic
+// and a per-tile overhead over the list, with flags recording whether the
+// list's A operand can be read as aligned 16-byte vectors.  The lists of the
+// SH / rotation stages are skinny (N and K of 1..64, M up to 1e7), where one
+// 64x64 tile pads most of its work.
 int gemm_pick_tile(const GemmDesc *h, int n);
 
 // Uploads a host descriptor list to a new device array.

@@ -337,30 +337,26 @@ int LearnedPC::ensure_batch(long B) {
 int LearnedPC::gemm(int M, int N, int K, const float *A, long ars, long acs, const float *Bm,
                     long brs, long bcs, float *C, hipStream_t st) {
   auto key = std::make_tuple(M, N, K, ars, acs, brs, bcs);
+  GemmDesc h{};
+  h.M = M;
+  h.N = N;
+  h.K = K;
+  h.a_rs = ars;
+  h.a_cs = acs;
+  h.b_rs = brs;
+  h.b_cs = bcs;
+  h.c_rs = N;
+  h.c_cs = 1;
   GemmDesc *d = nullptr;
   auto it = descs_.find(key);
   if (it != descs_.end()) {
     d = it->second;
   } else {
-    GemmDesc h{};
-    h.M = M;
-    h.N = N;
-    h.K = K;
-    h.a_rs = ars;
-    h.a_cs = acs;
-    h.b_rs = brs;
-    h.b_cs = bcs;
-    h.c_rs = N;
-    h.c_cs = 1;
     d = upload_descs(&h, 1);
     if (!d) return VES3D_MEMORY_ERROR;
     descs_[key] = d;
   }
-  GemmDesc shape{};
-  shape.M = M;
-  shape.N = N;
-  shape.K = K;
-  return gemm_grouped<float>(d, 1, 1, M, N, A, Bm, C, 1.0f, 0.0f, st, gemm_pick_tile(&shape, 1));
+  return gemm_grouped<float>(d, 1, 1, M, N, A, Bm, C, 1.0f, 0.0f, st, gemm_pick_tile(&h, 1));
 }
 
 int LearnedPC::forward(const float *Z, long B, float *Y, hipStream_t st) {
