@@ -67,8 +67,10 @@ def test_shc2pole(hip, p, N):
     assert rel(gp.cpu().numpy(), P) < 1e-13
 
 
-@pytest.mark.parametrize("p,N,dof", [(6, 2, 3), (8, 1, 12), (16, 2, 3)])
+@pytest.mark.parametrize("p,N,dof", [(6, 2, 3), (8, 1, 12), (16, 2, 3), (32, 1, 1)])
 def test_rotations(hip, p, N, dof):
+    """RotateAll / RotateTranspose against the oracle; p = 32 in FP64 takes
+    the scatter kernel's unstaged path (four rows exceed its LDS staging)."""
     nc = p * (p + 2)
     S = RNG.standard_normal((N * dof, nc))
     R = O.rotate_all(S, p, dof)

@@ -339,26 +339,27 @@ constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 constexpr int kTileShapeMask = 15, kTileVec8 = 16, kTileVec4 = 32;
 
 int gemm_pick_tile(const GemmDesc *h, int n) {
-  // Cost of a list in multiply-add equivalents: the padded multiply-adds
-  // (rows in whole 16-row wave blocks, K rounded up to 4), the A operand read
-  // once per column tile and C written once, at 39 multiply-adds per element
-  // (the MFMA rate over the HBM rate, the same in FP64 and FP32), and a fixed
-  // 16384 per tile for the prologue and the barriers.
+  // Cost of a list in multiply-add equivalents: the padded multiply-adds of
+  // whole tiles (a partly filled tile holds its block's residency even where
+  // its waves idle; K rounded up to 4), the A operand read once per column
+  // tile and C written once, at 39 multiply-adds per element (the MFMA rate
+  // over the HBM rate, the same in FP64 and FP32), and a fixed 16384 per
+  // tile for the prologue and the barriers.  The 256-row shape is left to
+  // forced selection: its 35 KB of FP64 LDS per block measured slower than
+  // the 64- and 128-row shapes on every SH / rotation plan.
   int best = 0;
   double best_cost = -1;
   for (int t = 0; t < kNumTiles; ++t) {
-    const int BM = 64 * kTiles[t][0], BN = 16 * kTiles[t][1], WR = 16 * kTiles[t][0];
+    if (kTiles[t][0] > 2) continue;
+    const int BM = 64 * kTiles[t][0], BN = 16 * kTiles[t][1];
     double cost = 0;
     for (int i = 0; i < n; ++i) {
       const GemmDesc &d = h[i];
       if (d.M <= 0 || d.N <= 0 || d.K <= 0) continue;
-      const long full = d.M / BM, rem = d.M % BM;
-      const long rows = full * BM + (rem ? cdiv(rem, WR) * WR : 0);
-      const long cols = cdiv(d.N, BN) * (long)BN;
-      const long tiles = (full + (rem ? 1 : 0)) * cdiv(d.N, BN);
-      cost += (double)rows * cols * (4 * cdiv(d.K, 4)) +
-              39.0 * ((double)d.M * d.K * cdiv(d.N, BN) + (double)d.M * d.N) +
-              16384.0 * tiles;
+      const long tiles_m = cdiv(d.M, BM), tiles_n = cdiv(d.N, BN);
+      cost += (double)tiles_m * BM * tiles_n * BN * (4 * cdiv(d.K, 4)) +
+              39.0 * ((double)d.M * d.K * tiles_n + (double)d.M * d.N) +
+              16384.0 * tiles_m * tiles_n;
     }
     if (best_cost < 0 || cost < best_cost) {
       best_cost = cost;

@@ -467,9 +467,26 @@ const RotMatsDev *rot_mats(int p) {
       phase[2 * (j * (p + 1) + k)] = std::cos(a);
       phase[2 * (j * (p + 1) + k) + 1] = std::sin(a);
     }
+  std::vector<int> scatter(4L * Ncoef);
+  for (int c = 0; c < Ncoef; ++c) {
+    int *e = &scatter[4L * c];
+    const int k = order[c];
+    e[2] = k;
+    if (k == 0 || k == p) {
+      e[0] = iperm[c];
+      e[1] = 0;
+      e[3] = 0;
+    } else {
+      const bool sn = is_sin[c] != 0;
+      e[0] = iperm[sn ? partner[c] : c];
+      e[1] = iperm[sn ? c : partner[c]];
+      e[3] = sn ? 2 : 1;
+    }
+  }
   auto d = std::make_unique<RotMatsDev>();
   d->h = h;
   d->phase = upload(phase);
+  d->scatter = upload(scatter);
   d->d64 = upload(h->data);
   d->d32 = upload(to_f32(h->data));
   d->perm = upload(h->perm);
@@ -478,7 +495,7 @@ const RotMatsDev *rot_mats(int p) {
   d->order = upload(order);
   d->is_sin = upload(is_sin);
   if (!d->d64 || !d->d32 || !d->perm || !d->iperm || !d->partner || !d->order || !d->is_sin ||
-      !d->phase)
+      !d->phase || !d->scatter)
     return nullptr;
   RotMatsDev *r = d.get();
   g_rdev[key] = std::move(d);

@@ -81,6 +81,10 @@ struct RotMatsDev {
   int *order = nullptr;    // coefficient index -> order m
   int *is_sin = nullptr;   // coefficient index -> 1 if sine coefficient
   double *phase = nullptr; // [p][p+1][2]: cos, sin of k j pi / p (longitude j, order k)
+  // per coefficient c of RotateTranspose's output: degree-major sources of
+  // c and of its cos/sin partner, its order, and 0 (m = 0 or p: a copy),
+  // 1 (cosine) or 2 (sine) -- [Ncoef][4]
+  int *scatter = nullptr;
 };
 const RotMatsDev *rot_mats(int p);
 

@@ -154,33 +154,48 @@ __global__ __launch_bounds__(256) void k_gather_coef(const T *__restrict__ in, l
 }
 
 // out[v][t][j][d][c] = phase_j^+(in)[c] reading the degree-major input
-// (src/SphericalHarmonics.cc:511-537).  Rows: (v, t, j, d).
-template <typename T>
-__global__ __launch_bounds__(256) void k_rot_scatter_phase(
-    const T *__restrict__ in, long rows, int p, int dof, int ncoef,
-    const int *__restrict__ iperm, const int *__restrict__ partner,
-    const int *__restrict__ order, const int *__restrict__ is_sin,
-    const double *__restrict__ phase, T *__restrict__ out) {
+// (src/SphericalHarmonics.cc:511-537).  Rows: (v, t, j, d).  tab[c] holds
+// c's two degree-major sources, its order and its kind (copy, cos, sin), so
+// each output takes one 16-byte table read before its data reads.
+template <typename T, bool STAGED>
+__global__ __launch_bounds__(256) void k_rot_scatter_phase(const T *__restrict__ in, long rows,
+                                                           int p, int dof, int ncoef,
+                                                           const int4 *__restrict__ tab,
+                                                           const double *__restrict__ phase,
+                                                           T *__restrict__ out) {
+  // STAGED: the block's four input rows go through LDS with contiguous
+  // reads, and the permuted reads hit LDS (rows up to kScatterStageBytes)
+  extern __shared__ unsigned char rsp_smem[];
   const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= rows) return;
-  const int j = (int)((row / dof) % p);
+  const bool live = row < rows;
   const T *s = in + row * ncoef;
+  if constexpr (STAGED) {
+    T *srow = (T *)rsp_smem + (threadIdx.x >> 6) * ncoef;
+    if (live)
+      for (int c = threadIdx.x & 63; c < ncoef; c += 64) srow[c] = s[c];
+    __syncthreads();
+    s = srow;
+  }
+  if (!live) return;
+  const int j = (int)((row / dof) % p);
+  const double *ph = phase + 2L * j * (p + 1);
   T *o = out + row * ncoef;
   for (int c = threadIdx.x & 63; c < ncoef; c += 64) {
-    const int k = order[c];
+    const int4 e = tab[c];
     T val;
-    if (k == 0 || k == p) {
-      val = s[iperm[c]];
+    if (e.w == 0) {
+      val = s[e.x];
     } else {
-      const bool sn = is_sin[c];
-      const T x0 = s[iperm[sn ? partner[c] : c]], x1 = s[iperm[sn ? c : partner[c]]];
-      const double cv = phase[2 * (j * (p + 1) + k)], sv = phase[2 * (j * (p + 1) + k) + 1];
-      const T r0 = (T)cv, r1 = (T)sv;
-      val = sn ? x0 * r1 + x1 * r0 : x0 * r0 - x1 * r1;
+      const T x0 = s[e.x], x1 = s[e.y];
+      const T r0 = (T)ph[2 * e.z], r1 = (T)ph[2 * e.z + 1];
+      val = e.w == 2 ? x0 * r1 + x1 * r0 : x0 * r0 - x1 * r1;
     }
     o[c] = val;
   }
 }
+
+// Largest LDS staging of four rows (the rotation scatter at p <= 31 in FP64).
+constexpr size_t kScatterStageBytes = 32768;
 
 // Blocks of four rows.
 inline unsigned row_blocks(long rows) { return (unsigned)((rows + 3) / 4); }
@@ -231,9 +246,13 @@ int rotate_transpose(const T *in, long Nv, int p, int dof, T *out, T *work, hipS
   V3D_CHECK(launch_status());
   V3D_CHECK(gemm_grouped<T>(pl->d, pl->n, (int)Nv, p * dof, 2 * p, Ap,
                             rot_ptr<T>(r), Bp, T(1), T(0), st, pl->tile));
-  k_rot_scatter_phase<T><<<row_blocks(rows), 256, 0, st>>>(Bp, rows, p, dof, nc, r->iperm,
-                                                           r->partner, r->order, r->is_sin,
-                                                           r->phase, out);
+  const size_t stage = 4 * nc * sizeof(T);
+  if (stage <= kScatterStageBytes)
+    k_rot_scatter_phase<T, true><<<row_blocks(rows), 256, stage, st>>>(
+        Bp, rows, p, dof, nc, (const int4 *)r->scatter, r->phase, out);
+  else
+    k_rot_scatter_phase<T, false><<<row_blocks(rows), 256, 0, st>>>(
+        Bp, rows, p, dof, nc, (const int4 *)r->scatter, r->phase, out);
   return launch_status();
 }
 
