@@ -410,6 +410,10 @@ __global__ void k_pack_far32(const double *__restrict__ spos, const double *__re
   rec[i] = r;
 }
 
+// The r^2 bias that removes a target's own group from the FP64 sums (see
+// kOwnGroupBias below): about 1e-15 |f| per own-group source.
+constexpr double kOwnGroupBias64 = 1e30;
+
 __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f64(
     const FarRec64 *__restrict__ rec, long spg, long n_groups, const double *__restrict__ tpos,
     long n_trg, long tpg, int nsplit, double csl, double *__restrict__ part) {
@@ -417,27 +421,27 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f64(
   const long tb = blockIdx.x / nsplit;
   const long t = tb * FF_BLOCK + threadIdx.x;
   const bool tvalid = t < n_trg;
-  double tx = 0, ty = 0, tz = 0;
-  if (tvalid) {
-    tx = tpos[3 * t];
-    ty = tpos[3 * t + 1];
-    tz = tpos[3 * t + 2];
-  }
-  const long my_g = tvalid ? t / tpg : -1;
+  // lanes past the last target compute for it and store nothing, and a
+  // target's own group enters through an r^2 bias, so the source loop's
+  // control flow is uniform (its addressing stays scalar)
+  const long tc = tvalid ? t : n_trg - 1;
+  const double tx = tpos[3 * tc], ty = tpos[3 * tc + 1], tz = tpos[3 * tc + 2];
+  const long my_g = tc / tpg;
   const long tfirst = tb * FF_BLOCK;
   const long tlast = min(tfirst + FF_BLOCK, n_trg) - 1;
   double Sx = 0, Sy = 0, Sz = 0;
   long g0, g1;
   slice_range(slice, nsplit, 0, spg, n_groups, g0, g1);
+  const int ns = (int)spg;
   for (long g = g0; g < g1; ++g) {
     if (tfirst / tpg == g && tlast / tpg == g) continue;  // block-uniform skip
-    if (!tvalid || my_g == g) continue;
+    const double bias = my_g == g ? kOwnGroupBias64 : 0.0;
     const FarRec64 *__restrict__ r = rec + g * spg;
 #pragma unroll 4
-    for (long j = 0; j < spg; ++j) {
+    for (int j = 0; j < ns; ++j) {
       const FarRec64 s = r[j];
       const double dx = tx - s.x, dy = ty - s.y, dz = tz - s.z;
-      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, bias)));
       const double ri = rsqrt_pair(r2);
       const double ri2 = ri * ri;
       const double ip = (s.fx * dx + s.fy * dy + s.fz * dz) * ri2;
@@ -454,67 +458,10 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f64(
   }
 }
 
-__global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c(
-    const FarRec32 *__restrict__ rec, const double *__restrict__ spos, long spg, long n_groups,
-    const double *__restrict__ tpos, long n_trg, long tpg, int nsplit, double csl,
-    double *__restrict__ part) {
-  const long slice = blockIdx.x % nsplit;
-  const long tb = blockIdx.x / nsplit;
-  const long t = tb * FF_BLOCK + threadIdx.x;
-  const bool tvalid = t < n_trg;
-  double tx = 0, ty = 0, tz = 0;
-  if (tvalid) {
-    tx = tpos[3 * t];
-    ty = tpos[3 * t + 1];
-    tz = tpos[3 * t + 2];
-  }
-  const long my_g = tvalid ? t / tpg : -1;
-  const long tfirst = tb * FF_BLOCK;
-  const long tlast = min(tfirst + FF_BLOCK, n_trg) - 1;
-  float Sh[3] = {0, 0, 0}, Sl[3] = {0, 0, 0};
-  long g0, g1;
-  slice_range(slice, nsplit, 0, spg, n_groups, g0, g1);
-  for (long g = g0; g < g1; ++g) {
-    if (tfirst / tpg == g && tlast / tpg == g) continue;  // block-uniform skip
-    if (!tvalid || my_g == g) continue;
-    const double *a = spos + 3 * g * spg;
-    const float lx = (float)(tx - a[0]), ly = (float)(ty - a[1]), lz = (float)(tz - a[2]);
-    const FarRec32 *__restrict__ r = rec + g * spg;
-    for (long j0 = 0; j0 < spg; j0 += FF_SUB) {
-      const long j1 = min(spg, j0 + FF_SUB);
-      float sx = 0, sy = 0, sz = 0;
-#pragma unroll 4
-      for (long j = j0; j < j1; ++j) {
-        const FarRec32 s = r[j];
-        const float dx = lx - s.x, dy = ly - s.y, dz = lz - s.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        const float ri = __frsqrt_rn(r2);
-        const float ri2 = ri * ri;
-        const float ip = (s.fx * dx + s.fy * dy + s.fz * dz) * ri2;
-        sx += ri * (s.fx + dx * ip);
-        sy += ri * (s.fy + dy * ip);
-        sz += ri * (s.fz + dz * ip);
-      }
-      float s_, e_;
-      two_sum(Sh[0], sx, s_, e_); Sh[0] = s_; Sl[0] += e_;
-      two_sum(Sh[1], sy, s_, e_); Sh[1] = s_; Sl[1] += e_;
-      two_sum(Sh[2], sz, s_, e_); Sh[2] = s_; Sl[2] += e_;
-    }
-  }
-  if (tvalid) {
-    double *o = part + (slice * n_trg + t) * 3;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) o[c] = csl * ((double)Sh[c] + (double)Sl[c]);
-  }
-}
-
-// Two targets per lane (t and t + FF_BLOCK): each source record loaded
-// through the scalar cache serves two pairs.  A target must not see its own
-// group's sources; instead of a lane-divergent skip (which would move the
-// source addressing out of SGPRs), its r^2 carries a bias of 1e30 for that
-// group, folded into the fma chain at no cost: each such source then adds
-// about 1e-15 |f| (including the coincident one, without a NaN), some 1e-14
-// of the sum and far below FP32 resolution.
+// FP32 partial sums merged into a TwoSum (hi, lo) pair, and the r^2 bias that
+// removes a target's own group: each own-group source then adds about
+// 1e-15 |f| (the coincident one included, without a NaN), far below FP32
+// resolution of the sum.
 struct Acc32 {
   float h[3], l[3];
 };
@@ -525,6 +472,72 @@ __device__ __forceinline__ void acc_merge(Acc32 &a, float x, float y, float z) {
   two_sum(a.h[2], z, s_, e_); a.h[2] = s_; a.l[2] += e_;
 }
 constexpr float kOwnGroupBias = 1e30f;
+
+__global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c(
+    const FarRec32 *__restrict__ rec, const double *__restrict__ spos, long spg, long n_groups,
+    const double *__restrict__ tpos, long n_trg, long tpg, int nsplit, double csl,
+    double *__restrict__ part) {
+  const long slice = blockIdx.x % nsplit;
+  const long tb = blockIdx.x / nsplit;
+  const long t = tb * FF_BLOCK + threadIdx.x;
+  const bool tvalid = t < n_trg;
+  // a lane past the last target computes for the last one and stores nothing,
+  // so that control flow inside the group loop stays uniform
+  const long tc = tvalid ? t : n_trg - 1;
+  const double tx = tpos[3 * tc], ty = tpos[3 * tc + 1], tz = tpos[3 * tc + 2];
+  const long my_g = tc / tpg;
+  const long tfirst = tb * FF_BLOCK;
+  const long tlast = min(tfirst + FF_BLOCK, n_trg) - 1;
+  Acc32 S = {{0, 0, 0}, {0, 0, 0}};
+  long g0, g1;
+  slice_range(slice, nsplit, 0, spg, n_groups, g0, g1);
+  for (long g = g0; g < g1; ++g) {
+    if (tfirst / tpg == g && tlast / tpg == g) continue;  // block-uniform skip
+    // a target's own group is not skipped lane by lane (that would make the
+    // loop below divergent): its r^2 carries kOwnGroupBias instead
+    const float bias = my_g == g ? kOwnGroupBias : 0.0f;
+    const double *a = spos + 3 * g * spg;
+    const float lx = (float)(tx - a[0]), ly = (float)(ty - a[1]), lz = (float)(tz - a[2]);
+    const FarRec32 *__restrict__ r = rec + g * spg;
+    auto chunk = [&](const FarRec32 *__restrict__ rr, int cnt, float &sx, float &sy,
+                     float &sz) {
+      auto pair = [&](const FarRec32 &src) {
+        const float dx = lx - src.x, dy = ly - src.y, dz = lz - src.z;
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, bias)));
+        const float ri = __frsqrt_rn(r2);
+        const float ri2 = ri * ri;
+        const float ip = (src.fx * dx + src.fy * dy + src.fz * dz) * ri2;
+        sx += ri * (src.fx + dx * ip);
+        sy += ri * (src.fy + dy * ip);
+        sz += ri * (src.fz + dz * ip);
+      };
+      if (cnt == FF_SUB) {
+#pragma unroll 8
+        for (int j = 0; j < FF_SUB; ++j) pair(rr[j]);
+      } else {
+        for (int j = 0; j < cnt; ++j) pair(rr[j]);
+      }
+    };
+    for (long j0 = 0; j0 < spg; j0 += FF_SUB) {
+      float sx = 0, sy = 0, sz = 0;
+      chunk(r + j0, (int)min((long)FF_SUB, spg - j0), sx, sy, sz);
+      acc_merge(S, sx, sy, sz);
+    }
+  }
+  if (tvalid) {
+    double *o = part + (slice * n_trg + t) * 3;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = csl * ((double)S.h[c] + (double)S.l[c]);
+  }
+}
+
+// Two targets per lane (t and t + FF_BLOCK): each source record loaded
+// through the scalar cache serves two pairs.  A target must not see its own
+// group's sources; instead of a lane-divergent skip (which would move the
+// source addressing out of SGPRs), its r^2 carries a bias of 1e30 for that
+// group, folded into the fma chain at no cost: each such source then adds
+// about 1e-15 |f| (including the coincident one, without a NaN), some 1e-14
+// of the sum and far below FP32 resolution.
 
 __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c2(
     const FarRec32 *__restrict__ rec, const double *__restrict__ spos, long spg, long n_groups,
@@ -650,13 +663,9 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c_x2(
   const long tb = blockIdx.x / nsplit;
   const long t = tb * FF_BLOCK + threadIdx.x;
   const bool tvalid = t < n_trg;
-  double tx = 0, ty = 0, tz = 0;
-  if (tvalid) {
-    tx = tpos[3 * t];
-    ty = tpos[3 * t + 1];
-    tz = tpos[3 * t + 2];
-  }
-  const long my_g = tvalid ? t / tpg : -1;
+  const long tc = tvalid ? t : n_trg - 1;  // uniform control flow, as k_far_groups_f32c
+  const double tx = tpos[3 * tc], ty = tpos[3 * tc + 1], tz = tpos[3 * tc + 2];
+  const long my_g = tc / tpg;
   const long tfirst = tb * FF_BLOCK;
   const long tlast = min(tfirst + FF_BLOCK, n_trg) - 1;
   Acc32 A = {{0, 0, 0}, {0, 0, 0}}, B = {{0, 0, 0}, {0, 0, 0}};
@@ -664,28 +673,33 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c_x2(
   slice_range(slice, nsplit, 0, spg, n_groups, g0, g1);
   for (long g = g0; g < g1; ++g) {
     if (tfirst / tpg == g && tlast / tpg == g) continue;  // block-uniform skip
-    if (!tvalid || my_g == g) continue;
+    const float bias = my_g == g ? kOwnGroupBias : 0.0f;
     const double *a = spos + 3 * g * spg;
     const float lx = (float)(tx - a[0]), ly = (float)(ty - a[1]), lz = (float)(tz - a[2]);
     const FarRec32x2 *__restrict__ r = rec + g * spg;
     for (long j0 = 0; j0 < spg; j0 += FF_SUB) {
-      const long j1 = min(spg, j0 + FF_SUB);
+      const int cnt = (int)min((long)FF_SUB, spg - j0);
+      const FarRec32x2 *__restrict__ rr = r + j0;
       float ux = 0, uy = 0, uz = 0, vx = 0, vy = 0, vz = 0;
-#pragma unroll 4
-      for (long j = j0; j < j1; ++j) {
-        const FarRec32x2 s = r[j];
-        const float dx = lx - s.x, dy = ly - s.y, dz = lz - s.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
+      auto pair = [&](const FarRec32x2 &src) {
+        const float dx = lx - src.x, dy = ly - src.y, dz = lz - src.z;
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, bias)));
         const float ri = __frsqrt_rn(r2);
         const float ri2 = ri * ri;
-        const float ip = (s.ax * dx + s.ay * dy + s.az * dz) * ri2;
-        ux += ri * (s.ax + dx * ip);
-        uy += ri * (s.ay + dy * ip);
-        uz += ri * (s.az + dz * ip);
-        const float iq = (s.bx * dx + s.by * dy + s.bz * dz) * ri2;
-        vx += ri * (s.bx + dx * iq);
-        vy += ri * (s.by + dy * iq);
-        vz += ri * (s.bz + dz * iq);
+        const float ip = (src.ax * dx + src.ay * dy + src.az * dz) * ri2;
+        ux += ri * (src.ax + dx * ip);
+        uy += ri * (src.ay + dy * ip);
+        uz += ri * (src.az + dz * ip);
+        const float iq = (src.bx * dx + src.by * dy + src.bz * dz) * ri2;
+        vx += ri * (src.bx + dx * iq);
+        vy += ri * (src.by + dy * iq);
+        vz += ri * (src.bz + dz * iq);
+      };
+      if (cnt == FF_SUB) {
+#pragma unroll 4
+        for (int j = 0; j < FF_SUB; ++j) pair(rr[j]);
+      } else {
+        for (int j = 0; j < cnt; ++j) pair(rr[j]);
       }
       acc_merge(A, ux, uy, uz);
       acc_merge(B, vx, vy, vz);
@@ -709,27 +723,24 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f64_x2(
   const long tb = blockIdx.x / nsplit;
   const long t = tb * FF_BLOCK + threadIdx.x;
   const bool tvalid = t < n_trg;
-  double tx = 0, ty = 0, tz = 0;
-  if (tvalid) {
-    tx = tpos[3 * t];
-    ty = tpos[3 * t + 1];
-    tz = tpos[3 * t + 2];
-  }
-  const long my_g = tvalid ? t / tpg : -1;
+  const long tc = tvalid ? t : n_trg - 1;  // uniform control flow, as k_far_groups_f64
+  const double tx = tpos[3 * tc], ty = tpos[3 * tc + 1], tz = tpos[3 * tc + 2];
+  const long my_g = tc / tpg;
   const long tfirst = tb * FF_BLOCK;
   const long tlast = min(tfirst + FF_BLOCK, n_trg) - 1;
   double Sx = 0, Sy = 0, Sz = 0, Tx = 0, Ty = 0, Tz = 0;
   long g0, g1;
   slice_range(slice, nsplit, 0, spg, n_groups, g0, g1);
+  const int ns = (int)spg;
   for (long g = g0; g < g1; ++g) {
     if (tfirst / tpg == g && tlast / tpg == g) continue;  // block-uniform skip
-    if (!tvalid || my_g == g) continue;
+    const double bias = my_g == g ? kOwnGroupBias64 : 0.0;
     const FarRec64x2 *__restrict__ r = rec + g * spg;
 #pragma unroll 2
-    for (long j = 0; j < spg; ++j) {
+    for (int j = 0; j < ns; ++j) {
       const FarRec64x2 s = r[j];
       const double dx = tx - s.x, dy = ty - s.y, dz = tz - s.z;
-      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, bias)));
       const double ri = rsqrt_pair(r2);
       const double ri2 = ri * ri;
       const double ip = (s.ax * dx + s.ay * dy + s.az * dz) * ri2;
