@@ -1,9 +1,12 @@
 // gemm.hip -- grouped strided batched GEMM (see gemm.hpp).
 //
-// 64x64 output tile per 256-thread workgroup, K staged through LDS in steps
-// of 16; each lane owns a 4x4 register tile.  Operand tiles are loaded with
-// the lane-to-element mapping chosen from the operand's unit stride so global
-// reads stay coalesced whether the operand is row- or column-contiguous.
+// Two kernels behind one entry.  The MFMA kernel (the default) takes its
+// output tile shape from the descriptor list's plan (64 or 128 rows by 16-64
+// columns; gemm_pick_tile).  The VALU kernel has a 64x64 tile in which each
+// lane owns a 4x4 register block.  Both stage K through LDS in slabs of 16,
+// with the lane-to-element mapping chosen from the operand's unit stride so
+// global reads stay coalesced whether the operand is row- or
+// column-contiguous.
 #include <atomic>
 #include <cstdint>
 #include <cstdlib>

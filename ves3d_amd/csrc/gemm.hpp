@@ -30,7 +30,7 @@ int gemm_grouped(const GemmDesc *d_descs, int ndesc, int batch, int max_m,
                  int max_n, const T *A, const T *B, T *C, T alpha, T beta,
                  hipStream_t st, int tile = -1);
 
-// MFMA output tile for a descriptor list: the shape (64..256 rows x 16..64
+// MFMA output tile for a descriptor list: the shape (64 or 128 rows x 16..64
 // columns) that minimises the padded multiply-add count plus operand traffic
 // and a per-tile overhead over the list, with flags recording whether the
 // list's A operand can be read as aligned 16-byte vectors.  The lists of the
