@@ -259,48 +259,52 @@ int rotate_transpose(const T *in, long Nv, int p, int dof, T *out, T *work, hipS
 // ------------------------------------------------------------- pole kernel
 // For each rotation r, pole t (0: x=-1, 1: x=+1) and upsampled source s:
 // the 6 symmetric tensor components of the SL (and/or DL) kernel times the
-// singular quadrature weight (src/SphericalHarmonics.cc:970-1073).
+// singular quadrature weight (src/SphericalHarmonics.cc:970-1073).  One
+// thread per (r, s) serves both poles, so the rotated geometry (X, X_theta,
+// X_phi: 72 bytes in FP64) is read once for the two 48-byte outputs.
 template <typename T>
 __global__ void k_pole_kernel(const T *__restrict__ X, const T *__restrict__ Xt,
                               const T *__restrict__ Xp, const T *__restrict__ trg,
                               const double *__restrict__ qw, long nrot, int p1,
                               T *__restrict__ SL0, T *__restrict__ DL0) {
   const int M1 = 2 * p1 * (p1 + 1);
-  const long total = nrot * 2 * (long)M1;
+  const long total = nrot * (long)M1;
   const double eps = 2.220446049250313e-16;
   const double csl = 1.0 / (8.0 * kPi), cdl = 3.0 / (4.0 * kPi);
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (long)gridDim.x * blockDim.x) {
     const int s = (int)(i % M1);
-    const long rt = i / M1;
-    const int t = (int)(rt & 1);
-    const long r = rt >> 1;
+    const long r = i / M1;
     const int j0 = s / (2 * p1);
-    const double w = qw[t ? j0 : p1 - j0];
-    double tx = trg[(r * 3 + 0) * 2 + t], ty = trg[(r * 3 + 1) * 2 + t], tz = trg[(r * 3 + 2) * 2 + t];
     const long b = r * 3 * (long)M1 + s;
-    const double dx = tx - (double)X[b], dy = ty - (double)X[b + M1], dz = tz - (double)X[b + 2 * M1];
+    const double x0 = X[b], y0 = X[b + M1], z0 = X[b + 2 * M1];
     const double xt = Xt[b], yt = Xt[b + M1], zt = Xt[b + 2 * M1];
     const double xp = Xp[b], yp = Xp[b + M1], zp = Xp[b + 2 * M1];
     const double nx = yt * zp - zt * yp, ny = zt * xp - xt * zp, nz = xt * yp - yt * xp;
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    double rinv = 1.0 / sqrt(r2);
-    if (r2 <= eps) rinv = 0;
-    const double rinv2 = rinv * rinv;
-    const double dd[6] = {dx * dx, dx * dy, dx * dz, dy * dy, dy * dz, dz * dz};
-    const long o = (rt * 6) * (long)M1 + s;
-    if (DL0) {
-      const double rn5 = cdl * w * (nx * dx + ny * dy + nz * dz) * rinv2 * rinv2 * rinv;
+    const double area = SL0 ? sqrt(nx * nx + ny * ny + nz * nz) : 0.0;
 #pragma unroll
-      for (int c = 0; c < 6; ++c) DL0[o + c * (long)M1] = (T)(dd[c] * rn5);
-    }
-    if (SL0) {
-      const double area = sqrt(nx * nx + ny * ny + nz * nz);
-      const double ar = csl * w * area * rinv;
-      const double ar2 = ar * rinv2;
+    for (int t = 0; t < 2; ++t) {
+      const double w = qw[t ? j0 : p1 - j0];
+      const double dx = trg[(r * 3 + 0) * 2 + t] - x0, dy = trg[(r * 3 + 1) * 2 + t] - y0,
+                   dz = trg[(r * 3 + 2) * 2 + t] - z0;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      double rinv = 1.0 / sqrt(r2);
+      if (r2 <= eps) rinv = 0;
+      const double rinv2 = rinv * rinv;
+      const double dd[6] = {dx * dx, dx * dy, dx * dz, dy * dy, dy * dz, dz * dz};
+      const long o = ((2 * r + t) * 6) * (long)M1 + s;
+      if (DL0) {
+        const double rn5 = cdl * w * (nx * dx + ny * dy + nz * dz) * rinv2 * rinv2 * rinv;
 #pragma unroll
-      for (int c = 0; c < 6; ++c)
-        SL0[o + c * (long)M1] = (T)(dd[c] * ar2 + ((c == 0 || c == 3 || c == 5) ? ar : 0.0));
+        for (int c = 0; c < 6; ++c) DL0[o + c * (long)M1] = (T)(dd[c] * rn5);
+      }
+      if (SL0) {
+        const double ar = csl * w * area * rinv;
+        const double ar2 = ar * rinv2;
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+          SL0[o + c * (long)M1] = (T)(dd[c] * ar2 + ((c == 0 || c == 3 || c == 5) ? ar : 0.0));
+      }
     }
   }
 }
@@ -445,7 +449,7 @@ int singular_build(const double *X0, long Nv, int p0, int p1, T *SL, T *DL, hipS
     for (int layer = 0; layer < 2; ++layer) {
       T *dst = layer == 0 ? SL : DL;
       if (!dst) continue;
-      k_pole_kernel<T><<<grid_for(rot * 2 * M1), 256, 0, st>>>(
+      k_pole_kernel<T><<<grid_for(rot * M1), 256, 0, st>>>(
           X, Xt, Xp, P, qw, rot, p1, layer == 0 ? K0 : nullptr, layer == 1 ? K0 : nullptr);
       V3D_CHECK(launch_status());
       V3D_CHECK(sht_shc2grid_transpose<T>(K0, rot * 12, p1, p0, K1, w, st));
