@@ -17,6 +17,12 @@ Workloads
                    preconditioner, trained beforehand (untimed) on
                    (A_loc e, e) pairs recorded during plain solves of separate
                    256-vesicle training suspensions at 10-25% volume fraction.
+                   Each preconditioner application is an inner BiCGStab on
+                   the vesicles' own operators (no far field) preconditioned
+                   by the network (--inner-max-iter, --inner-tol; 0 applies
+                   the network once).  The same inner solves with the
+                   diagonal preconditioner inside run afterwards, reported
+                   under `inner_solve`.
         The velocity error of both forms' first step is measured against an
         FP64 solve of the same system converged to relative residual 1e-10.
         `value` is the learned + mixed form's ms per step; the plain form's is
@@ -121,19 +127,38 @@ def bench_step(args):
             stats_plain.append(s)
 
     # ---- learned + mixed form from the same initial state
-    mixed = I.Stepper(nv, p, precision=args.mixed_precision, precond=2, **common)
+    mixed = I.Stepper(nv, p, precision=args.mixed_precision, precond=2,
+                      inner_max_iter=args.inner_max_iter, inner_tol=args.inner_tol, **common)
     I.attach(mixed, pc, collect=False)
     mixed.set_state(Xg)
     stats_mixed = []
     U_mixed1 = None
     for k in range(args.warmup + args.steps):
         s = mixed.step()
-        _log("learned+mixed step %d: %.1f ms, %d iters, relres %.2e true %.2e fell_back %d" %
-             (k, s["ms_total"], s["iterations"], s["relres"], s["true_relres"], s["fell_back"]))
+        _log("learned+mixed step %d: %.1f ms, %d iters (%d inner), relres %.2e true %.2e "
+             "fell_back %d" % (k, s["ms_total"], s["iterations"], s["inner_iters"], s["relres"],
+                               s["true_relres"], s["fell_back"]))
         if k == 0:
             _, U_mixed1, S_mixed1 = [t.clone() for t in mixed.get_state()]
         if k >= args.warmup:
             stats_mixed.append(s)
+
+    # the same inner solves with the diagonal preconditioner inside them
+    # instead of the network: what the network gains within this form
+    # (reported, not part of the headline)
+    stats_dinner = []
+    if args.inner_max_iter > 0:
+        dinner = I.Stepper(nv, p, precision=args.mixed_precision, precond=1,
+                           inner_max_iter=args.inner_max_iter, inner_tol=args.inner_tol,
+                           **common)
+        dinner.set_state(Xg)
+        for k in range(args.warmup + args.steps):
+            s = dinner.step()
+            _log("diagonal-inner+mixed step %d: %.1f ms, %d iters (%d inner), true %.2e" %
+                 (k, s["ms_total"], s["iterations"], s["inner_iters"], s["true_relres"]))
+            if k >= args.warmup:
+                stats_dinner.append(s)
+        dinner.close()
 
     # velocity error of both forms' first step against the FP64 solve of the
     # same system converged to relative residual args.ref_tol (untimed)
@@ -204,6 +229,14 @@ def bench_step(args):
                    "r_near": stats_plain[-1]["r_near"]},
         "ms_per_step_plain": ms_plain, "speedup_vs_plain": ms_plain / ms_mixed,
         "iterations_learned": it_mixed, "iterations_plain": it_plain,
+        "inner_solve": {"max_iter": args.inner_max_iter, "tol": args.inner_tol,
+                        "inner_iters_learned": [s["inner_iters"] for s in stats_mixed],
+                        "local_matvecs_learned": [s["local_matvecs"] for s in stats_mixed],
+                        "diagonal_inner_ms_per_step": (
+                            float(np.mean([s["ms_total"] for s in stats_dinner]))
+                            if stats_dinner else None),
+                        "diagonal_inner_iterations": [s["iterations"] for s in stats_dinner],
+                        "diagonal_inner_inner_iters": [s["inner_iters"] for s in stats_dinner]},
         "true_relres_learned": [s["true_relres"] for s in stats_mixed],
         "true_relres_plain": [s["true_relres"] for s in stats_plain],
         "fell_back": [s["fell_back"] for s in stats_mixed],
@@ -557,6 +590,10 @@ def main():
     ap.add_argument("--train-steps", type=int, default=2)
     ap.add_argument("--ref-tol", type=float, default=1e-10)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--inner-max-iter", type=int, default=6,
+                    help="learned form: inner BiCGStab iterations on the vesicles' own operators "
+                         "per preconditioner application (0: apply the network once)")
+    ap.add_argument("--inner-tol", type=float, default=1e-2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed-precision", type=int, default=2, choices=[1, 2],
                     help="precision of the learned form: 1 FP32 singular operator + FP32-"

@@ -361,6 +361,13 @@ typedef struct ves3d_step_params {
   double r_near;          /* near-zone radius (0: r_ves 2/sqrt(p_up), src/NearSingular.cc:83) */
   int post_off;           /* 1 skips the reparametrization and area / volume correction
                              that follow the position update (src/EvolveSurface.cc:330-335) */
+  int inner_max_iter;     /* > 0: the preconditioner becomes an inner BiCGStab on the
+                             vesicles' own (block-diagonal) operator, itself preconditioned
+                             by `precond` (diagonal or learned), stopped at inner_tol or
+                             inner_max_iter iterations.  0 (the reference's ImplicitPrecond
+                             form): `precond` is applied once.  No counterpart in the
+                             reference. */
+  double inner_tol;       /* relative residual of the inner solve (0: 1e-3) */
 } ves3d_step_params;
 
 typedef struct ves3d_step_stats {
@@ -382,6 +389,9 @@ typedef struct ves3d_step_stats {
   double ms_far2;         /* summed device time of the two-density far-field passes
                              (the right-hand side together with the first matvec) */
   int far2_calls;         /* number of two-density passes */
+  int inner_iters;        /* inner BiCGStab iterations summed over the step's
+                             preconditioner applications (0 without inner solves) */
+  int local_matvecs;      /* local-operator applications of the inner solves */
 } ves3d_step_stats;
 
 /* Creates / destroys a stepper (device state for nves vesicles). */
@@ -410,6 +420,14 @@ int ves3d_hip_stepper_rhs_matvec(void *handle, const double *x, double *b, doubl
 /* ImplicitPrecond with the diagonal spectral preconditioner
  * (src/InterfacialVelocity.cc:1054-1109): y = D x. */
 int ves3d_hip_stepper_precond(void *handle, const double *x, double *y);
+/* The inner-solve preconditioner of the step (ves3d_step_params.inner_max_iter
+ * > 0): y ~ A_loc^-1 x by BiCGStab on every vesicle's own operator from
+ * y = 0, preconditioned by the diagonal (learned = 0) or the attached learned
+ * preconditioner (learned = 1), to inner_tol (1e-3 if 0) within
+ * inner_max_iter (50 if 0) iterations; *iters_host gets the iteration
+ * count.  Requires a prepared step.  No counterpart in the reference. */
+int ves3d_hip_stepper_precond_inner(void *handle, const double *x, double *y, int learned,
+                                    int *iters_host);
 /* The relative residual |b - A x| / |b| of the current state: the step's
  * right-hand side b and operator A at the positions, x the packed velocity
  * and tension (the true-residual check of src/InterfacialVelocity.cc:181-188

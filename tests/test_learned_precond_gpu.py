@@ -3,10 +3,13 @@ oracle (oracle/learned.py), and what the trained network gains.
 
 Tolerances: canonical frames 1e-12 absolute; local operator 1e-9 relative
 (the FP64 matvec's own bar); P(r) 1e-5 relative to max|P(r)| (the network
-runs in FP32).  Gain: after training on solver-generated pairs from one
-suspension, the learned solve of another suspension takes at most 3/4 of
-the diagonal-preconditioned iterations, reaches true relative residual
-1e-6, and its velocity is within 1e-5 of a tightly converged FP64 solve.
+runs in FP32); the inner-solve preconditioner inverts the local operator to
+its 1e-12 inner tolerance (1e-9 against the oracle's operator).  Gain: after
+training on solver-generated pairs from one suspension, the learned solve of
+another suspension takes at most 3/4 of the diagonal-preconditioned
+iterations, the learned inner solves take fewer iterations than diagonal
+ones, every form reaches true relative residual 1e-6, and every velocity is
+within 1e-5 of a tightly converged FP64 solve.
 """
 import numpy as np
 import pytest
@@ -46,6 +49,29 @@ def test_frames_and_local_matvec(hip):
     xg = cu(x)
     y = st.matvec_local(xg).cpu().numpy()
     assert rel(y, orc.matvec(x, local=True)) < 1e-9
+    st.close()
+
+
+def test_inner_precond_inverts_local_operator(hip):
+    """y = P_inner(x) solves A_loc y = x (every vesicle's own operator)."""
+    nv, p = 3, 8
+    X = _ellipsoids(nv, p, 0.1, seed=35)
+    st = I.Stepper(nv, p, precision=0, precond=1, inner_max_iter=300, inner_tol=1e-12)
+    st.set_state(cu(X))
+    st.prepare()
+    print("inner precond nv=%d p=%d" % (nv, p), flush=True)
+    # a right-hand side in the local operator's range (its tension rows of
+    # degree > 2p/3 vanish)
+    z = cu(np.random.default_rng(3).standard_normal(st.n))
+    x = st.matvec_local(z)
+    y, its = st.precond_inner(x)
+    assert 0 < its <= 300
+    r = st.matvec_local(y) - x
+    assert float(torch.linalg.norm(r) / torch.linalg.norm(x)) < 1e-10
+    orc = ves.Implicit(X, p, dt=1e-3, shear=1.0, near=False)
+    xo = x.cpu().numpy()
+    ro = orc.matvec(y.cpu().numpy(), local=True) - xo
+    assert np.linalg.norm(ro) / np.linalg.norm(xo) < 1e-9
     st.close()
 
 
@@ -97,22 +123,33 @@ def test_trained_network_gain(hip):
     train.close()
     X = _ellipsoids(nv, p, 0.2, seed=44)
     its = {}
+    inner = {}
     U = {}
-    for name, kind, tol in (("ref", 1, 1e-11), ("diag", 1, 1e-6), ("learned", 2, 1e-6)):
-        st = I.Stepper(nv, p, precision=0, precond=kind, **dict(common, tol=tol))
+    for name, kind, tol, nin_it in (("ref", 1, 1e-11, 0), ("diag", 1, 1e-6, 0),
+                                    ("learned", 2, 1e-6, 0), ("diag_inner", 1, 1e-6, 20),
+                                    ("learned_inner", 2, 1e-6, 20)):
+        st = I.Stepper(nv, p, precision=0, precond=kind,
+                       **dict(common, tol=tol, inner_max_iter=nin_it, inner_tol=1e-3))
         if kind == 2:
             I.attach(st, pc, collect=False)
         st.set_state(cu(X))
         s = st.step()
-        print(name, s["iterations"], s["true_relres"], s["fell_back"], flush=True)
+        print(name, s["iterations"], s["inner_iters"], s["true_relres"], s["fell_back"],
+              flush=True)
         assert s["status"] == 100 and s["true_relres"] < tol * 1.05
         its[name] = s["iterations"]
+        inner[name] = s["inner_iters"]
         U[name] = st.get_state()[1].cpu().numpy()
         if kind == 2:
             assert s["fell_back"] == 0
         st.close()
     assert its["learned"] <= 0.75 * its["diag"], its
-    for name in ("diag", "learned"):
+    # the inner solves invert each vesicle's own operator: fewer outer
+    # iterations than one application of either preconditioner, and the
+    # network makes the inner solves cheaper than the diagonal does
+    assert its["learned_inner"] <= its["learned"] and its["diag_inner"] <= its["learned"], its
+    assert inner["learned_inner"] < inner["diag_inner"], inner
+    for name in ("diag", "learned", "diag_inner", "learned_inner"):
         e = np.linalg.norm(U[name] - U["ref"]) / np.linalg.norm(U["ref"])
         assert e < 1e-5, (name, e)
     pc.close()

@@ -106,6 +106,7 @@ _sig("ves3d_hip_stepper_matvec_local", P, P, P)
 _sig("ves3d_hip_stepper_residual", P, P)
 _sig("ves3d_hip_stepper_rhs_matvec", P, P, P, P)
 _sig("ves3d_hip_stepper_precond_learned", P, P, P)
+_sig("ves3d_hip_stepper_precond_inner", P, P, P, c_int, P)
 _sig("ves3d_hip_stepper_frames", P, P)
 _sig("ves3d_hip_stepper_step", P, P)
 _sig("ves3d_hip_stepper_attach_pc", P, P, c_int)

@@ -318,11 +318,23 @@ typedef int (*apply_fn)(const void *ctx, const double *x, double *y);
 
 // BiCGStab (src/BiCGStab.cc:45-138), right-preconditioned, FP64 dots.
 // Returns Error_t; *status gets the BiCGSReturn code (include/BiCGStab.h:20-24).
+// work: 8 n doubles of caller-owned scratch (the inner solves of the
+// preconditioner run many times per step), or null to allocate.
 int bicgstab(long n, apply_fn A, const void *actx, apply_fn P, const void *pctx, double *x,
              const double *b, int *max_iter, double *tol, int *status, hipStream_t st,
-             const double *Ax0 = nullptr) {
-  DBuf r, rt, p, v, s, t, ph, sh;
-  for (DBuf *d : {&r, &rt, &p, &v, &s, &t, &ph, &sh}) V3D_CHECK(d->alloc(n));
+             const double *Ax0 = nullptr, double *work = nullptr) {
+  DBuf own[8];
+  double *vec[8];
+  for (int i = 0; i < 8; ++i) {
+    if (work) {
+      vec[i] = work + i * n;
+    } else {
+      V3D_CHECK(own[i].alloc(n));
+      vec[i] = own[i].p;
+    }
+  }
+  double *r = vec[0], *rt = vec[1], *p = vec[2], *v = vec[3], *s = vec[4], *t = vec[5],
+         *ph = vec[6], *sh = vec[7];
   int err = 0;
   double normb = dev_norm(n, b, st, &err);
   V3D_CHECK(err);
@@ -443,6 +455,10 @@ struct Stepper {
   bool have_av0 = false;
   int rep_iters = 0, av_iters = 0;
   int last_iters = 0;              // iterations of the last converged solve
+  // inner solves of the preconditioner (inner_max_iter > 0)
+  DBuf inner_work;                 // BiCGStab scratch, 8 n
+  apply_fn inner_pc = nullptr;     // the inner solve's own preconditioner
+  int inner_iters = 0, local_mv = 0;
 
   ~Stepper() {
     if (cinfo) (void)hipFree(cinfo);
@@ -452,6 +468,8 @@ struct Stepper {
     prm = p_;
     const int p = prm.p;
     if (p < 2 || p > 64 || prm.nves < 1) return VES3D_INVALID_PARAMETER;
+    if (prm.inner_max_iter < 0 || prm.inner_tol < 0 || prm.inner_tol >= 1)
+      return VES3D_INVALID_PARAMETER;
     if (prm.p_up <= 0) prm.p_up = 2 * p;
     if (prm.filter_freq <= 0) prm.filter_freq = 2 * p / 3;
     nv = (long)prm.nves;
@@ -639,6 +657,29 @@ struct Stepper {
   static int Pdiag_cb(const void *ctx, const double *x, double *y) {
     return ((Stepper *)ctx)->precond_diag(x, y);
   }
+  // every vesicle's own operator (no far field, no near correction)
+  static int Aloc_cb(const void *ctx, const double *x, double *y) {
+    Stepper *s = (Stepper *)ctx;
+    ++s->local_mv;
+    return s->matvec(x, y, true);
+  }
+  // Preconditioner with an inner solve: y ~ A_loc^-1 x, the block-diagonal
+  // (per-vesicle) operator inverted by BiCGStab from y = 0 to inner_tol,
+  // preconditioned by inner_pc (the diagonal or the learned preconditioner).
+  // The outer solve's vector changes by at most inner_tol between two
+  // applications to the same input, far below the outer tolerance's effect.
+  static int Pinner_cb(const void *ctx, const double *x, double *y) {
+    Stepper *s = (Stepper *)ctx;
+    if (!s->inner_work.p) V3D_CHECK(s->inner_work.alloc(8 * s->n));
+    V3D_HIP(hipMemsetAsync(y, 0, sizeof(double) * s->n, s->st));
+    int it = s->prm.inner_max_iter;
+    double tol = s->prm.inner_tol > 0 ? s->prm.inner_tol : 1e-3;
+    int status = 0;
+    V3D_CHECK(bicgstab(s->n, Aloc_cb, s, s->inner_pc, s, y, x, &it, &tol, &status, s->st,
+                       nullptr, s->inner_work));
+    s->inner_iters += it;
+    return VES3D_SUCCESS;
+  }
   static int Plearn_cb(const void *ctx, const double *x, double *y) {
     Stepper *s = (Stepper *)ctx;
     V3D_CHECK(s->precond_diag(x, y));
@@ -803,6 +844,11 @@ struct Stepper {
     const bool learned = prm.precond == 2 && lpc && lpc->ready();
     if (prm.precond == 1 || (prm.precond == 2 && !learned)) P = Pdiag_cb;
     if (learned) P = Plearn_cb;
+    inner_iters = local_mv = 0;
+    if (prm.inner_max_iter > 0 && P) {
+      inner_pc = P;
+      P = Pinner_cb;
+    }
     // a learned solve gets an iteration budget tied to the last solve's count
     // (a third of max_iter before there is one), so that a preconditioner
     // that does not deliver costs little before the fallback below
@@ -864,6 +910,8 @@ struct Stepper {
       stats->ms_post = ms(t3b, t4);
       stats->ms_far2 = far2_ms;
       stats->far2_calls = far2_calls;
+      stats->inner_iters = inner_iters;
+      stats->local_matvecs = local_mv;
     }
     return VES3D_SUCCESS;
   }
@@ -993,6 +1041,23 @@ int ves3d_hip_stepper_precond(void *h, const double *x, double *y) {
   if (!h || !x || !y) return VES3D_INVALID_PARAMETER;
   Stepper *s = (Stepper *)h;
   V3D_CHECK(s->precond_diag(x, y));
+  return hip_status(hipStreamSynchronize(s->st));
+}
+
+int ves3d_hip_stepper_precond_inner(void *h, const double *x, double *y, int learned,
+                                    int *iters_host) {
+  if (!h || !x || !y) return VES3D_INVALID_PARAMETER;
+  Stepper *s = (Stepper *)h;
+  if (!s->prepared) return VES3D_INVALID_PARAMETER;
+  if (learned && !(s->lpc && s->lpc->ready())) return VES3D_INVALID_PARAMETER;
+  const ves3d_step_params keep = s->prm;
+  if (s->prm.inner_max_iter <= 0) s->prm.inner_max_iter = 50;
+  s->inner_pc = learned ? Stepper::Plearn_cb : Stepper::Pdiag_cb;
+  s->inner_iters = s->local_mv = 0;
+  const int e = Stepper::Pinner_cb(s, x, y);
+  s->prm = keep;
+  V3D_CHECK(e);
+  if (iters_host) *iters_host = s->inner_iters;
   return hip_status(hipStreamSynchronize(s->st));
 }
 

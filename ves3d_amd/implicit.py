@@ -16,7 +16,8 @@ class StepParams(ctypes.Structure):
                 ("tol", ctypes.c_double), ("max_iter", ctypes.c_int),
                 ("precision", ctypes.c_int), ("precond", ctypes.c_int),
                 ("near_off", ctypes.c_int), ("repul_dist", ctypes.c_double),
-                ("r_near", ctypes.c_double), ("post_off", ctypes.c_int)]
+                ("r_near", ctypes.c_double), ("post_off", ctypes.c_int),
+                ("inner_max_iter", ctypes.c_int), ("inner_tol", ctypes.c_double)]
 
 
 class StepStats(ctypes.Structure):
@@ -28,7 +29,8 @@ class StepStats(ctypes.Structure):
                 ("far_calls", ctypes.c_int), ("near_pairs", ctypes.c_int),
                 ("r_near", ctypes.c_double), ("reparam_iters", ctypes.c_int),
                 ("av_iters", ctypes.c_int), ("ms_post", ctypes.c_double),
-                ("ms_far2", ctypes.c_double), ("far2_calls", ctypes.c_int)]
+                ("ms_far2", ctypes.c_double), ("far2_calls", ctypes.c_int),
+                ("inner_iters", ctypes.c_int), ("local_matvecs", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -43,10 +45,10 @@ class Stepper:
 
     def __init__(self, nves, p, dt=1e-3, shear_rate=0.1, bending_modulus=1e-2, tol=1e-6,
                  max_iter=200, precision=0, precond=1, p_up=0, filter_freq=0, near=True,
-                 repul_dist=0.0, r_near=0.0, post=True):
+                 repul_dist=0.0, r_near=0.0, post=True, inner_max_iter=0, inner_tol=0.0):
         self.params = StepParams(nves, p, p_up, filter_freq, dt, shear_rate, bending_modulus, tol,
                                  max_iter, precision, precond, 0 if near else 1, repul_dist, r_near,
-                                 0 if post else 1)
+                                 0 if post else 1, inner_max_iter, inner_tol)
         self.nves, self.p = nves, p
         self.ng = 2 * p * (p + 1)
         h = ctypes.c_void_p()
@@ -132,6 +134,17 @@ class Stepper:
         torch.cuda.synchronize()
         call("ves3d_hip_stepper_matvec_local", self.h, ptr(x), ptr(y))
         return y
+
+    def precond_inner(self, x, learned=False):
+        """(y, iterations): y ~ A_loc^-1 x by the inner BiCGStab of the
+        inner-solve preconditioner (diagonal or learned inner preconditioner)."""
+        self._chk(x, self.n)
+        y = torch.empty_like(x)
+        it = ctypes.c_int()
+        torch.cuda.synchronize()
+        call("ves3d_hip_stepper_precond_inner", self.h, ptr(x), ptr(y), 1 if learned else 0,
+             ctypes.byref(it))
+        return y, it.value
 
     def precond_learned(self, x):
         self._chk(x, self.n)
