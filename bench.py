@@ -220,7 +220,9 @@ def bench_step(args):
     mixed.close()
     pc.close()
     return {
-        "metric": METRIC, "value": ms_mixed, "unit": "ms", "n_gpus": 1, "steps": args.steps,
+        "metric": METRIC if (nv, p) == (512, 16) else METRIC.replace(
+            "@512 ves p=16", "@%d ves p=%d" % (nv, p)),
+        "value": ms_mixed, "unit": "ms", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_mixed, "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": "implicit step, %d perturbed ellipsoids (1,1,0.6+-0.05) p=%d, "
