@@ -113,7 +113,8 @@ def bench_step(args):
     _log("trained on %d samples: loss %.3e in %.1f s" % (n_samples, loss, train_s))
 
     # ---- plain FP64 form
-    plain = I.Stepper(nv, p, precision=0, precond=1, **common)
+    timed = dict(common, extrapolate=args.extrapolate)
+    plain = I.Stepper(nv, p, precision=0, precond=1, **timed)
     plain.set_state(Xg)
     stats_plain = []
     U_plain1 = None
@@ -128,7 +129,7 @@ def bench_step(args):
 
     # ---- learned + mixed form from the same initial state
     mixed = I.Stepper(nv, p, precision=args.mixed_precision, precond=2,
-                      inner_max_iter=args.inner_max_iter, inner_tol=args.inner_tol, **common)
+                      inner_max_iter=args.inner_max_iter, inner_tol=args.inner_tol, **timed)
     I.attach(mixed, pc, collect=False)
     mixed.set_state(Xg)
     stats_mixed = []
@@ -150,7 +151,7 @@ def bench_step(args):
     if args.inner_max_iter > 0:
         dinner = I.Stepper(nv, p, precision=args.mixed_precision, precond=1,
                            inner_max_iter=args.inner_max_iter, inner_tol=args.inner_tol,
-                           **common)
+                           **timed)
         dinner.set_state(Xg)
         for k in range(args.warmup + args.steps):
             s = dinner.step()
@@ -231,6 +232,8 @@ def bench_step(args):
                    "r_near": stats_plain[-1]["r_near"]},
         "ms_per_step_plain": ms_plain, "speedup_vs_plain": ms_plain / ms_mixed,
         "iterations_learned": it_mixed, "iterations_plain": it_plain,
+        "initial_guess": ("2 x_{n-1} - x_{n-2} (both forms)" if args.extrapolate
+                          else "x_{n-1}, the previous step's solution (both forms)"),
         "inner_solve": {"max_iter": args.inner_max_iter, "tol": args.inner_tol,
                         "inner_iters_learned": [s["inner_iters"] for s in stats_mixed],
                         "local_matvecs_learned": [s["local_matvecs"] for s in stats_mixed],
@@ -596,6 +599,10 @@ def main():
                     help="learned form: inner BiCGStab iterations on the vesicles' own operators "
                          "per preconditioner application (0: apply the network once)")
     ap.add_argument("--inner-tol", type=float, default=1e-2)
+    ap.add_argument("--extrapolate", action="store_true",
+                    help="every timed form starts each solve from 2 x_{n-1} - x_{n-2} instead of "
+                         "x_{n-1} (the reference's initial guess); needs --warmup >= 2 for "
+                         "steady-state steps")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed-precision", type=int, default=2, choices=[1, 2],
                     help="precision of the learned form: 1 FP32 singular operator + FP32-"

@@ -368,6 +368,11 @@ typedef struct ves3d_step_params {
                              form): `precond` is applied once.  No counterpart in the
                              reference. */
   double inner_tol;       /* relative residual of the inner solve (0: 1e-3) */
+  int extrapolate;        /* 1: the solve starts from 2 x_{n-1} - x_{n-2}, the linear
+                             extrapolation of the last two steps' solutions; 0: from
+                             x_{n-1}, as the reference's AssembleInitial
+                             (src/InterfacialVelocity.cc:873-919).  No counterpart in
+                             the reference. */
 } ves3d_step_params;
 
 typedef struct ves3d_step_stats {

@@ -227,3 +227,28 @@ def test_rhs_matvec_one_pass(hip, far_kernel, precision, tol):
     assert rel(b1.cpu().numpy(), b0.cpu().numpy()) < tol
     assert rel(y1.cpu().numpy(), y0.cpu().numpy()) < tol
     st.close()
+
+
+def test_extrapolated_initial_guess(hip):
+    """extrapolate = 1 starts the solve from 2 x_{n-1} - x_{n-2}: the same
+    trajectory to the solver tolerance, and no more iterations once two
+    solutions exist."""
+    nv, p = 4, 8
+    X = W.implicit_step_case(nv, p, 0.15, seed=12)
+    runs = {}
+    for ex in (False, True):
+        st = I.Stepper(nv, p, dt=1e-3, shear_rate=1.0, precision=0, precond=1, tol=1e-8,
+                       max_iter=300, post=False, extrapolate=ex)
+        st.set_state(cu(X))
+        its = []
+        for k in range(4):
+            print("extrapolate %d step %d" % (ex, k), flush=True)
+            s = st.step()
+            assert s["status"] == 100 and s["true_relres"] < 1e-8 * 1.05, s
+            its.append(s["iterations"])
+        runs[ex] = (its, st.get_state()[1].cpu().numpy())
+        st.close()
+    (i0, U0), (i1, U1) = runs[False], runs[True]
+    assert i1[:2] == i0[:2]
+    assert all(a <= b for a, b in zip(i1[2:], i0[2:])), (i0, i1)
+    assert np.linalg.norm(U1 - U0) / np.linalg.norm(U0) < 1e-6

@@ -455,6 +455,10 @@ struct Stepper {
   bool have_av0 = false;
   int rep_iters = 0, av_iters = 0;
   int last_iters = 0;              // iterations of the last converged solve
+  // previous solution for the extrapolated initial guess (extrapolate = 1);
+  // solved counts the steps since the state was last set
+  DBuf xprev, xcur;
+  int solved = 0;
   // inner solves of the preconditioner (inner_max_iter > 0)
   DBuf inner_work;                 // BiCGStab scratch, 8 n
   apply_fn inner_pc = nullptr;     // the inner solve's own preconditioner
@@ -468,7 +472,8 @@ struct Stepper {
     prm = p_;
     const int p = prm.p;
     if (p < 2 || p > 64 || prm.nves < 1) return VES3D_INVALID_PARAMETER;
-    if (prm.inner_max_iter < 0 || prm.inner_tol < 0 || prm.inner_tol >= 1)
+    if (prm.inner_max_iter < 0 || prm.inner_tol < 0 || prm.inner_tol >= 1 ||
+        (prm.extrapolate != 0 && prm.extrapolate != 1))
       return VES3D_INVALID_PARAMETER;
     if (prm.p_up <= 0) prm.p_up = 2 * p;
     if (prm.filter_freq <= 0) prm.filter_freq = 2 * p / 3;
@@ -832,6 +837,15 @@ struct Stepper {
     V3D_HIP(hipStreamSynchronize(st));
     auto t1 = clk::now();
     V3D_CHECK(pack(U, sig, xsol));  // AssembleInitial (src/InterfacialVelocity.cc:873-919)
+    if (prm.extrapolate && solved >= 1) {
+      // (U, sigma) is the last step's solution x_{n-1}: start from
+      // 2 x_{n-1} - x_{n-2} once two solutions exist, and keep x_{n-1}
+      if (!xprev.p) V3D_CHECK(xprev.alloc(n));
+      if (!xcur.p) V3D_CHECK(xcur.alloc(n));
+      V3D_CHECK(dev_copy(n, xsol, xcur, st));
+      if (solved >= 2) V3D_CHECK(dev_axpby(n, -1.0, xprev, 2.0, xsol, st));
+      V3D_CHECK(dev_copy(n, xcur, xprev, st));
+    }
     // the right-hand side and the solver's first matvec A x0 in one pass
     if (!Ax0.p) V3D_CHECK(Ax0.alloc(n));
     V3D_CHECK(rhs_and_matvec(rhs, xsol, Ax0));
@@ -872,6 +886,7 @@ struct Stepper {
       V3D_CHECK(true_residual(xsol, rhs, &true_rel));
     }
     if (status == 100) last_iters = it;
+    ++solved;
     auto t3 = clk::now();
     V3D_CHECK(unpack(xsol, U, sig));
     V3D_CHECK(dev_axpby(3 * nv * ng, prm.dt, U, 1.0, X, st));  // dx = dt u
@@ -962,6 +977,7 @@ int ves3d_hip_stepper_set_state(void *h, const double *X, const double *U, const
   if (U) V3D_HIP(hipMemcpy(s->U.p, U, sizeof(double) * s->U.n, hipMemcpyDeviceToDevice));
   if (sigma) V3D_HIP(hipMemcpy(s->sig.p, sigma, sizeof(double) * s->sig.n, hipMemcpyDeviceToDevice));
   s->prepared = false;
+  s->solved = 0;  // a new trajectory: no solution history to extrapolate
   if (X) V3D_CHECK(s->record_area_volume());  // the targets of the area / volume correction
   return VES3D_SUCCESS;
 }

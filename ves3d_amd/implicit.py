@@ -17,7 +17,8 @@ class StepParams(ctypes.Structure):
                 ("precision", ctypes.c_int), ("precond", ctypes.c_int),
                 ("near_off", ctypes.c_int), ("repul_dist", ctypes.c_double),
                 ("r_near", ctypes.c_double), ("post_off", ctypes.c_int),
-                ("inner_max_iter", ctypes.c_int), ("inner_tol", ctypes.c_double)]
+                ("inner_max_iter", ctypes.c_int), ("inner_tol", ctypes.c_double),
+                ("extrapolate", ctypes.c_int)]
 
 
 class StepStats(ctypes.Structure):
@@ -45,10 +46,12 @@ class Stepper:
 
     def __init__(self, nves, p, dt=1e-3, shear_rate=0.1, bending_modulus=1e-2, tol=1e-6,
                  max_iter=200, precision=0, precond=1, p_up=0, filter_freq=0, near=True,
-                 repul_dist=0.0, r_near=0.0, post=True, inner_max_iter=0, inner_tol=0.0):
+                 repul_dist=0.0, r_near=0.0, post=True, inner_max_iter=0, inner_tol=0.0,
+                 extrapolate=False):
         self.params = StepParams(nves, p, p_up, filter_freq, dt, shear_rate, bending_modulus, tol,
                                  max_iter, precision, precond, 0 if near else 1, repul_dist, r_near,
-                                 0 if post else 1, inner_max_iter, inner_tol)
+                                 0 if post else 1, inner_max_iter, inner_tol,
+                                 1 if extrapolate else 0)
         self.nves, self.p = nves, p
         self.ng = 2 * p * (p + 1)
         h = ctypes.c_void_p()
