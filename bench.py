@@ -384,9 +384,9 @@ def bench_matvec(args):
     }
 
 
-# singular build, SL + DL, per vesicle: SURVEY-table accounting of the
-# reference's dense-DFT algorithm (RotateAll, upsampling SHT, pole kernel,
-# transpose SHT, RotateTranspose, final Grid2SHC)
+# singular build, SL + DL, per vesicle: 2 M N K summed over the GEMMs of
+# the reference's dense-DFT algorithm (RotateAll, upsampling SHT, pole
+# kernel, transpose SHT, RotateTranspose, final Grid2SHC)
 SING_GFLOP_PER_VES = {16: 1.772, 24: 12.74}
 PEAK_HBM_TBS = 8.0
 
