@@ -671,8 +671,9 @@ struct Stepper {
   // Preconditioner with an inner solve: y ~ A_loc^-1 x, the block-diagonal
   // (per-vesicle) operator inverted by BiCGStab from y = 0 to inner_tol,
   // preconditioned by inner_pc (the diagonal or the learned preconditioner).
-  // The outer solve's vector changes by at most inner_tol between two
-  // applications to the same input, far below the outer tolerance's effect.
+  // The result is A_loc^-1 x to relative accuracy inner_tol, so the outer
+  // BiCGStab sees a preconditioner that departs from a fixed linear map by
+  // that much; the true-residual check after the solve judges the solution.
   static int Pinner_cb(const void *ctx, const double *x, double *y) {
     Stepper *s = (Stepper *)ctx;
     if (!s->inner_work.p) V3D_CHECK(s->inner_work.alloc(8 * s->n));
