@@ -39,6 +39,10 @@ Workloads
         the SH transforms of the 512-vesicle step.
   build: one singular build (SL + DL) of --nves vesicles at --p, FP64 and
         FP32 transforms (a short program for hardware-counter runs).
+  trajectory: the step workload for --traj-steps steps (default 100) in the
+        plain and the learned + mixed form from the same state: every step's
+        time, iterations, fallbacks and true residual, and the positions'
+        drift between the two trajectories.
 """
 import argparse
 import json
@@ -75,23 +79,14 @@ def _log(*a):
 
 
 # ---------------------------------------------------------------- step
-def bench_step(args):
+def _train_pc(args, p, common):
+    """The learned preconditioner trained (untimed) on pairs that plain solves
+    of separate training suspensions generate: args.train_nves vesicles of
+    the same shape family at volume fractions 10% .. 25%.  Returns (pc,
+    fracs, collect seconds, train seconds, samples, loss)."""
     import torch
-    import ves3d_amd
     from ves3d_amd import implicit as I
     from ves3d_amd import workloads as W
-
-    ves3d_amd.init(0)
-    ves3d_amd.call("ves3d_hip_set_far_kernel", FAR_KINDS[args.far_kernel])
-    nv, p = args.nves, args.p
-    X0 = W.implicit_step_case(nv=nv, p=p)
-    Xg = torch.from_numpy(X0).cuda()
-    common = dict(dt=1e-3, shear_rate=1.0, bending_modulus=1e-2, tol=1e-6, max_iter=args.max_iter,
-                  r_near=args.r_near)
-
-    # ---- train the learned preconditioner (untimed) on pairs that plain
-    # solves of separate training suspensions generate: args.train_nves
-    # vesicles of the same shape family at volume fractions 10% .. 25%
     nin = 4 * p * (p + 2)
     pc = I.LearnedPC(nin, args.width, capacity=args.capacity, seed=7)
     t0 = time.perf_counter()
@@ -109,7 +104,24 @@ def bench_step(args):
     t0 = time.perf_counter()
     loss = pc.train(args.epochs, 512, args.lr)
     train_s = time.perf_counter() - t0
-    n_samples = pc.samples()
+    return pc, fracs, collect_s, train_s, pc.samples(), loss
+
+
+def bench_step(args):
+    import torch
+    import ves3d_amd
+    from ves3d_amd import implicit as I
+    from ves3d_amd import workloads as W
+
+    ves3d_amd.init(0)
+    ves3d_amd.call("ves3d_hip_set_far_kernel", FAR_KINDS[args.far_kernel])
+    nv, p = args.nves, args.p
+    X0 = W.implicit_step_case(nv=nv, p=p)
+    Xg = torch.from_numpy(X0).cuda()
+    common = dict(dt=1e-3, shear_rate=1.0, bending_modulus=1e-2, tol=1e-6, max_iter=args.max_iter,
+                  r_near=args.r_near)
+    nin = 4 * p * (p + 2)
+    pc, fracs, collect_s, train_s, n_samples, loss = _train_pc(args, p, common)
     _log("trained on %d samples: loss %.3e in %.1f s" % (n_samples, loss, train_s))
 
     # ---- plain FP64 form
@@ -258,6 +270,70 @@ def bench_step(args):
         "stage_ms_learned": {k: float(np.mean([s[k] for s in stats_mixed]))
                              for k in ("ms_setup", "ms_rhs", "ms_solve")},
         "roofline": roof, "cpu_baseline": cpu,
+    }
+
+
+def bench_trajectory(args):
+    """The step workload run for --traj-steps steps in both forms from the
+    same initial state (the multi-step form of the full-step configuration):
+    per-step ms, outer and inner iterations, fallbacks and true residuals of
+    every step, and how far the two trajectories' positions drift apart."""
+    import torch
+    import ves3d_amd
+    from ves3d_amd import implicit as I
+    from ves3d_amd import workloads as W
+
+    ves3d_amd.init(0)
+    ves3d_amd.call("ves3d_hip_set_far_kernel", FAR_KINDS[args.far_kernel])
+    nv, p, nsteps = args.nves, args.p, args.traj_steps
+    X0 = W.implicit_step_case(nv=nv, p=p)
+    Xg = torch.from_numpy(X0).cuda()
+    common = dict(dt=1e-3, shear_rate=1.0, bending_modulus=1e-2, tol=1e-6, max_iter=args.max_iter,
+                  r_near=args.r_near)
+    pc = _train_pc(args, p, common)[0]
+    forms = {
+        "plain": I.Stepper(nv, p, precision=0, precond=1, **common),
+        "learned": I.Stepper(nv, p, precision=args.mixed_precision, precond=2,
+                             inner_max_iter=args.inner_max_iter, inner_tol=args.inner_tol,
+                             **common),
+    }
+    I.attach(forms["learned"], pc, collect=False)
+    out, final = {}, {}
+    for name, st in forms.items():
+        st.set_state(Xg)
+        rec = []
+        for k in range(nsteps):
+            s = st.step()
+            rec.append(s)
+            _log("%s step %d: %.1f ms, %d iters (%d inner), true %.2e fell_back %d" %
+                 (name, k, s["ms_total"], s["iterations"], s["inner_iters"], s["true_relres"],
+                  s["fell_back"]))
+        final[name] = st.get_state()[0].clone()
+        warm = rec[1:] if len(rec) > 1 else rec
+        out[name] = {
+            "ms_per_step_warm_mean": float(np.mean([s["ms_total"] for s in warm])),
+            "ms_per_step_warm_max": float(np.max([s["ms_total"] for s in warm])),
+            "iterations": [s["iterations"] for s in rec],
+            "inner_iters": [s["inner_iters"] for s in rec],
+            "fell_back_steps": int(sum(s["fell_back"] for s in rec)),
+            "max_true_relres": float(np.max([s["true_relres"] for s in rec])),
+            "all_converged": bool(all(s["status"] == 100 for s in rec)),
+        }
+        st.close()
+    pc.close()
+    disp = float(torch.linalg.norm(final["plain"] - Xg))
+    drift = float(torch.linalg.norm(final["learned"] - final["plain"]))
+    ms_l = out["learned"]["ms_per_step_warm_mean"]
+    ms_p = out["plain"]["ms_per_step_warm_mean"]
+    return {
+        "metric": "ms per implicit step over a %d-step trajectory @%d ves p=%d "
+                  "(learned-PC+mixed vs plain) at 1e-6" % (nsteps, nv, p),
+        "value": ms_l, "unit": "ms", "n_gpus": 1, "steps": nsteps, "higher_is_better": False,
+        "data": "synthetic",
+        "config": {"workload": "implicit-step trajectory", "nves": nv, "p": p, "steps": nsteps},
+        "ms_per_step_plain": ms_p, "speedup_vs_plain": ms_p / ms_l,
+        "position_drift_vs_plain": drift / max(disp, 1e-300),
+        "position_displacement_plain": disp, "forms": out,
     }
 
 
@@ -583,7 +659,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="step",
-                    choices=["step", "far", "matvec", "probe", "stages", "build"])
+                    choices=["step", "far", "matvec", "probe", "stages", "build", "trajectory"])
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32c"])
     ap.add_argument("--nves", type=int, default=512)
     ap.add_argument("--p", type=int, default=16)
@@ -599,6 +675,8 @@ def main():
                     help="learned form: inner BiCGStab iterations on the vesicles' own operators "
                          "per preconditioner application (0: apply the network once)")
     ap.add_argument("--inner-tol", type=float, default=1e-2)
+    ap.add_argument("--traj-steps", type=int, default=100,
+                    help="trajectory workload: steps per form")
     ap.add_argument("--extrapolate", action="store_true",
                     help="every timed form starts each solve from 2 x_{n-1} - x_{n-2} instead of "
                          "x_{n-1} (the reference's initial guess); needs --warmup >= 2 for "
@@ -618,7 +696,8 @@ def main():
               "(across GPUs the path would shard by vesicle)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     res = {"step": bench_step, "far": bench_far, "matvec": bench_matvec,
-           "probe": bench_probe, "stages": bench_stages, "build": bench_build}[args.workload](args)
+           "probe": bench_probe, "stages": bench_stages, "build": bench_build,
+           "trajectory": bench_trajectory}[args.workload](args)
     print(json.dumps(res))
 
 
