@@ -84,27 +84,12 @@ def _train_pc(args, p, common):
     of separate training suspensions generate: args.train_nves vesicles of
     the same shape family at volume fractions 10% .. 25%.  Returns (pc,
     fracs, collect seconds, train seconds, samples, loss)."""
-    import torch
-    from ves3d_amd import implicit as I
-    from ves3d_amd import workloads as W
-    nin = 4 * p * (p + 2)
-    pc = I.LearnedPC(nin, args.width, capacity=args.capacity, seed=7)
-    t0 = time.perf_counter()
+    from ves3d_amd import training as T
     fracs = np.linspace(0.10, 0.25, max(1, args.train_steps))
-    trainer = I.Stepper(args.train_nves, p, precision=0, precond=1, **common)
-    I.attach(trainer, pc, collect=True)
-    for k, vf in enumerate(fracs):
-        Xt = W.implicit_step_case(nv=args.train_nves, p=p, volfrac=float(vf), seed=1000 + k)
-        trainer.set_state(torch.from_numpy(Xt).cuda())
-        s = trainer.step()
-        _log("training suspension %d (%d ves, %.0f%%): %d iters" %
-             (k, args.train_nves, 100 * vf, s["iterations"]))
-    trainer.close()
-    collect_s = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    loss = pc.train(args.epochs, 512, args.lr)
-    train_s = time.perf_counter() - t0
-    return pc, fracs, collect_s, train_s, pc.samples(), loss
+    pc, info = T.train_learned_pc(args.train_nves, p, fracs, common, width=args.width,
+                                  capacity=args.capacity, epochs=args.epochs, lr=args.lr,
+                                  log=_log)
+    return pc, fracs, info["collect_s"], info["train_s"], info["samples"], info["loss"]
 
 
 def bench_step(args):

@@ -456,7 +456,15 @@ int ves3d_hip_stepper_frames(void *handle, double *frames);
  * previous (velocity, tension) as initial guess, true-residual check,
  * X += dt U, then (unless post_off) InterfacialVelocity::reparam and
  * EvolveSurface::AreaVolumeCorrection against the area and volume recorded
- * by the last set_state with positions.  stats_host may be NULL. */
+ * by the last set_state with positions.  stats_host may be NULL.
+ * A solution is accepted when BiCGStab converged and its true residual is
+ * at most 1.05 tol; BiCGStab continues from its own result (at most twice,
+ * within the iteration budget) when only the true residual misses.  A
+ * learned solve that is not accepted is repeated from the same initial
+ * guess with the diagonal spectral preconditioner.  When no solution is
+ * accepted the step returns VES3D_DIVERGENCE_ERROR
+ * (src/InterfacialVelocity.cc:176-188), fills the statistics and leaves the
+ * positions, velocity and tension as they were. */
 int ves3d_hip_stepper_step(void *handle, ves3d_step_stats *stats_host);
 /* InterfacialVelocity::reparam without upsampling (src/InterfacialVelocity.cc:
  * 1639-1773) on the current positions, with the reference's default

@@ -318,11 +318,12 @@ typedef int (*apply_fn)(const void *ctx, const double *x, double *y);
 
 // BiCGStab (src/BiCGStab.cc:45-138), right-preconditioned, FP64 dots.
 // Returns Error_t; *status gets the BiCGSReturn code (include/BiCGStab.h:20-24).
-// work: 8 n doubles of caller-owned scratch (the inner solves of the
-// preconditioner run many times per step), or null to allocate.
+// work: 8 n doubles of caller-owned scratch (the step owns one for its
+// solves), or null to allocate.  x_zero: x holds zeros, so the initial
+// residual is b and the first operator application is skipped.
 int bicgstab(long n, apply_fn A, const void *actx, apply_fn P, const void *pctx, double *x,
              const double *b, int *max_iter, double *tol, int *status, hipStream_t st,
-             const double *Ax0 = nullptr, double *work = nullptr) {
+             const double *Ax0 = nullptr, double *work = nullptr, bool x_zero = false) {
   DBuf own[8];
   double *vec[8];
   for (int i = 0; i < 8; ++i) {
@@ -339,11 +340,15 @@ int bicgstab(long n, apply_fn A, const void *actx, apply_fn P, const void *pctx,
   double normb = dev_norm(n, b, st, &err);
   V3D_CHECK(err);
   if (normb == 0.0) normb = 1.0;
-  if (Ax0)  // A x computed by the caller (together with b)
-    V3D_CHECK(dev_copy(n, Ax0, r, st));
-  else
-    V3D_CHECK(A(actx, x, r));
-  V3D_CHECK(dev_axpby(n, 1.0, b, -1.0, r, st));  // r = b - Ax
+  if (x_zero) {
+    V3D_CHECK(dev_copy(n, b, r, st));  // r = b - A 0
+  } else {
+    if (Ax0)  // A x computed by the caller (together with b)
+      V3D_CHECK(dev_copy(n, Ax0, r, st));
+    else
+      V3D_CHECK(A(actx, x, r));
+    V3D_CHECK(dev_axpby(n, 1.0, b, -1.0, r, st));  // r = b - Ax
+  }
   V3D_CHECK(dev_copy(n, r, rt, st));
   double resid = dev_norm(n, r, st, &err) / normb;
   V3D_CHECK(err);
@@ -460,7 +465,9 @@ struct Stepper {
   DBuf xprev, xcur;
   int solved = 0;
   // inner solves of the preconditioner (inner_max_iter > 0)
-  DBuf inner_work;                 // BiCGStab scratch, 8 n
+  DBuf inner_work;                 // BiCGStab scratch of the inner solves, 8 n
+  DBuf solve_work;                 // BiCGStab scratch of the outer solve, 8 n
+  DBuf x0s, Axr;                   // the initial guess (for the fallback); A x of a restart
   apply_fn inner_pc = nullptr;     // the inner solve's own preconditioner
   int inner_iters = 0, local_mv = 0;
 
@@ -677,12 +684,12 @@ struct Stepper {
   static int Pinner_cb(const void *ctx, const double *x, double *y) {
     Stepper *s = (Stepper *)ctx;
     if (!s->inner_work.p) V3D_CHECK(s->inner_work.alloc(8 * s->n));
-    V3D_HIP(hipMemsetAsync(y, 0, sizeof(double) * s->n, s->st));
+    V3D_CHECK(dev_fill(s->n, 0.0, y, s->st));
     int it = s->prm.inner_max_iter;
     double tol = s->prm.inner_tol > 0 ? s->prm.inner_tol : 1e-3;
     int status = 0;
     V3D_CHECK(bicgstab(s->n, Aloc_cb, s, s->inner_pc, s, y, x, &it, &tol, &status, s->st,
-                       nullptr, s->inner_work));
+                       nullptr, s->inner_work, true));
     s->inner_iters += it;
     return VES3D_SUCCESS;
   }
@@ -829,6 +836,38 @@ struct Stepper {
     return hip_status(hipStreamSynchronize(st));
   }
 
+  // A solution is accepted when BiCGStab converged and its true residual
+  // is within 5% of the tolerance.
+  static constexpr double kTrueResSlack = 1.05;
+  static constexpr int kRestarts = 2;
+
+  // BiCGStab on the step's system from xsol (whose A x is Ax0), then the
+  // true residual of the result (src/InterfacialVelocity.cc:181-188).  When
+  // BiCGStab reports convergence but the true residual misses the
+  // tolerance (the recursive residual drifted from it), BiCGStab continues
+  // from the current solution, at most kRestarts times, within the same
+  // iteration budget.  it: iterations used in total.
+  int solve(apply_fn P, int budget, int *it, double *tol, int *status, double *true_rel) {
+    const double *ax = Ax0;
+    int used = 0;
+    for (int r = 0;; ++r) {
+      int k = budget - used;
+      double t = prm.tol;
+      V3D_CHECK(bicgstab(n, A_cb, this, P, this, xsol, rhs, &k, &t, status, st, ax, solve_work));
+      used += k;
+      *tol = t;
+      V3D_CHECK(true_residual(xsol, rhs, true_rel));  // yres = b - A xsol
+      if (*status != 100 || *true_rel <= prm.tol * kTrueResSlack || r == kRestarts ||
+          used >= budget)
+        break;
+      if (!Axr.p) V3D_CHECK(Axr.alloc(n));
+      V3D_CHECK(dev_axpy3(n, -1.0, yres, rhs, Axr, st));  // A xsol = b - (b - A xsol)
+      ax = Axr;
+    }
+    *it = used;
+    return VES3D_SUCCESS;
+  }
+
   int step(ves3d_step_stats *stats) {
     using clk = std::chrono::steady_clock;
     auto t0 = clk::now();
@@ -852,9 +891,6 @@ struct Stepper {
     V3D_CHECK(rhs_and_matvec(rhs, xsol, Ax0));
     V3D_HIP(hipStreamSynchronize(st));
     auto t2 = clk::now();
-    int it = prm.max_iter;
-    double tol = prm.tol;
-    int status = 0;
     apply_fn P = nullptr;
     const bool learned = prm.precond == 2 && lpc && lpc->ready();
     if (prm.precond == 1 || (prm.precond == 2 && !learned)) P = Pdiag_cb;
@@ -864,37 +900,49 @@ struct Stepper {
       inner_pc = P;
       P = Pinner_cb;
     }
+    if (!solve_work.p) V3D_CHECK(solve_work.alloc(8 * n));
+    if (learned) {  // the fallback restarts from the same initial guess, whose A x is Ax0
+      if (!x0s.p) V3D_CHECK(x0s.alloc(n));
+      V3D_CHECK(dev_copy(n, xsol, x0s, st));
+    }
     // a learned solve gets an iteration budget tied to the last solve's count
     // (a third of max_iter before there is one), so that a preconditioner
     // that does not deliver costs little before the fallback below
-    if (learned)
-      it = last_iters > 0 ? std::min(prm.max_iter, 2 * last_iters + 10)
-                          : std::max(10, prm.max_iter / 3);
-    V3D_CHECK(bicgstab(n, A_cb, this, P, this, xsol, rhs, &it, &tol, &status, st, Ax0));
-    // true residual of the returned solution (src/InterfacialVelocity.cc:181-188)
-    double true_rel = 0;
-    V3D_CHECK(true_residual(xsol, rhs, &true_rel));
+    const int budget = !learned ? prm.max_iter
+                       : last_iters > 0 ? std::min(prm.max_iter, 2 * last_iters + 10)
+                                        : std::max(10, prm.max_iter / 3);
+    int it = 0, status = 0;
+    double tol = prm.tol, true_rel = 0;
+    V3D_CHECK(solve(P, budget, &it, &tol, &status, &true_rel));
     bool fell_back = false;
-    if (learned && (status != 100 || !(true_rel <= prm.tol * 1.05))) {
-      // the learned preconditioner did not deliver: BiCGStab restarts from
-      // the initial guess with the diagonal spectral preconditioner
-      // (ImplicitPrecond) and its result stands
+    if (learned && !(status == 100 && true_rel <= prm.tol * kTrueResSlack)) {
+      // the learned preconditioner did not deliver: BiCGStab starts again
+      // from the initial guess with the diagonal spectral preconditioner
+      // (ImplicitPrecond)
       fell_back = true;
-      V3D_CHECK(pack(U, sig, xsol));
-      it = prm.max_iter;
+      V3D_CHECK(dev_copy(n, x0s, xsol, st));
       tol = prm.tol;
-      V3D_CHECK(bicgstab(n, A_cb, this, Pdiag_cb, this, xsol, rhs, &it, &tol, &status, st, Ax0));
-      V3D_CHECK(true_residual(xsol, rhs, &true_rel));
+      V3D_CHECK(solve(Pdiag_cb, prm.max_iter, &it, &tol, &status, &true_rel));
     }
-    if (status == 100) last_iters = it;
-    ++solved;
+    // src/InterfacialVelocity.cc:176-188: a solve that failed, or whose
+    // true residual misses the tolerance, is a DivergenceError; the state
+    // (positions, velocity, tension) is then left as it was
+    const bool converged = status == 100 && true_rel <= prm.tol * kTrueResSlack;
+    if (converged) {
+      last_iters = it;
+      ++solved;
+    } else {
+      solved = 0;  // no solution history to extrapolate from
+    }
     auto t3 = clk::now();
-    V3D_CHECK(unpack(xsol, U, sig));
-    V3D_CHECK(dev_axpby(3 * nv * ng, prm.dt, U, 1.0, X, st));  // dx = dt u
+    if (converged) {
+      V3D_CHECK(unpack(xsol, U, sig));
+      V3D_CHECK(dev_axpby(3 * nv * ng, prm.dt, U, 1.0, X, st));  // dx = dt u
+    }
     V3D_HIP(hipStreamSynchronize(st));
     auto t3b = clk::now();
     rep_iters = av_iters = 0;
-    if (!prm.post_off) {  // EvolveSurface::Evolve after the update (src/EvolveSurface.cc:330-335)
+    if (!prm.post_off && converged) {  // EvolveSurface::Evolve after the update (src/EvolveSurface.cc:330-335)
       V3D_CHECK(reparam());
       V3D_CHECK(area_volume_correction());
     }
@@ -929,7 +977,7 @@ struct Stepper {
       stats->inner_iters = inner_iters;
       stats->local_matvecs = local_mv;
     }
-    return VES3D_SUCCESS;
+    return converged ? VES3D_SUCCESS : VES3D_DIVERGENCE_ERROR;
   }
 };
 

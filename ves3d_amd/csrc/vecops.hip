@@ -13,10 +13,16 @@ namespace {
 constexpr int RB = 256;      // threads per block
 constexpr int RBLOCKS = 512; // first-stage blocks
 
-__global__ void k_axpby(long n, double a, const double *__restrict__ x, double b,
-                        double *__restrict__ y) {
+// x may alias y, so neither pointer is restrict-qualified
+__global__ void k_axpby(long n, double a, const double *x, double b, double *y) {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     y[i] = a * x[i] + b * y[i];
+}
+// y = a x without reading y (b == 0: y may hold uninitialised memory, whose
+// NaN bit patterns 0 * y would propagate)
+__global__ void k_scale(long n, double a, const double *x, double *y) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] = a * x[i];
 }
 __global__ void k_axpy3(long n, double a, const double *__restrict__ x,
                         const double *__restrict__ y, double *__restrict__ z) {
@@ -82,7 +88,10 @@ unsigned blocks_for(long n) {
 
 int dev_axpby(long n, double a, const double *x, double b, double *y, hipStream_t st) {
   if (n <= 0) return VES3D_SUCCESS;
-  k_axpby<<<blocks_for(n), RB, 0, st>>>(n, a, x, b, y);
+  if (b == 0.0)
+    k_scale<<<blocks_for(n), RB, 0, st>>>(n, a, x, y);
+  else
+    k_axpby<<<blocks_for(n), RB, 0, st>>>(n, a, x, b, y);
   return launch_status();
 }
 int dev_axpy3(long n, double a, const double *x, const double *y, double *z, hipStream_t st) {

@@ -5,7 +5,7 @@
 
 namespace v3d {
 
-// y = a*x + b*y  (x may alias y)
+// y = a*x + b*y  (x may alias y; b == 0 writes a*x without reading y)
 int dev_axpby(long n, double a, const double *x, double b, double *y, hipStream_t st);
 // z = a*x + y
 int dev_axpy3(long n, double a, const double *x, const double *y, double *z, hipStream_t st);

@@ -6,7 +6,18 @@ import numpy as np
 
 import torch
 
-from . import call, load, ptr, stream_ptr
+from . import Ves3dError, call, load, ptr, stream_ptr
+
+DIVERGENCE_ERROR = 31  # ErrorEvent::DivergenceError (include/Error.h:46-67)
+
+
+class StepDivergence(Ves3dError):
+    """The step's solve failed or its true residual missed the tolerance;
+    .stats holds the step statistics."""
+
+    def __init__(self, stats):
+        super().__init__("ves3d_hip_stepper_step", DIVERGENCE_ERROR)
+        self.stats = stats
 
 
 class StepParams(ctypes.Structure):
@@ -162,11 +173,22 @@ class Stepper:
         call("ves3d_hip_stepper_frames", self.h, ptr(F))
         return F
 
-    def step(self):
+    def step(self, check=True):
+        """One implicit step; returns its statistics.  A solve that fails or
+        whose true residual misses the tolerance is a DivergenceError
+        (src/InterfacialVelocity.cc:176-188): the state is left as it was,
+        and the call raises StepDivergence (carrying the statistics) or, with
+        check=False, returns them with "diverged" set."""
         st = StepStats()
         torch.cuda.synchronize()
-        call("ves3d_hip_stepper_step", self.h, ctypes.byref(st))
-        return st.as_dict()
+        rc = load().ves3d_hip_stepper_step(self.h, ctypes.byref(st))
+        d = st.as_dict()
+        d["diverged"] = rc == DIVERGENCE_ERROR
+        if rc == DIVERGENCE_ERROR and check:
+            raise StepDivergence(d)
+        if rc not in (0, DIVERGENCE_ERROR):
+            raise Ves3dError("ves3d_hip_stepper_step", rc)
+        return d
 
     def reparam(self):
         """Reparametrization of the current positions; returns the iterations."""
