@@ -12,7 +12,10 @@
   singular op  : unit-sphere closed forms  SL[c] = (2/3) c,  SL[n] = 0,
                  DL[c] = -c/2 (the grid's outward normal, the reference's
                  orientation), and spectral
-                 convergence on a perturbed shape.
+                 convergence on a perturbed shape; on an ellipsoid (where
+                 the closed forms would not hold at a wrongly rotated
+                 target) DL[c] = -c/2 and SL[n] = 0, and DL[c] = -c/2
+                 converges spectrally in p1 on a rough perturbed shape.
 """
 import os
 
@@ -20,7 +23,7 @@ import numpy as np
 import pytest
 from scipy.special import lpmv, factorial
 
-from oracle import sph
+from oracle import sph, ves
 from ves3d_amd import workloads as W
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -166,3 +169,51 @@ def test_singular_convergence_on_perturbed_shape():
         vals.append(F @ SL[0])
     e = [np.max(np.abs(v - vals[-1])) for v in vals[:-1]]
     assert e[0] > e[1] > e[2] and e[2] < 1e-6, e
+
+
+def _ellipsoid(p0, seed=5):
+    """One rotated ellipsoid of the full-step workload's family (semi-axes
+    1, 1, 0.6 +- 0.05), centred at the origin: [1][3][ng]."""
+    X = W.implicit_step_case(1, p0, 0.2, seed=seed)
+    return X - X.mean(axis=2, keepdims=True)
+
+
+@pytest.mark.parametrize("p0,sl_tol", [(12, 5e-7), (16, 3e-8)])
+def test_singular_identities_on_an_ellipsoid(p0, sl_tol):
+    """Identities that hold on every closed surface, so that they see which
+    point each target's rotation brings to the pole (the unit-sphere closed
+    forms hold at every point alike): the double layer of a constant density
+    is -c/2 on the surface (outward normal), and the single layer of the
+    unit normal vanishes (the Stokeslet is divergence-free).  The normal is
+    not band-limited, so SL[n] carries its truncation error, which decays
+    with p0."""
+    X = _ellipsoid(p0)
+    SL, DL = sph.stokes_singular_integ(X, p0, 2 * p0)
+    nc = p0 * (p0 + 2)
+    c = np.array([0.3, -1.1, 0.7])
+    f = np.repeat(c, X.shape[2]).reshape(3, -1)
+    F = sph.grid2shc(f, p0, p0).reshape(-1)
+    Vd = sph.shc2grid((F @ DL[0]).reshape(3, nc), p0, p0)
+    assert np.max(np.abs(Vd + 0.5 * f)) < 1e-12
+    n = ves.Surface(X, p0).normal[0]
+    Fn = sph.grid2shc(n, p0, p0).reshape(-1)
+    Vn = sph.shc2grid((Fn @ SL[0]).reshape(3, nc), p0, p0)
+    assert np.max(np.abs(Vn)) < sl_tol
+
+
+def test_double_layer_identity_converges_on_a_perturbed_shape():
+    """DL[c] = -c/2 on a rough perturbed sphere (amplitude 0.1/(l+1)^2 up to
+    l = p0): the singular quadrature converges spectrally in p1."""
+    p0 = 12
+    nc = p0 * (p0 + 2)
+    X = W.perturbed_shapes(1, p0, amp=0.1, seed=6)
+    c = np.array([0.3, -1.1, 0.7])
+    f = np.repeat(c, X.shape[2]).reshape(3, -1)
+    F = sph.grid2shc(f, p0, p0).reshape(-1)
+    e = []
+    for p1 in (16, 24, 36, 48):
+        _, DL = sph.stokes_singular_integ(X, p0, p1, want_sl=False)
+        Vd = sph.shc2grid((F @ DL[0]).reshape(3, nc), p0, p0)
+        e.append(np.max(np.abs(Vd + 0.5 * f)))
+    assert all(a > 20 * b for a, b in zip(e, e[1:])), e
+    assert e[-1] < 1e-11, e

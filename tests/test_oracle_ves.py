@@ -214,3 +214,111 @@ def test_bending_precond_singular_degree():
     assert np.allclose(bp[deg == 9], 1.0 / abs(1.0 - 1e-3 * 729))
     assert np.allclose(bp[deg == 12], 1.0 / abs(1.0 - 1e-3 * 1728))
     assert np.all(bp[deg == 0] == 1.0)
+
+
+def test_surface_dumbbell_known_answers():
+    """test/SurfaceTest.cc:66-176 on the p = 6 dumbbell (the first shape of
+    precomputed/shape_gallery_6.txt), with that test's tolerances: area,
+    volume, max |H|, the surface gradients and Laplacians of the coordinate
+    functions, dot(Lap x, n), H - dot(Lap x, n)/2, H + div(n)/2, the
+    linearised curvature of x itself, and n mapped to the tangent space."""
+    p = 6
+    X = np.load(os.path.join(GOLD, "dumbbell_6.npy"))[None]
+    S = ves.Surface(X, p)
+    assert abs(S.area()[0] / 16.2179377312307 - 1) < 5e-8
+    assert abs(S.volume()[0] / 5.24886489292959 - 1) < 5e-8
+    assert abs(np.max(np.abs(S.H)) / 1.376627062 - 1) < 8e-8
+    want = [(1.01100423438481, 2.239995450856133), (1.011004234384816, 2.239995450856133),
+            (1.004308085588217, 1.777133873450119)]
+    lap = []
+    for i, (g, l) in enumerate(want):
+        gr = S.grad(X[:, i])
+        lp = S.div(gr)
+        lap.append(lp)
+        assert abs(np.max(np.abs(gr)) / g - 1) < 1e-7, i
+        assert abs(np.max(np.abs(lp)) / l - 1) < 1e-7, i
+    hh = np.sum(np.stack(lap, axis=1) * S.normal, axis=1)
+    assert abs(np.max(np.abs(hh)) / 2.428980517523748 - 1) < 1e-7
+    assert abs(np.max(np.abs(S.H - 0.5 * hh)) / 0.348685011112687 - 1) < 1e-6
+    assert abs(np.max(np.abs(S.H + 0.5 * S.div(S.normal))) / 0.2437253515 - 1) < 1e-6
+    assert np.max(np.abs(S.linearized_mean_curv(S.X) - S.H)) < 1e-14
+    n = S.normal
+    assert np.max(np.abs(n - np.sum(n * S.normal, axis=1)[:, None] * S.normal)) < 1e-14
+
+
+def _ellipsoid(p, seed=5):
+    X = W.implicit_step_case(1, p, 0.2, seed=seed)
+    return X - X.mean(axis=2, keepdims=True)
+
+
+def _bending_energy(Y, p, q, kb):
+    """E = 2 kb int H^2 dA on the surface resampled to order q (unfiltered
+    curvature): the energy whose first variation the bending force is."""
+    Yq = ves.resample_vec(Y, p, q)
+    Sq = ves.Surface(Yq, q, filter_freq=q)
+    qw = np.repeat(sph.legendre_weights(q), 2 * q)
+    return 2 * kb * np.sum(Sq.H ** 2 * Sq.W * qw[None, :])
+
+
+def test_bending_force_is_the_first_variation_of_the_energy():
+    """On an ellipsoid (where both terms of the force are alive; on a sphere
+    they vanish whatever their coefficients): moving the surface by
+    eps phi n changes E = 2 kb int H^2 dA at the rate
+    dE/deps = -2 int F_b . (phi n) dA, the sign making F_b the force that
+    lowers the energy (bending_coeff = -kappa_b, src/VesicleProps.cc:41-43).
+    A wrong coefficient of either term, or the opposite sign, fails by O(1).
+    The linear bending force of the position itself is the bending force."""
+    p, kb, eps = 16, 0.01, 1e-4
+    X = _ellipsoid(p)
+    S = ves.Surface(X, p)
+    F = ves.Forces(S, 2 * p, kb)
+    Fb = F.bending()
+    phi = 0.1 * (X[0, 0] + 0.5 * X[0, 1] * X[0, 2])[None]
+    d = phi[:, None] * S.normal
+    dE = (_bending_energy(X + eps * d, p, 3 * p, kb) -
+          _bending_energy(X - eps * d, p, 3 * p, kb)) / (2 * eps)
+    qw = np.repeat(sph.legendre_weights(p), 2 * p)
+    work = np.sum(np.sum(Fb * d, axis=1) * S.W * qw[None, :])
+    assert abs(dE + 2 * work) < 1e-6 * abs(dE), (dE, work)
+    assert np.max(np.abs(F.linear_bending(S.X) - Fb)) < 1e-12 * np.max(np.abs(Fb))
+
+
+def test_tension_force_is_the_first_variation_of_the_area():
+    """A uniform tension sigma: moving the surface by eps phi n changes the
+    area at the rate dA/deps = -int F_s . (phi n) dA / sigma (F_s = 2 sigma H n
+    + grad sigma, src/InterfacialForce.cc:109-112), on an ellipsoid."""
+    p, eps, sig = 16, 1e-4, 0.7
+    X = _ellipsoid(p)
+    S = ves.Surface(X, p)
+    F = ves.Forces(S, 2 * p, 0.01)
+    Fs = F.tensile(np.full((1, X.shape[2]), sig))
+    phi = 0.1 * (X[0, 0] + 0.5 * X[0, 1] * X[0, 2])[None]
+    d = phi[:, None] * S.normal
+
+    def area(Y):
+        return ves.Surface(ves.resample_vec(Y, p, 3 * p), 3 * p).area()[0]
+
+    dA = (area(X + eps * d) - area(X - eps * d)) / (2 * eps)
+    qw = np.repeat(sph.legendre_weights(p), 2 * p)
+    work = np.sum(np.sum(Fs * d, axis=1) * S.W * qw[None, :])
+    assert abs(dA + work / sig) < 1e-6 * abs(dA), (dA, work)
+
+
+def test_lone_vesicle_step_is_inextensible_and_dissipative():
+    """A lone ellipsoid in fluid at rest: the step's velocity is surface-
+    divergence free (the tension rows) and lowers the bending energy (the
+    interfacial force does positive work on the fluid).  A sign slip
+    between the velocity and tension rows, or in the bending force, fails
+    one of the two."""
+    p, kb, dt = 12, 0.01, 1e-2
+    X = _ellipsoid(p)
+    imp = ves.Implicit(X, p, dt=dt, kb=kb, shear=0.0)
+    b = imp.rhs()
+    x, res, it, st = ves.bicgstab(imp.matvec, b, np.zeros_like(b), 1e-10, 500, imp.precond)
+    assert st == "success"
+    u, _ = imp.unpack(x)
+    S = ves.Surface(X, p)
+    assert np.max(np.abs(S.div(u))) < 1e-8 * np.max(np.abs(u))
+    h = 1e-4
+    dEdt = (_bending_energy(X + h * u, p, 2 * p, kb) - _bending_energy(X - h * u, p, 2 * p, kb)) / (2 * h)
+    assert dEdt < 0, dEdt

@@ -14,6 +14,7 @@ import pytest
 import torch
 
 from oracle import sph as O
+from oracle import ves
 from ves3d_amd import sph as G
 from ves3d_amd import workloads as W
 
@@ -145,6 +146,38 @@ def test_singular_closed_forms_gpu(hip, p0):
         assert np.max(np.abs(V - 2.0 / 3.0 * f)) < 1e-9
         assert np.max(np.abs(Vd + 0.5 * f)) < 1e-9
         assert np.max(np.abs(Vn)) < 1e-9
+
+
+@pytest.mark.parametrize("p0,sl_tol", [(16, 5e-8), (24, 1e-8)])
+def test_singular_identities_on_ellipsoids_gpu(hip, p0, sl_tol):
+    """Shape-sensitive identities at the workloads' orders, on two rotated
+    ellipsoids (semi-axes 1, 1, 0.6 +- 0.05), where a target rotated to the
+    wrong pole would break them: DL[c] = -c/2 on the surface and SL[n] = 0
+    (n truncated at p0, whose error decays with p0), p1 = 2 p0, for the
+    FP64 build and, to the FP32 build's accuracy, the FP32 one."""
+    nv = 2
+    X = W.implicit_step_case(nv, p0, 0.2, seed=9)
+    X = X - X.mean(axis=2, keepdims=True)
+    n = 3 * p0 * (p0 + 2)
+    nc = p0 * (p0 + 2)
+    c = np.array([0.3, -1.1, 0.7])
+    f = np.repeat(c, X.shape[2]).reshape(3, -1)
+    F = O.grid2shc(f, p0, p0).reshape(-1)
+    nrm = ves.Surface(X, p0).normal
+    for dt, dl_tol, sl_bar in ((torch.float64, 1e-9, sl_tol), (torch.float32, 1e-4, 1e-4)):
+        print("singular identities p0=%d %s" % (p0, dt), flush=True)
+        gsl = torch.zeros((nv, n, n), dtype=dt, device="cuda")
+        gdl = torch.zeros((nv, n, n), dtype=dt, device="cuda")
+        G.singular_build(cu(X), nv, p0, 2 * p0, gsl, gdl)
+        for v in range(nv):
+            SLv = gsl[v].double().cpu().numpy()
+            DLv = gdl[v].double().cpu().numpy()
+            Vd = O.shc2grid((F @ DLv).reshape(3, nc), p0, p0)
+            Fn = O.grid2shc(nrm[v], p0, p0).reshape(-1)
+            Vn = O.shc2grid((Fn @ SLv).reshape(3, nc), p0, p0)
+            assert np.max(np.abs(Vd + 0.5 * f)) < dl_tol, (dt, v)
+            assert np.max(np.abs(Vn)) < sl_bar, (dt, v)
+        del gsl, gdl
 
 
 @pytest.mark.parametrize("dt,tol", [(torch.float64, 1e-13), (torch.float32, 1e-6)])

@@ -150,14 +150,15 @@ __global__ void k_bend_s2(const double *__restrict__ a, const double *__restrict
   GRID_LOOP(i, n) s2[i] = a[i] * (H[i] * H[i] - K[i]);
 }
 
-// F = kb (s1 + 2 s2) n   (src/InterfacialForce.cc:37-40)
+// F = c (s1 + 2 s2) n with c the bending coefficient, -kappa_b
+// (src/InterfacialForce.cc:37-41, src/VesicleProps.cc:41-43)
 __global__ void k_bend_force(const double *__restrict__ s1, const double *__restrict__ s2,
-                             const double *__restrict__ nrm, double kb, long nv, long ng,
+                             const double *__restrict__ nrm, double coeff, long nv, long ng,
                              double *Fo) {
   GRID_LOOP(i, nv * ng) {
     const long v = i / ng, g = i % ng;
     const long b = v * 3 * ng + g;
-    const double s = kb * (s1[i] + 2.0 * s2[i]);
+    const double s = coeff * (s1[i] + 2.0 * s2[i]);
 #pragma unroll
     for (int d = 0; d < 3; ++d) Fo[b + d * ng] = s * nrm[b + d * ng];
   }
@@ -366,7 +367,7 @@ int ForcesG::bending(double *Fb, hipStream_t st) {  // src/InterfacialForce.cc:2
   V3D_CHECK(launch_status());
   V3D_CHECK(Su.grad(Su.H, gu, st));
   V3D_CHECK(Su.div(gu, su1, st));
-  k_bend_force<<<gblocks(nv * mu), 256, 0, st>>>(su1, su2, Su.nrm, kb, nv, mu, fu);
+  k_bend_force<<<gblocks(nv * mu), 256, 0, st>>>(su1, su2, Su.nrm, -kb, nv, mu, fu);
   V3D_CHECK(launch_status());
   return resample(fu, 3 * nv, pu, p, Fb, st);
 }
@@ -379,7 +380,7 @@ int ForcesG::linear_bending(const double *x_new, double *Fb, hipStream_t st) {  
   V3D_CHECK(launch_status());
   V3D_CHECK(Su.grad(su1, gu, st));
   V3D_CHECK(Su.div(gu, su1, st));
-  k_bend_force<<<gblocks(nv * mu), 256, 0, st>>>(su1, su2, Su.nrm, kb, nv, mu, fu);
+  k_bend_force<<<gblocks(nv * mu), 256, 0, st>>>(su1, su2, Su.nrm, -kb, nv, mu, fu);
   V3D_CHECK(launch_status());
   return resample(fu, 3 * nv, pu, p, Fb, st);
 }
