@@ -373,6 +373,15 @@ typedef struct ves3d_step_params {
                              x_{n-1}, as the reference's AssembleInitial
                              (src/InterfacialVelocity.cc:873-919).  No counterpart in
                              the reference. */
+  double far_coarse;      /* > 1: two-level far field.  A wave of 64 targets reads a
+                             vesicle's sources on its own p grid (with that grid's
+                             quadrature weights) instead of its upsampled ones when
+                             every target of the wave is at least far_coarse times the
+                             vesicle's radius from its centre (the p-grid quadrature is
+                             then as accurate as the upsampled one).  0: upsampled
+                             sources for every pair, the reference's far field
+                             (src/StokesVelocity.cc:477-490).  No counterpart in the
+                             reference, whose FMM makes its own approximation there. */
 } ves3d_step_params;
 
 typedef struct ves3d_step_stats {
@@ -397,6 +406,8 @@ typedef struct ves3d_step_stats {
   int inner_iters;        /* inner BiCGStab iterations summed over the step's
                              preconditioner applications (0 without inner solves) */
   int local_matvecs;      /* local-operator applications of the inner solves */
+  double far_pairs;       /* source-target pairs of one far-field evaluation of one
+                             density in this step (fewer with far_coarse) */
 } ves3d_step_stats;
 
 /* Creates / destroys a stepper (device state for nves vesicles). */

@@ -203,6 +203,33 @@ __global__ void k_far_setup(const double *__restrict__ X, const double *__restri
   }
 }
 
+// p-grid quadrature weight of the coarse far-field sources:
+// aqwp[v][g] = |x_t x x_p| w(lat) (as k_far_setup on the upsampled grid)
+__global__ void k_coarse_weights(const double *__restrict__ Xt, const double *__restrict__ Xp,
+                                 const double *__restrict__ qw, long nv, int p,
+                                 double *__restrict__ aqwp) {
+  const long ng = 2L * p * (p + 1);
+  GRID_LOOP(i, nv * ng) {
+    const long v = i / ng, s = i % ng;
+    const long b = v * 3 * ng + s;
+    const double xt = Xt[b], yt = Xt[b + ng], zt = Xt[b + 2 * ng];
+    const double xp = Xp[b], yp = Xp[b + ng], zp = Xp[b + 2 * ng];
+    const double nx = yt * zp - zt * yp, ny = zt * xp - xt * zp, nz = xt * yp - yt * xp;
+    aqwp[i] = sqrt(nx * nx + ny * ny + nz * nz) * qw[s / (2 * p)];
+  }
+}
+
+// qc[v ng + g][d] = f[v][d][g] * aqwp[v][g] (point-major coarse densities)
+__global__ void k_pack_coarse(const double *__restrict__ f, const double *__restrict__ aqwp,
+                              long nv, long ng, double *__restrict__ qc) {
+  GRID_LOOP(i, nv * ng) {
+    const long v = i / ng, g = i % ng;
+    const double w = aqwp[i];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) qc[i * 3 + d] = f[(v * 3 + d) * ng + g] * w;
+  }
+}
+
 // AxisMajor [nv][3][ng] -> point-major [nv*ng][3]
 __global__ void k_to_points(const double *__restrict__ X, long nv, long ng, double *P) {
   GRID_LOOP(i, nv * ng) {
@@ -397,6 +424,8 @@ int ForcesG::tensile(const double *sigma, double *Fs, hipStream_t st) {  // :94-
 // ------------------------------------------------------------- StokesG
 StokesG::~StokesG() {
   if (SL32) (void)hipFree(SL32);
+  if (cflags) (void)hipFree(cflags);
+  if (ccounts) (void)hipFree(ccounts);
   for (auto e : ev0) (void)hipEventDestroy(e);
   for (auto e : ev1) (void)hipEventDestroy(e);
 }
@@ -426,12 +455,14 @@ int StokesG::far_time(double *ms, int *calls, double *ms2, int *calls2) {
 }
 
 int StokesG::init(int p_, int pu_, long nv_, int precision_, int near_, double repul_dist,
-                  double r_near) {
+                  double r_near, double far_coarse_) {
   p = p_;
   pu = pu_;
   nv = nv_;
   precision = precision_;
   use_near = near_ != 0;
+  if (far_coarse_ != 0.0 && !(far_coarse_ > 1.0)) return VES3D_INVALID_PARAMETER;
+  far_coarse = far_coarse_;
   const long nc = sh_ncoef(p), ng = sh_ngrid(p), mu = sh_ngrid(pu);
   const long nmat = nv * 9 * nc * nc;
   if (precision != 1) {
@@ -447,6 +478,14 @@ int StokesG::init(int p_, int pu_, long nv_, int precision_, int near_, double r
   for (DBuf *b : {&F, &V, &shc}) V3D_CHECK(b->alloc(3 * nv * nc));
   for (DBuf *b : {&up, &tmpx, &tmpt, &tmpp}) V3D_CHECK(b->alloc(3 * nv * mu));
   V3D_CHECK(pole.alloc(3 * nv * 2));
+  if (far_coarse > 0) {
+    V3D_CHECK(aqwp.alloc(nv * ng));
+    V3D_CHECK(qc.alloc(nv * ng * 3));
+    const long nw = (nv * ng + 63) / 64;
+    if (hipMalloc(&cflags, (size_t)nw * nv) != hipSuccess) return VES3D_MEMORY_ERROR;
+    if (hipMalloc(&ccounts, 2 * sizeof(unsigned long long)) != hipSuccess)
+      return VES3D_MEMORY_ERROR;
+  }
   if (use_near) {
     V3D_CHECK(near.init(p, pu, nv, repul_dist, r_near));
     V3D_CHECK(frep.alloc(3 * nv * ng));
@@ -458,7 +497,7 @@ int StokesG::init(int p_, int pu_, long nv_, int precision_, int near_, double r
 }
 
 int StokesG::setup(const double *X, hipStream_t st) {
-  const long ng = sh_ngrid(p);
+  const long ng = sh_ngrid(p), mu = sh_ngrid(pu);
   if (precision != 1)
     V3D_CHECK(singular_build<double>(X, nv, p, pu, SL.p, (double *)nullptr, st));
   else
@@ -474,11 +513,36 @@ int StokesG::setup(const double *X, hipStream_t st) {
                                                                   pu, far_src, aqw);
   V3D_CHECK(launch_status());
   // targets: the re-synthesised p-grid positions (src/StokesVelocity.cc:227-239)
-  V3D_CHECK(sht_shc2grid<double>(shc, 3 * nv, p, p, tmpx, nullptr, nullptr, work, st));
+  const bool coarse = far_coarse > 0;
+  V3D_CHECK(sht_shc2grid<double>(shc, 3 * nv, p, p, tmpx, coarse ? tmpt.p : nullptr,
+                                 coarse ? tmpp.p : nullptr, work, st));
   k_to_points<<<gblocks(nv * ng), 256, 0, st>>>(tmpx, nv, ng, trg);
   V3D_CHECK(launch_status());
+  DBuf dqwp;
+  if (coarse) {  // the targets double as the coarse far-field sources
+    std::vector<double> qwp;
+    legendre_weights(p, qwp);
+    V3D_CHECK(upload(dqwp, qwp));
+    k_coarse_weights<<<gblocks(nv * ng), 256, 0, st>>>(tmpt, tmpp, dqwp, nv, p, aqwp);
+    V3D_CHECK(launch_status());
+    V3D_CHECK(far_coarse_flags(far_src, nv * (mu + 2), mu + 2, trg, nv * ng, ng, far_coarse,
+                               cflags, ccounts, st));
+    unsigned long long c[2];
+    V3D_HIP(hipMemcpyAsync(c, ccounts, sizeof(c), hipMemcpyDeviceToHost, st));
+    V3D_HIP(hipStreamSynchronize(st));
+    far_pairs = 64.0 * ((double)c[0] * ng + (double)c[1] * (mu + 2));
+  } else {
+    // every target against every other vesicle's upsampled sources
+    far_pairs = (double)(nv * ng) * (double)(nv - 1) * (double)(mu + 2);
+  }
   if (use_near) V3D_CHECK(near.setup(far_src, trg, st));
-  return hip_status(hipStreamSynchronize(st));  // dqw is freed on return
+  return hip_status(hipStreamSynchronize(st));  // dqw, dqwp are freed on return
+}
+
+int StokesG::pack_coarse(const double *f, double *qcoarse, hipStream_t st) {
+  const long ng = sh_ngrid(p);
+  k_pack_coarse<<<gblocks(nv * ng), 256, 0, st>>>(f, aqwp, nv, ng, qcoarse);
+  return launch_status();
 }
 
 int StokesG::self_and_density(const double *f, double *Fc, double *Vc, double *vel, double *upc,
@@ -534,9 +598,11 @@ int StokesG::apply(const double *f, double *vel, hipStream_t st, bool add_repul,
   }
   V3D_CHECK(self_and_density(f, F, V, vel, up, q, !local, st));
   if (local) return VES3D_SUCCESS;
+  FarCoarse fc{trg, qc, nullptr, (size_t)ng, cflags};
+  if (far_coarse > 0) V3D_CHECK(pack_coarse(f, qc, st));
   V3D_CHECK(event_begin(1, st));
   V3D_CHECK(far_field_between_groups(precision != 0, far_src, q, nv * (mu + 2), trg, farv, nv * ng,
-                                     mu + 2, ng, st));
+                                     mu + 2, ng, st, far_coarse > 0 ? &fc : nullptr));
   V3D_CHECK(event_end(st));
   k_add_points<<<gblocks(nv * ng), 256, 0, st>>>(farv, nv, ng, vel);
   V3D_CHECK(launch_status());
@@ -561,9 +627,17 @@ int StokesG::apply2(const double *f1, double *vel1, bool add_repul1, const doubl
   }
   V3D_CHECK(self_and_density(f1, F, V, vel1, up, q, true, st));
   V3D_CHECK(self_and_density(f2, F2, V2, vel2, up2, q2, true, st));
+  FarCoarse fc{trg, qc, qc2, (size_t)ng, cflags};
+  if (far_coarse > 0) {
+    if (!qc2.p) V3D_CHECK(qc2.alloc(nv * ng * 3));
+    fc.den2 = qc2;
+    V3D_CHECK(pack_coarse(f1, qc, st));
+    V3D_CHECK(pack_coarse(f2, qc2, st));
+  }
   V3D_CHECK(event_begin(2, st));
   V3D_CHECK(far_field_between_groups2(precision != 0, far_src, q, q2, nv * (mu + 2), trg, farv,
-                                      farv2, nv * ng, mu + 2, ng, st));
+                                      farv2, nv * ng, mu + 2, ng, st,
+                                      far_coarse > 0 ? &fc : nullptr));
   V3D_CHECK(event_end(st));
   k_add_points<<<gblocks(nv * ng), 256, 0, st>>>(farv, nv, ng, vel1);
   V3D_CHECK(launch_status());

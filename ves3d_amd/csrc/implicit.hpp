@@ -17,15 +17,39 @@ namespace v3d {
 // accumulation (f32c).  Distinct vesicles' surfaces are disjoint, so the
 // pair loop runs without the coincident-point guard of the direct-evaluation
 // entry points (stokes_direct.hip).
+//
+// coarse (optional): a second, coarser set of sources per group (cpg points
+// each, point-major positions and densities) that a wave of 64 targets uses
+// instead of the fine ones where flags[wave][group] is set (see
+// far_coarse_flags).  The FP32 form stores both levels relative to the
+// group's first fine source.  With coarse sources the scalar-cache kernels
+// run whatever ves3d_hip_set_far_kernel selected.
+struct FarCoarse {
+  const double *pos;           // [n_groups * cpg][3]
+  const double *den1, *den2;   // densities at the coarse points (den2: the second of a pair)
+  size_t cpg;
+  const unsigned char *flags;  // [ceil(n_trg / 64)][n_groups]
+};
 int far_field_between_groups(bool f32c, const double *spos, const double *sld, size_t n_src,
                              const double *tpos, double *tvel, size_t n_trg, size_t spg,
-                             size_t tpg, hipStream_t st);
+                             size_t tpg, hipStream_t st, const FarCoarse *coarse = nullptr);
 
 // The same with two densities on the same sources and targets, in one pass
 // (the pair geometry is computed once and serves both sums).
 int far_field_between_groups2(bool f32c, const double *spos, const double *sld1,
                               const double *sld2, size_t n_src, const double *tpos, double *tvel1,
-                              double *tvel2, size_t n_trg, size_t spg, size_t tpg, hipStream_t st);
+                              double *tvel2, size_t n_trg, size_t spg, size_t tpg, hipStream_t st,
+                              const FarCoarse *coarse = nullptr);
+
+// flags[w][g] = 1 where every target of wave w (targets 64 w .. 64 w + 63) is
+// at least kappa R_g from c_g, (c_g, R_g) the bounding sphere of group g's
+// fine sources (centre: their mean).  flags: ceil(n_trg / 64) * n_groups
+// bytes.  kappa > 1.  counts (device, 2 values): the (wave, group) pairs
+// that read the coarse sources, and those that read the fine ones (a wave
+// skips a group that holds all its targets).
+int far_coarse_flags(const double *spos, size_t n_src, size_t spg, const double *tpos,
+                     size_t n_trg, size_t tpg, double kappa, unsigned char *flags,
+                     unsigned long long *counts, hipStream_t st);
 
 // Device buffer owned by the step objects.
 struct DBuf {
@@ -85,8 +109,12 @@ class StokesG {
   // near: 1 applies the near-singular correction (src/NearSingular.cc) with
   // repulsion length repul_dist (<= 0: none) and near-zone radius r_near
   // (<= 0: the reference's r_ves 2/sqrt(p_up)).
+  // far_coarse > 1: the far field reads a vesicle's p-grid sources (its
+  // own quadrature) instead of its upsampled ones for every wave of 64
+  // targets at least far_coarse times the vesicle's radius from its centre;
+  // 0: the upsampled sources for every target (the reference's far field).
   int init(int p, int p_up, long nv, int precision, int near = 1, double repul_dist = 5e-2,
-           double r_near = 0.0);
+           double r_near = 0.0, double far_coarse = 0.0);
   int setup(const double *X, hipStream_t st);
   // add_repul: the density is f + NearSingular::ForceRepul (the right-hand
   // side of the implicit step, src/InterfacialVelocity.cc:733).  local: the
@@ -119,6 +147,16 @@ class StokesG {
   NearG near;
   DBuf frep, vup, vpole;
   DBuf F2, V2, up2, q2, farv2;  // second density of apply2
+  // coarse far-field sources: the p-grid targets with their quadrature
+  // weights aqwp (|x_t x x_p| w(lat)), densities qc, qc2, wave flags
+  double far_coarse = 0.0;
+  DBuf aqwp, qc, qc2;
+  unsigned char *cflags = nullptr;
+  unsigned long long *ccounts = nullptr;  // device: coarse, fine (wave, vesicle) pairs
+  // source-target pairs one far-field launch evaluates (per density), as of
+  // the last setup
+  double far_pairs = 0;
+  int pack_coarse(const double *f, double *qcoarse, hipStream_t st);
   ~StokesG();
 
  private:

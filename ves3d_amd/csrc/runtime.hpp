@@ -19,6 +19,8 @@ enum WorkspaceSlot {
   kWsSing4,
   kWsSolver,
   kWsFarRecords,
+  kWsFarCoarse,
+  kWsFarBounds,
   kWsCount
 };
 

@@ -24,6 +24,7 @@
 //     vesicle, sources of that vesicle staged through LDS.
 #include <atomic>
 #include "common.hpp"
+#include "implicit.hpp"
 #include "runtime.hpp"
 
 namespace v3d {
@@ -394,15 +395,18 @@ __global__ void k_pack_far64(const double *__restrict__ spos, const double *__re
   r._p0 = r._p1 = 0.0;
   rec[i] = r;
 }
+// FP32 records relative to the group's anchor anc[3 (i / spg) apg] (the
+// first fine source of the group, for both levels).
 __global__ void k_pack_far32(const double *__restrict__ spos, const double *__restrict__ sld,
-                             long n, long spg, FarRec32 *__restrict__ rec) {
+                             long n, long spg, const double *__restrict__ anc, long apg,
+                             FarRec32 *__restrict__ rec) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const long a = (i / spg) * spg;  // group anchor: its first source
+  const double *a = anc + 3 * (i / spg) * apg;
   FarRec32 r;
-  r.x = (float)(spos[3 * i] - spos[3 * a]);
-  r.y = (float)(spos[3 * i + 1] - spos[3 * a + 1]);
-  r.z = (float)(spos[3 * i + 2] - spos[3 * a + 2]);
+  r.x = (float)(spos[3 * i] - a[0]);
+  r.y = (float)(spos[3 * i + 1] - a[1]);
+  r.z = (float)(spos[3 * i + 2] - a[2]);
   r.fx = (float)sld[3 * i];
   r.fy = (float)sld[3 * i + 1];
   r.fz = (float)sld[3 * i + 2];
@@ -414,8 +418,32 @@ __global__ void k_pack_far32(const double *__restrict__ spos, const double *__re
 // kOwnGroupBias below): about 1e-15 |f| per own-group source.
 constexpr double kOwnGroupBias64 = 1e30;
 
+// Two-level sources.  Every group (vesicle) carries its fine sources (spg
+// records: the upsampled surface) and, optionally, coarse ones (cpg
+// records: the surface at its own order with its own quadrature weights).
+// cflag[w][g] != 0 lets wave w (targets 64 w .. 64 w + 63) read group g's
+// coarse records: the flag is set where every target of the wave is at
+// least kappa times g's radius from g's centre, so that the coarse
+// quadrature is as accurate as the fine one there.  The flag, the record
+// base and the count are wave-uniform (SGPRs), so the source loop keeps its
+// scalar addressing.  A wave whose targets all belong to group g skips g.
+struct WaveSpan {
+  long row, first, last;  // flag row (clamped to the last wave), first and last target
+};
+__device__ __forceinline__ WaveSpan wave_span(long tb, long n_trg) {
+  const long wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const long w = tb * (FF_BLOCK / 64) + wv;
+  const long n_waves = (n_trg + 63) / 64;
+  WaveSpan r;
+  r.row = w < n_waves ? w : n_waves - 1;
+  r.first = w * 64;
+  r.last = min(r.first + 64, n_trg) - 1;
+  return r;
+}
+
 __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f64(
-    const FarRec64 *__restrict__ rec, long spg, long n_groups, const double *__restrict__ tpos,
+    const FarRec64 *__restrict__ rec, long spg, const FarRec64 *__restrict__ crec, long cpg,
+    const unsigned char *__restrict__ cflag, long n_groups, const double *__restrict__ tpos,
     long n_trg, long tpg, int nsplit, double csl, double *__restrict__ part) {
   const long slice = blockIdx.x % nsplit;
   const long tb = blockIdx.x / nsplit;
@@ -427,16 +455,16 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f64(
   const long tc = tvalid ? t : n_trg - 1;
   const double tx = tpos[3 * tc], ty = tpos[3 * tc + 1], tz = tpos[3 * tc + 2];
   const long my_g = tc / tpg;
-  const long tfirst = tb * FF_BLOCK;
-  const long tlast = min(tfirst + FF_BLOCK, n_trg) - 1;
+  const WaveSpan ws = wave_span(tb, n_trg);
   double Sx = 0, Sy = 0, Sz = 0;
   long g0, g1;
   slice_range(slice, nsplit, 0, spg, n_groups, g0, g1);
-  const int ns = (int)spg;
   for (long g = g0; g < g1; ++g) {
-    if (tfirst / tpg == g && tlast / tpg == g) continue;  // block-uniform skip
+    if (ws.first / tpg == g && ws.last / tpg == g) continue;  // wave-uniform skip
     const double bias = my_g == g ? kOwnGroupBias64 : 0.0;
-    const FarRec64 *__restrict__ r = rec + g * spg;
+    const bool co = cflag && cflag[ws.row * n_groups + g];
+    const FarRec64 *__restrict__ r = co ? crec + g * cpg : rec + g * spg;
+    const int ns = (int)(co ? cpg : spg);
 #pragma unroll 4
     for (int j = 0; j < ns; ++j) {
       const FarRec64 s = r[j];
@@ -474,8 +502,9 @@ __device__ __forceinline__ void acc_merge(Acc32 &a, float x, float y, float z) {
 constexpr float kOwnGroupBias = 1e30f;
 
 __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c(
-    const FarRec32 *__restrict__ rec, const double *__restrict__ spos, long spg, long n_groups,
-    const double *__restrict__ tpos, long n_trg, long tpg, int nsplit, double csl,
+    const FarRec32 *__restrict__ rec, const double *__restrict__ spos, long spg,
+    const FarRec32 *__restrict__ crec, long cpg, const unsigned char *__restrict__ cflag,
+    long n_groups, const double *__restrict__ tpos, long n_trg, long tpg, int nsplit, double csl,
     double *__restrict__ part) {
   const long slice = blockIdx.x % nsplit;
   const long tb = blockIdx.x / nsplit;
@@ -486,20 +515,21 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c(
   const long tc = tvalid ? t : n_trg - 1;
   const double tx = tpos[3 * tc], ty = tpos[3 * tc + 1], tz = tpos[3 * tc + 2];
   const long my_g = tc / tpg;
-  const long tfirst = tb * FF_BLOCK;
-  const long tlast = min(tfirst + FF_BLOCK, n_trg) - 1;
+  const WaveSpan ws = wave_span(tb, n_trg);
   Acc32 S = {{0, 0, 0}, {0, 0, 0}};
   long g0, g1;
   slice_range(slice, nsplit, 0, spg, n_groups, g0, g1);
   for (long g = g0; g < g1; ++g) {
-    if (tfirst / tpg == g && tlast / tpg == g) continue;  // block-uniform skip
+    if (ws.first / tpg == g && ws.last / tpg == g) continue;  // wave-uniform skip
     // a target's own group is not skipped lane by lane (that would make the
     // loop below divergent): its r^2 carries kOwnGroupBias instead
     const float bias = my_g == g ? kOwnGroupBias : 0.0f;
-    const double *a = spos + 3 * g * spg;
+    const double *a = spos + 3 * g * spg;  // the group's anchor (both levels)
     const float lx = (float)(tx - a[0]), ly = (float)(ty - a[1]), lz = (float)(tz - a[2]);
-    const FarRec32 *__restrict__ r = rec + g * spg;
-    auto chunk = [&](const FarRec32 *__restrict__ rr, int cnt, float &sx, float &sy,
+    const bool co = cflag && cflag[ws.row * n_groups + g];
+    const FarRec32 *__restrict__ r = co ? crec + g * cpg : rec + g * spg;
+    const long cnt = co ? cpg : spg;
+    auto chunk = [&](const FarRec32 *__restrict__ rr, int n, float &sx, float &sy,
                      float &sz) {
       auto pair = [&](const FarRec32 &src) {
         const float dx = lx - src.x, dy = ly - src.y, dz = lz - src.z;
@@ -511,16 +541,16 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c(
         sy += ri * (src.fy + dy * ip);
         sz += ri * (src.fz + dz * ip);
       };
-      if (cnt == FF_SUB) {
+      if (n == FF_SUB) {
 #pragma unroll 8
         for (int j = 0; j < FF_SUB; ++j) pair(rr[j]);
       } else {
-        for (int j = 0; j < cnt; ++j) pair(rr[j]);
+        for (int j = 0; j < n; ++j) pair(rr[j]);
       }
     };
-    for (long j0 = 0; j0 < spg; j0 += FF_SUB) {
+    for (long j0 = 0; j0 < cnt; j0 += FF_SUB) {
       float sx = 0, sy = 0, sz = 0;
-      chunk(r + j0, (int)min((long)FF_SUB, spg - j0), sx, sy, sz);
+      chunk(r + j0, (int)min((long)FF_SUB, cnt - j0), sx, sy, sz);
       acc_merge(S, sx, sy, sz);
     }
   }
@@ -619,14 +649,15 @@ struct __align__(16) FarRec64x2 {
 
 __global__ void k_pack_far32x2(const double *__restrict__ spos, const double *__restrict__ s1,
                                const double *__restrict__ s2, long n, long spg,
+                               const double *__restrict__ anc, long apg,
                                FarRec32x2 *__restrict__ rec) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const long a = (i / spg) * spg;
+  const double *a = anc + 3 * (i / spg) * apg;
   FarRec32x2 r;
-  r.x = (float)(spos[3 * i] - spos[3 * a]);
-  r.y = (float)(spos[3 * i + 1] - spos[3 * a + 1]);
-  r.z = (float)(spos[3 * i + 2] - spos[3 * a + 2]);
+  r.x = (float)(spos[3 * i] - a[0]);
+  r.y = (float)(spos[3 * i + 1] - a[1]);
+  r.z = (float)(spos[3 * i + 2] - a[2]);
   r.ax = (float)s1[3 * i];
   r.ay = (float)s1[3 * i + 1];
   r.az = (float)s1[3 * i + 2];
@@ -656,8 +687,9 @@ __global__ void k_pack_far64x2(const double *__restrict__ spos, const double *__
 }
 
 __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c_x2(
-    const FarRec32x2 *__restrict__ rec, const double *__restrict__ spos, long spg, long n_groups,
-    const double *__restrict__ tpos, long n_trg, long tpg, int nsplit, double csl,
+    const FarRec32x2 *__restrict__ rec, const double *__restrict__ spos, long spg,
+    const FarRec32x2 *__restrict__ crec, long cpg, const unsigned char *__restrict__ cflag,
+    long n_groups, const double *__restrict__ tpos, long n_trg, long tpg, int nsplit, double csl,
     double *__restrict__ part1, double *__restrict__ part2) {
   const long slice = blockIdx.x % nsplit;
   const long tb = blockIdx.x / nsplit;
@@ -666,19 +698,20 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c_x2(
   const long tc = tvalid ? t : n_trg - 1;  // uniform control flow, as k_far_groups_f32c
   const double tx = tpos[3 * tc], ty = tpos[3 * tc + 1], tz = tpos[3 * tc + 2];
   const long my_g = tc / tpg;
-  const long tfirst = tb * FF_BLOCK;
-  const long tlast = min(tfirst + FF_BLOCK, n_trg) - 1;
+  const WaveSpan ws = wave_span(tb, n_trg);
   Acc32 A = {{0, 0, 0}, {0, 0, 0}}, B = {{0, 0, 0}, {0, 0, 0}};
   long g0, g1;
   slice_range(slice, nsplit, 0, spg, n_groups, g0, g1);
   for (long g = g0; g < g1; ++g) {
-    if (tfirst / tpg == g && tlast / tpg == g) continue;  // block-uniform skip
+    if (ws.first / tpg == g && ws.last / tpg == g) continue;  // wave-uniform skip
     const float bias = my_g == g ? kOwnGroupBias : 0.0f;
     const double *a = spos + 3 * g * spg;
     const float lx = (float)(tx - a[0]), ly = (float)(ty - a[1]), lz = (float)(tz - a[2]);
-    const FarRec32x2 *__restrict__ r = rec + g * spg;
-    for (long j0 = 0; j0 < spg; j0 += FF_SUB) {
-      const int cnt = (int)min((long)FF_SUB, spg - j0);
+    const bool co = cflag && cflag[ws.row * n_groups + g];
+    const FarRec32x2 *__restrict__ r = co ? crec + g * cpg : rec + g * spg;
+    const long n = co ? cpg : spg;
+    for (long j0 = 0; j0 < n; j0 += FF_SUB) {
+      const int cnt = (int)min((long)FF_SUB, n - j0);
       const FarRec32x2 *__restrict__ rr = r + j0;
       float ux = 0, uy = 0, uz = 0, vx = 0, vy = 0, vz = 0;
       auto pair = [&](const FarRec32x2 &src) {
@@ -716,7 +749,8 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c_x2(
 }
 
 __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f64_x2(
-    const FarRec64x2 *__restrict__ rec, long spg, long n_groups, const double *__restrict__ tpos,
+    const FarRec64x2 *__restrict__ rec, long spg, const FarRec64x2 *__restrict__ crec, long cpg,
+    const unsigned char *__restrict__ cflag, long n_groups, const double *__restrict__ tpos,
     long n_trg, long tpg, int nsplit, double csl, double *__restrict__ part1,
     double *__restrict__ part2) {
   const long slice = blockIdx.x % nsplit;
@@ -726,16 +760,16 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f64_x2(
   const long tc = tvalid ? t : n_trg - 1;  // uniform control flow, as k_far_groups_f64
   const double tx = tpos[3 * tc], ty = tpos[3 * tc + 1], tz = tpos[3 * tc + 2];
   const long my_g = tc / tpg;
-  const long tfirst = tb * FF_BLOCK;
-  const long tlast = min(tfirst + FF_BLOCK, n_trg) - 1;
+  const WaveSpan ws = wave_span(tb, n_trg);
   double Sx = 0, Sy = 0, Sz = 0, Tx = 0, Ty = 0, Tz = 0;
   long g0, g1;
   slice_range(slice, nsplit, 0, spg, n_groups, g0, g1);
-  const int ns = (int)spg;
   for (long g = g0; g < g1; ++g) {
-    if (tfirst / tpg == g && tlast / tpg == g) continue;  // block-uniform skip
+    if (ws.first / tpg == g && ws.last / tpg == g) continue;  // wave-uniform skip
     const double bias = my_g == g ? kOwnGroupBias64 : 0.0;
-    const FarRec64x2 *__restrict__ r = rec + g * spg;
+    const bool co = cflag && cflag[ws.row * n_groups + g];
+    const FarRec64x2 *__restrict__ r = co ? crec + g * cpg : rec + g * spg;
+    const int ns = (int)(co ? cpg : spg);
 #pragma unroll 2
     for (int j = 0; j < ns; ++j) {
       const FarRec64x2 s = r[j];
@@ -866,12 +900,110 @@ static int vesicle_stokes(const Real *src, const Real *nrm, const Real *den,
   return launch_status();
 }
 
+// Bounding sphere of every group's fine sources: bnd[g] = (centre, radius),
+// the centre the mean of the sources.  One workgroup per group.
+__global__ __launch_bounds__(256) void k_group_bounds(const double *__restrict__ spos, long spg,
+                                                      double4 *__restrict__ bnd) {
+  const long g = blockIdx.x;
+  const double *P = spos + 3 * g * spg;
+  __shared__ double red[4][4];
+  __shared__ double cen[3];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double c[3] = {0, 0, 0};
+  for (long j = threadIdx.x; j < spg; j += 256)
+    for (int d = 0; d < 3; ++d) c[d] += P[3 * j + d];
+  for (int d = 0; d < 3; ++d) {
+    const double t = wave_sum(c[d]);
+    if (lane == 0) red[wv][d] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3)
+    cen[threadIdx.x] =
+        (((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x]) /
+        (double)spg;
+  __syncthreads();
+  double m = 0;
+  for (long j = threadIdx.x; j < spg; j += 256) {
+    const double dx = P[3 * j] - cen[0], dy = P[3 * j + 1] - cen[1], dz = P[3 * j + 2] - cen[2];
+    m = fmax(m, dx * dx + dy * dy + dz * dz);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if (lane == 0) red[wv][3] = m;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    bnd[g] = make_double4(cen[0], cen[1], cen[2],
+                          sqrt(fmax(fmax(red[0][3], red[1][3]), fmax(red[2][3], red[3][3]))));
+}
+
+// flags[w][g] = 1 where every target of wave w is at least kappa R_g from c_g.
+// One workgroup per wave: its 64 targets in LDS, the threads over the groups.
+// counts[0] += the wave's coarse (wave, group) pairs, counts[1] += its fine
+// ones (groups the wave does not skip as its own).
+__global__ __launch_bounds__(256) void k_coarse_flags(const double *__restrict__ tpos, long n_trg,
+                                                      long tpg, const double4 *__restrict__ bnd,
+                                                      long n_groups, double kappa,
+                                                      unsigned char *__restrict__ flags,
+                                                      unsigned long long *__restrict__ counts) {
+  const long w = blockIdx.x;
+  __shared__ double T[64][3];
+  __shared__ unsigned cnt[2];
+  if (threadIdx.x < 64) {
+    const long t = min(w * 64 + threadIdx.x, n_trg - 1);
+    for (int d = 0; d < 3; ++d) T[threadIdx.x][d] = tpos[3 * t + d];
+  }
+  if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const long gf = (w * 64) / tpg, gl = min(w * 64 + 63, n_trg - 1) / tpg;
+  unsigned nc = 0, nf = 0;
+  for (long g = threadIdx.x; g < n_groups; g += 256) {
+    const double4 b = bnd[g];
+    double m = 1e300;
+    for (int k = 0; k < 64; ++k) {
+      const double dx = T[k][0] - b.x, dy = T[k][1] - b.y, dz = T[k][2] - b.z;
+      m = fmin(m, dx * dx + dy * dy + dz * dz);
+    }
+    const double lim = kappa * b.w;
+    const bool co = m >= lim * lim;
+    flags[w * n_groups + g] = co ? 1 : 0;
+    if (co) ++nc;
+    else if (!(gf == g && gl == g)) ++nf;
+  }
+  atomicAdd(&cnt[0], nc);
+  atomicAdd(&cnt[1], nf);
+  __syncthreads();
+  if (threadIdx.x < 2) atomicAdd(&counts[threadIdx.x], (unsigned long long)cnt[threadIdx.x]);
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+int far_coarse_flags(const double *spos, size_t n_src, size_t spg, const double *tpos,
+                     size_t n_trg, size_t tpg, double kappa, unsigned char *flags,
+                     unsigned long long *counts, hipStream_t st) {
+  if (!spos || !tpos || !flags || !counts || spg == 0 || tpg == 0 || n_src % spg || n_trg == 0 ||
+      !(kappa > 1.0))
+    return VES3D_INVALID_PARAMETER;
+  const long n_groups = (long)(n_src / spg);
+  double4 *bnd = (double4 *)workspace(kWsFarBounds, sizeof(double4) * n_groups);
+  if (!bnd) return VES3D_MEMORY_ERROR;
+  k_group_bounds<<<(unsigned)n_groups, 256, 0, st>>>(spos, (long)spg, bnd);
+  V3D_CHECK(launch_status());
+  k_zero_u64<<<1, 64, 0, st>>>(counts, 2);
+  V3D_CHECK(launch_status());
+  const long n_waves = ((long)n_trg + 63) / 64;
+  k_coarse_flags<<<(unsigned)n_waves, 256, 0, st>>>(tpos, (long)n_trg, (long)tpg, bnd, n_groups,
+                                                    kappa, flags, counts);
+  return launch_status();
+}
+
 int far_field_between_groups(bool f32c, const double *spos, const double *sld, size_t n_src,
                              const double *tpos, double *tvel, size_t n_trg, size_t spg,
-                             size_t tpg, hipStream_t st) {
+                             size_t tpg, hipStream_t st, const FarCoarse *coarse) {
   if (spg == 0 || tpg == 0) return VES3D_INVALID_PARAMETER;
   const double csl = 1.0 / (8.0 * kPi), cdl = 3.0 / (4.0 * kPi);
-  if (g_far_kind.load() == 0) {
+  if (!coarse && g_far_kind.load() == 0) {
     if (f32c)
       return far_eval<true>(spos, sld, nullptr, n_src, tpos, tvel, n_trg, spg, tpg, 0.0, csl, cdl, st, false);
     return far_eval<false>(spos, sld, nullptr, n_src, tpos, tvel, n_trg, spg, tpg, 0.0, csl, cdl, st, false);
@@ -879,31 +1011,48 @@ int far_field_between_groups(bool f32c, const double *spos, const double *sld, s
   if (n_trg == 0) return VES3D_SUCCESS;
   if (!spos || !sld || !tpos || !tvel) return VES3D_INVALID_PARAMETER;
   if (n_src % spg || n_trg % tpg || n_src / spg != n_trg / tpg) return VES3D_INVALID_PARAMETER;
+  if (coarse && (!coarse->pos || !coarse->den1 || !coarse->flags || coarse->cpg == 0))
+    return VES3D_INVALID_PARAMETER;
   const long n_groups = (long)(n_src / spg);
   const int ns = choose_nsplit((long)n_trg, (long)n_src, n_groups);
+  const size_t rsz = f32c ? sizeof(FarRec32) : sizeof(FarRec64);
   double *part = (double *)workspace(kWsFarPartial, (size_t)ns * n_trg * 3 * sizeof(double));
-  void *rec = workspace(kWsFarRecords, n_src * (f32c ? sizeof(FarRec32) : sizeof(FarRec64)));
-  if (!part || !rec) return VES3D_MEMORY_ERROR;
+  void *rec = workspace(kWsFarRecords, n_src * rsz);
+  const long cpg = coarse ? (long)coarse->cpg : 0, n_c = cpg * n_groups;
+  void *crec = coarse ? workspace(kWsFarCoarse, n_c * rsz) : nullptr;
+  if (!part || !rec || (coarse && !crec)) return VES3D_MEMORY_ERROR;
+  const unsigned char *cflag = coarse ? coarse->flags : nullptr;
   const long tblocks = ((long)n_trg + FF_BLOCK - 1) / FF_BLOCK;
   const dim3 grid((unsigned)(tblocks * ns));
   if (f32c) {
-    k_pack_far32<<<cdiv((long)n_src, 256), 256, 0, st>>>(spos, sld, (long)n_src, (long)spg,
-                                                          (FarRec32 *)rec);
+    k_pack_far32<<<cdiv((long)n_src, 256), 256, 0, st>>>(spos, sld, (long)n_src, (long)spg, spos,
+                                                          (long)spg, (FarRec32 *)rec);
     V3D_CHECK(launch_status());
-    if (g_far_kind.load() == 2) {
+    if (coarse) {
+      k_pack_far32<<<cdiv(n_c, 256), 256, 0, st>>>(coarse->pos, coarse->den1, n_c, cpg, spos,
+                                                    (long)spg, (FarRec32 *)crec);
+      V3D_CHECK(launch_status());
+    }
+    if (!coarse && g_far_kind.load() == 2) {
       const long tb2 = ((long)n_trg + 2 * FF_BLOCK - 1) / (2 * FF_BLOCK);
       k_far_groups_f32c2<<<dim3((unsigned)(tb2 * ns)), FF_BLOCK, 0, st>>>(
           (const FarRec32 *)rec, spos, (long)spg, n_groups, tpos, (long)n_trg, (long)tpg, ns, csl,
           part);
     } else {
       k_far_groups_f32c<<<grid, FF_BLOCK, 0, st>>>((const FarRec32 *)rec, spos, (long)spg,
-                                                   n_groups, tpos, (long)n_trg, (long)tpg, ns, csl,
-                                                   part);
+                                                   (const FarRec32 *)crec, cpg, cflag, n_groups,
+                                                   tpos, (long)n_trg, (long)tpg, ns, csl, part);
     }
   } else {
     k_pack_far64<<<cdiv((long)n_src, 256), 256, 0, st>>>(spos, sld, (long)n_src, (FarRec64 *)rec);
     V3D_CHECK(launch_status());
-    k_far_groups_f64<<<grid, FF_BLOCK, 0, st>>>((const FarRec64 *)rec, (long)spg, n_groups, tpos,
+    if (coarse) {
+      k_pack_far64<<<cdiv(n_c, 256), 256, 0, st>>>(coarse->pos, coarse->den1, n_c,
+                                                    (FarRec64 *)crec);
+      V3D_CHECK(launch_status());
+    }
+    k_far_groups_f64<<<grid, FF_BLOCK, 0, st>>>((const FarRec64 *)rec, (long)spg,
+                                                (const FarRec64 *)crec, cpg, cflag, n_groups, tpos,
                                                 (long)n_trg, (long)tpg, ns, csl, part);
   }
   V3D_CHECK(launch_status());
@@ -915,36 +1064,57 @@ int far_field_between_groups(bool f32c, const double *spos, const double *sld, s
 int far_field_between_groups2(bool f32c, const double *spos, const double *sld1,
                               const double *sld2, size_t n_src, const double *tpos, double *tvel1,
                               double *tvel2, size_t n_trg, size_t spg, size_t tpg,
-                              hipStream_t st) {
+                              hipStream_t st, const FarCoarse *coarse) {
   if (spg == 0 || tpg == 0) return VES3D_INVALID_PARAMETER;
-  if (g_far_kind.load() != 1) {  // the other kernels evaluate one density at a time
+  if (!coarse && g_far_kind.load() != 1) {  // the other kernels evaluate one density at a time
     V3D_CHECK(far_field_between_groups(f32c, spos, sld1, n_src, tpos, tvel1, n_trg, spg, tpg, st));
     return far_field_between_groups(f32c, spos, sld2, n_src, tpos, tvel2, n_trg, spg, tpg, st);
   }
   if (n_trg == 0) return VES3D_SUCCESS;
   if (!spos || !sld1 || !sld2 || !tpos || !tvel1 || !tvel2) return VES3D_INVALID_PARAMETER;
   if (n_src % spg || n_trg % tpg || n_src / spg != n_trg / tpg) return VES3D_INVALID_PARAMETER;
+  if (coarse && (!coarse->pos || !coarse->den1 || !coarse->den2 || !coarse->flags ||
+                 coarse->cpg == 0))
+    return VES3D_INVALID_PARAMETER;
   const double csl = 1.0 / (8.0 * kPi);
   const long n_groups = (long)(n_src / spg);
   const int ns = choose_nsplit((long)n_trg, (long)n_src, n_groups);
+  const size_t rsz = f32c ? sizeof(FarRec32x2) : sizeof(FarRec64x2);
   double *part = (double *)workspace(kWsFarPartial, (size_t)2 * ns * n_trg * 3 * sizeof(double));
-  void *rec = workspace(kWsFarRecords, n_src * (f32c ? sizeof(FarRec32x2) : sizeof(FarRec64x2)));
-  if (!part || !rec) return VES3D_MEMORY_ERROR;
+  void *rec = workspace(kWsFarRecords, n_src * rsz);
+  const long cpg = coarse ? (long)coarse->cpg : 0, n_c = cpg * n_groups;
+  void *crec = coarse ? workspace(kWsFarCoarse, n_c * rsz) : nullptr;
+  if (!part || !rec || (coarse && !crec)) return VES3D_MEMORY_ERROR;
+  const unsigned char *cflag = coarse ? coarse->flags : nullptr;
   double *part2 = part + (size_t)ns * n_trg * 3;
   const long tblocks = ((long)n_trg + FF_BLOCK - 1) / FF_BLOCK;
   const dim3 grid((unsigned)(tblocks * ns));
   if (f32c) {
     k_pack_far32x2<<<cdiv((long)n_src, 256), 256, 0, st>>>(spos, sld1, sld2, (long)n_src,
-                                                            (long)spg, (FarRec32x2 *)rec);
+                                                            (long)spg, spos, (long)spg,
+                                                            (FarRec32x2 *)rec);
     V3D_CHECK(launch_status());
+    if (coarse) {
+      k_pack_far32x2<<<cdiv(n_c, 256), 256, 0, st>>>(coarse->pos, coarse->den1, coarse->den2,
+                                                      n_c, cpg, spos, (long)spg,
+                                                      (FarRec32x2 *)crec);
+      V3D_CHECK(launch_status());
+    }
     k_far_groups_f32c_x2<<<grid, FF_BLOCK, 0, st>>>((const FarRec32x2 *)rec, spos, (long)spg,
+                                                    (const FarRec32x2 *)crec, cpg, cflag,
                                                     n_groups, tpos, (long)n_trg, (long)tpg, ns,
                                                     csl, part, part2);
   } else {
     k_pack_far64x2<<<cdiv((long)n_src, 256), 256, 0, st>>>(spos, sld1, sld2, (long)n_src,
                                                             (FarRec64x2 *)rec);
     V3D_CHECK(launch_status());
-    k_far_groups_f64_x2<<<grid, FF_BLOCK, 0, st>>>((const FarRec64x2 *)rec, (long)spg, n_groups,
+    if (coarse) {
+      k_pack_far64x2<<<cdiv(n_c, 256), 256, 0, st>>>(coarse->pos, coarse->den1, coarse->den2,
+                                                      n_c, (FarRec64x2 *)crec);
+      V3D_CHECK(launch_status());
+    }
+    k_far_groups_f64_x2<<<grid, FF_BLOCK, 0, st>>>((const FarRec64x2 *)rec, (long)spg,
+                                                   (const FarRec64x2 *)crec, cpg, cflag, n_groups,
                                                    tpos, (long)n_trg, (long)tpg, ns, csl, part,
                                                    part2);
   }
