@@ -62,9 +62,10 @@ PEAK_FP32_NONPACKED_TFLOPS = 78.6   # the same without packed FP32 (disabled in 
 SL_FLOP_PER_PAIR = 28      # src/CPUKernels.cc:1008
 # HBM bytes per launch of k_far_groups_f32c at 512 vesicles, p = 16, from two
 # rocprofv3 --pmc runs (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a
-# 512 MiB device copy gives the two counters in the same runs)
-FAR32_TRAFFIC_BYTES = 2.0 * 367348.5 * 1024 + 52224.0 * 1024
-FAR32_TRAFFIC_SOURCE = "profiles/k_far_groups_f32c_hbm_counters_512ves_p16.md"
+# 512 MiB device copy gives the two counters in the same runs), keyed by the
+# far_coarse setting they were measured at
+FAR32_TRAFFIC = {0.0: (2.0 * 367348.5 * 1024 + 52224.0 * 1024,
+                       "profiles/k_far_groups_f32c_hbm_counters_512ves_p16.md")}
 FAR_KINDS = {"lds": 0, "scalar": 1, "scalar2": 2}
 FAR32_NAMES = {"lds": "k_far_f32c<SL>", "scalar": "k_far_groups_f32c", "scalar2": "k_far_groups_f32c2"}
 DL_FLOP_PER_PAIR = 31      # src/PVFMMInterface.cc:365 (no Newton steps)
@@ -92,6 +93,43 @@ def _train_pc(args, p, common):
     return pc, fracs, info["collect_s"], info["train_s"], info["samples"], info["loss"]
 
 
+def _run_form(I, nv, p, X0g, kw, pc, nsteps, name, check=None, keep_first=False):
+    """Runs one form of the step for nsteps steps from X0g.  check: a
+    precision-0 stepper with the exact far field that judges every step's
+    solution by the FP64 operator (untimed: the residual is computed between
+    steps, outside the step's own clock).  Returns (stats list, FP64 relres
+    list, first-step (U, sigma) if keep_first)."""
+    import torch
+    st = I.Stepper(nv, p, **kw)
+    if pc is not None:
+        I.attach(st, pc, collect=False)
+    st.set_state(X0g)
+    stats, r64, first = [], [], None
+    for k in range(nsteps):
+        Xb = st.get_state()[0].clone() if check is not None else None
+        s = st.step()
+        _, U, S = st.get_state()
+        if check is not None:
+            check.set_state(Xb, U, S)
+            s["fp64_relres"] = check.residual()
+            r64.append(s["fp64_relres"])
+        if keep_first and k == 0:
+            first = (U.clone(), S.clone())
+        _log("%s step %d: %.1f ms, %d iters (%d inner), relres %.2e true %.2e%s, far %.1f ms "
+             "(%d + %d two-density), near %.1f ms (%d pairs), fell_back %d" %
+             (name, k, s["ms_total"], s["iterations"], s["inner_iters"], s["relres"],
+              s["true_relres"], (" fp64 %.2e" % s["fp64_relres"]) if check is not None else "",
+              s["ms_far"] + s["ms_far2"], s["far_calls"], s["far2_calls"], s["ms_near"],
+              s["near_pairs"], s["fell_back"]))
+        stats.append(s)
+    st.close()
+    return stats, r64, first
+
+
+def _mean(stats, key):
+    return float(np.mean([s[key] for s in stats]))
+
+
 def bench_step(args):
     import torch
     import ves3d_amd
@@ -108,55 +146,49 @@ def bench_step(args):
     nin = 4 * p * (p + 2)
     pc, fracs, collect_s, train_s, n_samples, loss = _train_pc(args, p, common)
     _log("trained on %d samples: loss %.3e in %.1f s" % (n_samples, loss, train_s))
-
-    # ---- plain FP64 form
+    W_, K = args.warmup, args.steps
     timed = dict(common, extrapolate=args.extrapolate)
-    plain = I.Stepper(nv, p, precision=0, precond=1, **timed)
-    plain.set_state(Xg)
-    stats_plain = []
-    U_plain1 = None
-    for k in range(args.warmup + args.steps):
-        s = plain.step()
-        _log("plain step %d: %.1f ms, %d iters, relres %.2e true %.2e" %
-             (k, s["ms_total"], s["iterations"], s["relres"], s["true_relres"]))
-        if k == 0:
-            U_plain1 = plain.get_state()[1].clone()
-        if k >= args.warmup:
-            stats_plain.append(s)
+    # the FP64 operator with the exact far field judges every learned step
+    check = I.Stepper(nv, p, precision=0, precond=1, **common)
+    learned_kw = dict(timed, precision=args.mixed_precision, precond=2,
+                      inner_max_iter=args.inner_max_iter, inner_tol=args.inner_tol,
+                      far_coarse=args.far_coarse)
 
-    # ---- learned + mixed form from the same initial state
-    mixed = I.Stepper(nv, p, precision=args.mixed_precision, precond=2,
-                      inner_max_iter=args.inner_max_iter, inner_tol=args.inner_tol, **timed)
-    I.attach(mixed, pc, collect=False)
-    mixed.set_state(Xg)
-    stats_mixed = []
-    U_mixed1 = None
-    for k in range(args.warmup + args.steps):
-        s = mixed.step()
-        _log("learned+mixed step %d: %.1f ms, %d iters (%d inner), relres %.2e true %.2e "
-             "fell_back %d" % (k, s["ms_total"], s["iterations"], s["inner_iters"], s["relres"],
-                               s["true_relres"], s["fell_back"]))
-        if k == 0:
-            _, U_mixed1, S_mixed1 = [t.clone() for t in mixed.get_state()]
-        if k >= args.warmup:
-            stats_mixed.append(s)
-
-    # the same inner solves with the diagonal preconditioner inside them
-    # instead of the network: what the network gains within this form
-    # (reported, not part of the headline)
-    stats_dinner = []
-    if args.inner_max_iter > 0:
-        dinner = I.Stepper(nv, p, precision=args.mixed_precision, precond=1,
-                           inner_max_iter=args.inner_max_iter, inner_tol=args.inner_tol,
-                           **timed)
-        dinner.set_state(Xg)
-        for k in range(args.warmup + args.steps):
-            s = dinner.step()
-            _log("diagonal-inner+mixed step %d: %.1f ms, %d iters (%d inner), true %.2e" %
-                 (k, s["ms_total"], s["iterations"], s["inner_iters"], s["true_relres"]))
-            if k >= args.warmup:
-                stats_dinner.append(s)
-        dinner.close()
+    # ---- the two headline forms from the same initial state
+    plain, _, first_plain = _run_form(I, nv, p, Xg, dict(timed, precision=0, precond=1), None,
+                                      W_ + K, "plain", keep_first=True)
+    learned, r64, first_learned = _run_form(I, nv, p, Xg, learned_kw, pc, W_ + K,
+                                            "learned+mixed", check=check, keep_first=True)
+    plain_t, learned_t = plain[W_:], learned[W_:]
+    # ---- attribution (reported, not part of the headline)
+    att = {}
+    if not args.no_attribution:
+        for name, kw, use_pc in (
+                ("learned_mixed_exact_far", dict(learned_kw, far_coarse=0.0), True),
+                ("diagonal_inner_mixed", dict(learned_kw, precond=1), False),
+                ("plain_two_level_far", dict(timed, precision=0, precond=1,
+                                             far_coarse=args.far_coarse), False)):
+            stt, _, _ = _run_form(I, nv, p, Xg, kw, pc if use_pc else None, W_ + K, name)
+            stt = stt[W_:]
+            att[name] = {"ms_per_step": _mean(stt, "ms_total"),
+                         "iterations": [s["iterations"] for s in stt],
+                         "inner_iters": [s["inner_iters"] for s in stt],
+                         "far_ms_per_step": _mean(stt, "ms_far") + _mean(stt, "ms_far2"),
+                         "true_relres": [s["true_relres"] for s in stt]}
+    # ---- the headline form with the reference's near zone r_ves 2/sqrt(p_up)
+    near_default = None
+    if not args.no_attribution:
+        stn, r64n, _ = _run_form(I, nv, p, Xg, dict(learned_kw, r_near=0.0), pc, W_ + K,
+                                 "learned+mixed r_near default",
+                                 check=I.Stepper(nv, p, precision=0, precond=1,
+                                                 **dict(common, r_near=0.0)))
+        stn = stn[W_:]
+        near_default = {"r_near": stn[-1]["r_near"], "near_pairs": [s["near_pairs"] for s in stn],
+                        "ms_per_step": _mean(stn, "ms_total"),
+                        "near_ms_per_step": _mean(stn, "ms_near"),
+                        "near_share": _mean(stn, "ms_near") / _mean(stn, "ms_total"),
+                        "iterations": [s["iterations"] for s in stn],
+                        "fp64_relres": r64n[W_:]}
 
     # velocity error of both forms' first step against the FP64 solve of the
     # same system converged to relative residual args.ref_tol (untimed)
@@ -165,43 +197,39 @@ def bench_step(args):
     sr = ref.step()
     _log("FP64 reference solve: %d iters, true relres %.2e" % (sr["iterations"], sr["true_relres"]))
     _, Ur, _ = ref.get_state()
-    nr = torch.linalg.norm(Ur)
-    vel_err = float(torch.linalg.norm(U_mixed1 - Ur) / nr)
-    vel_err_plain = float(torch.linalg.norm(U_plain1 - Ur) / nr)
-    # the learned + mixed form's first solution judged by the FP64 operator
-    ref.set_state(Xg, U_mixed1, S_mixed1)
-    relres64_mixed = ref.residual()
     ref.close()
+    nr = torch.linalg.norm(Ur)
+    vel_err = float(torch.linalg.norm(first_learned[0] - Ur) / nr)
+    vel_err_plain = float(torch.linalg.norm(first_plain[0] - Ur) / nr)
+    check.close()
 
-    ms_mixed = float(np.mean([s["ms_total"] for s in stats_mixed]))
-    ms_plain = float(np.mean([s["ms_total"] for s in stats_plain]))
-    it_mixed = [s["iterations"] for s in stats_mixed]
-    it_plain = [s["iterations"] for s in stats_plain]
+    ms_mixed = _mean(learned_t, "ms_total")
+    ms_plain = _mean(plain_t, "ms_total")
 
-    # roofline: far-field SL direct sum (dominant kernel), learned+mixed form
-    ng = 2 * p * (p + 1)
-    pu = 2 * p
-    mu = 2 * pu * (pu + 1) + 2
-    pairs = (nv * ng) * (nv - 1) * mu
+    # roofline: the far-field SL direct sum (dominant kernel) of the
+    # learned + mixed form; pairs per launch from the step's own count
+    pairs = float(learned_t[-1]["far_pairs"])
     flop = pairs * SL_FLOP_PER_PAIR
-    far_ms = sum(s["ms_far"] for s in stats_mixed) / max(1, sum(s["far_calls"] for s in stats_mixed))
-    far_ms_plain = sum(s["ms_far"] for s in stats_plain) / max(1, sum(s["far_calls"] for s in stats_plain))
+    far_ms = sum(s["ms_far"] for s in learned_t) / max(1, sum(s["far_calls"] for s in learned_t))
+    pairs_plain = float(plain_t[-1]["far_pairs"])
+    far_ms_plain = sum(s["ms_far"] for s in plain_t) / max(1, sum(s["far_calls"] for s in plain_t))
     achieved = flop / (far_ms * 1e-3) / 1e12
     scalar = args.far_kernel == "scalar" and args.nves == 512 and args.p == 16
+    traffic = FAR32_TRAFFIC.get(args.far_coarse) if scalar else None
     roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": FAR32_TRAFFIC_BYTES if scalar else None,
-            "traffic_source": FAR32_TRAFFIC_SOURCE if scalar else None,
+            "traffic": traffic[0] if traffic else None,
+            "traffic_source": traffic[1] if traffic else None,
             "peak_nonpacked": PEAK_FP32_NONPACKED_TFLOPS,
             "frac_nonpacked": achieved / PEAK_FP32_NONPACKED_TFLOPS,
             "kernel": FAR32_NAMES[args.far_kernel] + " (+k_pack_far32, k_sum_slices)",
             "kernel_ms": far_ms,
             "flop_per_launch": flop, "pairs_per_launch": pairs,
-            "two_density_pass_ms": (sum(s["ms_far2"] for s in stats_mixed)
-                                    / max(1, sum(s["far2_calls"] for s in stats_mixed))),
+            "two_density_pass_ms": (sum(s["ms_far2"] for s in learned_t)
+                                    / max(1, sum(s["far2_calls"] for s in learned_t))),
             "fp64_form": {"kernel": "k_far_f64<SL>" if args.far_kernel == "lds" else "k_far_groups_f64",
-                          "kernel_ms": far_ms_plain,
-                          "achieved": flop / (far_ms_plain * 1e-3) / 1e12,
+                          "kernel_ms": far_ms_plain, "pairs_per_launch": pairs_plain,
+                          "achieved": pairs_plain * SL_FLOP_PER_PAIR / (far_ms_plain * 1e-3) / 1e12,
                           "peak": PEAK_FP64_TFLOPS}}
 
     # the sustained FP32 VALU rate of this GPU, measured in the same run
@@ -212,48 +240,53 @@ def bench_step(args):
     roof["probe_tflops_fp32"] = 2 * r32.value / 1e12
     roof["frac_of_probe"] = achieved / roof["probe_tflops_fp32"]
 
-    cpu = None if args.no_cpu else cpu_baseline_step(nv, p, X0, np.mean(
-        [s["matvecs"] for s in stats_plain]), stats_plain[-1]["near_pairs"])
-    plain.close()
-    mixed.close()
+    cpu = None if args.no_cpu else cpu_baseline_step(nv, p, X0, _mean(plain_t, "matvecs"),
+                                                     plain_t[-1]["near_pairs"])
     pc.close()
     return {
         "metric": METRIC if (nv, p) == (512, 16) else METRIC.replace(
             "@512 ves p=16", "@%d ves p=%d" % (nv, p)),
-        "value": ms_mixed, "unit": "ms", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_mixed, "higher_is_better": False,
+        "value": ms_mixed, "unit": "ms", "n_gpus": 1, "steps": K,
+        "warmup": W_, "ms_per_step": ms_mixed, "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": "implicit step, %d perturbed ellipsoids (1,1,0.6+-0.05) p=%d, "
                                "20%% volume fraction, shear 1.0, dt 1e-3, BiCGStab 1e-6" % (nv, p),
                    "nves": nv, "p": p, "p_up": 2 * p, "unknowns": nv * nin,
-                   "r_near": stats_plain[-1]["r_near"]},
+                   "r_near": plain_t[-1]["r_near"]},
         "ms_per_step_plain": ms_plain, "speedup_vs_plain": ms_plain / ms_mixed,
-        "iterations_learned": it_mixed, "iterations_plain": it_plain,
+        "learned_form": "precision %d (FP32-compensated far field%s), learned preconditioner in "
+                        "inner solves (%d iterations, %g), two-level far field (p-grid sources "
+                        "beyond %g radii)" % (args.mixed_precision,
+                                              ", FP32 singular operator" if args.mixed_precision == 1
+                                              else ", FP64 singular operator",
+                                              args.inner_max_iter, args.inner_tol, args.far_coarse),
+        "plain_form": "FP64, diagonal spectral preconditioner, upsampled sources for every pair",
+        "iterations_learned": [s["iterations"] for s in learned_t],
+        "iterations_plain": [s["iterations"] for s in plain_t],
         "initial_guess": ("2 x_{n-1} - x_{n-2} (both forms)" if args.extrapolate
                           else "x_{n-1}, the previous step's solution (both forms)"),
         "inner_solve": {"max_iter": args.inner_max_iter, "tol": args.inner_tol,
-                        "inner_iters_learned": [s["inner_iters"] for s in stats_mixed],
-                        "local_matvecs_learned": [s["local_matvecs"] for s in stats_mixed],
-                        "diagonal_inner_ms_per_step": (
-                            float(np.mean([s["ms_total"] for s in stats_dinner]))
-                            if stats_dinner else None),
-                        "diagonal_inner_iterations": [s["iterations"] for s in stats_dinner],
-                        "diagonal_inner_inner_iters": [s["inner_iters"] for s in stats_dinner]},
-        "true_relres_learned": [s["true_relres"] for s in stats_mixed],
-        "true_relres_plain": [s["true_relres"] for s in stats_plain],
-        "fell_back": [s["fell_back"] for s in stats_mixed],
+                        "inner_iters_learned": [s["inner_iters"] for s in learned_t],
+                        "local_matvecs_learned": [s["local_matvecs"] for s in learned_t]},
+        "attribution": att,
+        "near_default_r_near": near_default,
+        "true_relres_learned": [s["true_relres"] for s in learned_t],
+        "fp64_relres_learned_every_step": r64,
+        "true_relres_plain": [s["true_relres"] for s in plain_t],
+        "fell_back": [s["fell_back"] for s in learned_t],
         "velocity_rel_err_vs_fp64": vel_err, "velocity_rel_err_plain_vs_fp64": vel_err_plain,
         "fp64_reference_relres": sr["true_relres"],
-        "fp64_relres_of_learned_first_step": relres64_mixed, "pc_train_seconds": train_s,
-        "pc_collect_seconds": collect_s,
+        "pc_train_seconds": train_s, "pc_collect_seconds": collect_s,
         "pc_train_suspensions": "%d x %d ves at %s%% volume fraction" % (
             len(fracs), args.train_nves, "/".join("%.0f" % (100 * f) for f in fracs)),
-        "near_pairs": [s["near_pairs"] for s in stats_plain], "r_near": stats_plain[-1]["r_near"],
+        "near_pairs": [s["near_pairs"] for s in plain_t], "r_near": plain_t[-1]["r_near"],
         "pc_train_samples": int(n_samples), "pc_train_loss": loss,
-        "stage_ms_plain": {k: float(np.mean([s[k] for s in stats_plain]))
-                           for k in ("ms_setup", "ms_rhs", "ms_solve")},
-        "stage_ms_learned": {k: float(np.mean([s[k] for s in stats_mixed]))
-                             for k in ("ms_setup", "ms_rhs", "ms_solve")},
+        "stage_ms_plain": {k: _mean(plain_t, k) for k in ("ms_setup", "ms_rhs", "ms_solve",
+                                                          "ms_post", "ms_near")},
+        "stage_ms_learned": {k: _mean(learned_t, k) for k in ("ms_setup", "ms_rhs", "ms_solve",
+                                                              "ms_post", "ms_near")},
+        "far_ms_per_step": {"plain": _mean(plain_t, "ms_far") + _mean(plain_t, "ms_far2"),
+                            "learned": _mean(learned_t, "ms_far") + _mean(learned_t, "ms_far2")},
         "roofline": roof, "cpu_baseline": cpu,
     }
 
@@ -280,7 +313,7 @@ def bench_trajectory(args):
         "plain": I.Stepper(nv, p, precision=0, precond=1, **common),
         "learned": I.Stepper(nv, p, precision=args.mixed_precision, precond=2,
                              inner_max_iter=args.inner_max_iter, inner_tol=args.inner_tol,
-                             **common),
+                             far_coarse=args.far_coarse, **common),
     }
     I.attach(forms["learned"], pc, collect=False)
     out, final = {}, {}
@@ -366,14 +399,34 @@ def cpu_baseline_step(nv, p, X, n_matvec, near_pairs):
     up = sph.shc2grid(shc, p, pu).reshape(nv, 3, mu)
     src = np.ascontiguousarray(np.transpose(up, (0, 2, 1)).reshape(-1, 3))
     den = np.ones_like(src) * 1e-3
-    ntr = 256
-    trg = np.ascontiguousarray(np.transpose(X, (0, 2, 1)).reshape(-1, 3)[:: (nv * 2 * p * (p + 1)) // ntr][:ntr])
-    out = np.zeros((ntr, 3))
-    orc = OL.load()
-    t0 = time.perf_counter()
-    orc.orc_far_field(OL.p(src), OL.p(den), None, OL.cs(src.shape[0]), OL.p(trg), OL.p(out), OL.cs(ntr),
-                      OL.cs(0), OL.cs(0))
-    t_far = (time.perf_counter() - t0) * (nv * 2 * p * (p + 1)) / ntr
+    ng = 2 * p * (p + 1)
+    pairs_matvec = float(nv * ng) * (nv - 1) * (mu + 2)  # far pairs per matvec, exact far field
+    ref = OL.load_ref()
+    if ref is not None:
+        # the far-field leg on the reference's own StokesAlltoAll
+        # (src/CPUKernels.cc, compiled from source into oracle/_ref): all-to-all
+        # among the first n_all upsampled sources, scaled by pairs
+        n_all = min(src.shape[0], 60000)
+        s_all = np.ascontiguousarray(src[:n_all])
+        d_all = np.ascontiguousarray(den[:n_all])
+        out = np.zeros((n_all, 3))
+        t0 = time.perf_counter()
+        ref.ref_stokes_all2all_f64(OL.p(s_all), OL.p(d_all), OL.cs(n_all), OL.p(out))
+        t_far = (time.perf_counter() - t0) * pairs_matvec / (float(n_all) * n_all)
+        far_leg = ("the reference's own StokesAlltoAll (src/CPUKernels.cc built from source "
+                   "into oracle/_ref) all-to-all on %d upsampled sources, scaled by pairs to "
+                   "%.3g pairs per matvec" % (n_all, pairs_matvec))
+    else:
+        ntr = 256
+        trg = np.ascontiguousarray(np.transpose(X, (0, 2, 1)).reshape(-1, 3)[:: (nv * ng) // ntr][:ntr])
+        out = np.zeros((ntr, 3))
+        orc = OL.load()
+        t0 = time.perf_counter()
+        orc.orc_far_field(OL.p(src), OL.p(den), None, OL.cs(src.shape[0]), OL.p(trg), OL.p(out),
+                          OL.cs(ntr), OL.cs(0), OL.cs(0))
+        t_far = (time.perf_counter() - t0) * (nv * ng) / ntr
+        far_leg = ("the oracle's C far field of all %d sources on 256 targets (x%d)" %
+                   (src.shape[0], (nv * ng) // ntr))
     imp = ves.Implicit(X[:2], p, dt=1e-3, shear=1.0, near=False)
     x = np.random.default_rng(0).standard_normal(4 * 2 * p * (p + 2))
     t0 = time.perf_counter()
@@ -383,14 +436,15 @@ def cpu_baseline_step(nv, p, X, n_matvec, near_pairs):
     total = t_sing * nv + n_matvec * (t_far + t_local) + \
         near_pairs * (near_setup + n_matvec * near_apply)
     del st
+    far_share = n_matvec * t_far / total
     return {"value": total * 1e3, "unit": "ms per implicit step (extrapolated)",
-            "cores": _cpu_threads(), "kind": "port",
-            "sample": "FP64 oracle (numpy; C far field with OpenMP): singular operator of 2 "
-                      "vesicles (x%d), far field of all %d sources on 256 targets (x%d), one "
-                      "2-vesicle matvec (x%d), times %.0f matvecs per step (the GPU plain run's "
-                      "count), plus near-singular setup + %.0f applications per pair on two close "
-                      "vesicles (x%d pairs)" % (nv, src.shape[0], (nv * 2 * p * (p + 1)) // ntr,
-                                                nv // 2, n_matvec, n_matvec, near_pairs),
+            "cores": _cpu_threads(), "kind": "reference" if ref is not None else "port",
+            "sample": "far field (%.0f%% of the total): %s; the rest on the FP64 oracle port "
+                      "(numpy): singular operator of 2 vesicles (x%d), one 2-vesicle matvec "
+                      "(x%d), near-singular setup + %.0f applications per pair on two close "
+                      "vesicles (x%d pairs); all times %.0f matvecs per step (the GPU plain "
+                      "run's count)" % (100 * far_share, far_leg, nv, nv // 2, n_matvec,
+                                        near_pairs, n_matvec),
             "components_s": {"singular_per_vesicle": t_sing, "far_per_matvec": t_far,
                              "local_per_matvec": t_local, "near_setup_per_pair": near_setup,
                              "near_apply_per_pair": near_apply}}
@@ -411,9 +465,14 @@ def bench_matvec(args):
     ves3d_amd.call("ves3d_hip_set_far_kernel", FAR_KINDS[args.far_kernel])
     nv, p = args.nves, args.p
     prec = args.mixed_precision if args.dtype == "f32c" else 0
-    st = I.Stepper(nv, p, dt=1e-3, shear_rate=1.0, precision=prec, precond=1, r_near=args.r_near)
+    # the mixed form's kernel with its two-level far field, the FP64 form's
+    # over the upsampled sources (as the step workload's two headline forms)
+    coarse = args.far_coarse if prec else 0.0
+    st = I.Stepper(nv, p, dt=1e-3, shear_rate=1.0, precision=prec, precond=1, r_near=args.r_near,
+                   far_coarse=coarse)
     st.set_state(torch.from_numpy(W.implicit_step_case(nv=nv, p=p)).cuda())
     st.prepare()
+    pairs = st.far_pairs()
     x = torch.from_numpy(np.random.default_rng(3).standard_normal(st.n)).cuda()
     cal_bytes = 1 << 29                          # 512 MiB read + 512 MiB written
     a = torch.ones(cal_bytes // 8, dtype=torch.float64, device="cuda")
@@ -429,9 +488,6 @@ def bench_matvec(args):
         st.matvec(x)
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) / max(1, args.steps) * 1e3
-    ng = 2 * p * (p + 1)
-    mu = 2 * (2 * p) * (2 * p + 1) + 2
-    pairs = (nv * ng) * (nv - 1) * mu
     st.close()
     return {
         "metric": "ms per Galerkin matvec @512 ves p=16", "value": ms, "unit": "ms", "n_gpus": 1,
@@ -440,7 +496,7 @@ def bench_matvec(args):
         "data": "synthetic",
         "config": {"workload": "implicit-step matvec, 512 perturbed ellipsoids p=16, 20% volume "
                                "fraction", "precision": prec, "far_kernel": args.far_kernel,
-                   "far_pairs": pairs},
+                   "far_coarse": coarse, "far_pairs": pairs},
         "roofline": None, "cpu_baseline": None,
     }
 
@@ -673,6 +729,12 @@ def main():
     ap.add_argument("--far-kernel", default="scalar", choices=sorted(FAR_KINDS),
                     help="far field between vesicles: sources through the scalar cache (scalar2: "
                          "two targets per lane in FP32) or LDS tiles")
+    ap.add_argument("--far-coarse", type=float, default=3.0,
+                    help="learned form: two-level far field, p-grid sources for waves of "
+                         "targets beyond this many vesicle radii (0: upsampled sources "
+                         "everywhere, as the plain form)")
+    ap.add_argument("--no-attribution", action="store_true",
+                    help="step workload: skip the attribution forms and the near-default line")
     ap.add_argument("--r-near", type=float, default=0.1,
                     help="near-singular zone radius (0: the reference's r_ves 2/sqrt(p_up))")
     args = ap.parse_args()

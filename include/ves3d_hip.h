@@ -408,6 +408,7 @@ typedef struct ves3d_step_stats {
   int local_matvecs;      /* local-operator applications of the inner solves */
   double far_pairs;       /* source-target pairs of one far-field evaluation of one
                              density in this step (fewer with far_coarse) */
+  double ms_near;         /* summed device time of the near-singular corrections */
 } ves3d_step_stats;
 
 /* Creates / destroys a stepper (device state for nves vesicles). */
@@ -450,6 +451,10 @@ int ves3d_hip_stepper_precond_inner(void *handle, const double *x, double *y, in
  * for a given solution).  With a precision-0 stepper it judges a solution
  * of any form against the FP64 operator. */
 int ves3d_hip_stepper_residual(void *handle, double *relres_host);
+/* Source-target pairs of one far-field evaluation (one density) as of the
+ * last prepare: every target against every other vesicle's upsampled
+ * sources, or fewer with far_coarse.  No counterpart in the reference. */
+int ves3d_hip_stepper_far_pairs(void *handle, double *pairs_host);
 /* No counterpart in the reference (it feeds the learned preconditioner):
  * ImplicitApply with every vesicle's own operator only, the singular
  * self-interaction without the far field and the near-singular correction:

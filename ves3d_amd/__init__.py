@@ -103,6 +103,7 @@ _sig("ves3d_hip_stepper_rhs", P, P)
 _sig("ves3d_hip_stepper_matvec", P, P, P)
 _sig("ves3d_hip_stepper_precond", P, P, P)
 _sig("ves3d_hip_stepper_matvec_local", P, P, P)
+_sig("ves3d_hip_stepper_far_pairs", P, P)
 _sig("ves3d_hip_stepper_residual", P, P)
 _sig("ves3d_hip_stepper_rhs_matvec", P, P, P, P)
 _sig("ves3d_hip_stepper_precond_learned", P, P, P)

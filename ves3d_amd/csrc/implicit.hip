@@ -435,9 +435,10 @@ void StokesG::timing_begin() {
   nev = 0;
 }
 
-int StokesG::far_time(double *ms, int *calls, double *ms2, int *calls2) {
-  double tot[3] = {0, 0, 0};
-  int cnt[3] = {0, 0, 0};
+int StokesG::far_time(double *ms, int *calls, double *ms2, int *calls2, double *ms_near,
+                      int *calls_near) {
+  double tot[4] = {0, 0, 0, 0};
+  int cnt[4] = {0, 0, 0, 0};
   for (int i = 0; i < nev; ++i) {
     V3D_HIP(hipEventSynchronize(ev1[i]));
     float t = 0;
@@ -449,6 +450,8 @@ int StokesG::far_time(double *ms, int *calls, double *ms2, int *calls2) {
   if (calls) *calls = cnt[1];
   if (ms2) *ms2 = tot[2];
   if (calls2) *calls2 = cnt[2];
+  if (ms_near) *ms_near = tot[3];
+  if (calls_near) *calls_near = cnt[3];
   timing = false;
   nev = 0;
   return VES3D_SUCCESS;
@@ -607,7 +610,9 @@ int StokesG::apply(const double *f, double *vel, hipStream_t st, bool add_repul,
   k_add_points<<<gblocks(nv * ng), 256, 0, st>>>(farv, nv, ng, vel);
   V3D_CHECK(launch_status());
   if (!near_on) return VES3D_SUCCESS;
-  return near_correct(q, V, vel, st);
+  V3D_CHECK(event_begin(3, st));
+  V3D_CHECK(near_correct(q, V, vel, st));
+  return event_end(st);
 }
 
 int StokesG::apply2(const double *f1, double *vel1, bool add_repul1, const double *f2,
@@ -644,8 +649,10 @@ int StokesG::apply2(const double *f1, double *vel1, bool add_repul1, const doubl
   k_add_points<<<gblocks(nv * ng), 256, 0, st>>>(farv2, nv, ng, vel2);
   V3D_CHECK(launch_status());
   if (!near_on) return VES3D_SUCCESS;
+  V3D_CHECK(event_begin(3, st));
   V3D_CHECK(near_correct(q, V, vel1, st));
-  return near_correct(q2, V2, vel2, st);
+  V3D_CHECK(near_correct(q2, V2, vel2, st));
+  return event_end(st);
 }
 
 }  // namespace v3d

@@ -132,7 +132,9 @@ class StokesG {
   // elapsed times are summed by far_time() after the stream has drained:
   // one-density launches into (ms, calls), two-density ones into (ms2, calls2)
   void timing_begin();
-  int far_time(double *ms, int *calls, double *ms2 = nullptr, int *calls2 = nullptr);
+  // near-singular corrections (kind 3) into (ms_near, calls_near)
+  int far_time(double *ms, int *calls, double *ms2 = nullptr, int *calls2 = nullptr,
+               double *ms_near = nullptr, int *calls_near = nullptr);
 
   int p = 0, pu = 0, precision = 0;
   long nv = 0;

@@ -950,7 +950,8 @@ struct Stepper {
     auto t4 = clk::now();
     double far_ms = 0, far2_ms = 0;
     int far_calls = 0, far2_calls = 0;
-    V3D_CHECK(stokes.far_time(&far_ms, &far_calls, &far2_ms, &far2_calls));
+    double near_ms = 0;
+    V3D_CHECK(stokes.far_time(&far_ms, &far_calls, &far2_ms, &far2_calls, &near_ms));
     prepared = false;
     if (stats) {
       auto ms = [](clk::time_point a, clk::time_point b) {
@@ -978,6 +979,7 @@ struct Stepper {
       stats->inner_iters = inner_iters;
       stats->local_matvecs = local_mv;
       stats->far_pairs = stokes.far_pairs;
+      stats->ms_near = near_ms;
     }
     return converged ? VES3D_SUCCESS : VES3D_DIVERGENCE_ERROR;
   }
@@ -1078,6 +1080,14 @@ int ves3d_hip_stepper_residual(void *h, double *relres_host) {
   V3D_CHECK(s->pack(s->U, s->sig, s->xsol));
   V3D_CHECK(s->true_residual(s->xsol, s->rhs, relres_host));
   return hip_status(hipStreamSynchronize(s->st));
+}
+
+int ves3d_hip_stepper_far_pairs(void *h, double *pairs_host) {
+  if (!h || !pairs_host) return VES3D_INVALID_PARAMETER;
+  Stepper *s = (Stepper *)h;
+  if (!s->prepared) return VES3D_INVALID_PARAMETER;
+  *pairs_host = s->stokes.far_pairs;
+  return VES3D_SUCCESS;
 }
 
 int ves3d_hip_stepper_matvec_local(void *h, const double *x, double *y) {

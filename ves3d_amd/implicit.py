@@ -43,7 +43,7 @@ class StepStats(ctypes.Structure):
                 ("av_iters", ctypes.c_int), ("ms_post", ctypes.c_double),
                 ("ms_far2", ctypes.c_double), ("far2_calls", ctypes.c_int),
                 ("inner_iters", ctypes.c_int), ("local_matvecs", ctypes.c_int),
-                ("far_pairs", ctypes.c_double)]
+                ("far_pairs", ctypes.c_double), ("ms_near", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -106,6 +106,12 @@ class Stepper:
 
     def prepare(self):
         call("ves3d_hip_stepper_prepare", self.h)
+
+    def far_pairs(self):
+        """Source-target pairs of one far-field evaluation (after prepare)."""
+        v = ctypes.c_double()
+        call("ves3d_hip_stepper_far_pairs", self.h, ctypes.byref(v))
+        return v.value
 
     def rhs(self):
         b = torch.empty(self.n, dtype=torch.float64, device="cuda")
