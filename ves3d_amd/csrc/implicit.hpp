@@ -41,6 +41,14 @@ int far_field_between_groups2(bool f32c, const double *spos, const double *sld1,
                               double *tvel2, size_t n_trg, size_t spg, size_t tpg, hipStream_t st,
                               const FarCoarse *coarse = nullptr);
 
+// ves3d_hip_direct_eval_f64 with caller-provided scratch for the per-slice
+// partial sums (part, part_bytes; at least n_trg * 24 bytes, else
+// VES3D_INVALID_PARAMETER): allocates nothing.
+int direct_eval_f64_fixed(const double *src_pos, const double *sl_den, const double *dl_den,
+                          size_t n_src, const double *trg_pos, double *trg_vel, size_t n_trg,
+                          size_t spg, size_t tpg, double *part, size_t part_bytes,
+                          hipStream_t st);
+
 // flags[w][g] = 1 where every target of wave w (targets 64 w .. 64 w + 63) is
 // at least kappa R_g from c_g, (c_g, R_g) the bounding sphere of group g's
 // fine sources (centre: their mean).  flags: ceil(n_trg / 64) * n_groups

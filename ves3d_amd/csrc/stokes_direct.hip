@@ -328,11 +328,15 @@ static void launch_far(dim3 grid, hipStream_t st, const double *spos,
 }
 
 // Shared driver of the all-pairs and far-field entry points.
+// part_fixed (optional): caller-provided scratch of part_cap bytes for the
+// per-slice partial sums instead of the library's workspace; the slice count
+// is then lowered until the partials fit.
 template <bool F32>
 static int far_eval(const double *spos, const double *sld, const double *dld,
                     size_t n_src, const double *tpos, double *tvel, size_t n_trg,
                     size_t spg, size_t tpg, double eps2, double csl, double cdl,
-                    hipStream_t st, bool guard = true) {
+                    hipStream_t st, bool guard = true, double *part_fixed = nullptr,
+                    size_t part_cap = 0) {
   if (n_trg == 0) return VES3D_SUCCESS;
   if (!tpos || !tvel) return VES3D_INVALID_PARAMETER;
   if (n_src == 0 || (!sld && !dld)) {
@@ -347,8 +351,16 @@ static int far_eval(const double *spos, const double *sld, const double *dld,
       return VES3D_INVALID_PARAMETER;
     n_groups = (long)(n_src / spg);
   }
-  const int ns = choose_nsplit((long)n_trg, (long)n_src, n_groups);
-  double *part = (double *)workspace(kWsFarPartial, (size_t)ns * n_trg * 3 * sizeof(double));
+  int ns = choose_nsplit((long)n_trg, (long)n_src, n_groups);
+  double *part = nullptr;
+  if (part_fixed) {
+    const size_t per_slice = n_trg * 3 * sizeof(double);
+    if (per_slice > part_cap) return VES3D_INVALID_PARAMETER;
+    while ((size_t)ns * per_slice > part_cap) ns = ns > 8 ? ns / 2 : ns - 1;
+    part = part_fixed;
+  } else {
+    part = (double *)workspace(kWsFarPartial, (size_t)ns * n_trg * 3 * sizeof(double));
+  }
   if (!part) return VES3D_MEMORY_ERROR;
   const long tblocks = ((long)n_trg + FF_BLOCK - 1) / FF_BLOCK;
   const dim3 grid((unsigned)(tblocks * ns));
@@ -900,6 +912,9 @@ static int vesicle_stokes(const Real *src, const Real *nrm, const Real *den,
   return launch_status();
 }
 
+// Coincident-point guard of the PVFMMEval stand-in (r^2 <= 128 eps is skipped).
+static const double kEvalEps2 = 128.0 * 2.220446049250313e-16;
+
 // Bounding sphere of every group's fine sources: bnd[g] = (centre, radius),
 // the centre the mean of the sources.  One workgroup per group.
 __global__ __launch_bounds__(256) void k_group_bounds(const double *__restrict__ spos, long spg,
@@ -977,6 +992,16 @@ __global__ __launch_bounds__(256) void k_coarse_flags(const double *__restrict__
 
 __global__ void k_zero_u64(unsigned long long *p, int n) {
   if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+int direct_eval_f64_fixed(const double *src_pos, const double *sl_den, const double *dl_den,
+                          size_t n_src, const double *trg_pos, double *trg_vel, size_t n_trg,
+                          size_t spg, size_t tpg, double *part, size_t part_bytes,
+                          hipStream_t st) {
+  if (!part) return VES3D_INVALID_PARAMETER;
+  return far_eval<false>(src_pos, sl_den, dl_den, n_src, trg_pos, trg_vel, n_trg, spg, tpg,
+                         kEvalEps2, 1.0 / (8.0 * kPi), 3.0 / (4.0 * kPi), st, true, part,
+                         part_bytes);
 }
 
 int far_coarse_flags(const double *spos, size_t n_src, size_t spg, const double *tpos,
@@ -1186,7 +1211,6 @@ int ves3d_hip_set_far_kernel(int kind) {
   return VES3D_SUCCESS;
 }
 
-static const double kEvalEps2 = 128.0 * 2.220446049250313e-16;
 
 int ves3d_hip_direct_eval_f64(const double *src_pos, const double *sl_den,
                               const double *dl_den, size_t n_src,
