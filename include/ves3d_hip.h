@@ -223,6 +223,21 @@ int ves3d_hip_direct_eval_f32c(const double *src_pos, const double *sl_den,
                                const double *trg_pos, double *trg_vel,
                                size_t n_trg, size_t src_per_group,
                                size_t trg_per_group, ves3d_stream_t stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ves3d_hip_direct_eval_f64 with the same arguments and
+ * results, whose one scratch buffer (the per-slice FP64 partial sums) is a
+ * fixed 64 MiB device array per GPU, held until the process ends, so the
+ * call performs no hipMalloc / hipFree.  n_trg may be at most 2,796,202
+ * (n_trg * 24 bytes <= 64 MiB), else VES3D_INVALID_PARAMETER; where the
+ * partials of the default slice count do not fit, fewer slices are used
+ * (the same sums in another order).  All calls on one GPU share the array:
+ * calls that overlap on that GPU (different streams, threads) race. */
+int ves3d_hip_direct_eval_f64_noalloc(const double *src_pos, const double *sl_den,
+                                      const double *dl_den, size_t n_src,
+                                      const double *trg_pos, double *trg_vel,
+                                      size_t n_trg, size_t src_per_group,
+                                      size_t trg_per_group, ves3d_stream_t stream);
 
 /* ---- spherical-harmonic transforms (SphericalHarmonics<Real>,
  *      include/SphericalHarmonics.h:14-24, src/SphericalHarmonics.cc:3-360) --

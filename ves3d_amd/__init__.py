@@ -78,6 +78,7 @@ _sig("ves3d_hip_stokes_all2all_f64", P, P, S, P, P)
 _sig("ves3d_hip_stokes_all2all_f32c", P, P, S, P, P)
 _sig("ves3d_hip_direct_eval_f64", P, P, P, S, P, P, S, S, S, P)
 _sig("ves3d_hip_direct_eval_f32c", P, P, P, S, P, P, S, S, S, P)
+_sig("ves3d_hip_direct_eval_f64_noalloc", P, P, P, S, P, P, S, S, S, P)
 for _t in ("f64", "f32"):
     _sig("ves3d_hip_shc2grid_" + _t, P, S, c_int, c_int, P, P, P, P)
     _sig("ves3d_hip_grid2shc_" + _t, P, S, c_int, c_int, P, P)
