@@ -506,6 +506,20 @@ int ves3d_hip_stepper_reparam(void *handle, int *iters_host);
  * steps on the upsampled surface along (-2H n, n) until area and volume
  * match the recorded ones to 1e-12 (at most 10).  Synchronous. */
 int ves3d_hip_stepper_area_volume(void *handle, int *iters_host);
+/* Per-vesicle properties (VesicleProperties, src/VesicleProps.cc:19-55, as
+ * read by ves3d_read_vesicle_props): bending moduli, viscosity contrasts and
+ * excess densities, host arrays [nves], and the gravity field (host [3],
+ * Parameters::gravity_field, default (0, 0, -1)); NULL leaves an entry as it
+ * is.  The step's forces then carry bending_coeff = -bending_modulus per
+ * vesicle and, when some vesicle has an excess density, the gravity traction
+ * excess_density (g . x) n in both the right-hand side (of the positions)
+ * and the matvec (of dt u) (src/InterfacialForce.cc:126-184).  A viscosity
+ * contrast other than 1 returns VES3D_NOT_IMPLEMENTED.  Until called, every
+ * vesicle has the step parameters' bending modulus and no excess density. */
+int ves3d_hip_stepper_set_vesicle_props(void *handle, const double *bending_modulus_host,
+                                        const double *viscosity_contrast_host,
+                                        const double *excess_density_host,
+                                        const double *gravity_field_host);
 /* Overrides the recorded area / volume targets (host arrays [nves]). */
 int ves3d_hip_stepper_set_area_volume(void *handle, const double *area_host,
                                       const double *vol_host);
@@ -558,6 +572,14 @@ int ves3d_checkpoint_read(const char *path, ves3d_checkpoint_params *prm, size_t
                           double *X, size_t x_capacity, double *bending_modulus,
                           double *viscosity_contrast, double *excess_density,
                           size_t props_capacity);
+
+/* The vesicle-properties file of Simulation::setup_from_options
+ * (src/ves3d_simulation.cc:174-196): the first 3 nves numbers of the ASCII
+ * file, one row (bending modulus, viscosity contrast, excess density) per
+ * vesicle, into props_host [3][nves] ordered by property (the reference's
+ * Transpose, VesicleProperties::getPropIdx order, src/VesicleProps.cc:61-72).
+ * Fewer than 3 nves numbers: VES3D_IO_BAD_STREAM. */
+int ves3d_read_vesicle_props(const char *path, size_t nves, double *props_host);
 
 /* InitializeShapes (src/HelperFuns.cc:366-432): X_host [nves][3][(p+1)2p] from
  * a shape gallery (shapes [3][(p+1)2p] back to back, gallery_len values) and

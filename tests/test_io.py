@@ -78,3 +78,44 @@ def test_initialize_shapes():
                                                   gallery.size, bad.ctypes.data_as(ctypes.c_void_p),
                                                   X.ctypes.data_as(ctypes.c_void_p))
     assert rc == 10
+
+
+def test_read_vesicle_props():
+    """The reference's vesicle-properties file (precomputed/
+    vesprops_rand_polysed_512.txt): one row (bending modulus, viscosity
+    contrast, excess density) per vesicle, returned by property."""
+    from ves3d_amd import workloads as W
+    path = os.path.join(GOLD, "vesprops_rand_polysed_512.txt")
+    want = np.loadtxt(path)
+    for nv in (512, 17):
+        kb, vc, rho = W.read_vesicle_props(path, nv)
+        assert np.array_equal(kb, want[:nv, 0])
+        assert np.array_equal(vc, want[:nv, 1])
+        assert np.array_equal(rho, want[:nv, 2])
+    out = np.zeros(3 * 513)
+    rc = ves3d_amd.load().ves3d_read_vesicle_props(path.encode(), 513,
+                                                   out.ctypes.data_as(ctypes.c_void_p))
+    assert rc == 51  # IOBadStream: fewer than 3 nves numbers
+
+
+def test_polydisperse_suspension_inputs():
+    """The reference's polydisperse sedimentation inputs
+    (experiment/polydisperse_config.in): 512 copies of the p = 16 gallery's
+    first shape, scaled 0.65 or 1.05, whose positions InitializeShapes
+    builds; the vesicles are disjoint (every pair's closest points are
+    farther apart than 0.01)."""
+    p = 16
+    gallery = _read(os.path.join(GOLD, "shape_gallery_16_shape0.txt"))
+    spec = _read(os.path.join(GOLD, "geometry_spec_polysed_512.txt"))
+    assert gallery.size == 3 * 2 * p * (p + 1) and spec.size == 8 * 512
+    from ves3d_amd import workloads as W
+    X = W.suspension_from_files(os.path.join(GOLD, "shape_gallery_16_shape0.txt"),
+                                os.path.join(GOLD, "geometry_spec_polysed_512.txt"), p)
+    assert X.shape == (512, 3, 2 * p * (p + 1))
+    c = X.mean(axis=2)
+    r = np.sqrt(np.max(np.sum((X - c[:, :, None]) ** 2, axis=1), axis=1))
+    d = np.linalg.norm(c[:, None] - c[None], axis=2)
+    close = np.argwhere((d < r[:, None] + r[None, :]) & (np.arange(512)[:, None] < np.arange(512)))
+    for a, b in close:
+        dmin = np.min(np.linalg.norm(X[a][:, :, None] - X[b][:, None, :], axis=0))
+        assert dmin > 0.01, (a, b, dmin)

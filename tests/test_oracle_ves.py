@@ -322,3 +322,20 @@ def test_lone_vesicle_step_is_inextensible_and_dissipative():
     h = 1e-4
     dEdt = (_bending_energy(X + h * u, p, 2 * p, kb) - _bending_energy(X - h * u, p, 2 * p, kb)) / (2 * h)
     assert dEdt < 0, dEdt
+
+
+def test_gravity_force_total_is_weight():
+    """InterfacialForce::gravityForce: the traction rho (g . x) n integrates
+    over a closed surface to rho V g (divergence theorem), on an ellipsoid;
+    zero without an excess density."""
+    p = 16
+    X = _ellipsoid(p) + np.array([0.3, -0.2, 1.5])[None, :, None]
+    S = ves.Surface(X, p)
+    g = np.array([0.2, -0.1, -1.0])
+    F = ves.Forces(S, 2 * p, 0.01, excess_density=0.7, gravity=g)
+    Fg = F.gravity(X)
+    qw = np.repeat(sph.legendre_weights(p), 2 * p)
+    tot = np.sum(Fg[0] * (S.W[0] * qw)[None, :], axis=1)
+    assert np.allclose(tot, 0.7 * S.volume()[0] * g, atol=1e-10), (tot, 0.7 * S.volume()[0] * g)
+    F0 = ves.Forces(S, 2 * p, 0.01)
+    assert np.max(np.abs(F0.gravity(X))) == 0.0

@@ -112,11 +112,13 @@ _sig("ves3d_hip_stepper_precond_inner", P, P, P, c_int, P)
 _sig("ves3d_hip_stepper_frames", P, P)
 _sig("ves3d_hip_stepper_step", P, P)
 _sig("ves3d_hip_stepper_attach_pc", P, P, c_int)
+_sig("ves3d_hip_stepper_set_vesicle_props", P, P, P, P, P)
 _sig("ves3d_hip_stepper_reparam", P, P)
 _sig("ves3d_hip_stepper_area_volume", P, P)
 _sig("ves3d_hip_stepper_set_area_volume", P, P, P)
 _sig("ves3d_read_ascii", ctypes.c_char_p, P, S, P)
 _sig("ves3d_initialize_shapes", c_int, S, P, S, P, P)
+_sig("ves3d_read_vesicle_props", ctypes.c_char_p, S, P)
 _sig("ves3d_checkpoint_defaults", c_int, S, P)
 _sig("ves3d_checkpoint_write", ctypes.c_char_p, P, S, P, P, P, P)
 _sig("ves3d_checkpoint_read", ctypes.c_char_p, P, P, P, S, P, P, P, S)

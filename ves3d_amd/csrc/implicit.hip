@@ -150,17 +150,30 @@ __global__ void k_bend_s2(const double *__restrict__ a, const double *__restrict
   GRID_LOOP(i, n) s2[i] = a[i] * (H[i] * H[i] - K[i]);
 }
 
-// F = c (s1 + 2 s2) n with c the bending coefficient, -kappa_b
+// F = c_v (s1 + 2 s2) n with c_v vesicle v's bending coefficient, -kappa_b
 // (src/InterfacialForce.cc:37-41, src/VesicleProps.cc:41-43)
 __global__ void k_bend_force(const double *__restrict__ s1, const double *__restrict__ s2,
-                             const double *__restrict__ nrm, double coeff, long nv, long ng,
-                             double *Fo) {
+                             const double *__restrict__ nrm, const double *__restrict__ coeff,
+                             long nv, long ng, double *Fo) {
   GRID_LOOP(i, nv * ng) {
     const long v = i / ng, g = i % ng;
     const long b = v * 3 * ng + g;
-    const double s = coeff * (s1[i] + 2.0 * s2[i]);
+    const double s = coeff[v] * (s1[i] + 2.0 * s2[i]);
 #pragma unroll
     for (int d = 0; d < 3; ++d) Fo[b + d * ng] = s * nrm[b + d * ng];
+  }
+}
+
+// F += rho_v (g . x) n   (src/InterfacialForce.cc:140-159)
+__global__ void k_gravity_add(const double *__restrict__ x, const double *__restrict__ nrm,
+                              const double *__restrict__ rho, double gx, double gy, double gz,
+                              long nv, long ng, double *F) {
+  GRID_LOOP(i, nv * ng) {
+    const long v = i / ng, g = i % ng;
+    const long b = v * 3 * ng + g;
+    const double s = rho[v] * (gx * x[b] + gy * x[b + ng] + gz * x[b + 2 * ng]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) F[b + d * ng] += s * nrm[b + d * ng];
   }
 }
 
@@ -360,7 +373,13 @@ int ForcesG::init(int p_, int pu_, int f, long nv_, double kb_) {
   p = p_;
   pu = pu_;
   nv = nv_;
-  kb = kb_;
+  V3D_CHECK(bcoef.alloc(nv));
+  V3D_CHECK(rho.alloc(nv));
+  {
+    std::vector<double> c(nv, -kb_), z(nv, 0.0);
+    V3D_HIP(hipMemcpy(bcoef.p, c.data(), sizeof(double) * nv, hipMemcpyHostToDevice));
+    V3D_HIP(hipMemcpy(rho.p, z.data(), sizeof(double) * nv, hipMemcpyHostToDevice));
+  }
   V3D_CHECK(S.init(p, f, nv));
   V3D_CHECK(Su.init(pu, f * pu / p, nv));  // filter kept relative (src/Surface.cc:589-590)
   const long mu = sh_ngrid(pu);
@@ -394,7 +413,7 @@ int ForcesG::bending(double *Fb, hipStream_t st) {  // src/InterfacialForce.cc:2
   V3D_CHECK(launch_status());
   V3D_CHECK(Su.grad(Su.H, gu, st));
   V3D_CHECK(Su.div(gu, su1, st));
-  k_bend_force<<<gblocks(nv * mu), 256, 0, st>>>(su1, su2, Su.nrm, -kb, nv, mu, fu);
+  k_bend_force<<<gblocks(nv * mu), 256, 0, st>>>(su1, su2, Su.nrm, bcoef, nv, mu, fu);
   V3D_CHECK(launch_status());
   return resample(fu, 3 * nv, pu, p, Fb, st);
 }
@@ -407,9 +426,33 @@ int ForcesG::linear_bending(const double *x_new, double *Fb, hipStream_t st) {  
   V3D_CHECK(launch_status());
   V3D_CHECK(Su.grad(su1, gu, st));
   V3D_CHECK(Su.div(gu, su1, st));
-  k_bend_force<<<gblocks(nv * mu), 256, 0, st>>>(su1, su2, Su.nrm, -kb, nv, mu, fu);
+  k_bend_force<<<gblocks(nv * mu), 256, 0, st>>>(su1, su2, Su.nrm, bcoef, nv, mu, fu);
   V3D_CHECK(launch_status());
   return resample(fu, 3 * nv, pu, p, Fb, st);
+}
+
+int ForcesG::set_props(const double *kb_host, const double *rho_host, const double *g_host) {
+  if (kb_host) {
+    std::vector<double> c(nv);
+    for (long v = 0; v < nv; ++v) c[v] = -kb_host[v];  // bending_coeff = -bending_modulus
+    V3D_HIP(hipMemcpy(bcoef.p, c.data(), sizeof(double) * nv, hipMemcpyHostToDevice));
+  }
+  if (rho_host) {
+    V3D_HIP(hipMemcpy(rho.p, rho_host, sizeof(double) * nv, hipMemcpyHostToDevice));
+    has_rho = false;  // VesicleProperties::update (src/VesicleProps.cc:50-52)
+    for (long v = 0; v < nv; ++v) has_rho |= std::fabs(rho_host[v]) > 1e-12;
+  }
+  if (g_host)
+    for (int d = 0; d < 3; ++d) grav[d] = g_host[d];
+  return VES3D_SUCCESS;
+}
+
+int ForcesG::gravity_add(const double *x, double *F, hipStream_t st) {
+  if (!has_rho) return VES3D_SUCCESS;
+  const long ng = S.ng;
+  k_gravity_add<<<gblocks(nv * ng), 256, 0, st>>>(x, S.nrm, rho, grav[0], grav[1], grav[2], nv,
+                                                  ng, F);
+  return launch_status();
 }
 
 int ForcesG::tensile(const double *sigma, double *Fs, hipStream_t st) {  // :94-124

@@ -97,13 +97,24 @@ class ForcesG {
   int bending(double *Fb, hipStream_t st);                            // explicit
   int linear_bending(const double *x_new, double *Fb, hipStream_t st);
   int tensile(const double *sigma, double *Fs, hipStream_t st);
+  // F += rho_v (g . x) n on the p grid (InterfacialForce::gravityForce,
+  // src/InterfacialForce.cc:126-160); nothing unless some vesicle has an
+  // excess density (VesicleProperties::has_excess_density)
+  int gravity_add(const double *x, double *F, hipStream_t st);
+  // per-vesicle properties (VesicleProperties, src/VesicleProps.cc:19-55):
+  // bending moduli and excess densities [nv] on the host (NULL: unchanged)
+  // and the gravity field g[3] (NULL: unchanged)
+  int set_props(const double *kb_host, const double *rho_host, const double *g_host);
   // resample [nf][grid(p_from)] -> [nf][grid(p_to)]
   int resample(const double *in, long nf, int p_from, int p_to, double *out, hipStream_t st);
 
   SurfaceG S, Su;
   int p = 0, pu = 0;
   long nv = 0;
-  double kb = 0;
+  DBuf bcoef;             // bending coefficient per vesicle, -kappa_b (src/VesicleProps.cc:41-43)
+  DBuf rho;               // excess density per vesicle
+  double grav[3] = {0.0, 0.0, -1.0};  // Parameters::gravity_field default (src/Parameters.cc:31-33)
+  bool has_rho = false;
   DBuf xu, gu, fu, su1, su2, rcoef, rwork;
 };
 

@@ -2,7 +2,8 @@
 // (DataIO::ReadDataStl, src/DataIO_templates.cc:72-98), the shape gallery +
 // geometry specification initialisation (InitializeShapes,
 // src/HelperFuns.cc:366-432, called by Simulation::setup_from_options,
-// src/ves3d_simulation.cc:126-168).  Host code; no device work.
+// src/ves3d_simulation.cc:126-168) and the vesicle-properties file
+// (src/ves3d_simulation.cc:174-196).  Host code; no device work.
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -88,5 +89,22 @@ extern "C" int ves3d_initialize_shapes(int p, size_t nves, const double *gallery
       }
     }
   }
+  return VES3D_SUCCESS;
+}
+
+extern "C" int ves3d_read_vesicle_props(const char *path, size_t nves, double *props_host) {
+  if (!path || (nves > 0 && !props_host)) return VES3D_INVALID_PARAMETER;
+  const size_t nprops = 3;  // VesicleProperties::n_props (src/VesicleProps.cc:16)
+  size_t n = 0;
+  int e = ves3d_read_ascii(path, nullptr, 0, &n);
+  if (e != VES3D_SUCCESS) return e;
+  if (n < nprops * nves) return VES3D_IO_BAD_STREAM;
+  std::vector<double> all(n);
+  e = ves3d_read_ascii(path, all.data(), n, &n);
+  if (e != VES3D_SUCCESS) return e;
+  // the file holds one row of (bending modulus, viscosity contrast, excess
+  // density) per vesicle; the properties come out by column
+  for (size_t v = 0; v < nves; ++v)
+    for (size_t k = 0; k < nprops; ++k) props_host[k * nves + v] = all[v * nprops + k];
   return VES3D_SUCCESS;
 }

@@ -581,7 +581,9 @@ struct Stepper {
     if (!local) ++matvecs;
     V3D_CHECK(unpack(x, ug, sg));
     V3D_CHECK(dev_axpby(3 * nv * ng, prm.dt, ug, 0.0, f2, st));  // Du = dt u
+    // implicitTractionJump (src/InterfacialForce.cc:172-184)
     V3D_CHECK(forces.linear_bending(f2, f, st));
+    V3D_CHECK(forces.gravity_add(f2, f, st));
     V3D_CHECK(forces.tensile(sg, f2, st));
     V3D_CHECK(dev_axpby(3 * nv * ng, 1.0, f2, 1.0, f, st));
     V3D_CHECK(stokes.apply(f, Sf, st, false, local));
@@ -597,7 +599,8 @@ struct Stepper {
     if (!prepared) return VES3D_INVALID_PARAMETER;
     k_shear<<<gb(nv * ng), 256, 0, st>>>(X, nv, ng, prm.shear_rate, ug);
     V3D_CHECK(launch_status());
-    V3D_CHECK(forces.bending(f, st));
+    V3D_CHECK(forces.bending(f, st));  // explicitTractionJump (src/InterfacialForce.cc:162-170)
+    V3D_CHECK(forces.gravity_add(X, f, st));
     V3D_CHECK(stokes.apply(f, Sf, st, true));
     V3D_CHECK(dev_axpby(3 * nv * ng, 1.0, Sf, 1.0, ug, st));
     V3D_CHECK(forces.S.div(ug, ten, st));
@@ -615,12 +618,14 @@ struct Stepper {
     V3D_CHECK(unpack(x, ug, sg));
     V3D_CHECK(dev_axpby(3 * nv * ng, prm.dt, ug, 0.0, f2, st));
     V3D_CHECK(forces.linear_bending(f2, f, st));
+    V3D_CHECK(forces.gravity_add(f2, f, st));
     V3D_CHECK(forces.tensile(sg, f2, st));
     V3D_CHECK(dev_axpby(3 * nv * ng, 1.0, f2, 1.0, f, st));
     // density of the right-hand side (as in assemble_rhs)
     k_shear<<<gb(nv * ng), 256, 0, st>>>(X, nv, ng, prm.shear_rate, ub);
     V3D_CHECK(launch_status());
     V3D_CHECK(forces.bending(fb, st));
+    V3D_CHECK(forces.gravity_add(X, fb, st));
     V3D_CHECK(stokes.apply2(fb, Sfb, true, f, Sf, st));
     // right-hand side
     V3D_CHECK(dev_axpby(3 * nv * ng, 1.0, Sfb, 1.0, ub, st));
@@ -1169,6 +1174,20 @@ int ves3d_hip_stepper_set_area_volume(void *h, const double *area_host, const do
   V3D_HIP(hipMemcpy(s->a0.p, area_host, sizeof(double) * s->nv, hipMemcpyHostToDevice));
   V3D_HIP(hipMemcpy(s->v0.p, vol_host, sizeof(double) * s->nv, hipMemcpyHostToDevice));
   s->have_av0 = true;
+  return VES3D_SUCCESS;
+}
+
+int ves3d_hip_stepper_set_vesicle_props(void *h, const double *bending_modulus_host,
+                                        const double *viscosity_contrast_host,
+                                        const double *excess_density_host,
+                                        const double *gravity_field_host) {
+  if (!h) return VES3D_INVALID_PARAMETER;
+  Stepper *s = (Stepper *)h;
+  if (viscosity_contrast_host)
+    for (long v = 0; v < s->nv; ++v)
+      if (std::fabs(viscosity_contrast_host[v] - 1.0) > 1e-12) return VES3D_NOT_IMPLEMENTED;
+  V3D_CHECK(s->forces.set_props(bending_modulus_host, excess_density_host, gravity_field_host));
+  s->prepared = false;
   return VES3D_SUCCESS;
 }
 

@@ -180,6 +180,21 @@ class Stepper:
         call("ves3d_hip_stepper_frames", self.h, ptr(F))
         return F
 
+    def set_vesicle_props(self, bending_modulus=None, viscosity_contrast=None,
+                          excess_density=None, gravity=None):
+        """Per-vesicle properties ([nves] each) and the gravity field (3);
+        None leaves an entry unchanged."""
+        def arr(a, n):
+            if a is None:
+                return None, None
+            a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (n,)))
+            return a, a.ctypes.data_as(ctypes.c_void_p)
+        kb, pkb = arr(bending_modulus, self.nves)
+        vc, pvc = arr(viscosity_contrast, self.nves)
+        rho, prho = arr(excess_density, self.nves)
+        g, pg = arr(gravity, 3)
+        call("ves3d_hip_stepper_set_vesicle_props", self.h, pkb, pvc, prho, pg)
+
     def step(self, check=True):
         """One implicit step; returns its statistics.  A solve that fails or
         whose true residual misses the tolerance is a DivergenceError

@@ -232,3 +232,13 @@ def suspension_from_files(gallery_path, spec_path, p, nves=None):
     call("ves3d_initialize_shapes", p, nv, gallery.ctypes.data_as(ctypes.c_void_p), gallery.size,
          spec.ctypes.data_as(ctypes.c_void_p), X.ctypes.data_as(ctypes.c_void_p))
     return X
+
+
+def read_vesicle_props(path, nves):
+    """The vesicle-properties file (src/ves3d_simulation.cc:174-196):
+    (bending modulus, viscosity contrast, excess density), each [nves]."""
+    import ctypes
+    from . import call
+    out = np.zeros(3 * nves)
+    call("ves3d_read_vesicle_props", path.encode(), nves, out.ctypes.data_as(ctypes.c_void_p))
+    return out[:nves].copy(), out[nves:2 * nves].copy(), out[2 * nves:].copy()
