@@ -354,6 +354,29 @@ int ves3d_hip_bicgstab(size_t n, ves3d_apply_fn A, const void *A_ctx,
                        const double *b, int *max_iter, double *tol,
                        int *bicgs_status, ves3d_stream_t stream);
 
+/* ---- StokesVelocity<Real> (src/StokesVelocity.cc:146-539) --------------
+ * The Stokes velocity of a single-layer density f and a double-layer
+ * density d on nves vesicles at order p (SH grid points [nves][3][(p+1)2p],
+ * AxisMajor), upsampled to p_up (0: 2p) for the far field and the
+ * near-singular correction, in FP64: the singular self term (the SL and DL
+ * self operators of StokesSingularInteg), the direct sum over the other
+ * vesicles' upsampled sources that stands in for PVFMMEval, and the
+ * near-singular correction (NearSingular; r_near <= 0: the reference's
+ * r_ves 2/sqrt(p_up)) without repulsion.  create / destroy a handle
+ * (the singular operators of nves vesicles: 2 x 72 p^2 (p+2)^2 bytes per
+ * vesicle); set_src = SetSrcCoord (synchronous; builds the operators and
+ * the near-pair setup); eval = SetDensitySL(f_sl) + SetDensityDL(f_dl) +
+ * operator(): trg == NULL gives the velocity on the surfaces themselves
+ * (trg_is_surf), vel [nves][3][(p+1)2p]; otherwise the velocity at n_trg
+ * targets off the surfaces (SetTrgCoord), trg and vel point-major
+ * [n_trg][3], without the self term.  Either density may be NULL, not
+ * both.  eval is synchronous. */
+int ves3d_hip_stokes_create(int p, int p_up, size_t nves, double r_near, void **handle_host);
+int ves3d_hip_stokes_destroy(void *handle);
+int ves3d_hip_stokes_set_src(void *handle, const double *X, ves3d_stream_t stream);
+int ves3d_hip_stokes_eval(void *handle, const double *f_sl, const double *f_dl,
+                          const double *trg, size_t n_trg, double *vel, ves3d_stream_t stream);
+
 /* Parameters of the globally implicit interfacial-velocity step
  * (InterfacialVelocity::updateImplicit with solve_for_velocity,
  * src/InterfacialVelocity.cc:203-560): shear background flow
@@ -514,8 +537,13 @@ int ves3d_hip_stepper_area_volume(void *handle, int *iters_host);
  * vesicle and, when some vesicle has an excess density, the gravity traction
  * excess_density (g . x) n in both the right-hand side (of the positions)
  * and the matvec (of dt u) (src/InterfacialForce.cc:126-184).  A viscosity
- * contrast other than 1 returns VES3D_NOT_IMPLEMENTED.  Until called, every
- * vesicle has the step parameters' bending modulus and no excess density. */
+ * contrast lambda != 1 on some vesicle adds the double layer of
+ * (1 - lambda) u to the matvec's Stokes evaluation and scales its u by
+ * (1 + lambda) / 2 (src/InterfacialVelocity.cc:945-968); the double-layer
+ * self operators are then built at every step, and that step's far field
+ * runs over the upsampled sources whatever far_coarse says.  Until called,
+ * every vesicle has the step parameters' bending modulus, viscosity
+ * contrast 1 and no excess density. */
 int ves3d_hip_stepper_set_vesicle_props(void *handle, const double *bending_modulus_host,
                                         const double *viscosity_contrast_host,
                                         const double *excess_density_host,

@@ -1,7 +1,8 @@
-"""FP64 numpy restatement of NearSingular<Real> (src/NearSingular.cc) for
-surface targets with a single-layer density and no periodic box: near-target
-search, quadratic-patch projection, repulsion force, SubtractDirect and the
-near-singular velocity interpolation.
+"""FP64 numpy restatement of NearSingular<Real> (src/NearSingular.cc) without
+a periodic box: near-target search, quadratic-patch projection, repulsion
+force, SubtractDirect and the near-singular velocity interpolation, for
+surface targets (trg_is_surf, the step's own surface points) and for targets
+off the surfaces, with single- and double-layer densities.
 
 TEST INFRASTRUCTURE ONLY (the checker of ves3d_amd/csrc/near.hip).
 
@@ -49,11 +50,13 @@ def r_near_of(S, nv, p_up):
 
 def near_pairs(S, T, nv, p_up, r_near, trg_per_ves):
     """(target, vesicle, closest point) for every target within r_near of a
-    point of another vesicle (:797-918).  Ordered by vesicle, then target."""
+    point of another vesicle (:797-918).  Ordered by vesicle, then target.
+    trg_per_ves None: targets off the surfaces, which belong to no vesicle."""
     M = 2 + 2 * p_up * (p_up + 1)
     P = S.reshape(nv, M, 3)
     r2n = r_near * r_near
-    tves = np.arange(T.shape[0]) // trg_per_ves
+    tves = (np.arange(T.shape[0]) // trg_per_ves if trg_per_ves
+            else np.full(T.shape[0], -1))
     out = []
     for v in range(nv):
         d2 = np.sum((T[:, None, :] - P[v][None, :, :]) ** 2, axis=2)
@@ -227,12 +230,28 @@ def stokes_sl(src, q, trg):
     return v.sum(axis=1) / (8 * np.pi)
 
 
-class NearSingular:
-    """NearSingular<Real> with trg_is_surf = true (src/StokesVelocity.cc:240-
-    242): coordinate setup (:80-1120), ForceRepul (:1122-1126), SubtractDirect
-    (:1143-1212) and operator() (:1375-1551), single-layer density only."""
+def stokes_dl(src, qd, trg):
+    """DL kernel 3/(4 pi) sum (d.n)(d.g) d / r^5, d = trg - src, qd = [g (3),
+    n (3)] per source (src/PVFMMInterface.cc:284-369)."""
+    d = trg[:, None, :] - src[None, :, :]
+    r2 = np.sum(d * d, axis=2)
+    ri = np.where(r2 > 0, 1.0 / np.sqrt(np.where(r2 > 0, r2, 1.0)), 0.0)
+    dn = np.sum(qd[None, :, 3:] * d, axis=2)
+    dg = np.sum(qd[None, :, :3] * d, axis=2)
+    v = (dn * dg * ri ** 5)[:, :, None] * d
+    return v.sum(axis=1) * (3.0 / (4 * np.pi))
 
-    def __init__(self, S, T, nv, p_up, trg_per_ves, repul_dist=5e-2, r_near=None):
+
+class NearSingular:
+    """NearSingular<Real> (src/StokesVelocity.cc:240-242, :146-150):
+    coordinate setup (:80-1120), ForceRepul (:1122-1126), SubtractDirect
+    (:1143-1212) and operator() (:1375-1551), for surface targets
+    (trg_is_surf, trg_per_ves targets per vesicle) or targets off the
+    surfaces (trg_is_surf False, trg_per_ves None)."""
+
+    def __init__(self, S, T, nv, p_up, trg_per_ves, repul_dist=5e-2, r_near=None,
+                 trg_is_surf=True):
+        self.trg_is_surf = trg_is_surf
         self.nv, self.pu, self.ntrg = nv, p_up, T.shape[0]
         self.M = 2 + 2 * p_up * (p_up + 1)
         self.S = np.ascontiguousarray(S, dtype=np.float64).reshape(nv * self.M, 3)
@@ -269,7 +288,7 @@ class NearSingular:
         for i, (t, v, k) in enumerate(self.pairs):
             dR = self.T[t] - self.proj[i]
             dn = math.sqrt(float(dR @ dR))
-            if not self.extr[i]:
+            if not self.extr[i] and self.trg_is_surf:  # always the exterior normal
                 dn = -dn
             oo = 1.0 / dn if dn != 0 else 0.0
             if abs(dn) < EPS_SQRT:
@@ -291,30 +310,48 @@ class NearSingular:
         """Per-target repulsion force [ntrg][3] (point-major)."""
         return self.scatter(self.repl)
 
-    def subtract_direct(self, q):
-        """Direct SL of vesicle v at each of its near targets, scattered to
-        the targets (the part SubtractDirect removes from the far field)."""
+    def subtract_direct(self, q, qd=None):
+        """Direct SL (q, [M][3]) and DL (qd, [M][6], or None) sums of vesicle
+        v at each of its near targets, scattered to the targets (the part
+        SubtractDirect removes from the far field)."""
         out = np.zeros((len(self.pairs), 3))
         for v in np.unique(self.pairs[:, 1]) if len(self.pairs) else []:
             sel = np.nonzero(self.pairs[:, 1] == v)[0]
             sl = slice(v * self.M, (v + 1) * self.M)
-            out[sel] = stokes_sl(self.S[sl], q[sl], self.T[self.pairs[sel, 0]])
+            T = self.T[self.pairs[sel, 0]]
+            if q is not None:
+                out[sel] = stokes_sl(self.S[sl], q[sl], T)
+            if qd is not None:
+                out[sel] += stokes_dl(self.S[sl], qd[sl], T)
         return self.scatter(out)
 
-    def interp(self, q, s_vel_up):
+    def interp(self, q, s_vel_up, qd=None, u_dl=None):
         """operator(): near-singular velocity at each near target from v's
         surface velocity at the projection and the direct sum at 7 points
-        along the normal, scattered to the targets."""
+        along the normal, scattered to the targets.  With a double-layer
+        density (qd [M][6] packed as [g w, n], u_dl the density itself on
+        the upsampled grid with the poles first, point-major) the direct
+        sums carry its DL potential and the surface value its jump,
+        +1/2 u_dl (-1/2 at an interior target off the surfaces, :1458-1469)."""
         npair = len(self.pairs)
         out = np.zeros((npair, 3))
         for v in np.unique(self.pairs[:, 1]) if npair else []:
             sel = np.nonzero(self.pairs[:, 1] == v)[0]
             sl = slice(v * self.M, (v + 1) * self.M)
-            iv = stokes_sl(self.S[sl], q[sl], self.ipts[sel].reshape(-1, 3))
+            P = self.ipts[sel].reshape(-1, 3)
+            iv = np.zeros((P.shape[0], 3))
+            if q is not None:
+                iv += stokes_sl(self.S[sl], q[sl], P)
+            if qd is not None:
+                iv += stokes_dl(self.S[sl], qd[sl], P)
             iv = iv.reshape(len(sel), INTERP_DEG - 1, 3)
             for a, i in enumerate(sel):
                 k = self.pairs[i, 2]
-                patch = QuadraticPatch(patch_mesh(self.pu, k, s_vel_up[sl], self.Mint, self.Mfix))
+                mesh = patch_mesh(self.pu, k, s_vel_up[sl], self.Mint, self.Mfix)
+                if u_dl is not None:
+                    scal = -0.5 if (not self.extr[i] and not self.trg_is_surf) else 0.5
+                    mesh = mesh + scal * patch_mesh(self.pu, k, u_dl[sl], self.Mint, self.Mfix)
+                patch = QuadraticPatch(mesh)
                 pv = patch.eval(*self.xy[i])
                 if self.w[i, 0] == 1.0 and not self.w[i, 1:].any():
                     out[i] = pv

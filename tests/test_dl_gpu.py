@@ -1,0 +1,156 @@
+"""GPU parity of the double-layer path and of targets off the surfaces
+against the oracle (oracle/ves.py StokesSurface with a DL density and
+eval_off, Implicit with a viscosity contrast), and the constant-DL identity
+at the workloads' order.
+
+Bars (relative to the oracle output's max-norm unless stated):
+  StokesVelocity on the surfaces / off them, FP64      2e-8 (near pairs)
+  step right-hand side and matvec with a contrast, FP64  2e-8
+  step velocity with a contrast                        1e-5 vs the oracle's
+                                                        solve to 1e-10
+  mixed-precision matvec with a contrast               2e-5 (L2)
+  rigid translation with a contrast (matvec (U,0) = (U,0))  1e-5
+  constant DL density at p = 16: -c/2 on the own surface 1e-9, 0 on the
+  other vesicle's 1e-5, -c inside 1e-5, 0 outside 1e-4 (absolute, |c| ~ 1)
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import sph, ves
+from ves3d_amd import implicit as I
+from ves3d_amd import stokes as K
+from ves3d_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+C = np.array([0.3, -1.1, 0.7])
+
+
+def cu(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _case(nv, p, gap, seed=5):
+    from test_step_forms_gpu import close_pair
+    X = W.implicit_step_case(nv, p, 0.05, seed=seed)
+    return close_pair(X, p, 0, 1, gap)
+
+
+def _smooth(nv, p, seed):
+    rng = np.random.default_rng(seed)
+    c = np.zeros((3 * nv, p * (p + 2)))
+    off, L = sph.block_offsets(p)
+    for j in range(2 * p):
+        m = (j + 1) // 2
+        for k in range(L[j]):
+            c[:, off[j] + k] = rng.standard_normal(3 * nv) / (1 + m + k) ** 1.5
+    return sph.shc2grid(c, p, p).reshape(nv, 3, -1)
+
+
+def test_stokes_velocity_vs_oracle(hip):
+    nv, p = 3, 8
+    X = _case(nv, p, 0.1)
+    f, d = _smooth(nv, p, 1), _smooth(nv, p, 2)
+    orc = ves.StokesSurface(X, p, 2 * p, near=True, want_dl=True)
+    assert len(orc.near.pairs) > 0
+    sv = K.StokesVelocity(nv, p)
+    sv.set_src(cu(X))
+    for a, b in ((f, None), (None, d), (f, d)):
+        print("surface targets: SL %s DL %s" % (a is not None, b is not None), flush=True)
+        got = sv(cu(a) if a is not None else None, cu(b) if b is not None else None).cpu().numpy()
+        assert rel(got, orc(a, d=b)) < 2e-8
+    # targets off the surfaces: inside and outside, near and far
+    n = ves.Surface(X, p).normal
+    rng = np.random.default_rng(3)
+    T = []
+    for v in range(nv):
+        idx = rng.choice(X.shape[2], 20, replace=False)
+        for h in (-0.2, -0.05, 0.05, 0.2, 1.5):
+            T.append((X[v][:, idx] + h * n[v][:, idx]).T)
+    T = np.ascontiguousarray(np.concatenate(T))
+    for a, b in ((f, None), (None, d), (f, d)):
+        print("off-surface targets: SL %s DL %s" % (a is not None, b is not None), flush=True)
+        got = sv(cu(a) if a is not None else None, cu(b) if b is not None else None,
+                 trg=cu(T)).cpu().numpy()
+        want = orc.eval_off(T, a, b)
+        assert len(orc.near_off.pairs) > 0
+        assert rel(got, want) < 2e-8, rel(got, want)
+    sv.close()
+
+
+def test_constant_double_layer_p16(hip):
+    p = 16
+    X = _case(2, p, 0.3)
+    ng = X.shape[2]
+    sv = K.StokesVelocity(2, p)
+    sv.set_src(cu(X))
+    d = np.zeros((2, 3, ng))
+    d[0] = C[:, None]
+    v = sv(None, cu(d)).cpu().numpy()
+    print("constant DL at p=16: own %.2e other %.2e" %
+          (np.max(np.abs(v[0] + 0.5 * C[:, None])), np.max(np.abs(v[1]))), flush=True)
+    assert np.max(np.abs(v[0] + 0.5 * C[:, None])) < 1e-9
+    assert np.max(np.abs(v[1])) < 1e-5
+    n = ves.Surface(X, p).normal[0]
+    idx = np.random.default_rng(1).choice(ng, 64, replace=False)
+    for h in (0.02, 0.1, 0.3):
+        To = cu((X[0][:, idx] + h * n[:, idx]).T)
+        Ti = cu((X[0][:, idx] - h * n[:, idx]).T)
+        vo = sv(None, cu(d), trg=To).cpu().numpy()
+        vi = sv(None, cu(d), trg=Ti).cpu().numpy()
+        print("  h %.2f: outside %.2e inside %.2e" % (h, np.max(np.abs(vo)), np.max(np.abs(vi + C))),
+              flush=True)
+        assert np.max(np.abs(vo)) < 1e-4 and np.max(np.abs(vi + C)) < 1e-5
+    sv.close()
+
+
+def test_viscosity_contrast_step_vs_oracle(hip):
+    nv, p = 3, 8
+    X = _case(nv, p, 0.1, seed=7)
+    lam = np.array([5.0, 0.2, 1.0])
+    orc = ves.Implicit(X, p, dt=1e-3, shear=1.0, viscosity_contrast=lam)
+    tol = 2e-8
+    for precision in (0, 2):
+        st = I.Stepper(nv, p, dt=1e-3, shear_rate=1.0, precision=precision, precond=1, tol=1e-6,
+                       max_iter=300, post=False)
+        st.set_vesicle_props(viscosity_contrast=lam)
+        st.set_state(cu(X))
+        st.prepare()
+        x = np.random.default_rng(4).standard_normal(st.n)
+        y = st.matvec(cu(x)).cpu().numpy()
+        y0 = orc.matvec(x)
+        b2, y2 = [t.cpu().numpy() for t in st.rhs_matvec(cu(x))]
+        if precision == 0:
+            b = st.rhs().cpu().numpy()
+            assert rel(b, orc.rhs()) < tol
+            assert rel(y, y0) < tol, rel(y, y0)
+            assert rel(y2, y0) < tol and rel(b2, b) < 1e-14
+            # rigid translation: (U, 0) -> (U, 0)
+            U = np.random.default_rng(5).standard_normal((nv, 3))
+            ng = X.shape[2]
+            xu = orc.pack(np.repeat(U[:, :, None], ng, axis=2), np.zeros((nv, ng)))
+            yu = st.matvec(cu(xu)).cpu().numpy()
+            assert np.max(np.abs(yu - xu)) < 1e-5 * np.max(np.abs(xu))
+        else:
+            e = np.linalg.norm(y - y0) / np.linalg.norm(y0)
+            assert e < 2e-5, e
+        st.close()
+    b0 = orc.rhs()
+    x0, _, it0, s0 = ves.bicgstab(orc.matvec, b0, np.zeros_like(b0), 1e-10, 500, orc.precond)
+    assert s0 == "success"
+    u0, _ = orc.unpack(x0)
+    st = I.Stepper(nv, p, dt=1e-3, shear_rate=1.0, precision=0, precond=1, tol=1e-6, max_iter=300,
+                   post=False)
+    st.set_vesicle_props(viscosity_contrast=lam)
+    st.set_state(cu(X))
+    s = st.step()
+    U = st.get_state()[1].cpu().numpy()
+    e = np.linalg.norm(U - u0) / np.linalg.norm(u0)
+    print("viscosity contrast step: %d iterations (oracle %d), velocity err %.2e" %
+          (s["iterations"], it0, e), flush=True)
+    assert s["status"] == 100 and e < 1e-5
+    st.close()

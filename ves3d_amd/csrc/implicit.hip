@@ -192,10 +192,12 @@ __global__ void k_tens_force(const double *__restrict__ sig, const double *__res
 
 // Far-field source setup (src/StokesVelocity.cc:196-277, :307-337):
 // far[v][0..1] = poles, far[v][2+j] = upsampled points; aqw = |x_t x x_p| w(lat).
+// nrm (optional): the unit normals (x_t x x_p) / |x_t x x_p|, AxisMajor
+// [nv][3][mu] (scoord_norm, src/StokesVelocity.cc:244-273).
 __global__ void k_far_setup(const double *__restrict__ X, const double *__restrict__ Xt,
                             const double *__restrict__ Xp, const double *__restrict__ pole,
                             const double *__restrict__ qw, long nv, int pu, double *far,
-                            double *aqw) {
+                            double *aqw, double *nrm) {
   const long mu = 2L * pu * (pu + 1);
   GRID_LOOP(i, nv * (mu + 2)) {
     const long v = i / (mu + 2), j = i % (mu + 2);
@@ -211,7 +213,36 @@ __global__ void k_far_setup(const double *__restrict__ X, const double *__restri
       const double xt = Xt[b], yt = Xt[b + mu], zt = Xt[b + 2 * mu];
       const double xp = Xp[b], yp = Xp[b + mu], zp = Xp[b + 2 * mu];
       const double nx = yt * zp - zt * yp, ny = zt * xp - xt * zp, nz = xt * yp - yt * xp;
-      aqw[v * mu + s] = sqrt(nx * nx + ny * ny + nz * nz) * qw[s / (2 * pu)];
+      const double area = sqrt(nx * nx + ny * ny + nz * nz);
+      aqw[v * mu + s] = area * qw[s / (2 * pu)];
+      if (nrm) {
+        const double ia = 1.0 / area;
+        nrm[b] = nx * ia;
+        nrm[b + mu] = ny * ia;
+        nrm[b + 2 * mu] = nz * ia;
+      }
+    }
+  }
+}
+
+// qd[v][0..1] = 0, qd[v][2+s] = (d_up[v][:][s] aqw[v][s], n_up[v][:][s])
+// (qforce_double, src/StokesVelocity.cc:338-378)
+__global__ void k_pack_dl(const double *__restrict__ du, const double *__restrict__ aqw,
+                          const double *__restrict__ nrm, long nv, long mu, double *qd) {
+  GRID_LOOP(i, nv * (mu + 2)) {
+    const long v = i / (mu + 2), j = i % (mu + 2);
+    double *o = qd + i * 6;
+    if (j < 2) {
+#pragma unroll
+      for (int d = 0; d < 6; ++d) o[d] = 0.0;
+    } else {
+      const long s = j - 2;
+      const double w = aqw[v * mu + s];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        o[d] = du[(v * 3 + d) * mu + s] * w;
+        o[3 + d] = nrm[(v * 3 + d) * mu + s];
+      }
     }
   }
 }
@@ -467,6 +498,7 @@ int ForcesG::tensile(const double *sigma, double *Fs, hipStream_t st) {  // :94-
 // ------------------------------------------------------------- StokesG
 StokesG::~StokesG() {
   if (SL32) (void)hipFree(SL32);
+  if (DL32) (void)hipFree(DL32);
   if (cflags) (void)hipFree(cflags);
   if (ccounts) (void)hipFree(ccounts);
   for (auto e : ev0) (void)hipEventDestroy(e);
@@ -545,9 +577,10 @@ int StokesG::init(int p_, int pu_, long nv_, int precision_, int near_, double r
 int StokesG::setup(const double *X, hipStream_t st) {
   const long ng = sh_ngrid(p), mu = sh_ngrid(pu);
   if (precision != 1)
-    V3D_CHECK(singular_build<double>(X, nv, p, pu, SL.p, (double *)nullptr, st));
+    V3D_CHECK(singular_build<double>(X, nv, p, pu, SL.p, want_dl ? DL.p : nullptr, st));
   else
-    V3D_CHECK(singular_build<float>(X, nv, p, pu, SL32, (float *)nullptr, st));
+    V3D_CHECK(singular_build<float>(X, nv, p, pu, SL32, want_dl ? DL32 : nullptr, st));
+  near_off_ready = false;  // new sources
   V3D_CHECK(sht_grid2shc<double>(X, 3 * nv, p, p, shc, work, st));
   V3D_CHECK(sht_shc2grid<double>(shc, 3 * nv, p, pu, tmpx, tmpt, tmpp, work, st));
   V3D_CHECK(sht_shc2pole<double>(shc, 3 * nv, p, pole, st));
@@ -555,8 +588,8 @@ int StokesG::setup(const double *X, hipStream_t st) {
   legendre_weights(pu, qw);
   DBuf dqw;
   V3D_CHECK(upload(dqw, qw));
-  k_far_setup<<<gblocks(nv * (sh_ngrid(pu) + 2)), 256, 0, st>>>(tmpx, tmpt, tmpp, pole, dqw, nv,
-                                                                  pu, far_src, aqw);
+  k_far_setup<<<gblocks(nv * (sh_ngrid(pu) + 2)), 256, 0, st>>>(
+      tmpx, tmpt, tmpp, pole, dqw, nv, pu, far_src, aqw, want_dl ? nrm_up.p : nullptr);
   V3D_CHECK(launch_status());
   // targets: the re-synthesised p-grid positions (src/StokesVelocity.cc:227-239)
   const bool coarse = far_coarse > 0;
@@ -592,26 +625,89 @@ int StokesG::pack_coarse(const double *f, double *qcoarse, hipStream_t st) {
 }
 
 int StokesG::self_and_density(const double *f, double *Fc, double *Vc, double *vel, double *upc,
-                              double *qc, bool pack, hipStream_t st) {
+                              double *qc, bool pack, hipStream_t st, const double *d) {
   const long mu = sh_ngrid(pu);
-  V3D_CHECK(sht_grid2shc<double>(f, 3 * nv, p, p, Fc, work, st));
+  if (f) V3D_CHECK(sht_grid2shc<double>(f, 3 * nv, p, p, Fc, work, st));
+  if (d) V3D_CHECK(sht_grid2shc<double>(d, 3 * nv, p, p, G, work, st));
   if (precision != 1)
-    V3D_CHECK(self_apply<double>(SL.p, Fc, nullptr, nullptr, nv, p, Vc, st));
+    V3D_CHECK(self_apply<double>(f ? SL.p : nullptr, Fc, d ? DL.p : nullptr, G.p, nv, p, Vc, st));
   else
-    V3D_CHECK(self_apply<float>(SL32, Fc, nullptr, nullptr, nv, p, Vc, st));
+    V3D_CHECK(self_apply<float>(f ? SL32 : nullptr, Fc, d ? DL32 : nullptr, G.p, nv, p, Vc, st));
   V3D_CHECK(sht_shc2grid<double>(Vc, 3 * nv, p, p, vel, nullptr, nullptr, work, st));
   if (!pack) return VES3D_SUCCESS;
-  V3D_CHECK(sht_shc2grid<double>(Fc, 3 * nv, p, pu, upc, nullptr, nullptr, work, st));
-  k_pack_density<<<gblocks(nv * (mu + 2)), 256, 0, st>>>(upc, aqw, nv, mu, qc);
-  return launch_status();
+  if (f) {
+    V3D_CHECK(sht_shc2grid<double>(Fc, 3 * nv, p, pu, upc, nullptr, nullptr, work, st));
+    k_pack_density<<<gblocks(nv * (mu + 2)), 256, 0, st>>>(upc, aqw, nv, mu, qc);
+    V3D_CHECK(launch_status());
+  }
+  if (d) {  // uforce_double / qforce_double (src/StokesVelocity.cc:338-378)
+    V3D_CHECK(sht_shc2grid<double>(G, 3 * nv, p, pu, dup, nullptr, nullptr, work, st));
+    V3D_CHECK(sht_shc2pole<double>(G, 3 * nv, p, dpole, st));
+    k_pack_dl<<<gblocks(nv * (mu + 2)), 256, 0, st>>>(dup, aqw, nrm_up, nv, mu, qd);
+    V3D_CHECK(launch_status());
+  }
+  return VES3D_SUCCESS;
 }
 
-int StokesG::near_correct(const double *qc, const double *Vc, double *vel, hipStream_t st) {
+int StokesG::near_correct(const double *qc, const double *Vc, double *vel, hipStream_t st,
+                          bool dl) {
   // near-singular correction with the self velocity on the upsampled grid
   // and at the poles (S_vel_up, src/StokesVelocity.cc:452-470)
   V3D_CHECK(sht_shc2grid<double>(Vc, 3 * nv, p, pu, vup, nullptr, nullptr, work, st));
   V3D_CHECK(sht_shc2pole<double>(Vc, 3 * nv, p, vpole, st));
-  return near.apply(qc, vup, vpole, vel, st);
+  return near.apply(qc, dl ? qd.p : nullptr, vup, vpole, dl ? dup.p : nullptr,
+                    dl ? dpole.p : nullptr, vel, st);
+}
+
+int StokesG::enable_dl() {
+  if (want_dl) return VES3D_SUCCESS;
+  const long nc = sh_ncoef(p), mu = sh_ngrid(pu), ng = sh_ngrid(p);
+  const long nmat = nv * 9 * nc * nc;
+  if (precision != 1) {
+    V3D_CHECK(DL.alloc(nmat));
+  } else {
+    if (hipMalloc(&DL32, sizeof(float) * nmat) != hipSuccess) return VES3D_MEMORY_ERROR;
+  }
+  V3D_CHECK(nrm_up.alloc(3 * nv * mu));
+  V3D_CHECK(G.alloc(3 * nv * nc));
+  V3D_CHECK(qd.alloc(nv * (mu + 2) * 6));
+  V3D_CHECK(dup.alloc(3 * nv * mu));
+  V3D_CHECK(dpole.alloc(3 * nv * 2));
+  V3D_CHECK(selfv.alloc(3 * nv * ng));
+  if (!vup.p) {
+    V3D_CHECK(vup.alloc(3 * nv * mu));
+    V3D_CHECK(vpole.alloc(3 * nv * 2));
+  }
+  want_dl = true;
+  return VES3D_SUCCESS;
+}
+
+int StokesG::eval_off(const double *f, const double *d, const double *trgo, long n, double *vel,
+                      hipStream_t st) {
+  const long mu = sh_ngrid(pu);
+  if ((!f && !d) || (d && !want_dl) || !trgo || !vel || n < 0) return VES3D_INVALID_PARAMETER;
+  if (n == 0) return VES3D_SUCCESS;
+  if (!selfv.p) V3D_CHECK(selfv.alloc(3 * nv * sh_ngrid(p)));
+  if (!vup.p) {
+    V3D_CHECK(vup.alloc(3 * nv * mu));
+    V3D_CHECK(vpole.alloc(3 * nv * 2));
+  }
+  V3D_CHECK(self_and_density(f, F, V, selfv, up, q, true, st, d));
+  V3D_CHECK(far_field_sl_dl(false, far_src, f ? q.p : nullptr, d ? qd.p : nullptr, nv * (mu + 2),
+                            trgo, vel, (size_t)n, 0, 0, true, st));
+  // near-singular correction (near_singular1): the same r_near as the
+  // surface targets'
+  if (!near_off_ready) {
+    const double rn = use_near ? near.r_near : 0.0;
+    V3D_CHECK(near_off.init(p, pu, nv, 0.0, rn, false, 256));
+    near_off_ready = true;
+  }
+  V3D_CHECK(near_off.setup(far_src, trgo, st, n));
+  if (near_off.npairs == 0) return VES3D_SUCCESS;
+  V3D_CHECK(sht_shc2grid<double>(V, 3 * nv, p, pu, vup, nullptr, nullptr, work, st));
+  V3D_CHECK(sht_shc2pole<double>(V, 3 * nv, p, vpole, st));
+  return near_off.apply(f ? q.p : nullptr, d ? qd.p : nullptr, vup, vpole, d ? dup.p : nullptr,
+                        d ? dpole.p : nullptr, vel, st);
 }
 
 int StokesG::event_begin(int kind, hipStream_t st) {
@@ -634,16 +730,32 @@ int StokesG::event_end(hipStream_t st) {
   return VES3D_SUCCESS;
 }
 
-int StokesG::apply(const double *f, double *vel, hipStream_t st, bool add_repul, bool local) {
+int StokesG::apply(const double *f, double *vel, hipStream_t st, bool add_repul, bool local,
+                   const double *d) {
   const long ng = sh_ngrid(p), mu = sh_ngrid(pu);
   const bool near_on = use_near && near.npairs > 0;
+  if ((d && !want_dl) || (!f && (!d || add_repul))) return VES3D_INVALID_PARAMETER;
   if (add_repul && near_on && near.repul > 0) {
     V3D_HIP(hipMemcpyAsync(frep, f, sizeof(double) * 3 * nv * ng, hipMemcpyDeviceToDevice, st));
     V3D_CHECK(near.add_repulsion(frep, st));
     f = frep;
   }
-  V3D_CHECK(self_and_density(f, F, V, vel, up, q, !local, st));
+  V3D_CHECK(self_and_density(f, F, V, vel, up, q, !local, st, d));
   if (local) return VES3D_SUCCESS;
+  if (d) {
+    // single and double layer over the upsampled sources (qforce_single,
+    // qforce_double), own vesicle skipped
+    V3D_CHECK(event_begin(1, st));
+    V3D_CHECK(far_field_sl_dl(precision != 0, far_src, f ? q.p : nullptr, qd, nv * (mu + 2), trg,
+                              farv, nv * ng, mu + 2, ng, false, st));
+    V3D_CHECK(event_end(st));
+    k_add_points<<<gblocks(nv * ng), 256, 0, st>>>(farv, nv, ng, vel);
+    V3D_CHECK(launch_status());
+    if (!near_on) return VES3D_SUCCESS;
+    V3D_CHECK(event_begin(3, st));
+    V3D_CHECK(near_correct(f ? q.p : nullptr, V, vel, st, true));
+    return event_end(st);
+  }
   FarCoarse fc{trg, qc, nullptr, (size_t)ng, cflags};
   if (far_coarse > 0) V3D_CHECK(pack_coarse(f, qc, st));
   V3D_CHECK(event_begin(1, st));

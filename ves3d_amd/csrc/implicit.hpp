@@ -41,6 +41,14 @@ int far_field_between_groups2(bool f32c, const double *spos, const double *sld1,
                               double *tvel2, size_t n_trg, size_t spg, size_t tpg, hipStream_t st,
                               const FarCoarse *coarse = nullptr);
 
+// Single- and double-layer far field (the PVFMMEval stand-in's kernels):
+// sld [n_src][3] and/or dld [n_src][6] = (density times weight, normal),
+// grouped (spg, tpg > 0: own-group pairs skipped) or not; guard: pairs
+// closer than sqrt(128 eps) are skipped (targets that may sit on a source).
+int far_field_sl_dl(bool f32c, const double *spos, const double *sld, const double *dld,
+                    size_t n_src, const double *tpos, double *tvel, size_t n_trg, size_t spg,
+                    size_t tpg, bool guard, hipStream_t st);
+
 // ves3d_hip_direct_eval_f64 with caller-provided scratch for the per-slice
 // partial sums (part, part_bytes; at least n_trg * 24 bytes, else
 // VES3D_INVALID_PARAMETER): allocates nothing.
@@ -139,8 +147,20 @@ class StokesG {
   // side of the implicit step, src/InterfacialVelocity.cc:733).  local: the
   // self-interaction (singular operator) alone, without the far field and
   // the near-singular correction.
+  // d (optional): a double-layer density [nv][3][ng] (SetDensityDL); f may
+  // then be null (no single layer).  The double layer needs enable_dl() and
+  // runs over the upsampled sources whatever far_coarse says.
   int apply(const double *f, double *vel, hipStream_t st, bool add_repul = false,
-            bool local = false);
+            bool local = false, const double *d = nullptr);
+  // The double-layer self operators (built at the next setup) and buffers.
+  int enable_dl();
+  // Velocity at n targets off the surfaces (trg [n][3] -> vel [n][3];
+  // SetTrgCoord, src/StokesVelocity.cc:146-150): the direct sum over every
+  // vesicle's upsampled sources (coincident points skipped) with the
+  // near-singular correction of targets within r_near of a vesicle, no
+  // self term; f (SL) and d (DL) as in apply, either may be null.  FP64.
+  int eval_off(const double *f, const double *d, const double *trg, long n, double *vel,
+               hipStream_t st);
   // apply for two densities at once (one far-field pass over both): the
   // right-hand side's (f1, with the repulsion when add_repul1) and the
   // solver's first matvec's (f2)
@@ -178,14 +198,23 @@ class StokesG {
   // the last setup
   double far_pairs = 0;
   int pack_coarse(const double *f, double *qcoarse, hipStream_t st);
+  // double layer: operators, upsampled unit normals [nv][3][mu], the
+  // density's coefficients G, its packed far-field form qd [nv][M][6],
+  // itself on the upsampled grid and at the poles
+  bool want_dl = false;
+  DBuf DL, nrm_up, G, qd, dup, dpole, selfv;
+  float *DL32 = nullptr;
+  NearG near_off;              // near-singular correction of targets off the surfaces
+  bool near_off_ready = false;
   ~StokesG();
 
  private:
-  // density SHT, self apply (vel = S_self f on the p grid) and, with pack,
-  // the upsampled far-field density qc
+  // density SHTs, self apply (vel = S_self f + D_self d on the p grid) and,
+  // with pack, the upsampled far-field densities qc (and qd, dup, dpole)
   int self_and_density(const double *f, double *Fc, double *Vc, double *vel, double *upc,
-                       double *qc, bool pack, hipStream_t st);
-  int near_correct(const double *qc, const double *Vc, double *vel, hipStream_t st);
+                       double *qc, bool pack, hipStream_t st, const double *d = nullptr);
+  int near_correct(const double *qc, const double *Vc, double *vel, hipStream_t st,
+                   bool dl = false);
   int event_begin(int kind, hipStream_t st);
   int event_end(hipStream_t st);
 };

@@ -1,5 +1,7 @@
 // near.hip -- near-singular correction (NearSingular<Real>,
-// src/NearSingular.cc) on gfx950, FP64 throughout.
+// src/NearSingular.cc) on gfx950, FP64 throughout, for the step's surface
+// targets (trg_is_surf) or targets off the surfaces, with single- and
+// double-layer densities.
 //
 // Setup (once per step, positions fixed):
 //   k_bounds     per-vesicle source centroid / radius, target radius
@@ -103,17 +105,62 @@ __global__ __launch_bounds__(256) void k_bounds(const double *__restrict__ src,
   }
 }
 
-// flags[tv][v] = 1 when some target of tv may lie within r_near of a source of
-// v (bounding spheres about v's source centroid; conservative margin).
+// Targets off the surfaces, in groups of tpg: centroid tcen and squared
+// radius rt2 of every group.
+__global__ __launch_bounds__(256) void k_tbounds(const double *__restrict__ trg, long n_trg,
+                                                 long tpg, double *tcen, double *rt2) {
+  const long g = blockIdx.x;
+  const long a = g * tpg, b = min(a + tpg, n_trg);
+  __shared__ double sh[3][4];
+  __shared__ double c[3];
+  __shared__ double mx[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double a0 = 0, a1 = 0, a2 = 0;
+  for (long k = a + threadIdx.x; k < b; k += 256) {
+    a0 += trg[3 * k];
+    a1 += trg[3 * k + 1];
+    a2 += trg[3 * k + 2];
+  }
+  a0 = wave_sum(a0);
+  a1 = wave_sum(a1);
+  a2 = wave_sum(a2);
+  if (lane == 0) {
+    sh[0][w] = a0;
+    sh[1][w] = a1;
+    sh[2][w] = a2;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int d = threadIdx.x;
+    c[d] = (((sh[d][0] + sh[d][1]) + sh[d][2]) + sh[d][3]) / (double)(b - a);
+    tcen[g * 3 + d] = c[d];
+  }
+  __syncthreads();
+  double mt = 0;
+  for (long k = a + threadIdx.x; k < b; k += 256) {
+    const double dx = trg[3 * k] - c[0], dy = trg[3 * k + 1] - c[1], dz = trg[3 * k + 2] - c[2];
+    mt = fmax(mt, dx * dx + dy * dy + dz * dz);
+  }
+  mt = wave_max(mt);
+  if (lane == 0) mx[w] = mt;
+  __syncthreads();
+  if (threadIdx.x == 0) rt2[g] = fmax(fmax(mx[0], mx[1]), fmax(mx[2], mx[3]));
+}
+
+// flags[tg][v] = 1 when some target of target group tg may lie within
+// r_near of a source of v (bounding spheres: targets within sqrt(rt2[tg]) of
+// tcen[tg], sources within sqrt(rs2[v]) of cen[v]; conservative margin).
+// surf: the groups are the vesicles' own surface points (tcen = cen), which
+// never pair with their own vesicle.
 __global__ void k_cand(const double *__restrict__ cen, const double *__restrict__ rs2,
-                       const double *__restrict__ rt2, long nv, double r_near,
-                       unsigned char *flags) {
-  NGL(i, nv * nv) {
+                       const double *__restrict__ tcen, const double *__restrict__ rt2, long ntg,
+                       long nv, double r_near, int surf, unsigned char *flags) {
+  NGL(i, ntg * nv) {
     const long tv = i / nv, v = i % nv;
     unsigned char f = 0;
-    if (tv != v) {
-      const double dx = cen[3 * tv] - cen[3 * v], dy = cen[3 * tv + 1] - cen[3 * v + 1],
-                   dz = cen[3 * tv + 2] - cen[3 * v + 2];
+    if (!surf || tv != v) {
+      const double dx = tcen[3 * tv] - cen[3 * v], dy = tcen[3 * tv + 1] - cen[3 * v + 1],
+                   dz = tcen[3 * tv + 2] - cen[3 * v + 2];
       // targets of tv lie within sqrt(rt2[tv]) of c_tv and sources of v
       // within sqrt(rs2[v]) of c_v, so a pair closer than r_near needs
       // |c_tv - c_v| < sqrt(rt2[tv]) + sqrt(rs2[v]) + r_near
@@ -126,10 +173,11 @@ __global__ void k_cand(const double *__restrict__ cen, const double *__restrict_
 
 // One workgroup per candidate pair (v, tv): closest source of v for every
 // target of tv that can be within r_near of v.
+// Targets of group tv are tv * ng .. min(tv * ng + ng, n_trg) - 1.
 __global__ __launch_bounds__(256) void k_search(const double *__restrict__ src,
                                                 const double *__restrict__ trg,
                                                 const int *__restrict__ cand, long M, long ng,
-                                                const double *__restrict__ cen,
+                                                long n_trg, const double *__restrict__ cen,
                                                 const double *__restrict__ rs2, double r_near,
                                                 int *near_k, int *cnt) {
 #pragma clang fp contract(off)
@@ -146,7 +194,7 @@ __global__ __launch_bounds__(256) void k_search(const double *__restrict__ src,
     const long j = j0 + threadIdx.x;
     double tx = 0, ty = 0, tz = 0;
     bool act = false;
-    if (j < ng) {
+    if (j < ng && tv * ng + j < n_trg) {
       const double *tp = trg + (tv * ng + j) * 3;
       tx = tp[0];
       ty = tp[1];
@@ -426,11 +474,15 @@ __device__ bool patch_project(const Patch &P, const double t[3], double &x, doub
 
 // Per pair: projection, repulsion, interpolation points (index 0 = the
 // target itself) and Lagrange weights (src/NearSingular.cc:1040-1098,
-// :1426-1456, :1508-1540).  pxy = (x, y, 1 if the target sits on the patch).
+// :1426-1456, :1508-1540).  pxy = (x, y, 1 if the target sits on the patch,
+// the factor of the double-layer density's jump in the patch value: +1/2,
+// or -1/2 for an interior target off the surfaces, :1458-1469).  surf: the
+// targets are surface points, whose interpolation always runs along the
+// exterior normal.
 __global__ void k_project(const double *__restrict__ src, const double *__restrict__ trg,
                           const int *__restrict__ pairs, long npairs, long M, int pu,
                           const double *__restrict__ mint, const double *__restrict__ mfix,
-                          double r_near, double r2repul_inv, double *ipts, double *wts,
+                          double r_near, double r2repul_inv, int surf, double *ipts, double *wts,
                           double *pxy, double *rep) {
 #pragma clang fp contract(off)
   NGL(i, npairs) {
@@ -460,7 +512,7 @@ __global__ void k_project(const double *__restrict__ src, const double *__restri
     for (int d = 0; d < 3; ++d) rep[3 * i + d] = n[d] * f;
 
     double dn = sqrt(dot3(dR, dR));
-    if (!extr) dn = -dn;
+    if (!extr && surf) dn = -dn;
     double oo = 1.0 / dn;
     if (fabs(dn) < kEpsSqrt) {
       dn = 0.0;
@@ -481,17 +533,23 @@ __global__ void k_project(const double *__restrict__ src, const double *__restri
         if (q != l) yl *= (ix - inter_points(q)) / (x0 - inter_points(q));
       w[l] = ix == 0.0 ? (l == 0 ? 1.0 : 0.0) : yl;
     }
-    pxy[3 * i] = x;
-    pxy[3 * i + 1] = y;
-    pxy[3 * i + 2] = ix == 0.0 ? 1.0 : 0.0;
+    pxy[4 * i] = x;
+    pxy[4 * i + 1] = y;
+    pxy[4 * i + 2] = ix == 0.0 ? 1.0 : 0.0;
+    pxy[4 * i + 3] = (!extr && !surf) ? -0.5 : 0.5;
   }
 }
 
 // ------------------------------------------------------------- apply
 // tiles[3 i] = (v, first pair, pair count <= kTile).  Thread (pair, l)
-// evaluates 1/(8 pi) sum_s q_s / r + (q_s . d) d / r^3 at point l of pair.
+// evaluates at point l of its pair the single layer of q,
+// 1/(8 pi) sum_s q_s / r + (q_s . d) d / r^3, and the double layer of
+// qd = [g (3), n (3)] per source, 3/(4 pi) sum_s (d . n_s)(d . g_s) d / r^5,
+// d = x - y_s (src/PVFMMInterface.cc:161-238, :284-369).
+template <bool SL, bool DL>
 __global__ __launch_bounds__(256) void k_direct(const double *__restrict__ src,
                                                 const double *__restrict__ q,
+                                                const double *__restrict__ qd,
                                                 const double *__restrict__ ipts,
                                                 const int *__restrict__ tiles, long M,
                                                 double *dvel) {
@@ -507,50 +565,70 @@ __global__ __launch_bounds__(256) void k_direct(const double *__restrict__ src,
     tz = ipts[pt + 2];
   }
   const double *s = src + (long)v * M * 3;
-  const double *g = q + (long)v * M * 3;
-  __shared__ double ls[6 * 128];
-  double u0 = 0, u1 = 0, u2 = 0;
+  const double *g = SL ? q + (long)v * M * 3 : nullptr;
+  const double *gd = DL ? qd + (long)v * M * 6 : nullptr;
+  constexpr int R = 3 + (SL ? 3 : 0) + (DL ? 6 : 0);  // doubles per staged source
+  __shared__ double ls[R * 128];
+  double u0 = 0, u1 = 0, u2 = 0, w0 = 0, w1 = 0, w2 = 0;
   for (long k0 = 0; k0 < M; k0 += 128) {
     if (threadIdx.x < 128) {
       const long kk = k0 + threadIdx.x;
       if (kk < M) {
-        for (int d = 0; d < 3; ++d) {
-          ls[6 * threadIdx.x + d] = s[3 * kk + d];
-          ls[6 * threadIdx.x + 3 + d] = g[3 * kk + d];
-        }
+        double *o = ls + R * threadIdx.x;
+        for (int d = 0; d < 3; ++d) o[d] = s[3 * kk + d];
+        if (SL)
+          for (int d = 0; d < 3; ++d) o[3 + d] = g[3 * kk + d];
+        if (DL)
+          for (int d = 0; d < 6; ++d) o[(SL ? 6 : 3) + d] = gd[6 * kk + d];
       }
     }
     __syncthreads();
     const int kn = (int)std::min<long>(128, M - k0);
     if (act) {
       for (int k = 0; k < kn; ++k) {
-        const double dx = tx - ls[6 * k], dy = ty - ls[6 * k + 1], dz = tz - ls[6 * k + 2];
+        const double *o = ls + R * k;
+        const double dx = tx - o[0], dy = ty - o[1], dz = tz - o[2];
         const double r2 = dx * dx + dy * dy + dz * dz;
         if (r2 > 0.0) {
           const double ri = 1.0 / sqrt(r2);
-          const double fx = ls[6 * k + 3], fy = ls[6 * k + 4], fz = ls[6 * k + 5];
-          const double fd = (fx * dx + fy * dy + fz * dz) * ri * ri * ri;
-          u0 += fx * ri + fd * dx;
-          u1 += fy * ri + fd * dy;
-          u2 += fz * ri + fd * dz;
+          if (SL) {
+            const double fx = o[3], fy = o[4], fz = o[5];
+            const double fd = (fx * dx + fy * dy + fz * dz) * ri * ri * ri;
+            u0 += fx * ri + fd * dx;
+            u1 += fy * ri + fd * dy;
+            u2 += fz * ri + fd * dz;
+          }
+          if (DL) {
+            const double *h = o + (SL ? 6 : 3);
+            const double ri2 = ri * ri;
+            const double c = (dx * h[3] + dy * h[4] + dz * h[5]) * (dx * h[0] + dy * h[1] + dz * h[2]) *
+                             ri2 * ri2 * ri;
+            w0 += c * dx;
+            w1 += c * dy;
+            w2 += c * dz;
+          }
         }
       }
     }
     __syncthreads();
   }
   if (act) {
-    const double sc = 1.0 / (8.0 * kPi);
-    dvel[pt] = u0 * sc;
-    dvel[pt + 1] = u1 * sc;
-    dvel[pt + 2] = u2 * sc;
+    const double sc = 1.0 / (8.0 * kPi), dc = 3.0 / (4.0 * kPi);
+    dvel[pt] = u0 * sc + w0 * dc;
+    dvel[pt + 1] = u1 * sc + w1 * dc;
+    dvel[pt + 2] = u2 * sc + w2 * dc;
   }
 }
 
 // corr[i] = sum_l w_l y_l - direct(target), y_0 = patch value of the self
 // velocity at the projection, y_l = direct value at interpolation point l.
+// With a double-layer density (dup / dpole: the density on the upsampled
+// grid and at the poles) the patch value carries its jump, pxy[4 i + 3]
+// times the density (src/NearSingular.cc:1458-1469).
 __global__ void k_combine(const int *__restrict__ pairs, long npairs, int pu,
                           const double *__restrict__ mint, const double *__restrict__ mfix,
                           const double *__restrict__ vup, const double *__restrict__ vpole,
+                          const double *__restrict__ dup, const double *__restrict__ dpole,
                           const double *__restrict__ pxy, const double *__restrict__ wts,
                           const double *__restrict__ dvel, double *corr) {
 #pragma clang fp contract(off)
@@ -560,15 +638,22 @@ __global__ void k_combine(const int *__restrict__ pairs, long npairs, int pu,
     const int k = pairs[3 * i + 2];
     double mesh[9][3];
     patch_mesh(PointsAM{vup, vpole, mu}, v, k, pu, mint, mfix, mesh);
+    if (dup) {
+      double md[9][3];
+      patch_mesh(PointsAM{dup, dpole, mu}, v, k, pu, mint, mfix, md);
+      const double sc = pxy[4 * i + 3];
+      for (int r = 0; r < 9; ++r)
+        for (int d = 0; d < 3; ++d) mesh[r][d] += sc * md[r][d];
+    }
     Patch P;
     patch_from_mesh(mesh, P);
     double pv[3];
-    patch_eval(P, pxy[3 * i], pxy[3 * i + 1], pv);
+    patch_eval(P, pxy[4 * i], pxy[4 * i + 1], pv);
     const double *dv = dvel + i * kDeg * 3;
     const double *w = wts + i * kDeg;
     for (int d = 0; d < 3; ++d) {
       double y = pv[d];
-      if (pxy[3 * i + 2] == 0.0) {
+      if (pxy[4 * i + 2] == 0.0) {
         double s = 0.0;
         for (int l = 1; l < kDeg; ++l) s += w[l] * dv[3 * l + d];
         y = w[0] * pv[d] + s;
@@ -579,12 +664,13 @@ __global__ void k_combine(const int *__restrict__ pairs, long npairs, int pu,
 }
 
 // out[tv][d][j] += sum of per_pair over the pairs of target tgt[i] = tv ng + j,
-// in pair order (perm[tptr[i] .. tptr[i+1]) lists them).
+// in pair order (perm[tptr[i] .. tptr[i+1]) lists them); ng == 0: the
+// targets are point-major, out[t][d].
 __global__ void k_scatter(const int *__restrict__ tgt, const int *__restrict__ tptr,
                           const int *__restrict__ perm, long nt, long ng,
                           const double *__restrict__ per_pair, double *out) {
   NGL(i, nt) {
-    const long t = tgt[i], tv = t / ng, j = t % ng;
+    const long t = tgt[i];
     double s0 = 0, s1 = 0, s2 = 0;
     for (int b = tptr[i]; b < tptr[i + 1]; ++b) {
       const long pidx = perm[b];
@@ -592,9 +678,16 @@ __global__ void k_scatter(const int *__restrict__ tgt, const int *__restrict__ t
       s1 += per_pair[3 * pidx + 1];
       s2 += per_pair[3 * pidx + 2];
     }
-    out[(tv * 3 + 0) * ng + j] += s0;
-    out[(tv * 3 + 1) * ng + j] += s1;
-    out[(tv * 3 + 2) * ng + j] += s2;
+    if (ng) {
+      const long tv = t / ng, j = t % ng;
+      out[(tv * 3 + 0) * ng + j] += s0;
+      out[(tv * 3 + 1) * ng + j] += s1;
+      out[(tv * 3 + 2) * ng + j] += s2;
+    } else {
+      out[3 * t] += s0;
+      out[3 * t + 1] += s1;
+      out[3 * t + 2] += s2;
+    }
   }
 }
 
@@ -678,12 +771,16 @@ NearG::Buf<T>::~Buf() {
 
 NearG::~NearG() = default;
 
-int NearG::init(int p_, int pu_, long nv_, double repul_dist, double r_near_override) {
-  if (p_ < 1 || pu_ < 1 || nv_ < 1) return VES3D_INVALID_PARAMETER;
+int NearG::init(int p_, int pu_, long nv_, double repul_dist, double r_near_override,
+                bool trg_is_surf_, long trg_per_group) {
+  if (p_ < 1 || pu_ < 1 || nv_ < 1 || (!trg_is_surf_ && trg_per_group < 1))
+    return VES3D_INVALID_PARAMETER;
   p = p_;
   pu = pu_;
   nv = nv_;
+  trg_is_surf = trg_is_surf_;
   ng = 2L * p * (p + 1);
+  tpg = trg_is_surf ? ng : trg_per_group;
   M = 2 + 2L * pu * (pu + 1);
   repul = repul_dist;
   r_near_fixed = r_near_override;
@@ -699,14 +796,27 @@ int NearG::init(int p_, int pu_, long nv_, double repul_dist, double r_near_over
   return VES3D_SUCCESS;
 }
 
-int NearG::setup(const double *src, const double *trg, hipStream_t st) {
+int NearG::setup(const double *src, const double *trg, hipStream_t st, long n_trg_) {
   // earlier applies may still read the per-pair buffers that reserve() below
   // can replace
   V3D_HIP(hipStreamSynchronize(st));
   npairs = ntrg_near = ntiles = 0;
   src_ = src;
-  k_bounds<<<(unsigned)nv, 256, 0, st>>>(src, trg, M, ng, cen.p, rs2.p, rt2.p);
+  n_trg = trg_is_surf ? nv * ng : n_trg_;
+  if (n_trg <= 0) return trg_is_surf ? VES3D_INVALID_PARAMETER : VES3D_SUCCESS;
+  const long ntg = trg_is_surf ? nv : (n_trg + tpg - 1) / tpg;  // target groups
+  V3D_CHECK(rt2.reserve(std::max(ntg, nv)));  // k_bounds writes nv entries first
+  V3D_CHECK(flags.reserve(ntg * nv));
+  k_bounds<<<(unsigned)nv, 256, 0, st>>>(src, trg_is_surf ? trg : src, M, trg_is_surf ? ng : 0,
+                                          cen.p, rs2.p, rt2.p);
   V3D_CHECK(launch_status());
+  const double *tc = cen.p;
+  if (!trg_is_surf) {
+    V3D_CHECK(tcen.reserve(3 * ntg));
+    k_tbounds<<<(unsigned)ntg, 256, 0, st>>>(trg, n_trg, tpg, tcen.p, rt2.p);
+    V3D_CHECK(launch_status());
+    tc = tcen.p;
+  }
   std::vector<double> h_rs2(nv);
   V3D_HIP(hipMemcpyAsync(h_rs2.data(), rs2.p, sizeof(double) * nv, hipMemcpyDeviceToHost, st));
   V3D_HIP(hipStreamSynchronize(st));
@@ -715,14 +825,15 @@ int NearG::setup(const double *src, const double *trg, hipStream_t st) {
   // src/NearSingular.cc:83, :118-160
   r_near = r_near_fixed > 0 ? r_near_fixed : std::sqrt(m2) * (2.0 / std::sqrt((double)pu));
 
-  k_cand<<<nblk(nv * nv), 256, 0, st>>>(cen.p, rs2.p, rt2.p, nv, r_near, flags.p);
+  k_cand<<<nblk(ntg * nv), 256, 0, st>>>(cen.p, rs2.p, tc, rt2.p, ntg, nv, r_near,
+                                          trg_is_surf ? 1 : 0, flags.p);
   V3D_CHECK(launch_status());
-  std::vector<unsigned char> h_flags(nv * nv);
-  V3D_HIP(hipMemcpyAsync(h_flags.data(), flags.p, nv * nv, hipMemcpyDeviceToHost, st));
+  std::vector<unsigned char> h_flags(ntg * nv);
+  V3D_HIP(hipMemcpyAsync(h_flags.data(), flags.p, ntg * nv, hipMemcpyDeviceToHost, st));
   V3D_HIP(hipStreamSynchronize(st));
   h_cand.clear();
   for (long v = 0; v < nv; ++v)
-    for (long tv = 0; tv < nv; ++tv)
+    for (long tv = 0; tv < ntg; ++tv)
       if (h_flags[tv * nv + v]) {
         h_cand.push_back((int)v);
         h_cand.push_back((int)tv);
@@ -732,9 +843,9 @@ int NearG::setup(const double *src, const double *trg, hipStream_t st) {
   V3D_CHECK(cand.reserve(2 * ncand));
   V3D_CHECK(cnt.reserve(ncand));
   V3D_CHECK(off.reserve(ncand));
-  V3D_CHECK(near_k.reserve(ncand * ng));
+  V3D_CHECK(near_k.reserve(ncand * tpg));
   V3D_HIP(hipMemcpyAsync(cand.p, h_cand.data(), sizeof(int) * 2 * ncand, hipMemcpyHostToDevice, st));
-  k_search<<<(unsigned)ncand, 256, 0, st>>>(src, trg, cand.p, M, ng, cen.p, rs2.p, r_near,
+  k_search<<<(unsigned)ncand, 256, 0, st>>>(src, trg, cand.p, M, tpg, n_trg, cen.p, rs2.p, r_near,
                                               near_k.p, cnt.p);
   V3D_CHECK(launch_status());
   h_cnt.resize(ncand);
@@ -750,7 +861,7 @@ int NearG::setup(const double *src, const double *trg, hipStream_t st) {
   if (npairs == 0) return VES3D_SUCCESS;
   V3D_HIP(hipMemcpyAsync(off.p, h_off.data(), sizeof(int) * ncand, hipMemcpyHostToDevice, st));
   V3D_CHECK(pairs.reserve(3 * npairs));
-  k_compact<<<(unsigned)ncand, 256, 0, st>>>(near_k.p, off.p, cand.p, ng, pairs.p);
+  k_compact<<<(unsigned)ncand, 256, 0, st>>>(near_k.p, off.p, cand.p, tpg, pairs.p);
   V3D_CHECK(launch_status());
   h_pairs.resize(3 * npairs);
   V3D_HIP(hipMemcpyAsync(h_pairs.data(), pairs.p, sizeof(int) * 3 * npairs,
@@ -799,32 +910,41 @@ int NearG::setup(const double *src, const double *trg, hipStream_t st) {
   V3D_CHECK(ipts.reserve(npairs * kDeg * 3));
   V3D_CHECK(dvel.reserve(npairs * kDeg * 3));
   V3D_CHECK(wts.reserve(npairs * kDeg));
-  V3D_CHECK(pxy.reserve(npairs * 3));
+  V3D_CHECK(pxy.reserve(npairs * 4));
   V3D_CHECK(rep.reserve(npairs * 3));
   V3D_CHECK(corr.reserve(npairs * 3));
   const double r2inv = repul > 0 ? 1.0 / (repul * repul) : 0.0;
   k_project<<<nblk(npairs), 256, 0, st>>>(src, trg, pairs.p, npairs, M, pu, mint.p, mfix.p, r_near,
-                                           r2inv, ipts.p, wts.p, pxy.p, rep.p);
+                                           r2inv, trg_is_surf ? 1 : 0, ipts.p, wts.p, pxy.p,
+                                           rep.p);
   V3D_CHECK(launch_status());
   // the host vectors above are released on return
   return hip_status(hipStreamSynchronize(st));
 }
 
 int NearG::add_repulsion(double *f, hipStream_t st) const {
-  if (npairs == 0 || repul <= 0) return VES3D_SUCCESS;
+  if (npairs == 0 || repul <= 0 || !trg_is_surf) return VES3D_SUCCESS;
   k_scatter<<<nblk(ntrg_near), 256, 0, st>>>(tgt.p, tptr.p, perm.p, ntrg_near, ng, rep.p, f);
   return launch_status();
 }
 
-int NearG::apply(const double *q, const double *vup, const double *vpole, double *vel,
-                 hipStream_t st) {
+int NearG::apply(const double *q, const double *qd, const double *vup, const double *vpole,
+                 const double *dup, const double *dpole, double *vel, hipStream_t st) {
   if (npairs == 0) return VES3D_SUCCESS;
-  k_direct<<<(unsigned)ntiles, 256, 0, st>>>(src_, q, ipts.p, tiles.p, M, dvel.p);
+  if ((!q && !qd) || (qd && (!dup || !dpole))) return VES3D_INVALID_PARAMETER;
+  if (q && qd)
+    k_direct<true, true><<<(unsigned)ntiles, 256, 0, st>>>(src_, q, qd, ipts.p, tiles.p, M, dvel.p);
+  else if (q)
+    k_direct<true, false><<<(unsigned)ntiles, 256, 0, st>>>(src_, q, qd, ipts.p, tiles.p, M, dvel.p);
+  else
+    k_direct<false, true><<<(unsigned)ntiles, 256, 0, st>>>(src_, q, qd, ipts.p, tiles.p, M, dvel.p);
   V3D_CHECK(launch_status());
-  k_combine<<<nblk(npairs), 256, 0, st>>>(pairs.p, npairs, pu, mint.p, mfix.p, vup, vpole, pxy.p,
-                                           wts.p, dvel.p, corr.p);
+  k_combine<<<nblk(npairs), 256, 0, st>>>(pairs.p, npairs, pu, mint.p, mfix.p, vup, vpole,
+                                           qd ? dup : nullptr, dpole, pxy.p, wts.p, dvel.p,
+                                           corr.p);
   V3D_CHECK(launch_status());
-  k_scatter<<<nblk(ntrg_near), 256, 0, st>>>(tgt.p, tptr.p, perm.p, ntrg_near, ng, corr.p, vel);
+  k_scatter<<<nblk(ntrg_near), 256, 0, st>>>(tgt.p, tptr.p, perm.p, ntrg_near,
+                                              trg_is_surf ? ng : 0, corr.p, vel);
   return launch_status();
 }
 
@@ -839,7 +959,7 @@ extern "C" int ves3d_hip_near_create(int p, int p_up, size_t nves, double repul_
   *handle_host = nullptr;
   NearG *g = new (std::nothrow) NearG();
   if (!g) return VES3D_MEMORY_ERROR;
-  const int s = g->init(p, p_up, (long)nves, repul_dist, r_near);
+  const int s = g->init(p, p_up, (long)nves, repul_dist, r_near, true, 0);
   if (s != VES3D_SUCCESS) {
     delete g;
     return s;
@@ -875,5 +995,6 @@ extern "C" int ves3d_hip_near_force_repul(void *handle, double *f, ves3d_stream_
 extern "C" int ves3d_hip_near_apply(void *handle, const double *q, const double *vup,
                                     const double *vpole, double *vel, ves3d_stream_t stream) {
   if (!handle || !q || !vup || !vpole || !vel) return VES3D_INVALID_PARAMETER;
-  return static_cast<NearG *>(handle)->apply(q, vup, vpole, vel, (hipStream_t)stream);
+  return static_cast<NearG *>(handle)->apply(q, nullptr, vup, vpole, nullptr, nullptr, vel,
+                                             (hipStream_t)stream);
 }

@@ -301,6 +301,21 @@ __global__ void k_scale_om(const double *__restrict__ x, long nf, int nc,
     y[o] = x[o] * scale[c];
   }
 }
+// out[v][:] = c[v] x[v][:] on vector fields [nv][3 ng] (per-vesicle factors)
+__global__ void k_ves_scale(const double *__restrict__ x, const double *__restrict__ c, long nv,
+                            long n3, double *__restrict__ out) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nv * n3;
+       i += (long)gridDim.x * blockDim.x)
+    out[i] = c[i / n3] * x[i];
+}
+// y[v][:] = c[v] y[v][:] - s[v][:]
+__global__ void k_ves_axmy(double *y, const double *__restrict__ c, const double *__restrict__ s,
+                           long nv, long n3) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nv * n3;
+       i += (long)gridDim.x * blockDim.x)
+    y[i] = c[i / n3] * y[i] - s[i];
+}
+
 // u_inf = (rate * z, 0, 0)  (src/BgFlow.cc:58-76)
 __global__ void k_shear(const double *__restrict__ X, long nv, long ng, double rate,
                         double *__restrict__ u) {
@@ -468,6 +483,10 @@ struct Stepper {
   DBuf inner_work;                 // BiCGStab scratch of the inner solves, 8 n
   DBuf solve_work;                 // BiCGStab scratch of the outer solve, 8 n
   DBuf x0s, Axr;                   // the initial guess (for the fallback); A x of a restart
+  // viscosity contrast (VesicleProperties::update, src/VesicleProps.cc:28-52):
+  // vel_coeff = (1 + lambda) / 2, dl_coeff = 1 - lambda per vesicle
+  bool contrast = false;
+  DBuf vcoef, dcoef, dd;
   apply_fn inner_pc = nullptr;     // the inner solve's own preconditioner
   int inner_iters = 0, local_mv = 0;
 
@@ -586,10 +605,19 @@ struct Stepper {
     V3D_CHECK(forces.gravity_add(f2, f, st));
     V3D_CHECK(forces.tensile(sg, f2, st));
     V3D_CHECK(dev_axpby(3 * nv * ng, 1.0, f2, 1.0, f, st));
-    V3D_CHECK(stokes.apply(f, Sf, st, false, local));
+    if (contrast) {  // SetDensityDL(dl_coeff u) (src/InterfacialVelocity.cc:945-951)
+      k_ves_scale<<<gb(3 * nv * ng), 256, 0, st>>>(ug, dcoef, nv, 3 * ng, dd);
+      V3D_CHECK(launch_status());
+    }
+    V3D_CHECK(stokes.apply(f, Sf, st, false, local, contrast ? dd.p : nullptr));
     V3D_CHECK(forces.S.div(Sf, ten, st));
     V3D_CHECK(dev_axpby(nv * ng, 0.0, ten, -1.0, ten, st));      // ten = -div
-    V3D_CHECK(dev_axpby(3 * nv * ng, -1.0, Sf, 1.0, ug, st));    // vox = u - Sf
+    if (contrast) {  // vox = vel_coeff u - Sf (:965-968)
+      k_ves_axmy<<<gb(3 * nv * ng), 256, 0, st>>>(ug, vcoef, Sf, nv, 3 * ng);
+      V3D_CHECK(launch_status());
+    } else {
+      V3D_CHECK(dev_axpby(3 * nv * ng, -1.0, Sf, 1.0, ug, st));  // vox = u - Sf
+    }
     return pack(ug, ten, y);
   }
 
@@ -611,6 +639,10 @@ struct Stepper {
   // evaluations share one far-field pass (two densities, same sources)
   int rhs_and_matvec(double *b, const double *x, double *y) {
     if (!prepared) return VES3D_INVALID_PARAMETER;
+    if (contrast) {  // the matvec carries a double layer the right-hand side has not
+      V3D_CHECK(assemble_rhs(b));
+      return matvec(x, y);
+    }
     if (!ub.p)
       for (DBuf *d : {&ub, &fb, &Sfb}) V3D_CHECK(d->alloc(3 * nv * ng));
     ++matvecs;
@@ -1183,9 +1215,28 @@ int ves3d_hip_stepper_set_vesicle_props(void *h, const double *bending_modulus_h
                                         const double *gravity_field_host) {
   if (!h) return VES3D_INVALID_PARAMETER;
   Stepper *s = (Stepper *)h;
-  if (viscosity_contrast_host)
-    for (long v = 0; v < s->nv; ++v)
-      if (std::fabs(viscosity_contrast_host[v] - 1.0) > 1e-12) return VES3D_NOT_IMPLEMENTED;
+  if (viscosity_contrast_host) {
+    bool con = false;
+    std::vector<double> vc(s->nv), dc(s->nv);
+    for (long v = 0; v < s->nv; ++v) {
+      const double lam = viscosity_contrast_host[v];
+      if (!(lam >= 0.0)) return VES3D_INVALID_PARAMETER;
+      con |= std::fabs(lam - 1.0) > 1e-12;  // has_contrast (src/VesicleProps.cc:47-48)
+      vc[v] = 0.5 + 0.5 * lam;
+      dc[v] = 1.0 - lam;
+    }
+    if (con) {
+      V3D_CHECK(s->stokes.enable_dl());
+      if (!s->vcoef.p) {
+        V3D_CHECK(s->vcoef.alloc(s->nv));
+        V3D_CHECK(s->dcoef.alloc(s->nv));
+        V3D_CHECK(s->dd.alloc(3 * s->nv * s->ng));
+      }
+      V3D_HIP(hipMemcpy(s->vcoef.p, vc.data(), sizeof(double) * s->nv, hipMemcpyHostToDevice));
+      V3D_HIP(hipMemcpy(s->dcoef.p, dc.data(), sizeof(double) * s->nv, hipMemcpyHostToDevice));
+    }
+    s->contrast = con;
+  }
   V3D_CHECK(s->forces.set_props(bending_modulus_host, excess_density_host, gravity_field_host));
   s->prepared = false;
   return VES3D_SUCCESS;

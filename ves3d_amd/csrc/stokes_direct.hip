@@ -994,6 +994,18 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
   if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
+int far_field_sl_dl(bool f32c, const double *spos, const double *sld, const double *dld,
+                    size_t n_src, const double *tpos, double *tvel, size_t n_trg, size_t spg,
+                    size_t tpg, bool guard, hipStream_t st) {
+  const double csl = 1.0 / (8.0 * kPi), cdl = 3.0 / (4.0 * kPi);
+  const double eps2 = guard ? kEvalEps2 : 0.0;
+  if (f32c)
+    return far_eval<true>(spos, sld, dld, n_src, tpos, tvel, n_trg, spg, tpg, eps2, csl, cdl, st,
+                          guard);
+  return far_eval<false>(spos, sld, dld, n_src, tpos, tvel, n_trg, spg, tpg, eps2, csl, cdl, st,
+                         guard);
+}
+
 int direct_eval_f64_fixed(const double *src_pos, const double *sl_den, const double *dl_den,
                           size_t n_src, const double *trg_pos, double *trg_vel, size_t n_trg,
                           size_t spg, size_t tpg, double *part, size_t part_bytes,

@@ -5,6 +5,8 @@ Arguments are torch tensors on the GPU in the reference layouts; results are
 written in place.  Each wrapper checks shapes and dtypes on the host before
 the launch.
 """
+import ctypes
+
 import torch
 
 from . import call, ptr, stream_ptr
@@ -73,3 +75,48 @@ def direct_eval(src_pos, sl_den, dl_den, trg_pos, trg_vel, src_per_group=0,
     call("ves3d_hip_direct_eval_" + ("f32c" if mixed else "f64"), ptr(src_pos),
          ptr(sl_den), ptr(dl_den), ns, ptr(trg_pos), ptr(trg_vel), nt,
          src_per_group, trg_per_group, stream_ptr(stream))
+
+
+class StokesVelocity:
+    """StokesVelocity<Real> (src/StokesVelocity.cc) on the GPU, FP64: the
+    velocity of single- and double-layer densities on the vesicles' surfaces
+    or at targets off them (ves3d_hip_stokes_*)."""
+
+    def __init__(self, nves, p, p_up=0, r_near=0.0):
+        h = ctypes.c_void_p()
+        call("ves3d_hip_stokes_create", p, p_up, nves, r_near, ctypes.byref(h))
+        self.h, self.nves, self.p = h, nves, p
+        self.ng = 2 * p * (p + 1)
+
+    def close(self):
+        if self.h:
+            call("ves3d_hip_stokes_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_src(self, X):
+        _chk(X, torch.float64, 3 * self.nves * self.ng, "X")
+        torch.cuda.synchronize()
+        call("ves3d_hip_stokes_set_src", self.h, ptr(X), None)
+
+    def __call__(self, f_sl=None, f_dl=None, trg=None):
+        """Velocity [nves][3][ng] on the surfaces (trg None) or [n][3] at
+        targets trg [n][3] off the surfaces."""
+        n = 3 * self.nves * self.ng
+        _chk(f_sl, torch.float64, n, "f_sl")
+        _chk(f_dl, torch.float64, n, "f_dl")
+        if trg is None:
+            vel = torch.empty((self.nves, 3, self.ng), dtype=torch.float64, device="cuda")
+            nt = 0
+        else:
+            nt = trg.numel() // 3
+            _chk(trg, torch.float64, 3 * nt, "trg")
+            vel = torch.empty((nt, 3), dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        call("ves3d_hip_stokes_eval", self.h, ptr(f_sl), ptr(f_dl), ptr(trg), nt, ptr(vel), None)
+        return vel
