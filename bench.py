@@ -370,7 +370,7 @@ def _near_cost_per_pair(p):
     nr = N.NearSingular(st.scoord_far, st.trg, 2, 2 * p, ng)
     t_setup = time.perf_counter() - t0
     st.near = nr
-    _, V, q, _ = st.terms(np.random.default_rng(1).uniform(-1, 1, X.shape))
+    _, V, q, _, _, _ = st.terms(np.random.default_rng(1).uniform(-1, 1, X.shape))
     t0 = time.perf_counter()
     st.near_velocity(q, V)
     t_apply = time.perf_counter() - t0

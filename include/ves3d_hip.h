@@ -380,7 +380,9 @@ int ves3d_hip_stokes_eval(void *handle, const double *f_sl, const double *f_dl,
 /* Parameters of the globally implicit interfacial-velocity step
  * (InterfacialVelocity::updateImplicit with solve_for_velocity,
  * src/InterfacialVelocity.cc:203-560): shear background flow
- * u = (shear_rate z, 0, 0), no viscosity contrast, no gravity. */
+ * u = (shear_rate z, 0, 0); per-vesicle properties (viscosity contrast,
+ * excess density under gravity, bending moduli) are set by
+ * ves3d_hip_stepper_set_vesicle_props. */
 typedef struct ves3d_step_params {
   size_t nves;            /* number of vesicles */
   int p;                  /* SH order of the surfaces */
@@ -543,7 +545,8 @@ int ves3d_hip_stepper_area_volume(void *handle, int *iters_host);
  * self operators are then built at every step, and that step's far field
  * runs over the upsampled sources whatever far_coarse says.  Until called,
  * every vesicle has the step parameters' bending modulus, viscosity
- * contrast 1 and no excess density. */
+ * contrast 1 and no excess density.  A negative viscosity contrast is
+ * VES3D_INVALID_PARAMETER (nothing changes). */
 int ves3d_hip_stepper_set_vesicle_props(void *handle, const double *bending_modulus_host,
                                         const double *viscosity_contrast_host,
                                         const double *excess_density_host,

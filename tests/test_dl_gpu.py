@@ -13,7 +13,8 @@ Bars (relative to the oracle output's max-norm unless stated):
   step velocity with a contrast                        1e-5 vs the oracle's
                                                         solve to 1e-10
   mixed-precision matvec with a contrast               2e-5 (L2)
-  rigid translation with a contrast (matvec (U,0) = (U,0))  1e-5
+  rigid translation with a contrast: GPU vs oracle 2e-8; matvec (U,0) =
+  (U,0) to 1e-3 (p = 8, a 0.1 gap: the near scheme's accuracy)
   constant DL density at p = 16: -c/2 on the own surface 1e-9, 0 on the
   other vesicle's 1e-5, -c inside 1e-5, 0 outside 1e-4 (absolute, |c| ~ 1)
 """
@@ -140,7 +141,11 @@ def test_viscosity_contrast_step_vs_oracle(hip):
             ng = X.shape[2]
             xu = orc.pack(np.repeat(U[:, :, None], ng, axis=2), np.zeros((nv, ng)))
             yu = st.matvec(cu(xu)).cpu().numpy()
-            assert np.max(np.abs(yu - xu)) < 1e-5 * np.max(np.abs(xu))
+            assert rel(yu, orc.matvec(xu)) < tol
+            # at p = 8 with a 0.1 gap the other vesicle's DL of a constant
+            # vanishes only to the near scheme's accuracy (the oracle's own
+            # deviation is 5.6e-4 here; 1e-5 at p = 12 with a 0.3 gap)
+            assert np.max(np.abs(yu - xu)) < 1e-3 * np.max(np.abs(xu))
         else:
             e = np.linalg.norm(y - y0) / np.linalg.norm(y0)
             assert e < 2e-5, e

@@ -61,7 +61,7 @@ def test_near_pairs_repulsion_velocity(hip, p, seed):
         torch.cuda.synchronize()
         assert rel(f.cpu().numpy().reshape(nv, 3, ng), st.force_repul()) < 1e-9
         dens = np.random.default_rng(seed).uniform(-1, 1, X.shape)
-        s_vel, V, q, far = st.terms(dens)
+        s_vel, V, q, _, _, far = st.terms(dens)
         vu, vp = st.self_velocity_up(V)
         want = np.transpose(st.near_velocity(q, V).reshape(nv, ng, 3), (0, 2, 1))
         vel = torch.zeros(nv * 3 * ng, dtype=torch.float64, device="cuda")

@@ -68,8 +68,8 @@ def test_props_and_gravity_vs_oracle(hip):
     # gravity alone moves the heavy vesicles down
     assert np.mean(U[3, 2]) < np.mean(U[0, 2])
     with pytest.raises(Ves3dError) as ei:
-        st.set_vesicle_props(viscosity_contrast=[1.0, 2.0, 1.0, 1.0])
-    assert ei.value.code == 7
+        st.set_vesicle_props(viscosity_contrast=[1.0, -2.0, 1.0, 1.0])
+    assert ei.value.code == 10
     st.close()
 
 
@@ -91,7 +91,7 @@ def test_polydisperse_sedimentation_step(hip):
         _, u, sig = st.get_state()
         st.close()
         chk = I.Stepper(nv, p, precision=0, precond=1, **common)
-        chk.set_vesicle_props(bending_modulus=kb, excess_density=rho)
+        chk.set_vesicle_props(bending_modulus=kb, viscosity_contrast=vc, excess_density=rho)
         chk.set_state(cu(X), u, sig)
         r64 = chk.residual()
         chk.close()
