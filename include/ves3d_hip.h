@@ -713,6 +713,14 @@ int ves3d_hip_lpc_samples(void *handle, size_t *n_host);
  * also zeroes the Adam moments and step count. */
 int ves3d_hip_lpc_set_params(void *handle, const float *params);
 int ves3d_hip_lpc_get_params(void *handle, float *params);
+/* bf16 != 0: every product of the forward and backward passes takes BF16
+ * operands (the activations, and the weights rounded to nearest even from
+ * the FP32 master copy after each update) on BF16 MFMA tiles
+ * (v_mfma_f32_16x16x32_bf16) with FP32 accumulation; pre-activations,
+ * biases, gradients and Adam stay FP32.  0 (the default): FP32 activations
+ * on FP32 MFMA tiles.  BF16 needs nin and width multiples of 8
+ * (VES3D_INVALID_PARAMETER otherwise). */
+int ves3d_hip_lpc_set_activations(void *handle, int bf16);
 /* Y [B][nin] = g(Z [B][nin]). */
 int ves3d_hip_lpc_forward(void *handle, const float *Z, size_t B, float *Y,
                           ves3d_stream_t stream);

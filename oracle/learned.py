@@ -72,13 +72,14 @@ def tension_gauge(p, tcut):
     return keep
 
 
-def precond(imp, params, x, F, tcut=None):
+def precond(imp, params, x, F, tcut=None, bf16_ops=False):
+    """bf16_ops: the network's BF16-activation form (oracle/mlp.py)."""
     p, nc = imp.p, imp.nc
     tcut = 2 * p // 3 if tcut is None else tcut
     z = rows(imp, imp.precond(x))
     s = np.sqrt(np.sum(z * z, axis=1)) + 1e-300
     zc = rotate_rows(z, F, nc, True) / s[:, None]
-    g = 0.5 * (mlp.forward(params, zc)[0] - mlp.forward(params, -zc)[0])
+    g = 0.5 * (mlp.forward(params, zc, bf16_ops)[0] - mlp.forward(params, -zc, bf16_ops)[0])
     g = rotate_rows(g, F, nc, False) * s[:, None]
     g[:, ~tension_gauge(p, tcut)] = 0.0
     return unrows(imp, z + g)

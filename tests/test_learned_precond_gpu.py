@@ -3,13 +3,15 @@ oracle (oracle/learned.py), and what the trained network gains.
 
 Tolerances: canonical frames 1e-12 absolute; local operator 1e-9 relative
 (the FP64 matvec's own bar); P(r) 1e-5 relative to max|P(r)| (the network
-runs in FP32); the inner-solve preconditioner inverts the local operator to
+runs in FP32), 2e-3 with BF16 activations against the oracle's BF16 form
+(oracle/mlp.py); the inner-solve preconditioner inverts the local operator to
 its 1e-12 inner tolerance (1e-9 against the oracle's operator).  Gain: after
 training on solver-generated pairs from one suspension, the learned solve of
 another suspension takes at most 3/4 of the diagonal-preconditioned
 iterations, the learned inner solves take fewer iterations than diagonal
 ones, every form reaches true relative residual 1e-6, and every velocity is
-within 1e-5 of a tightly converged FP64 solve.
+within 1e-5 of a tightly converged FP64 solve -- with FP32 and with BF16
+activations.
 """
 import numpy as np
 import pytest
@@ -98,6 +100,13 @@ def test_precond_learned_vs_oracle(hip):
     y = st.precond_learned(xg).cpu().numpy()
     want = L.precond(orc, p32, x, L.frames(X, p))
     assert rel(y, want) < 1e-5
+    # BF16 activations against the oracle's BF16 form
+    pc.set_activations("bf16")
+    yb = st.precond_learned(xg).cpu().numpy()
+    wantb = L.precond(orc, p32, x, L.frames(X, p), bf16_ops=True)
+    print("bf16 activations: vs BF16 oracle %.2e, vs FP64 %.2e" % (rel(yb, wantb), rel(yb, want)),
+          flush=True)
+    assert rel(yb, wantb) < 2e-3
     # the gauge: tension coefficients above the divergence filter keep D r
     keep = np.tile(L.tension_gauge(p, 2 * p // 3), (nv, 1))
     d = L.rows(orc, orc.precond(x))
@@ -106,12 +115,15 @@ def test_precond_learned_vs_oracle(hip):
     pc.close()
 
 
-def test_trained_network_gain(hip):
-    """Trained on pairs from one suspension, evaluated on another."""
+@pytest.mark.parametrize("activations", ["fp32", "bf16"])
+def test_trained_network_gain(hip, activations):
+    """Trained on pairs from one suspension, evaluated on another, with FP32
+    or BF16 activations."""
     nv, p = 8, 8
     common = dict(dt=1e-3, shear_rate=1.0, tol=1e-6, max_iter=300, post=False)
     nin = 4 * p * (p + 2)
     pc = I.LearnedPC(nin, 256, capacity=8192, seed=7)
+    pc.set_activations(activations)
     train = I.Stepper(nv, p, precision=0, precond=1, **common)
     I.attach(train, pc, collect=True)
     for seed in (41, 42, 43):

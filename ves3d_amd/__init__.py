@@ -144,6 +144,7 @@ _sig("ves3d_hip_lpc_num_params", P, P)
 _sig("ves3d_hip_lpc_samples", P, P)
 _sig("ves3d_hip_lpc_set_params", P, P)
 _sig("ves3d_hip_lpc_get_params", P, P)
+_sig("ves3d_hip_lpc_set_activations", P, c_int)
 _sig("ves3d_hip_lpc_forward", P, P, S, P, P)
 _sig("ves3d_hip_lpc_loss_grad", P, P, P, S, P, P, P)
 _sig("ves3d_hip_lpc_adam", P, ctypes.c_float, P)

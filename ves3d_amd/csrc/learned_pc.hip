@@ -86,6 +86,127 @@ __global__ void k_colsum(const float *__restrict__ D, long B, int w, float *g) {
     g[c] = s;
   }
 }
+// ---- BF16 activations on MFMA
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// y = bf16(x), round to nearest even
+__global__ void k_to_bf16(const float *__restrict__ x, long n, __bf16 *y) {
+  GL(i, n) y[i] = (__bf16)x[i];
+}
+// A += b (FP32, kept for the backward pass); H = bf16(gelu(A))
+__global__ void k_bias_gelu_bf(float *A, const float *__restrict__ b, long B, int w, __bf16 *H) {
+  GL(i, B * w) {
+    const float a = A[i] + b[i % w];
+    A[i] = a;
+    H[i] = (__bf16)gelu_f(a);
+  }
+}
+// dA *= gelu'(A) (FP32, for the bias gradient); dAh = bf16(dA)
+__global__ void k_gelu_back_bf(float *dA, const float *__restrict__ A, long n, __bf16 *dAh) {
+  GL(i, n) {
+    const float g = dA[i] * gelu_g(A[i]);
+    dA[i] = g;
+    dAh[i] = (__bf16)g;
+  }
+}
+
+// C[M][N] (FP32, row stride ldc) = sum_k A(m, k) B(k, n) with BF16 operands
+// on v_mfma_f32_16x16x32_bf16 and FP32 accumulation.
+//   A(m, k) = A[m lda + k] (AK: k contiguous) or A[k lda + m];
+//   B(k, n) = B[n ldb + k] (BK: k contiguous) or B[k ldb + n].
+// A workgroup of 4 waves computes a 64 x 64 tile, each wave 32 x 32 as 2 x 2
+// MFMA tiles, over K slabs of 32 staged through LDS with k contiguous, so
+// that a lane's 8 operands are one 16-byte LDS read: lane l supplies row
+// (l % 16) of A and column (l % 16) of B at the slab's k = 8 (l / 16) .. +7,
+// the same k for both operands (the sum does not depend on which k the
+// instruction assigns to which register, only on A and B agreeing), and
+// holds results of rows 4 (l / 16) + r, column l % 16 (r = 0..3).  Global
+// loads are 16-byte vectors along the contiguous direction; the host
+// requires that extent and the leading dimension to be multiples of 8 and
+// both bases 16-byte aligned, so a vector is either whole in range or whole
+// out of it (then zero).
+constexpr int kBT = 64, kBK = 32, kBPad = 8;
+template <bool AK, bool BK>
+__global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K,
+                                                   const __bf16 *__restrict__ A, long lda,
+                                                   const __bf16 *__restrict__ B, long ldb,
+                                                   float *__restrict__ C, long ldc) {
+  __shared__ __attribute__((aligned(16))) __bf16 As[kBT][kBK + kBPad];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[kBT][kBK + kBPad];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const long m0 = (long)blockIdx.y * kBT, n0 = (long)blockIdx.x * kBT;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  const int li = lane & 15, lg = lane >> 4;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const bf16x8 zero8 = {};
+  for (int k0 = 0; k0 < K; k0 += kBK) {
+    // one 8-vector of each operand per thread: 64 x 32 = 256 vectors
+    if constexpr (AK) {
+      const int m = tid >> 2, kk = (tid & 3) * 8;
+      const long gm = m0 + m, gk = k0 + kk;
+      const bf16x8 v = (gm < M && gk < K) ? *(const bf16x8 *)(A + gm * lda + gk) : zero8;
+      *(bf16x8 *)&As[m][kk] = v;
+    } else {
+      const int kk = tid >> 3, mm = (tid & 7) * 8;
+      const long gm = m0 + mm, gk = k0 + kk;
+      const bf16x8 v = (gm < M && gk < K) ? *(const bf16x8 *)(A + gk * lda + gm) : zero8;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) As[mm + u][kk] = v[u];
+    }
+    if constexpr (BK) {
+      const int n = tid >> 2, kk = (tid & 3) * 8;
+      const long gn = n0 + n, gk = k0 + kk;
+      const bf16x8 v = (gn < N && gk < K) ? *(const bf16x8 *)(B + gn * ldb + gk) : zero8;
+      *(bf16x8 *)&Bs[n][kk] = v;
+    } else {
+      const int kk = tid >> 3, nn = (tid & 7) * 8;
+      const long gn = n0 + nn, gk = k0 + kk;
+      const bf16x8 v = (gn < N && gk < K) ? *(const bf16x8 *)(B + gk * ldb + gn) : zero8;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) Bs[nn + u][kk] = v[u];
+    }
+    __syncthreads();
+    bf16x8 a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) a[i] = *(const bf16x8 *)&As[wm + 16 * i + li][8 * lg];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) b[j] = *(const bf16x8 *)&Bs[wn + 16 * j + li][8 * lg];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long gm = m0 + wm + 16 * i + 4 * lg + r, gn = n0 + wn + 16 * j + li;
+        if (gm < M && gn < N) C[gm * ldc + gn] = acc[i][j][r];
+      }
+}
+
+template <bool AK, bool BK>
+int gemm_bf16(int M, int N, int K, const __bf16 *A, long lda, const __bf16 *B, long ldb, float *C,
+              long ldc, hipStream_t st) {
+  if (M < 1 || N < 1 || K < 1) return VES3D_INVALID_PARAMETER;
+  if ((AK ? K : M) % 8 || lda % 8 || (BK ? K : N) % 8 || ldb % 8) return VES3D_INVALID_PARAMETER;
+  if (((uintptr_t)A | (uintptr_t)B) & 15) return VES3D_INVALID_PARAMETER;
+  const long gy = (M + kBT - 1) / kBT;
+  if (gy > 65535) return VES3D_INVALID_PARAMETER;
+  dim3 grid((unsigned)((N + kBT - 1) / kBT), (unsigned)gy);
+  k_gemm_bf16<AK, BK><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
+  return launch_status();
+}
+
 // Adam (Kingma & Ba 2015, Algorithm 1)
 // out = -in
 __global__ void k_negate(const float *__restrict__ in, long n, float *out) {
@@ -279,7 +400,38 @@ LearnedPC::~LearnedPC() {
   if (perm_) (void)hipFree(perm_);
   if (vidx_) (void)hipFree(vidx_);
   if (Pbest_) (void)hipFree(Pbest_);
+  for (__bf16 *p : {Wh_, Zh_, H1h_, H2h_, dYh_, dA2h_, dA1h_})
+    if (p) (void)hipFree(p);
   for (auto &kv : descs_) (void)hipFree(kv.second);
+}
+
+int LearnedPC::set_activations(int mode) {
+  if (mode != 0 && mode != 1) return VES3D_INVALID_PARAMETER;
+  if (mode == 1 && (nin_ % 8 || width_ % 8)) return VES3D_INVALID_PARAMETER;
+  if (mode == 1 && !Wh_) {
+    const long nw = (long)width_ * nin_ * 2 + (long)width_ * width_;
+    if (hipMalloc(&Wh_, sizeof(__bf16) * nw) != hipSuccess) return VES3D_MEMORY_ERROR;
+  }
+  act_ = mode;
+  wdirty_ = true;
+  // reallocate the batch buffers at the current capacity with the new
+  // mode's operand buffers
+  const long b = bcap_;
+  bcap_ = 0;
+  return b > 0 ? ensure_batch(b) : VES3D_SUCCESS;
+}
+
+int LearnedPC::refresh_bf16(hipStream_t st) {
+  if (!wdirty_) return VES3D_SUCCESS;
+  const long w = width_, n = nin_;
+  k_to_bf16<<<gb(w * n), 256, 0, st>>>(P_ + oW1_, w * n, Wh_);
+  V3D_CHECK(launch_status());
+  k_to_bf16<<<gb(w * w), 256, 0, st>>>(P_ + oW2_, w * w, Wh_ + w * n);
+  V3D_CHECK(launch_status());
+  k_to_bf16<<<gb(n * w), 256, 0, st>>>(P_ + oW3_, n * w, Wh_ + w * n + w * w);
+  V3D_CHECK(launch_status());
+  wdirty_ = false;
+  return VES3D_SUCCESS;
 }
 
 int LearnedPC::init(int nin, int width, long capacity, unsigned long long seed) {
@@ -329,6 +481,15 @@ int LearnedPC::ensure_batch(long B) {
   for (float **p : nb)
     if (hipMalloc(p, sizeof(float) * B * n) != hipSuccess) return VES3D_MEMORY_ERROR;
   if (hipMalloc(&scale_, sizeof(double) * B) != hipSuccess) return VES3D_MEMORY_ERROR;
+  for (__bf16 *p : {Zh_, H1h_, H2h_, dYh_, dA2h_, dA1h_})
+    if (p) (void)hipFree(p);
+  Zh_ = H1h_ = H2h_ = dYh_ = dA2h_ = dA1h_ = nullptr;
+  if (act_ == 1) {
+    for (__bf16 **p : {&H1h_, &H2h_, &dA2h_, &dA1h_})
+      if (hipMalloc(p, sizeof(__bf16) * B * w) != hipSuccess) return VES3D_MEMORY_ERROR;
+    for (__bf16 **p : {&Zh_, &dYh_})
+      if (hipMalloc(p, sizeof(__bf16) * B * n) != hipSuccess) return VES3D_MEMORY_ERROR;
+  }
   bcap_ = B;
   return VES3D_SUCCESS;
 }
@@ -362,6 +523,23 @@ int LearnedPC::gemm(int M, int N, int K, const float *A, long ars, long acs, con
 int LearnedPC::forward(const float *Z, long B, float *Y, hipStream_t st) {
   V3D_CHECK(ensure_batch(B));
   const int w = width_, n = nin_;
+  if (act_ == 1) {
+    // BF16 operands (weights rounded from the FP32 master copy), FP32
+    // accumulation, FP32 pre-activations and output
+    V3D_CHECK(refresh_bf16(st));
+    const __bf16 *W1h = Wh_, *W2h = Wh_ + (long)w * n, *W3h = W2h + (long)w * w;
+    k_to_bf16<<<gb(B * n), 256, 0, st>>>(Z, B * n, Zh_);
+    V3D_CHECK(launch_status());
+    V3D_CHECK((gemm_bf16<true, true>((int)B, w, n, Zh_, n, W1h, n, A1_, w, st)));
+    k_bias_gelu_bf<<<gb(B * w), 256, 0, st>>>(A1_, P_ + ob1_, B, w, H1h_);
+    V3D_CHECK(launch_status());
+    V3D_CHECK((gemm_bf16<true, true>((int)B, w, w, H1h_, w, W2h, w, A2_, w, st)));
+    k_bias_gelu_bf<<<gb(B * w), 256, 0, st>>>(A2_, P_ + ob2_, B, w, H2h_);
+    V3D_CHECK(launch_status());
+    V3D_CHECK((gemm_bf16<true, true>((int)B, n, w, H2h_, w, W3h, w, Y, n, st)));
+    k_add_bias<<<gb(B * n), 256, 0, st>>>(Y, P_ + ob3_, B, n);
+    return launch_status();
+  }
   V3D_CHECK(gemm((int)B, w, n, Z, n, 1, P_ + oW1_, 1, n, A1_, st));   // Z W1^T
   k_bias_gelu<<<gb(B * w), 256, 0, st>>>(A1_, P_ + ob1_, B, w, H1_);
   V3D_CHECK(launch_status());
@@ -383,6 +561,34 @@ int LearnedPC::loss_grad(const float *Z, const float *T, long B, double *loss_ho
   V3D_CHECK(launch_status());
   k_sum_parts<<<1, 64, 0, st>>>(lossbuf_, nb, 0.5 / (double)tot, lossbuf_ + 4100);
   V3D_CHECK(launch_status());
+  if (act_ == 1) {
+    // BF16 operands of every product, FP32 gradients and bias sums
+    const __bf16 *W2h = Wh_ + (long)w * n, *W3h = W2h + (long)w * w;
+    const int Bi = (int)B;
+    k_to_bf16<<<gb(tot), 256, 0, st>>>(dY_, tot, dYh_);
+    V3D_CHECK(launch_status());
+    V3D_CHECK((gemm_bf16<false, false>(n, w, Bi, dYh_, n, H2h_, w, G_ + oW3_, w, st)));
+    k_colsum<<<gb(n), 256, 0, st>>>(dY_, B, n, G_ + ob3_);
+    V3D_CHECK(launch_status());
+    V3D_CHECK((gemm_bf16<true, false>(Bi, w, n, dYh_, n, W3h, w, dA2_, w, st)));
+    k_gelu_back_bf<<<gb(B * w), 256, 0, st>>>(dA2_, A2_, B * w, dA2h_);
+    V3D_CHECK(launch_status());
+    V3D_CHECK((gemm_bf16<false, false>(w, w, Bi, dA2h_, w, H1h_, w, G_ + oW2_, w, st)));
+    k_colsum<<<gb(w), 256, 0, st>>>(dA2_, B, w, G_ + ob2_);
+    V3D_CHECK(launch_status());
+    V3D_CHECK((gemm_bf16<true, false>(Bi, w, w, dA2h_, w, W2h, w, dA1_, w, st)));
+    k_gelu_back_bf<<<gb(B * w), 256, 0, st>>>(dA1_, A1_, B * w, dA1h_);
+    V3D_CHECK(launch_status());
+    V3D_CHECK((gemm_bf16<false, false>(w, n, Bi, dA1h_, w, Zh_, n, G_ + oW1_, n, st)));
+    k_colsum<<<gb(w), 256, 0, st>>>(dA1_, B, w, G_ + ob1_);
+    V3D_CHECK(launch_status());
+    if (loss_host) {
+      V3D_HIP(hipMemcpyAsync(loss_host, lossbuf_ + 4100, sizeof(double), hipMemcpyDeviceToHost,
+                             st));
+      V3D_HIP(hipStreamSynchronize(st));
+    }
+    return VES3D_SUCCESS;
+  }
   // layer 3
   V3D_CHECK(gemm(n, w, (int)B, dY_, 1, n, H2_, w, 1, G_ + oW3_, st));  // dY^T H2
   k_colsum<<<gb(n), 256, 0, st>>>(dY_, B, n, G_ + ob3_);
@@ -419,6 +625,7 @@ int LearnedPC::adam(float lr, hipStream_t st) {
   ++t_;
   const float c1 = 1.0f - std::pow(0.9f, (float)t_), c2 = 1.0f - std::pow(0.999f, (float)t_);
   k_adam<<<gb(np_), 256, 0, st>>>(P_, G_, M_, V_, np_, lr, c1, c2, wd_);
+  wdirty_ = true;
   return launch_status();
 }
 
@@ -505,6 +712,7 @@ int LearnedPC::train(int epochs, int batch, float lr, double *last_loss, hipStre
     }
   }
   V3D_HIP(hipMemcpyAsync(P_, Pbest_, sizeof(float) * np_, hipMemcpyDeviceToDevice, st));
+  wdirty_ = true;
   if (last_loss) *last_loss = best;
   trained_ = true;
   return hip_status(hipStreamSynchronize(st));

@@ -8,10 +8,11 @@
 // preconditioner and, per vesicle v, z_v = (D r)_v (velocity then tension
 // coefficients, 4 p(p+2) values) and s_v = |z_v|:
 //     P(r)_v = z_v + s_v * g(z_v / s_v),
-// g a 3-layer MLP (nin -> width -> width -> nin, tanh-GELU, FP32 weights).
-// g is trained so that P(A e) ~ e, i.e. on inputs z/s and targets
-// (e_v - z_v)/s.  The solve's recursive and true residuals decide whether
-// the result is used (the stepper falls back to unpreconditioned BiCGStab).
+// g a 3-layer MLP (nin -> width -> width -> nin, tanh-GELU, FP32 weights,
+// FP32 or BF16 activations).  g is trained so that P(A e) ~ e, i.e. on
+// inputs z/s and targets (e_v - z_v)/s.  The solve's recursive and true
+// residuals decide whether the result is used (the stepper falls back to
+// the diagonal preconditioner).
 #pragma once
 #include <map>
 #include <tuple>
@@ -68,9 +69,19 @@ class LearnedPC {
   // zeroes the Adam moments and step count (new parameters start afresh)
   int reset_optimizer();
   void set_ready(bool r) { trained_ = r; }
+  // the parameters were written from outside (the BF16 weight copy is stale)
+  void params_changed() { wdirty_ = true; }
+  // 0: FP32 activations on FP32 MFMA tiles; 1: BF16 operands of every
+  // product (activations, and the weights rounded from the FP32 master copy
+  // after each update) on BF16 MFMA with FP32 accumulation; pre-activations,
+  // gradients and the optimizer stay FP32.  Needs nin and width multiples
+  // of 8.
+  int set_activations(int mode);
+  int activations() const { return act_; }
 
  private:
   int ensure_batch(long B);
+  int refresh_bf16(hipStream_t st);
   int gemm(int M, int N, int K, const float *A, long ars, long acs, const float *Bm, long brs,
            long bcs, float *C, hipStream_t st);
 
@@ -92,6 +103,11 @@ class LearnedPC {
   bool trained_ = false;
   unsigned long long seed_ = 0;
   std::map<std::tuple<int, int, int, long, long, long, long>, GemmDesc *> descs_;
+  int act_ = 0;
+  bool wdirty_ = true;
+  // BF16 weights [W1 | W2 | W3] and activation / gradient operands
+  __bf16 *Wh_ = nullptr, *Zh_ = nullptr, *H1h_ = nullptr, *H2h_ = nullptr, *dYh_ = nullptr,
+         *dA2h_ = nullptr, *dA1h_ = nullptr;
 };
 
 }  // namespace v3d

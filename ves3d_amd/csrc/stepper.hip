@@ -1287,8 +1287,14 @@ int ves3d_hip_lpc_set_params(void *h, const float *params) {
   LearnedPC *l = (LearnedPC *)h;
   V3D_HIP(hipMemcpy(l->params(), params, sizeof(float) * l->num_params(), hipMemcpyDeviceToDevice));
   V3D_CHECK(l->reset_optimizer());
+  l->params_changed();
   l->set_ready(true);
   return VES3D_SUCCESS;
+}
+
+int ves3d_hip_lpc_set_activations(void *h, int bf16) {
+  if (!h) return VES3D_INVALID_PARAMETER;
+  return ((LearnedPC *)h)->set_activations(bf16 ? 1 : 0);
 }
 
 int ves3d_hip_lpc_get_params(void *h, float *params) {

@@ -242,6 +242,7 @@ class LearnedPC:
         call("ves3d_hip_lpc_create", nin, width, capacity, seed, ctypes.byref(h))
         self.h = h
         self.nin, self.width = nin, width
+        self.activations = "fp32"
         n = ctypes.c_size_t()
         call("ves3d_hip_lpc_num_params", self.h, ctypes.byref(n))
         self.num_params = n.value
@@ -275,6 +276,14 @@ class LearnedPC:
     def set_params(self, p):
         torch.cuda.synchronize()
         call("ves3d_hip_lpc_set_params", self.h, ptr(p))
+
+    def set_activations(self, kind):
+        """"fp32" (FP32 MFMA tiles) or "bf16" (BF16 operands, FP32
+        accumulation, FP32 master weights)."""
+        if kind not in ("fp32", "bf16"):
+            raise ValueError(kind)
+        call("ves3d_hip_lpc_set_activations", self.h, 1 if kind == "bf16" else 0)
+        self.activations = kind
 
     def samples(self):
         n = ctypes.c_size_t()
