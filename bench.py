@@ -81,16 +81,69 @@ def _log(*a):
 
 # ---------------------------------------------------------------- step
 def _train_pc(args, p, common):
-    """The learned preconditioner trained (untimed) on pairs that plain solves
-    of separate training suspensions generate: args.train_nves vesicles of
-    the same shape family at volume fractions 10% .. 25%.  Returns (pc,
-    fracs, collect seconds, train seconds, samples, loss)."""
+    """The learned preconditioner, untimed.  By default the stored network
+    (ves3d_amd/weights, trained by `python -m ves3d_amd.training` on the
+    first 200 plain-solve steps of 256-vesicle suspensions at 10-25% volume
+    fraction); with --pc-weights train, or when none is stored for (p,
+    width), trained in this run on pairs that plain solves of
+    args.train_steps separate training suspensions generate (args.train_nves
+    vesicles, 10% .. 25%).  Returns (pc, record) with the record of how the
+    network was made."""
     from ves3d_amd import training as T
+    if args.pc_weights != "train":
+        pc, meta = T.load_stored(p, args.width, args.activations,
+                                 None if args.pc_weights == "stored" else args.pc_weights)
+        if pc is not None:
+            _log("stored network: %d plain-solve steps, %d samples, held-out loss %.3e" %
+                 (meta["plain_solve_steps"], meta["samples"], meta["heldout_loss"]))
+            rec = {"source": "stored (%s)" % os.path.relpath(
+                       (None if args.pc_weights == "stored" else args.pc_weights)
+                       or T.weights_path(p, args.width), os.path.dirname(os.path.abspath(__file__))),
+                   "activations": args.activations}
+            rec.update({k: meta[k] for k in (
+                "training_suspensions", "plain_solve_steps", "step", "samples", "samples_kept",
+                "epochs", "batch", "heldout_loss", "dataset_rel_fit", "collect_seconds",
+                "train_seconds", "activations", "command") if k in meta})
+            rec["trained_with_activations"] = meta.get("activations")
+            rec["activations"] = args.activations
+            return pc, rec
+        if args.pc_weights != "stored":
+            raise FileNotFoundError(args.pc_weights + ".npy")
+        _log("no stored network for p=%d width=%d: training in this run" % (p, args.width))
     fracs = np.linspace(0.10, 0.25, max(1, args.train_steps))
     pc, info = T.train_learned_pc(args.train_nves, p, fracs, common, width=args.width,
                                   capacity=args.capacity, epochs=args.epochs, lr=args.lr,
-                                  log=_log)
-    return pc, fracs, info["collect_s"], info["train_s"], info["samples"], info["loss"]
+                                  log=_log, activations=args.activations)
+    _log("trained on %d samples: loss %.3e in %.1f s" % (info["samples"], info["loss"],
+                                                         info["train_s"]))
+    return pc, {"source": "trained in this run", "activations": args.activations,
+                "training_suspensions": "%d x %d ves at %s%% volume fraction" % (
+                    len(fracs), args.train_nves, "/".join("%.0f" % (100 * f) for f in fracs)),
+                "plain_solve_steps": info["steps"], "samples": int(info["samples"]),
+                "epochs": args.epochs, "heldout_loss": info["loss"],
+                "collect_seconds": info["collect_s"], "train_seconds": info["train_s"]}
+
+
+def _network_bf16_error(I, nv, p, Xg, kw, pc):
+    """P(b) of the learned preconditioner with BF16 against FP32 activations
+    for the first step's right-hand side b: relative to |P(b)| and to the
+    network's own correction |P(b) - D b|."""
+    st = I.Stepper(nv, p, **kw)
+    I.attach(st, pc, collect=False)
+    st.set_state(Xg)
+    st.prepare()
+    b = st.rhs()
+    act = pc.activations
+    out = {}
+    for kind in ("bf16", "fp32"):
+        pc.set_activations(kind)
+        out[kind] = st.precond_learned(b).double()
+    d = st.precond(b).double()
+    pc.set_activations(act)
+    st.close()
+    diff = (out["bf16"] - out["fp32"]).norm().item()
+    return {"rel_to_P": diff / out["fp32"].norm().item(),
+            "rel_to_correction": diff / (out["fp32"] - d).norm().item()}
 
 
 def _run_form(I, nv, p, X0g, kw, pc, nsteps, name, check=None, keep_first=False):
@@ -144,8 +197,7 @@ def bench_step(args):
     common = dict(dt=1e-3, shear_rate=1.0, bending_modulus=1e-2, tol=1e-6, max_iter=args.max_iter,
                   r_near=args.r_near)
     nin = 4 * p * (p + 2)
-    pc, fracs, collect_s, train_s, n_samples, loss = _train_pc(args, p, common)
-    _log("trained on %d samples: loss %.3e in %.1f s" % (n_samples, loss, train_s))
+    pc, pc_rec = _train_pc(args, p, common)
     W_, K = args.warmup, args.steps
     timed = dict(common, extrapolate=args.extrapolate)
     # the FP64 operator with the exact far field judges every learned step
@@ -202,6 +254,7 @@ def bench_step(args):
     vel_err = float(torch.linalg.norm(first_learned[0] - Ur) / nr)
     vel_err_plain = float(torch.linalg.norm(first_plain[0] - Ur) / nr)
     check.close()
+    pc_rec["bf16_vs_fp32_on_first_rhs"] = _network_bf16_error(I, nv, p, Xg, learned_kw, pc)
 
     ms_mixed = _mean(learned_t, "ms_total")
     ms_plain = _mean(plain_t, "ms_total")
@@ -276,11 +329,8 @@ def bench_step(args):
         "fell_back": [s["fell_back"] for s in learned_t],
         "velocity_rel_err_vs_fp64": vel_err, "velocity_rel_err_plain_vs_fp64": vel_err_plain,
         "fp64_reference_relres": sr["true_relres"],
-        "pc_train_seconds": train_s, "pc_collect_seconds": collect_s,
-        "pc_train_suspensions": "%d x %d ves at %s%% volume fraction" % (
-            len(fracs), args.train_nves, "/".join("%.0f" % (100 * f) for f in fracs)),
+        "network": pc_rec,
         "near_pairs": [s["near_pairs"] for s in plain_t], "r_near": plain_t[-1]["r_near"],
-        "pc_train_samples": int(n_samples), "pc_train_loss": loss,
         "stage_ms_plain": {k: _mean(plain_t, k) for k in ("ms_setup", "ms_rhs", "ms_solve",
                                                           "ms_post", "ms_near")},
         "stage_ms_learned": {k: _mean(learned_t, k) for k in ("ms_setup", "ms_rhs", "ms_solve",
@@ -709,7 +759,12 @@ def main():
     ap.add_argument("--capacity", type=int, default=200000)
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--train-nves", type=int, default=256)
-    ap.add_argument("--train-steps", type=int, default=2)
+    ap.add_argument("--train-steps", type=int, default=2,
+                    help="training suspensions (one plain step each) with --pc-weights train")
+    ap.add_argument("--pc-weights", default="stored",
+                    help="'stored' (the committed network), a path without suffix, or 'train'")
+    ap.add_argument("--activations", default="bf16", choices=["fp32", "bf16"],
+                    help="the network's activations (BF16 on BF16 MFMA, or FP32)")
     ap.add_argument("--ref-tol", type=float, default=1e-10)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--inner-max-iter", type=int, default=6,

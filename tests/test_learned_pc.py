@@ -189,3 +189,22 @@ def test_gpu_network_bf16_vs_oracle(hip, nin, w, B):
                          Z.astype(np.float64))
     assert _rel(Y2, Y2o) < 1e-5
     pc.close()
+
+
+def test_stored_network_files():
+    """The committed network: FP32 parameters of the p = 16, width 256
+    layout, finite, and a record of the 200 plain-solve steps it was trained
+    on (read with loaders that execute nothing)."""
+    import json
+    import os
+    from ves3d_amd import training as T
+    base = T.weights_path(16, 256)
+    flat = np.load(base + ".npy", allow_pickle=False)
+    nin, w = 4 * 16 * 18, 256
+    assert flat.dtype == np.float32 and flat.shape == (w * nin * 2 + w * w + 2 * w + nin,)
+    assert np.all(np.isfinite(flat))
+    meta = json.load(open(base + ".json"))
+    assert meta["p"] == 16 and meta["width"] == w and meta["nin"] == nin
+    assert meta["plain_solve_steps"] == 200 and len(meta["plain_iterations"]) == 200
+    assert meta["activations"] == "bf16" and meta["samples"] > 500000
+    assert os.path.basename(base) == "lpc_p16_w256"
