@@ -64,8 +64,8 @@ SL_FLOP_PER_PAIR = 28      # src/CPUKernels.cc:1008
 # rocprofv3 --pmc runs (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a
 # 512 MiB device copy gives the two counters in the same runs), keyed by the
 # far_coarse setting they were measured at
-FAR32_TRAFFIC = {3.0: (2.0 * 258954.5 * 1024 + 52224.0 * 1024,
-                       "profiles/k_far_groups_f32c_hbm_counters_512ves_p16.md")}
+FAR32_TRAFFIC = {(3.0, 0.0): (2.0 * 258954.5 * 1024 + 52224.0 * 1024,
+                              "profiles/k_far_groups_f32c_hbm_counters_512ves_p16.md")}
 FAR_KINDS = {"lds": 0, "scalar": 1, "scalar2": 2}
 FAR32_NAMES = {"lds": "k_far_f32c<SL>", "scalar": "k_far_groups_f32c", "scalar2": "k_far_groups_f32c2"}
 DL_FLOP_PER_PAIR = 31      # src/PVFMMInterface.cc:365 (no Newton steps)
@@ -204,7 +204,7 @@ def bench_step(args):
     check = I.Stepper(nv, p, precision=0, precond=1, **common)
     learned_kw = dict(timed, precision=args.mixed_precision, precond=2,
                       inner_max_iter=args.inner_max_iter, inner_tol=args.inner_tol,
-                      far_coarse=args.far_coarse)
+                      far_coarse=args.far_coarse, far_coarse2=args.far_coarse2)
 
     # ---- the two headline forms from the same initial state
     plain, _, first_plain = _run_form(I, nv, p, Xg, dict(timed, precision=0, precond=1), None,
@@ -216,10 +216,13 @@ def bench_step(args):
     att = {}
     if not args.no_attribution:
         for name, kw, use_pc in (
-                ("learned_mixed_exact_far", dict(learned_kw, far_coarse=0.0), True),
+                ("learned_mixed_exact_far", dict(learned_kw, far_coarse=0.0, far_coarse2=0.0),
+                 True),
+                ("learned_mixed_two_level_far", dict(learned_kw, far_coarse2=0.0), True),
                 ("diagonal_inner_mixed", dict(learned_kw, precond=1), False),
-                ("plain_two_level_far", dict(timed, precision=0, precond=1,
-                                             far_coarse=args.far_coarse), False)):
+                ("plain_multi_level_far", dict(timed, precision=0, precond=1,
+                                               far_coarse=args.far_coarse,
+                                               far_coarse2=args.far_coarse2), False)):
             stt, _, _ = _run_form(I, nv, p, Xg, kw, pc if use_pc else None, W_ + K, name)
             stt = stt[W_:]
             att[name] = {"ms_per_step": _mean(stt, "ms_total"),
@@ -268,7 +271,7 @@ def bench_step(args):
     far_ms_plain = sum(s["ms_far"] for s in plain_t) / max(1, sum(s["far_calls"] for s in plain_t))
     achieved = flop / (far_ms * 1e-3) / 1e12
     scalar = args.far_kernel == "scalar" and args.nves == 512 and args.p == 16
-    traffic = FAR32_TRAFFIC.get(args.far_coarse) if scalar else None
+    traffic = FAR32_TRAFFIC.get((args.far_coarse, args.far_coarse2)) if scalar else None
     roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": traffic[0] if traffic else None,
@@ -308,11 +311,13 @@ def bench_step(args):
                    "r_near": plain_t[-1]["r_near"]},
         "ms_per_step_plain": ms_plain, "speedup_vs_plain": ms_plain / ms_mixed,
         "learned_form": "precision %d (FP32-compensated far field%s), learned preconditioner in "
-                        "inner solves (%d iterations, %g), two-level far field (p-grid sources "
-                        "beyond %g radii)" % (args.mixed_precision,
-                                              ", FP32 singular operator" if args.mixed_precision == 1
-                                              else ", FP64 singular operator",
-                                              args.inner_max_iter, args.inner_tol, args.far_coarse),
+                        "inner solves (%d iterations, %g), multi-level far field (p-grid sources "
+                        "beyond %g radii, order-p/2 sources beyond %g)" % (
+                            args.mixed_precision,
+                            ", FP32 singular operator" if args.mixed_precision == 1
+                            else ", FP64 singular operator",
+                            args.inner_max_iter, args.inner_tol, args.far_coarse,
+                            args.far_coarse2),
         "plain_form": "FP64, diagonal spectral preconditioner, upsampled sources for every pair",
         "iterations_learned": [s["iterations"] for s in learned_t],
         "iterations_plain": [s["iterations"] for s in plain_t],
@@ -363,7 +368,8 @@ def bench_trajectory(args):
         "plain": I.Stepper(nv, p, precision=0, precond=1, **common),
         "learned": I.Stepper(nv, p, precision=args.mixed_precision, precond=2,
                              inner_max_iter=args.inner_max_iter, inner_tol=args.inner_tol,
-                             far_coarse=args.far_coarse, **common),
+                             far_coarse=args.far_coarse, far_coarse2=args.far_coarse2,
+                             **common),
     }
     I.attach(forms["learned"], pc, collect=False)
     out, final = {}, {}
@@ -515,11 +521,12 @@ def bench_matvec(args):
     ves3d_amd.call("ves3d_hip_set_far_kernel", FAR_KINDS[args.far_kernel])
     nv, p = args.nves, args.p
     prec = args.mixed_precision if args.dtype == "f32c" else 0
-    # the mixed form's kernel with its two-level far field, the FP64 form's
+    # the mixed form's kernel with its multi-level far field, the FP64 form's
     # over the upsampled sources (as the step workload's two headline forms)
     coarse = args.far_coarse if prec else 0.0
+    coarse2 = args.far_coarse2 if prec else 0.0
     st = I.Stepper(nv, p, dt=1e-3, shear_rate=1.0, precision=prec, precond=1, r_near=args.r_near,
-                   far_coarse=coarse)
+                   far_coarse=coarse, far_coarse2=coarse2)
     st.set_state(torch.from_numpy(W.implicit_step_case(nv=nv, p=p)).cuda())
     st.prepare()
     pairs = st.far_pairs()
@@ -546,7 +553,7 @@ def bench_matvec(args):
         "data": "synthetic",
         "config": {"workload": "implicit-step matvec, 512 perturbed ellipsoids p=16, 20% volume "
                                "fraction", "precision": prec, "far_kernel": args.far_kernel,
-                   "far_coarse": coarse, "far_pairs": pairs},
+                   "far_coarse": coarse, "far_coarse2": coarse2, "far_pairs": pairs},
         "roofline": None, "cpu_baseline": None,
     }
 
@@ -788,6 +795,9 @@ def main():
                     help="learned form: two-level far field, p-grid sources for waves of "
                          "targets beyond this many vesicle radii (0: upsampled sources "
                          "everywhere, as the plain form)")
+    ap.add_argument("--far-coarse2", type=float, default=6.0,
+                    help="learned form: third far-field level, order-p/2 sources for waves "
+                         "beyond this many radii (0: two levels)")
     ap.add_argument("--no-attribution", action="store_true",
                     help="step workload: skip the attribution forms and the near-default line")
     ap.add_argument("--r-near", type=float, default=0.1,

@@ -422,6 +422,13 @@ typedef struct ves3d_step_params {
                              sources for every pair, the reference's far field
                              (src/StokesVelocity.cc:477-490).  No counterpart in the
                              reference, whose FMM makes its own approximation there. */
+  double far_coarse2;     /* > far_coarse (needs far_coarse): three-level far field.
+                             Beyond far_coarse2 radii a wave reads the vesicle at order
+                             p/2: the degree-p surface at the order-p/2 grid points with
+                             the density times the area element (per unit-sphere
+                             measure) truncated to degree p/2 and the order-p/2
+                             Gauss-Legendre weights.  0: two levels.  No counterpart in
+                             the reference. */
 } ves3d_step_params;
 
 typedef struct ves3d_step_stats {

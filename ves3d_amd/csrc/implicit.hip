@@ -263,6 +263,30 @@ __global__ void k_coarse_weights(const double *__restrict__ Xt, const double *__
   }
 }
 
+// out[v][d][g] = f[v][d][g] * w[v][g]
+__global__ void k_mul_pointwise(const double *__restrict__ f, const double *__restrict__ w, long nv,
+                                long ng, double *__restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * nv * ng) return;
+  out[i] = f[i] * w[(i / (3 * ng)) * ng + i % ng];
+}
+// c[f][k] *= mask[k] (zeroes the coefficients above a degree)
+__global__ void k_mask_coef(double *__restrict__ c, const double *__restrict__ mask, long n,
+                            long nc) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) c[i] *= mask[i % nc];
+}
+// out[v ng2 + i][d] = grid[v][d][i] * w[latitude of i] (order-q2 grid,
+// 2 q2 longitudes per latitude)
+__global__ void k_pack_level2(const double *__restrict__ grid, const double *__restrict__ w,
+                              long nv, long ng2, int q2, double *__restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv * ng2) return;
+  const long v = i / ng2, k = i % ng2;
+  const double wk = w[k / (2 * q2)];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) out[i * 3 + d] = grid[(v * 3 + d) * ng2 + k] * wk;
+}
 // qc[v ng + g][d] = f[v][d][g] * aqwp[v][g] (point-major coarse densities)
 __global__ void k_pack_coarse(const double *__restrict__ f, const double *__restrict__ aqwp,
                               long nv, long ng, double *__restrict__ qc) {
@@ -533,14 +557,17 @@ int StokesG::far_time(double *ms, int *calls, double *ms2, int *calls2, double *
 }
 
 int StokesG::init(int p_, int pu_, long nv_, int precision_, int near_, double repul_dist,
-                  double r_near, double far_coarse_) {
+                  double r_near, double far_coarse_, double far_coarse2_) {
   p = p_;
   pu = pu_;
   nv = nv_;
   precision = precision_;
   use_near = near_ != 0;
   if (far_coarse_ != 0.0 && !(far_coarse_ > 1.0)) return VES3D_INVALID_PARAMETER;
+  if (far_coarse2_ != 0.0 && !(far_coarse_ > 1.0 && far_coarse2_ > far_coarse_ && p_ >= 4))
+    return VES3D_INVALID_PARAMETER;
   far_coarse = far_coarse_;
+  far_coarse2 = far_coarse2_;
   const long nc = sh_ncoef(p), ng = sh_ngrid(p), mu = sh_ngrid(pu);
   const long nmat = nv * 9 * nc * nc;
   if (precision != 1) {
@@ -561,8 +588,27 @@ int StokesG::init(int p_, int pu_, long nv_, int precision_, int near_, double r
     V3D_CHECK(qc.alloc(nv * ng * 3));
     const long nw = (nv * ng + 63) / 64;
     if (hipMalloc(&cflags, (size_t)nw * nv) != hipSuccess) return VES3D_MEMORY_ERROR;
-    if (hipMalloc(&ccounts, 2 * sizeof(unsigned long long)) != hipSuccess)
+    if (hipMalloc(&ccounts, 3 * sizeof(unsigned long long)) != hipSuccess)
       return VES3D_MEMORY_ERROR;
+  }
+  if (far_coarse2 > 0) {
+    ord2 = p / 2;
+    const long ng2 = sh_ngrid(ord2);
+    V3D_CHECK(c2pos.alloc(nv * ng2 * 3));
+    V3D_CHECK(c2gw.alloc(nv * ng));
+    V3D_CHECK(c2den.alloc(nv * ng2 * 3));
+    V3D_CHECK(c2prod.alloc(3 * nv * ng));
+    V3D_CHECK(c2coef.alloc(3 * nv * nc));
+    V3D_CHECK(c2grid.alloc(3 * nv * ng2));
+    std::vector<double> mask(nc, 0.0), x, w;
+    for (int j = 0; j < 2 * p; ++j) {
+      const int m = (j + 1) / 2;
+      for (int k = 0; k < sh_blocklen(p, j); ++k) mask[sh_blockoff(p, j) + k] = m + k <= ord2;
+    }
+    V3D_CHECK(upload(c2mask, mask));
+    gauss_legendre(ord2 + 1, x, w);
+    for (double &v : w) v *= kPi / ord2;
+    V3D_CHECK(upload(c2w, w));
   }
   if (use_near) {
     V3D_CHECK(near.init(p, pu, nv, repul_dist, r_near));
@@ -571,7 +617,8 @@ int StokesG::init(int p_, int pu_, long nv_, int precision_, int near_, double r
     V3D_CHECK(vpole.alloc(3 * nv * 2));
   }
   return work.alloc(std::max({shc2grid_work(3 * nv, p, pu), grid2shc_work(3 * nv, p, p),
-                              shc2grid_work(3 * nv, p, p)}));
+                              shc2grid_work(3 * nv, p, p),
+                              ord2 ? shc2grid_work(3 * nv, p, ord2) : 0L}));
 }
 
 int StokesG::setup(const double *X, hipStream_t st) {
@@ -604,12 +651,28 @@ int StokesG::setup(const double *X, hipStream_t st) {
     V3D_CHECK(upload(dqwp, qwp));
     k_coarse_weights<<<gblocks(nv * ng), 256, 0, st>>>(tmpt, tmpp, dqwp, nv, p, aqwp);
     V3D_CHECK(launch_status());
+    long ng2 = 0;
+    DBuf dinvs;
+    if (far_coarse2 > 0) {
+      // the area element per unit-sphere measure, |x_t x x_p| / sin(theta),
+      // and the degree-p surface at the order-q2 grid points
+      std::vector<double> x, w, invs(p + 1);
+      gauss_legendre(p + 1, x, w);
+      for (int i = 0; i <= p; ++i) invs[i] = 1.0 / std::sqrt(1.0 - x[i] * x[i]);
+      V3D_CHECK(upload(dinvs, invs));
+      k_coarse_weights<<<gblocks(nv * ng), 256, 0, st>>>(tmpt, tmpp, dinvs, nv, p, c2gw);
+      V3D_CHECK(launch_status());
+      ng2 = sh_ngrid(ord2);
+      V3D_CHECK(sht_shc2grid<double>(shc, 3 * nv, p, ord2, c2grid, nullptr, nullptr, work, st));
+      k_to_points<<<gblocks(nv * ng2), 256, 0, st>>>(c2grid, nv, ng2, c2pos);
+      V3D_CHECK(launch_status());
+    }
     V3D_CHECK(far_coarse_flags(far_src, nv * (mu + 2), mu + 2, trg, nv * ng, ng, far_coarse,
-                               cflags, ccounts, st));
-    unsigned long long c[2];
+                               far_coarse2, cflags, ccounts, st));
+    unsigned long long c[3];
     V3D_HIP(hipMemcpyAsync(c, ccounts, sizeof(c), hipMemcpyDeviceToHost, st));
     V3D_HIP(hipStreamSynchronize(st));
-    far_pairs = 64.0 * ((double)c[0] * ng + (double)c[1] * (mu + 2));
+    far_pairs = 64.0 * ((double)c[0] * ng + (double)c[1] * (mu + 2) + (double)c[2] * ng2);
   } else {
     // every target against every other vesicle's upsampled sources
     far_pairs = (double)(nv * ng) * (double)(nv - 1) * (double)(mu + 2);
@@ -622,6 +685,48 @@ int StokesG::pack_coarse(const double *f, double *qcoarse, hipStream_t st) {
   const long ng = sh_ngrid(p);
   k_pack_coarse<<<gblocks(nv * ng), 256, 0, st>>>(f, aqwp, nv, ng, qcoarse);
   return launch_status();
+}
+
+// Level-2 densities: (f c2gw) truncated to degree ord2, synthesised at the
+// order-ord2 grid points, times the order-ord2 quadrature weights.
+int StokesG::pack_level2(const double *f, double *out, hipStream_t st) {
+  const long ng = sh_ngrid(p), nc = sh_ncoef(p), ng2 = sh_ngrid(ord2);
+  k_mul_pointwise<<<gblocks(3 * nv * ng), 256, 0, st>>>(f, c2gw, nv, ng, c2prod);
+  V3D_CHECK(launch_status());
+  V3D_CHECK(sht_grid2shc<double>(c2prod, 3 * nv, p, p, c2coef, work, st));
+  k_mask_coef<<<gblocks(3 * nv * nc), 256, 0, st>>>(c2coef, c2mask, 3 * nv * nc, nc);
+  V3D_CHECK(launch_status());
+  V3D_CHECK(sht_shc2grid<double>(c2coef, 3 * nv, p, ord2, c2grid, nullptr, nullptr, work, st));
+  k_pack_level2<<<gblocks(nv * ng2), 256, 0, st>>>(c2grid, c2w, nv, ng2, ord2, out);
+  return launch_status();
+}
+
+FarCoarse StokesG::coarse_levels(const double *f1, const double *f2, hipStream_t st, int *err) {
+  const long ng = sh_ngrid(p);
+  FarCoarse fc{trg, qc, nullptr, (size_t)ng, cflags};
+  *err = VES3D_SUCCESS;
+  auto chk = [err](int e) {
+    if (e != VES3D_SUCCESS && *err == VES3D_SUCCESS) *err = e;
+  };
+  if (f2 && !qc2.p) chk(qc2.alloc(nv * ng * 3));
+  chk(pack_coarse(f1, qc, st));
+  if (f2) {
+    fc.den2 = qc2;
+    chk(pack_coarse(f2, qc2, st));
+  }
+  if (far_coarse2 > 0) {
+    const long ng2 = sh_ngrid(ord2);
+    fc.pos2 = c2pos;
+    fc.cpg2 = (size_t)ng2;
+    fc.l2den1 = c2den;
+    chk(pack_level2(f1, c2den, st));
+    if (f2) {
+      if (!c2den2.p) chk(c2den2.alloc(nv * ng2 * 3));
+      fc.l2den2 = c2den2;
+      chk(pack_level2(f2, c2den2, st));
+    }
+  }
+  return fc;
 }
 
 int StokesG::self_and_density(const double *f, double *Fc, double *Vc, double *vel, double *upc,
@@ -756,8 +861,12 @@ int StokesG::apply(const double *f, double *vel, hipStream_t st, bool add_repul,
     V3D_CHECK(near_correct(f ? q.p : nullptr, V, vel, st, true));
     return event_end(st);
   }
-  FarCoarse fc{trg, qc, nullptr, (size_t)ng, cflags};
-  if (far_coarse > 0) V3D_CHECK(pack_coarse(f, qc, st));
+  FarCoarse fc{};
+  if (far_coarse > 0) {
+    int e = VES3D_SUCCESS;
+    fc = coarse_levels(f, nullptr, st, &e);
+    V3D_CHECK(e);
+  }
   V3D_CHECK(event_begin(1, st));
   V3D_CHECK(far_field_between_groups(precision != 0, far_src, q, nv * (mu + 2), trg, farv, nv * ng,
                                      mu + 2, ng, st, far_coarse > 0 ? &fc : nullptr));
@@ -787,12 +896,11 @@ int StokesG::apply2(const double *f1, double *vel1, bool add_repul1, const doubl
   }
   V3D_CHECK(self_and_density(f1, F, V, vel1, up, q, true, st));
   V3D_CHECK(self_and_density(f2, F2, V2, vel2, up2, q2, true, st));
-  FarCoarse fc{trg, qc, qc2, (size_t)ng, cflags};
+  FarCoarse fc{};
   if (far_coarse > 0) {
-    if (!qc2.p) V3D_CHECK(qc2.alloc(nv * ng * 3));
-    fc.den2 = qc2;
-    V3D_CHECK(pack_coarse(f1, qc, st));
-    V3D_CHECK(pack_coarse(f2, qc2, st));
+    int e = VES3D_SUCCESS;
+    fc = coarse_levels(f1, f2, st, &e);
+    V3D_CHECK(e);
   }
   V3D_CHECK(event_begin(2, st));
   V3D_CHECK(far_field_between_groups2(precision != 0, far_src, q, q2, nv * (mu + 2), trg, farv,

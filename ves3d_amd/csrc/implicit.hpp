@@ -18,17 +18,21 @@ namespace v3d {
 // pair loop runs without the coincident-point guard of the direct-evaluation
 // entry points (stokes_direct.hip).
 //
-// coarse (optional): a second, coarser set of sources per group (cpg points
-// each, point-major positions and densities) that a wave of 64 targets uses
-// instead of the fine ones where flags[wave][group] is set (see
-// far_coarse_flags).  The FP32 form stores both levels relative to the
-// group's first fine source.  With coarse sources the scalar-cache kernels
-// run whatever ves3d_hip_set_far_kernel selected.
+// coarse (optional): one or two coarser sets of sources per group (cpg and
+// cpg2 points each, point-major positions and densities) that a wave of 64
+// targets uses instead of the fine ones where flags[wave][group] names
+// level 1 or 2 (see far_coarse_flags).  The FP32 form stores every level
+// relative to the group's first fine source.  With coarse sources the
+// scalar-cache kernels run whatever ves3d_hip_set_far_kernel selected.
 struct FarCoarse {
-  const double *pos;           // [n_groups * cpg][3]
-  const double *den1, *den2;   // densities at the coarse points (den2: the second of a pair)
+  const double *pos;           // level 1: [n_groups * cpg][3]
+  const double *den1, *den2;   // densities at the level-1 points (den2: the second of a pair)
   size_t cpg;
-  const unsigned char *flags;  // [ceil(n_trg / 64)][n_groups]
+  const unsigned char *flags;  // [ceil(n_trg / 64)][n_groups]: the level, 0..2
+  // level 2 (cpg2 = 0: none): [n_groups * cpg2][3] points and densities
+  const double *pos2 = nullptr;
+  const double *l2den1 = nullptr, *l2den2 = nullptr;
+  size_t cpg2 = 0;
 };
 int far_field_between_groups(bool f32c, const double *spos, const double *sld, size_t n_src,
                              const double *tpos, double *tvel, size_t n_trg, size_t spg,
@@ -57,14 +61,15 @@ int direct_eval_f64_fixed(const double *src_pos, const double *sl_den, const dou
                           size_t spg, size_t tpg, double *part, size_t part_bytes,
                           hipStream_t st);
 
-// flags[w][g] = 1 where every target of wave w (targets 64 w .. 64 w + 63) is
-// at least kappa R_g from c_g, (c_g, R_g) the bounding sphere of group g's
-// fine sources (centre: their mean).  flags: ceil(n_trg / 64) * n_groups
-// bytes.  kappa > 1.  counts (device, 2 values): the (wave, group) pairs
-// that read the coarse sources, and those that read the fine ones (a wave
-// skips a group that holds all its targets).
+// flags[w][g] = 2 where every target of wave w (targets 64 w .. 64 w + 63)
+// is at least kappa2 R_g from c_g, else 1 where at least kappa R_g, else 0;
+// (c_g, R_g) the bounding sphere of group g's fine sources (centre: their
+// mean).  flags: ceil(n_trg / 64) * n_groups bytes.  kappa > 1; kappa2 = 0
+// (no level 2) or kappa2 > kappa.  counts (device, 3 values): the (wave,
+// group) pairs that read level 1, those that read the fine sources (a wave
+// skips a group that holds all its targets), and those that read level 2.
 int far_coarse_flags(const double *spos, size_t n_src, size_t spg, const double *tpos,
-                     size_t n_trg, size_t tpg, double kappa, unsigned char *flags,
+                     size_t n_trg, size_t tpg, double kappa, double kappa2, unsigned char *flags,
                      unsigned long long *counts, hipStream_t st);
 
 // Device buffer owned by the step objects.
@@ -140,8 +145,13 @@ class StokesG {
   // own quadrature) instead of its upsampled ones for every wave of 64
   // targets at least far_coarse times the vesicle's radius from its centre;
   // 0: the upsampled sources for every target (the reference's far field).
+  // far_coarse2 > far_coarse (needs far_coarse): beyond far_coarse2 radii
+  // the wave reads the vesicle at order p/2 instead: the degree-p surface
+  // at the order-p/2 grid points, with the density f times the area element
+  // (per unit-sphere measure) truncated to degree p/2 and the order-p/2
+  // Gauss-Legendre weights; 0: two levels.
   int init(int p, int p_up, long nv, int precision, int near = 1, double repul_dist = 5e-2,
-           double r_near = 0.0, double far_coarse = 0.0);
+           double r_near = 0.0, double far_coarse = 0.0, double far_coarse2 = 0.0);
   int setup(const double *X, hipStream_t st);
   // add_repul: the density is f + NearSingular::ForceRepul (the right-hand
   // side of the implicit step, src/InterfacialVelocity.cc:733).  local: the
@@ -193,11 +203,23 @@ class StokesG {
   double far_coarse = 0.0;
   DBuf aqwp, qc, qc2;
   unsigned char *cflags = nullptr;
-  unsigned long long *ccounts = nullptr;  // device: coarse, fine (wave, vesicle) pairs
+  // device: level-1, fine and level-2 (wave, vesicle) pairs
+  unsigned long long *ccounts = nullptr;
+  // level 2 (far_coarse2): order ord2 = p/2, its points c2pos [nv][ng2][3],
+  // the area element per unit-sphere measure c2gw [nv][ng] on the p grid,
+  // densities c2den, c2den2 [nv][ng2][3]; scratch for the product f c2gw
+  // [3 nv][ng], its coefficients [3 nv][nc], their order-ord2 synthesis
+  // [3 nv][ng2]; the degree mask [nc] and the latitude weights [ord2 + 1]
+  double far_coarse2 = 0.0;
+  int ord2 = 0;
+  DBuf c2pos, c2gw, c2den, c2den2, c2prod, c2coef, c2grid, c2mask, c2w;
   // source-target pairs one far-field launch evaluates (per density), as of
   // the last setup
   double far_pairs = 0;
   int pack_coarse(const double *f, double *qcoarse, hipStream_t st);
+  int pack_level2(const double *f, double *out, hipStream_t st);
+  // the far-field levels of apply / apply2 (den2 / l2den2: the second density)
+  FarCoarse coarse_levels(const double *f1, const double *f2, hipStream_t st, int *err);
   // double layer: operators, upsampled unit normals [nv][3][mu], the
   // density's coefficients G, its packed far-field form qd [nv][M][6],
   // itself on the upsampled grid and at the poles

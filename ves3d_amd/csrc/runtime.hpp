@@ -20,6 +20,7 @@ enum WorkspaceSlot {
   kWsSolver,
   kWsFarRecords,
   kWsFarCoarse,
+  kWsFarCoarse2,
   kWsFarBounds,
   kWsCount
 };

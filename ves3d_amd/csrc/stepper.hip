@@ -500,7 +500,8 @@ struct Stepper {
     if (p < 2 || p > 64 || prm.nves < 1) return VES3D_INVALID_PARAMETER;
     if (prm.inner_max_iter < 0 || prm.inner_tol < 0 || prm.inner_tol >= 1 ||
         (prm.extrapolate != 0 && prm.extrapolate != 1) ||
-        (prm.far_coarse != 0.0 && !(prm.far_coarse > 1.0)))
+        (prm.far_coarse != 0.0 && !(prm.far_coarse > 1.0)) ||
+        (prm.far_coarse2 != 0.0 && !(prm.far_coarse > 1.0 && prm.far_coarse2 > prm.far_coarse)))
       return VES3D_INVALID_PARAMETER;
     if (prm.p_up <= 0) prm.p_up = 2 * p;
     if (prm.filter_freq <= 0) prm.filter_freq = 2 * p / 3;
@@ -511,7 +512,7 @@ struct Stepper {
     V3D_CHECK(forces.init(p, prm.p_up, prm.filter_freq, nv, prm.bending_modulus));
     if (prm.repul_dist == 0.0) prm.repul_dist = 5e-2;  // src/Parameters.cc:46
     V3D_CHECK(stokes.init(p, prm.p_up, nv, prm.precision, prm.near_off ? 0 : 1, prm.repul_dist,
-                          prm.r_near, prm.far_coarse));
+                          prm.r_near, prm.far_coarse, prm.far_coarse2));
     for (DBuf *b : {&X, &U, &f, &f2, &Sf, &ug}) V3D_CHECK(b->alloc(3 * nv * ng));
     for (DBuf *b : {&sig, &ten, &sg}) V3D_CHECK(b->alloc(nv * ng));
     V3D_CHECK(cf.alloc(3 * nv * nc));

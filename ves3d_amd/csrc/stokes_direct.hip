@@ -430,13 +430,13 @@ __global__ void k_pack_far32(const double *__restrict__ spos, const double *__re
 // kOwnGroupBias below): about 1e-15 |f| per own-group source.
 constexpr double kOwnGroupBias64 = 1e30;
 
-// Two-level sources.  Every group (vesicle) carries its fine sources (spg
-// records: the upsampled surface) and, optionally, coarse ones (cpg
-// records: the surface at its own order with its own quadrature weights).
-// cflag[w][g] != 0 lets wave w (targets 64 w .. 64 w + 63) read group g's
-// coarse records: the flag is set where every target of the wave is at
-// least kappa times g's radius from g's centre, so that the coarse
-// quadrature is as accurate as the fine one there.  The flag, the record
+// Multi-level sources.  Every group (vesicle) carries its fine sources (spg
+// records: the upsampled surface) and, optionally, coarser ones: level 1
+// the surface at its own order with its own quadrature weights, level 2 a
+// lower order still.  flags[w][g] names the level wave w (targets 64 w ..
+// 64 w + 63) reads group g at: level l where every target of the wave is at
+// least kappa_l times g's radius from g's centre, so that the coarser
+// quadrature is as accurate as the fine one there.  The level, the record
 // base and the count are wave-uniform (SGPRs), so the source loop keeps its
 // scalar addressing.  A wave whose targets all belong to group g skips g.
 struct WaveSpan {
@@ -453,9 +453,22 @@ __device__ __forceinline__ WaveSpan wave_span(long tb, long n_trg) {
   return r;
 }
 
+// The source levels of the far field between groups: level 0 the fine
+// records (spg per group), levels 1 and 2 coarser ones (cnt[l] per group,
+// 0: absent); flags[w][g] names the level wave w reads group g at (null:
+// level 0 everywhere).  The flags never name an absent level.
+template <typename Rec>
+struct FarLevels {
+  const Rec *rec[3];
+  long cnt[3];
+  const unsigned char *flags;
+  __device__ __forceinline__ int level(long row, long n_groups, long g) const {
+    return flags ? (int)flags[row * n_groups + g] : 0;
+  }
+};
+
 __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f64(
-    const FarRec64 *__restrict__ rec, long spg, const FarRec64 *__restrict__ crec, long cpg,
-    const unsigned char *__restrict__ cflag, long n_groups, const double *__restrict__ tpos,
+    const FarLevels<FarRec64> L, long spg, long n_groups, const double *__restrict__ tpos,
     long n_trg, long tpg, int nsplit, double csl, double *__restrict__ part) {
   const long slice = blockIdx.x % nsplit;
   const long tb = blockIdx.x / nsplit;
@@ -474,9 +487,9 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f64(
   for (long g = g0; g < g1; ++g) {
     if (ws.first / tpg == g && ws.last / tpg == g) continue;  // wave-uniform skip
     const double bias = my_g == g ? kOwnGroupBias64 : 0.0;
-    const bool co = cflag && cflag[ws.row * n_groups + g];
-    const FarRec64 *__restrict__ r = co ? crec + g * cpg : rec + g * spg;
-    const int ns = (int)(co ? cpg : spg);
+    const int lv = L.level(ws.row, n_groups, g);
+    const FarRec64 *__restrict__ r = L.rec[lv] + g * L.cnt[lv];
+    const int ns = (int)L.cnt[lv];
 #pragma unroll 4
     for (int j = 0; j < ns; ++j) {
       const FarRec64 s = r[j];
@@ -514,9 +527,8 @@ __device__ __forceinline__ void acc_merge(Acc32 &a, float x, float y, float z) {
 constexpr float kOwnGroupBias = 1e30f;
 
 __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c(
-    const FarRec32 *__restrict__ rec, const double *__restrict__ spos, long spg,
-    const FarRec32 *__restrict__ crec, long cpg, const unsigned char *__restrict__ cflag,
-    long n_groups, const double *__restrict__ tpos, long n_trg, long tpg, int nsplit, double csl,
+    const FarLevels<FarRec32> L, const double *__restrict__ spos, long spg, long n_groups,
+    const double *__restrict__ tpos, long n_trg, long tpg, int nsplit, double csl,
     double *__restrict__ part) {
   const long slice = blockIdx.x % nsplit;
   const long tb = blockIdx.x / nsplit;
@@ -536,11 +548,11 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c(
     // a target's own group is not skipped lane by lane (that would make the
     // loop below divergent): its r^2 carries kOwnGroupBias instead
     const float bias = my_g == g ? kOwnGroupBias : 0.0f;
-    const double *a = spos + 3 * g * spg;  // the group's anchor (both levels)
+    const double *a = spos + 3 * g * spg;  // the group's anchor (every level)
     const float lx = (float)(tx - a[0]), ly = (float)(ty - a[1]), lz = (float)(tz - a[2]);
-    const bool co = cflag && cflag[ws.row * n_groups + g];
-    const FarRec32 *__restrict__ r = co ? crec + g * cpg : rec + g * spg;
-    const long cnt = co ? cpg : spg;
+    const int lv = L.level(ws.row, n_groups, g);
+    const FarRec32 *__restrict__ r = L.rec[lv] + g * L.cnt[lv];
+    const long cnt = L.cnt[lv];
     auto chunk = [&](const FarRec32 *__restrict__ rr, int n, float &sx, float &sy,
                      float &sz) {
       auto pair = [&](const FarRec32 &src) {
@@ -699,9 +711,8 @@ __global__ void k_pack_far64x2(const double *__restrict__ spos, const double *__
 }
 
 __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c_x2(
-    const FarRec32x2 *__restrict__ rec, const double *__restrict__ spos, long spg,
-    const FarRec32x2 *__restrict__ crec, long cpg, const unsigned char *__restrict__ cflag,
-    long n_groups, const double *__restrict__ tpos, long n_trg, long tpg, int nsplit, double csl,
+    const FarLevels<FarRec32x2> L, const double *__restrict__ spos, long spg, long n_groups,
+    const double *__restrict__ tpos, long n_trg, long tpg, int nsplit, double csl,
     double *__restrict__ part1, double *__restrict__ part2) {
   const long slice = blockIdx.x % nsplit;
   const long tb = blockIdx.x / nsplit;
@@ -719,9 +730,9 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c_x2(
     const float bias = my_g == g ? kOwnGroupBias : 0.0f;
     const double *a = spos + 3 * g * spg;
     const float lx = (float)(tx - a[0]), ly = (float)(ty - a[1]), lz = (float)(tz - a[2]);
-    const bool co = cflag && cflag[ws.row * n_groups + g];
-    const FarRec32x2 *__restrict__ r = co ? crec + g * cpg : rec + g * spg;
-    const long n = co ? cpg : spg;
+    const int lv = L.level(ws.row, n_groups, g);
+    const FarRec32x2 *__restrict__ r = L.rec[lv] + g * L.cnt[lv];
+    const long n = L.cnt[lv];
     for (long j0 = 0; j0 < n; j0 += FF_SUB) {
       const int cnt = (int)min((long)FF_SUB, n - j0);
       const FarRec32x2 *__restrict__ rr = r + j0;
@@ -761,8 +772,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c_x2(
 }
 
 __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f64_x2(
-    const FarRec64x2 *__restrict__ rec, long spg, const FarRec64x2 *__restrict__ crec, long cpg,
-    const unsigned char *__restrict__ cflag, long n_groups, const double *__restrict__ tpos,
+    const FarLevels<FarRec64x2> L, long spg, long n_groups, const double *__restrict__ tpos,
     long n_trg, long tpg, int nsplit, double csl, double *__restrict__ part1,
     double *__restrict__ part2) {
   const long slice = blockIdx.x % nsplit;
@@ -779,9 +789,9 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f64_x2(
   for (long g = g0; g < g1; ++g) {
     if (ws.first / tpg == g && ws.last / tpg == g) continue;  // wave-uniform skip
     const double bias = my_g == g ? kOwnGroupBias64 : 0.0;
-    const bool co = cflag && cflag[ws.row * n_groups + g];
-    const FarRec64x2 *__restrict__ r = co ? crec + g * cpg : rec + g * spg;
-    const int ns = (int)(co ? cpg : spg);
+    const int lv = L.level(ws.row, n_groups, g);
+    const FarRec64x2 *__restrict__ r = L.rec[lv] + g * L.cnt[lv];
+    const int ns = (int)L.cnt[lv];
 #pragma unroll 2
     for (int j = 0; j < ns; ++j) {
       const FarRec64x2 s = r[j];
@@ -951,26 +961,28 @@ __global__ __launch_bounds__(256) void k_group_bounds(const double *__restrict__
                           sqrt(fmax(fmax(red[0][3], red[1][3]), fmax(red[2][3], red[3][3]))));
 }
 
-// flags[w][g] = 1 where every target of wave w is at least kappa R_g from c_g.
-// One workgroup per wave: its 64 targets in LDS, the threads over the groups.
-// counts[0] += the wave's coarse (wave, group) pairs, counts[1] += its fine
-// ones (groups the wave does not skip as its own).
+// flags[w][g] = 2 where every target of wave w is at least kappa2 R_g from
+// c_g (kappa2 > 0), else 1 where at least kappa R_g, else 0.  One workgroup
+// per wave: its 64 targets in LDS, the threads over the groups.
+// counts[0] += the wave's level-1 (wave, group) pairs, counts[1] += its fine
+// ones (groups the wave does not skip as its own), counts[2] += its level-2
+// ones.
 __global__ __launch_bounds__(256) void k_coarse_flags(const double *__restrict__ tpos, long n_trg,
                                                       long tpg, const double4 *__restrict__ bnd,
-                                                      long n_groups, double kappa,
+                                                      long n_groups, double kappa, double kappa2,
                                                       unsigned char *__restrict__ flags,
                                                       unsigned long long *__restrict__ counts) {
   const long w = blockIdx.x;
   __shared__ double T[64][3];
-  __shared__ unsigned cnt[2];
+  __shared__ unsigned cnt[3];
   if (threadIdx.x < 64) {
     const long t = min(w * 64 + threadIdx.x, n_trg - 1);
     for (int d = 0; d < 3; ++d) T[threadIdx.x][d] = tpos[3 * t + d];
   }
-  if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
+  if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
   __syncthreads();
   const long gf = (w * 64) / tpg, gl = min(w * 64 + 63, n_trg - 1) / tpg;
-  unsigned nc = 0, nf = 0;
+  unsigned nc = 0, nf = 0, nc2 = 0;
   for (long g = threadIdx.x; g < n_groups; g += 256) {
     const double4 b = bnd[g];
     double m = 1e300;
@@ -978,16 +990,18 @@ __global__ __launch_bounds__(256) void k_coarse_flags(const double *__restrict__
       const double dx = T[k][0] - b.x, dy = T[k][1] - b.y, dz = T[k][2] - b.z;
       m = fmin(m, dx * dx + dy * dy + dz * dz);
     }
-    const double lim = kappa * b.w;
-    const bool co = m >= lim * lim;
-    flags[w * n_groups + g] = co ? 1 : 0;
-    if (co) ++nc;
+    const double lim = kappa * b.w, lim2 = kappa2 * b.w;
+    const int lv = (kappa2 > 0.0 && m >= lim2 * lim2) ? 2 : (m >= lim * lim ? 1 : 0);
+    flags[w * n_groups + g] = (unsigned char)lv;
+    if (lv == 2) ++nc2;
+    else if (lv == 1) ++nc;
     else if (!(gf == g && gl == g)) ++nf;
   }
   atomicAdd(&cnt[0], nc);
   atomicAdd(&cnt[1], nf);
+  atomicAdd(&cnt[2], nc2);
   __syncthreads();
-  if (threadIdx.x < 2) atomicAdd(&counts[threadIdx.x], (unsigned long long)cnt[threadIdx.x]);
+  if (threadIdx.x < 3) atomicAdd(&counts[threadIdx.x], (unsigned long long)cnt[threadIdx.x]);
 }
 
 __global__ void k_zero_u64(unsigned long long *p, int n) {
@@ -1016,22 +1030,29 @@ int direct_eval_f64_fixed(const double *src_pos, const double *sl_den, const dou
                          part_bytes);
 }
 
+// A level-2 set needs a level-1 set; two densities need both of each set.
+static bool coarse_valid(const FarCoarse &c, bool two) {
+  if (!c.pos || !c.den1 || !c.flags || c.cpg == 0 || (two && !c.den2)) return false;
+  if (c.cpg2 == 0) return true;
+  return c.pos2 && c.l2den1 && (!two || c.l2den2);
+}
+
 int far_coarse_flags(const double *spos, size_t n_src, size_t spg, const double *tpos,
-                     size_t n_trg, size_t tpg, double kappa, unsigned char *flags,
+                     size_t n_trg, size_t tpg, double kappa, double kappa2, unsigned char *flags,
                      unsigned long long *counts, hipStream_t st) {
   if (!spos || !tpos || !flags || !counts || spg == 0 || tpg == 0 || n_src % spg || n_trg == 0 ||
-      !(kappa > 1.0))
+      !(kappa > 1.0) || !(kappa2 == 0.0 || kappa2 > kappa))
     return VES3D_INVALID_PARAMETER;
   const long n_groups = (long)(n_src / spg);
   double4 *bnd = (double4 *)workspace(kWsFarBounds, sizeof(double4) * n_groups);
   if (!bnd) return VES3D_MEMORY_ERROR;
   k_group_bounds<<<(unsigned)n_groups, 256, 0, st>>>(spos, (long)spg, bnd);
   V3D_CHECK(launch_status());
-  k_zero_u64<<<1, 64, 0, st>>>(counts, 2);
+  k_zero_u64<<<1, 64, 0, st>>>(counts, 3);
   V3D_CHECK(launch_status());
   const long n_waves = ((long)n_trg + 63) / 64;
   k_coarse_flags<<<(unsigned)n_waves, 256, 0, st>>>(tpos, (long)n_trg, (long)tpg, bnd, n_groups,
-                                                    kappa, flags, counts);
+                                                    kappa, kappa2, flags, counts);
   return launch_status();
 }
 
@@ -1048,16 +1069,17 @@ int far_field_between_groups(bool f32c, const double *spos, const double *sld, s
   if (n_trg == 0) return VES3D_SUCCESS;
   if (!spos || !sld || !tpos || !tvel) return VES3D_INVALID_PARAMETER;
   if (n_src % spg || n_trg % tpg || n_src / spg != n_trg / tpg) return VES3D_INVALID_PARAMETER;
-  if (coarse && (!coarse->pos || !coarse->den1 || !coarse->flags || coarse->cpg == 0))
-    return VES3D_INVALID_PARAMETER;
+  if (coarse && !coarse_valid(*coarse, false)) return VES3D_INVALID_PARAMETER;
   const long n_groups = (long)(n_src / spg);
   const int ns = choose_nsplit((long)n_trg, (long)n_src, n_groups);
   const size_t rsz = f32c ? sizeof(FarRec32) : sizeof(FarRec64);
   double *part = (double *)workspace(kWsFarPartial, (size_t)ns * n_trg * 3 * sizeof(double));
   void *rec = workspace(kWsFarRecords, n_src * rsz);
   const long cpg = coarse ? (long)coarse->cpg : 0, n_c = cpg * n_groups;
+  const long cpg2 = coarse ? (long)coarse->cpg2 : 0, n_c2 = cpg2 * n_groups;
   void *crec = coarse ? workspace(kWsFarCoarse, n_c * rsz) : nullptr;
-  if (!part || !rec || (coarse && !crec)) return VES3D_MEMORY_ERROR;
+  void *crec2 = cpg2 ? workspace(kWsFarCoarse2, n_c2 * rsz) : nullptr;
+  if (!part || !rec || (coarse && !crec) || (cpg2 && !crec2)) return VES3D_MEMORY_ERROR;
   const unsigned char *cflag = coarse ? coarse->flags : nullptr;
   const long tblocks = ((long)n_trg + FF_BLOCK - 1) / FF_BLOCK;
   const dim3 grid((unsigned)(tblocks * ns));
@@ -1070,15 +1092,22 @@ int far_field_between_groups(bool f32c, const double *spos, const double *sld, s
                                                     (long)spg, (FarRec32 *)crec);
       V3D_CHECK(launch_status());
     }
+    if (cpg2) {
+      k_pack_far32<<<cdiv(n_c2, 256), 256, 0, st>>>(coarse->pos2, coarse->l2den1, n_c2, cpg2,
+                                                     spos, (long)spg, (FarRec32 *)crec2);
+      V3D_CHECK(launch_status());
+    }
     if (!coarse && g_far_kind.load() == 2) {
       const long tb2 = ((long)n_trg + 2 * FF_BLOCK - 1) / (2 * FF_BLOCK);
       k_far_groups_f32c2<<<dim3((unsigned)(tb2 * ns)), FF_BLOCK, 0, st>>>(
           (const FarRec32 *)rec, spos, (long)spg, n_groups, tpos, (long)n_trg, (long)tpg, ns, csl,
           part);
     } else {
-      k_far_groups_f32c<<<grid, FF_BLOCK, 0, st>>>((const FarRec32 *)rec, spos, (long)spg,
-                                                   (const FarRec32 *)crec, cpg, cflag, n_groups,
-                                                   tpos, (long)n_trg, (long)tpg, ns, csl, part);
+      const FarLevels<FarRec32> L{
+          {(const FarRec32 *)rec, (const FarRec32 *)crec, (const FarRec32 *)crec2},
+          {(long)spg, cpg, cpg2}, cflag};
+      k_far_groups_f32c<<<grid, FF_BLOCK, 0, st>>>(L, spos, (long)spg, n_groups, tpos,
+                                                   (long)n_trg, (long)tpg, ns, csl, part);
     }
   } else {
     k_pack_far64<<<cdiv((long)n_src, 256), 256, 0, st>>>(spos, sld, (long)n_src, (FarRec64 *)rec);
@@ -1088,9 +1117,16 @@ int far_field_between_groups(bool f32c, const double *spos, const double *sld, s
                                                     (FarRec64 *)crec);
       V3D_CHECK(launch_status());
     }
-    k_far_groups_f64<<<grid, FF_BLOCK, 0, st>>>((const FarRec64 *)rec, (long)spg,
-                                                (const FarRec64 *)crec, cpg, cflag, n_groups, tpos,
-                                                (long)n_trg, (long)tpg, ns, csl, part);
+    if (cpg2) {
+      k_pack_far64<<<cdiv(n_c2, 256), 256, 0, st>>>(coarse->pos2, coarse->l2den1, n_c2,
+                                                     (FarRec64 *)crec2);
+      V3D_CHECK(launch_status());
+    }
+    const FarLevels<FarRec64> L{
+        {(const FarRec64 *)rec, (const FarRec64 *)crec, (const FarRec64 *)crec2},
+        {(long)spg, cpg, cpg2}, cflag};
+    k_far_groups_f64<<<grid, FF_BLOCK, 0, st>>>(L, (long)spg, n_groups, tpos, (long)n_trg,
+                                                (long)tpg, ns, csl, part);
   }
   V3D_CHECK(launch_status());
   const long n = (long)n_trg * 3;
@@ -1110,9 +1146,7 @@ int far_field_between_groups2(bool f32c, const double *spos, const double *sld1,
   if (n_trg == 0) return VES3D_SUCCESS;
   if (!spos || !sld1 || !sld2 || !tpos || !tvel1 || !tvel2) return VES3D_INVALID_PARAMETER;
   if (n_src % spg || n_trg % tpg || n_src / spg != n_trg / tpg) return VES3D_INVALID_PARAMETER;
-  if (coarse && (!coarse->pos || !coarse->den1 || !coarse->den2 || !coarse->flags ||
-                 coarse->cpg == 0))
-    return VES3D_INVALID_PARAMETER;
+  if (coarse && !coarse_valid(*coarse, true)) return VES3D_INVALID_PARAMETER;
   const double csl = 1.0 / (8.0 * kPi);
   const long n_groups = (long)(n_src / spg);
   const int ns = choose_nsplit((long)n_trg, (long)n_src, n_groups);
@@ -1120,8 +1154,10 @@ int far_field_between_groups2(bool f32c, const double *spos, const double *sld1,
   double *part = (double *)workspace(kWsFarPartial, (size_t)2 * ns * n_trg * 3 * sizeof(double));
   void *rec = workspace(kWsFarRecords, n_src * rsz);
   const long cpg = coarse ? (long)coarse->cpg : 0, n_c = cpg * n_groups;
+  const long cpg2 = coarse ? (long)coarse->cpg2 : 0, n_c2 = cpg2 * n_groups;
   void *crec = coarse ? workspace(kWsFarCoarse, n_c * rsz) : nullptr;
-  if (!part || !rec || (coarse && !crec)) return VES3D_MEMORY_ERROR;
+  void *crec2 = cpg2 ? workspace(kWsFarCoarse2, n_c2 * rsz) : nullptr;
+  if (!part || !rec || (coarse && !crec) || (cpg2 && !crec2)) return VES3D_MEMORY_ERROR;
   const unsigned char *cflag = coarse ? coarse->flags : nullptr;
   double *part2 = part + (size_t)ns * n_trg * 3;
   const long tblocks = ((long)n_trg + FF_BLOCK - 1) / FF_BLOCK;
@@ -1137,10 +1173,17 @@ int far_field_between_groups2(bool f32c, const double *spos, const double *sld1,
                                                       (FarRec32x2 *)crec);
       V3D_CHECK(launch_status());
     }
-    k_far_groups_f32c_x2<<<grid, FF_BLOCK, 0, st>>>((const FarRec32x2 *)rec, spos, (long)spg,
-                                                    (const FarRec32x2 *)crec, cpg, cflag,
-                                                    n_groups, tpos, (long)n_trg, (long)tpg, ns,
-                                                    csl, part, part2);
+    if (cpg2) {
+      k_pack_far32x2<<<cdiv(n_c2, 256), 256, 0, st>>>(coarse->pos2, coarse->l2den1,
+                                                       coarse->l2den2, n_c2, cpg2, spos,
+                                                       (long)spg, (FarRec32x2 *)crec2);
+      V3D_CHECK(launch_status());
+    }
+    const FarLevels<FarRec32x2> L{
+        {(const FarRec32x2 *)rec, (const FarRec32x2 *)crec, (const FarRec32x2 *)crec2},
+        {(long)spg, cpg, cpg2}, cflag};
+    k_far_groups_f32c_x2<<<grid, FF_BLOCK, 0, st>>>(L, spos, (long)spg, n_groups, tpos,
+                                                    (long)n_trg, (long)tpg, ns, csl, part, part2);
   } else {
     k_pack_far64x2<<<cdiv((long)n_src, 256), 256, 0, st>>>(spos, sld1, sld2, (long)n_src,
                                                             (FarRec64x2 *)rec);
@@ -1150,10 +1193,16 @@ int far_field_between_groups2(bool f32c, const double *spos, const double *sld1,
                                                       n_c, (FarRec64x2 *)crec);
       V3D_CHECK(launch_status());
     }
-    k_far_groups_f64_x2<<<grid, FF_BLOCK, 0, st>>>((const FarRec64x2 *)rec, (long)spg,
-                                                   (const FarRec64x2 *)crec, cpg, cflag, n_groups,
-                                                   tpos, (long)n_trg, (long)tpg, ns, csl, part,
-                                                   part2);
+    if (cpg2) {
+      k_pack_far64x2<<<cdiv(n_c2, 256), 256, 0, st>>>(coarse->pos2, coarse->l2den1,
+                                                       coarse->l2den2, n_c2, (FarRec64x2 *)crec2);
+      V3D_CHECK(launch_status());
+    }
+    const FarLevels<FarRec64x2> L{
+        {(const FarRec64x2 *)rec, (const FarRec64x2 *)crec, (const FarRec64x2 *)crec2},
+        {(long)spg, cpg, cpg2}, cflag};
+    k_far_groups_f64_x2<<<grid, FF_BLOCK, 0, st>>>(L, (long)spg, n_groups, tpos, (long)n_trg,
+                                                   (long)tpg, ns, csl, part, part2);
   }
   V3D_CHECK(launch_status());
   const long n = (long)n_trg * 3;
