@@ -648,6 +648,37 @@ def bench_stages(args):
             "config": {"workload": "stages"}, "stages": out}
 
 
+def bench_learned(args):
+    """The headline form alone (learned preconditioner + mixed precision +
+    multi-level far field) for warmup + steps steps: a short program for
+    kernel traces and counters of the form that the step workload times."""
+    import torch
+    import ves3d_amd
+    from ves3d_amd import implicit as I
+    from ves3d_amd import workloads as W
+    ves3d_amd.init(0)
+    ves3d_amd.call("ves3d_hip_set_far_kernel", FAR_KINDS[args.far_kernel])
+    nv, p = args.nves, args.p
+    Xg = torch.from_numpy(W.implicit_step_case(nv=nv, p=p)).cuda()
+    common = dict(dt=1e-3, shear_rate=1.0, bending_modulus=1e-2, tol=1e-6, max_iter=args.max_iter,
+                  r_near=args.r_near)
+    pc, pc_rec = _train_pc(args, p, common)
+    kw = dict(common, extrapolate=args.extrapolate, precision=args.mixed_precision, precond=2,
+              inner_max_iter=args.inner_max_iter, inner_tol=args.inner_tol,
+              far_coarse=args.far_coarse, far_coarse2=args.far_coarse2)
+    stats, _, _ = _run_form(I, nv, p, Xg, kw, pc, args.warmup + args.steps, "learned+mixed")
+    st = stats[args.warmup:]
+    ms = _mean(st, "ms_total")
+    return {"metric": "ms per implicit step @512 ves p=16, learned-PC+mixed form alone",
+            "value": ms, "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": ms, "higher_is_better": False, "data": "synthetic",
+            "config": {"workload": "learned form alone", "nves": nv, "p": p},
+            "iterations": [s["iterations"] for s in st], "network": pc_rec,
+            "stage_ms": {k: _mean(st, k) for k in ("ms_setup", "ms_rhs", "ms_solve", "ms_post",
+                                                   "ms_near", "ms_far", "ms_far2")},
+            "far_pairs": st[-1]["far_pairs"]}
+
+
 def bench_build(args):
     """One singular-operator build (SL + DL) of args.nves vesicles at args.p in
     FP64 and in FP32 transforms: a short program for hardware-counter runs
@@ -757,7 +788,8 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="step",
-                    choices=["step", "far", "matvec", "probe", "stages", "build", "trajectory"])
+                    choices=["step", "far", "matvec", "probe", "stages", "build", "trajectory",
+                             "learned"])
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32c"])
     ap.add_argument("--nves", type=int, default=512)
     ap.add_argument("--p", type=int, default=16)
@@ -809,7 +841,7 @@ def main():
         sys.exit(2)
     res = {"step": bench_step, "far": bench_far, "matvec": bench_matvec,
            "probe": bench_probe, "stages": bench_stages, "build": bench_build,
-           "trajectory": bench_trajectory}[args.workload](args)
+           "trajectory": bench_trajectory, "learned": bench_learned}[args.workload](args)
     print(json.dumps(res))
 
 

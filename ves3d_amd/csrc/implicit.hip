@@ -263,29 +263,30 @@ __global__ void k_coarse_weights(const double *__restrict__ Xt, const double *__
   }
 }
 
+// y = float(x), round to nearest
+__global__ void k_round_f32(const double *__restrict__ x, long n, float *__restrict__ y) {
+  GRID_LOOP(i, n) y[i] = (float)x[i];
+}
 // out[v][d][g] = f[v][d][g] * w[v][g]
 __global__ void k_mul_pointwise(const double *__restrict__ f, const double *__restrict__ w, long nv,
                                 long ng, double *__restrict__ out) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= 3 * nv * ng) return;
-  out[i] = f[i] * w[(i / (3 * ng)) * ng + i % ng];
+  GRID_LOOP(i, 3 * nv * ng) out[i] = f[i] * w[(i / (3 * ng)) * ng + i % ng];
 }
 // c[f][k] *= mask[k] (zeroes the coefficients above a degree)
 __global__ void k_mask_coef(double *__restrict__ c, const double *__restrict__ mask, long n,
                             long nc) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) c[i] *= mask[i % nc];
+  GRID_LOOP(i, n) c[i] *= mask[i % nc];
 }
 // out[v ng2 + i][d] = grid[v][d][i] * w[latitude of i] (order-q2 grid,
 // 2 q2 longitudes per latitude)
 __global__ void k_pack_level2(const double *__restrict__ grid, const double *__restrict__ w,
                               long nv, long ng2, int q2, double *__restrict__ out) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv * ng2) return;
-  const long v = i / ng2, k = i % ng2;
-  const double wk = w[k / (2 * q2)];
+  GRID_LOOP(i, nv * ng2) {
+    const long v = i / ng2, k = i % ng2;
+    const double wk = w[k / (2 * q2)];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) out[i * 3 + d] = grid[(v * 3 + d) * ng2 + k] * wk;
+    for (int d = 0; d < 3; ++d) out[i * 3 + d] = grid[(v * 3 + d) * ng2 + k] * wk;
+  }
 }
 // qc[v ng + g][d] = f[v][d][g] * aqwp[v][g] (point-major coarse densities)
 __global__ void k_pack_coarse(const double *__restrict__ f, const double *__restrict__ aqwp,
@@ -570,11 +571,11 @@ int StokesG::init(int p_, int pu_, long nv_, int precision_, int near_, double r
   far_coarse2 = far_coarse2_;
   const long nc = sh_ncoef(p), ng = sh_ngrid(p), mu = sh_ngrid(pu);
   const long nmat = nv * 9 * nc * nc;
-  if (precision != 1) {
-    V3D_CHECK(SL.alloc(nmat));
-  } else {
+  if (precision != 1) V3D_CHECK(SL.alloc(nmat));
+  // precision 1: the FP32 build; precision 2: an FP32 copy of the FP64
+  // operator for the local (inner-solve) applications
+  if (precision != 0)
     if (hipMalloc(&SL32, sizeof(float) * nmat) != hipSuccess) return VES3D_MEMORY_ERROR;
-  }
   V3D_CHECK(far_src.alloc(nv * (mu + 2) * 3));
   V3D_CHECK(q.alloc(nv * (mu + 2) * 3));
   V3D_CHECK(trg.alloc(nv * ng * 3));
@@ -627,6 +628,11 @@ int StokesG::setup(const double *X, hipStream_t st) {
     V3D_CHECK(singular_build<double>(X, nv, p, pu, SL.p, want_dl ? DL.p : nullptr, st));
   else
     V3D_CHECK(singular_build<float>(X, nv, p, pu, SL32, want_dl ? DL32 : nullptr, st));
+  if (precision == 2) {
+    const long nmat = nv * 9 * (long)sh_ncoef(p) * sh_ncoef(p);
+    k_round_f32<<<gblocks(nmat), 256, 0, st>>>(SL.p, nmat, SL32);
+    V3D_CHECK(launch_status());
+  }
   near_off_ready = false;  // new sources
   V3D_CHECK(sht_grid2shc<double>(X, 3 * nv, p, p, shc, work, st));
   V3D_CHECK(sht_shc2grid<double>(shc, 3 * nv, p, pu, tmpx, tmpt, tmpp, work, st));
@@ -734,7 +740,11 @@ int StokesG::self_and_density(const double *f, double *Fc, double *Vc, double *v
   const long mu = sh_ngrid(pu);
   if (f) V3D_CHECK(sht_grid2shc<double>(f, 3 * nv, p, p, Fc, work, st));
   if (d) V3D_CHECK(sht_grid2shc<double>(d, 3 * nv, p, p, G, work, st));
-  if (precision != 1)
+  // precision 2, the self-interaction alone (an inner solve's operator, a
+  // preconditioner's): the FP32 copy halves the HBM-bound apply
+  if (precision == 2 && !pack && !d)
+    V3D_CHECK(self_apply<float>(f ? SL32 : nullptr, Fc, nullptr, G.p, nv, p, Vc, st));
+  else if (precision != 1)
     V3D_CHECK(self_apply<double>(f ? SL.p : nullptr, Fc, d ? DL.p : nullptr, G.p, nv, p, Vc, st));
   else
     V3D_CHECK(self_apply<float>(f ? SL32 : nullptr, Fc, d ? DL32 : nullptr, G.p, nv, p, Vc, st));
