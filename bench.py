@@ -204,7 +204,8 @@ def bench_step(args):
     check = I.Stepper(nv, p, precision=0, precond=1, **common)
     learned_kw = dict(timed, precision=args.mixed_precision, precond=2,
                       inner_max_iter=args.inner_max_iter, inner_tol=args.inner_tol,
-                      far_coarse=args.far_coarse, far_coarse2=args.far_coarse2)
+                      far_coarse=args.far_coarse, far_coarse2=args.far_coarse2,
+                      fallback=args.fallback)
 
     # ---- the two headline forms from the same initial state
     plain, _, first_plain = _run_form(I, nv, p, Xg, dict(timed, precision=0, precond=1), None,
@@ -332,6 +333,9 @@ def bench_step(args):
         "fp64_relres_learned_every_step": r64,
         "true_relres_plain": [s["true_relres"] for s in plain_t],
         "fell_back": [s["fell_back"] for s in learned_t],
+        "fallback": ("diagonal spectral preconditioner (unpreconditioned BiCGStab, the "
+                     "workload's fallback, takes > 300 iterations here; --fallback none)"
+                     if args.fallback == "diagonal" else "unpreconditioned BiCGStab"),
         "velocity_rel_err_vs_fp64": vel_err, "velocity_rel_err_plain_vs_fp64": vel_err_plain,
         "fp64_reference_relres": sr["true_relres"],
         "network": pc_rec,
@@ -827,6 +831,9 @@ def main():
                     help="learned form: two-level far field, p-grid sources for waves of "
                          "targets beyond this many vesicle radii (0: upsampled sources "
                          "everywhere, as the plain form)")
+    ap.add_argument("--fallback", default="diagonal", choices=["diagonal", "none"],
+                    help="learned form: the solve that repeats a failed learned solve, "
+                         "diagonal-preconditioned or unpreconditioned")
     ap.add_argument("--far-coarse2", type=float, default=6.0,
                     help="learned form: third far-field level, order-p/2 sources for waves "
                          "beyond this many radii (0: two levels)")

@@ -429,6 +429,10 @@ typedef struct ves3d_step_params {
                              measure) truncated to degree p/2 and the order-p/2
                              Gauss-Legendre weights.  0: two levels.  No counterpart in
                              the reference. */
+  int fallback;           /* when a learned solve fails or its true residual misses the
+                             tolerance, the step solves again from the same initial
+                             guess: 0 with the diagonal spectral preconditioner
+                             (ImplicitPrecond), 1 unpreconditioned. */
 } ves3d_step_params;
 
 typedef struct ves3d_step_stats {

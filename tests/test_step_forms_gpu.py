@@ -162,6 +162,34 @@ def test_fallback_restarts_from_the_extrapolated_guess(hip):
     pc.close()
 
 
+def test_unpreconditioned_fallback(hip):
+    """fallback="none": a learned solve that fails (a NaN network) is
+    repeated from the same initial guess by unpreconditioned BiCGStab, which
+    still meets the tolerance and reaches the diagonal solve's velocity."""
+    nv, p = 2, 6
+    X = workload(nv, p, seed=104)
+    nin = 4 * p * (p + 2)
+    pc = I.LearnedPC(nin, 32, capacity=0, seed=3)
+    pc.set_params(torch.full((pc.num_params,), float("nan"), dtype=torch.float32, device="cuda"))
+    U = {}
+    for name, precond, fb in (("diag", 1, "diagonal"), ("none", 2, "none")):
+        st = I.Stepper(nv, p, precision=0, precond=precond, fallback=fb,
+                       **dict(COMMON, tol=1e-6, max_iter=2000))
+        if precond == 2:
+            I.attach(st, pc, collect=False)
+        st.set_state(cu(X))
+        s = st.step()
+        print("fallback %s: %d iterations, true relres %.2e, fell_back %d" %
+              (name, s["iterations"], s["true_relres"], s["fell_back"]), flush=True)
+        assert s["status"] == 100 and s["true_relres"] <= 1e-6 * 1.05, s
+        assert s["fell_back"] == (1 if precond == 2 else 0)
+        U[name] = st.get_state()[1].cpu().numpy()
+        st.close()
+    e = np.linalg.norm(U["none"] - U["diag"]) / np.linalg.norm(U["diag"])
+    assert e < 1e-4, e
+    pc.close()
+
+
 def test_divergence_leaves_the_state(hip):
     """A solve that cannot converge within max_iter is a DivergenceError
     (src/InterfacialVelocity.cc:176-178): the step raises it with its

@@ -263,6 +263,16 @@ __global__ void k_coarse_weights(const double *__restrict__ Xt, const double *__
   }
 }
 
+// out[f][k] = in[f][map[k]] (map[k] >= 0) or 0: zero-padding of SH
+// coefficients to a higher order
+__global__ void k_pad_coef(const double *__restrict__ in, long nf, long nci,
+                           const double *__restrict__ map, long nco, double *__restrict__ out) {
+  GRID_LOOP(i, nf * nco) {
+    const long f = i / nco;
+    const double m = map[i % nco];
+    out[i] = m >= 0.0 ? in[f * nci + (long)m] : 0.0;
+  }
+}
 // y = float(x), round to nearest
 __global__ void k_round_f32(const double *__restrict__ x, long n, float *__restrict__ y) {
   GRID_LOOP(i, n) y[i] = (float)x[i];
@@ -359,6 +369,8 @@ int SurfaceG::init(int p_, int f_, long nv_) {
   for (DBuf *b : {&E, &F, &G, &W, &H, &K, &s1}) V3D_CHECK(b->alloc(nv * ng));
   for (DBuf *b : {&cu, &cv, &nrm, &t1, &t2, &t3}) V3D_CHECK(b->alloc(3 * nv * ng));
   V3D_CHECK(shc.alloc(3 * nv * nc));
+  V3D_CHECK(c1.alloc(nv * nc));
+  V3D_CHECK(c3.alloc(3 * nv * nc));
   V3D_CHECK(work.alloc(std::max(shc2grid_work(3 * nv, p, p), grid2shc_work(3 * nv, p, p))));
   return upload(box, box_filter(p, f));
 }
@@ -374,11 +386,15 @@ int SurfaceG::backward(const double *S, long nf, int du, int dv, double *X, hipS
   return sht_synth<double>(S, nf, p, p, du, dv, X, work, st, du == 1 ? -1.0 : 1.0);
 }
 
-int SurfaceG::low_pass(double *field, long nf, hipStream_t st) {
-  V3D_CHECK(forward(field, nf, shc, st));
-  k_scale_rows<<<gblocks(nf * nc), 256, 0, st>>>(shc, nf, (int)nc, box);
+int SurfaceG::low_pass_c(double *field, long nf, double *coef, bool want_grid, hipStream_t st) {
+  V3D_CHECK(forward(field, nf, coef, st));
+  k_scale_rows<<<gblocks(nf * nc), 256, 0, st>>>(coef, nf, (int)nc, box);
   V3D_CHECK(launch_status());
-  return backward(shc, nf, 0, 0, field, st);
+  return want_grid ? backward(coef, nf, 0, 0, field, st) : VES3D_SUCCESS;
+}
+
+int SurfaceG::low_pass(double *field, long nf, hipStream_t st) {
+  return low_pass_c(field, nf, shc, true, st);
 }
 
 int SurfaceG::update(const double *X, hipStream_t st) {
@@ -396,32 +412,45 @@ int SurfaceG::update(const double *X, hipStream_t st) {
   return low_pass(H, nv, st);
 }
 
-int SurfaceG::grad(const double *fin, double *out, hipStream_t st) {
-  V3D_CHECK(forward(fin, nv, shc, st));
-  V3D_CHECK(backward(shc, nv, 1, 0, t1, st));
-  V3D_CHECK(backward(shc, nv, 0, 1, t2, st));
+int SurfaceG::grad_c(const double *c, double *out, double *cout, bool want_grid,
+                     hipStream_t st) {
+  V3D_CHECK(backward(c, nv, 1, 0, t1, st));
+  V3D_CHECK(backward(c, nv, 0, 1, t2, st));
   k_grad_comb<<<gblocks(nv * ng), 256, 0, st>>>(t1, t2, cu, cv, nv, ng, out);
   V3D_CHECK(launch_status());
-  return low_pass(out, 3 * nv, st);
+  return low_pass_c(out, 3 * nv, cout, want_grid, st);
 }
 
-int SurfaceG::div(const double *V, double *out, hipStream_t st) {
-  V3D_CHECK(forward(V, 3 * nv, shc, st));
-  V3D_CHECK(backward(shc, 3 * nv, 1, 0, t1, st));
-  V3D_CHECK(backward(shc, 3 * nv, 0, 1, t2, st));
+int SurfaceG::grad(const double *fin, double *out, hipStream_t st) {
+  V3D_CHECK(forward(fin, nv, shc, st));
+  return grad_c(shc, out, c3, true, st);
+}
+
+int SurfaceG::div_c(const double *c, double *out, hipStream_t st) {
+  V3D_CHECK(backward(c, 3 * nv, 1, 0, t1, st));
+  V3D_CHECK(backward(c, 3 * nv, 0, 1, t2, st));
   k_div_comb<<<gblocks(nv * ng), 256, 0, st>>>(t1, t2, cu, cv, nv, ng, out);
   V3D_CHECK(launch_status());
   return low_pass(out, nv, st);
 }
 
-int SurfaceG::lin_mean_curv(const double *x, double *out, hipStream_t st) {
-  V3D_CHECK(forward(x, 3 * nv, shc, st));
-  V3D_CHECK(backward(shc, 3 * nv, 2, 0, t1, st));
-  V3D_CHECK(backward(shc, 3 * nv, 1, 1, t2, st));
-  V3D_CHECK(backward(shc, 3 * nv, 0, 2, t3, st));
+int SurfaceG::div(const double *V, double *out, hipStream_t st) {
+  V3D_CHECK(forward(V, 3 * nv, shc, st));
+  return div_c(shc, out, st);
+}
+
+int SurfaceG::lin_mean_curv_c(const double *c, double *out, double *cout, hipStream_t st) {
+  V3D_CHECK(backward(c, 3 * nv, 2, 0, t1, st));
+  V3D_CHECK(backward(c, 3 * nv, 1, 1, t2, st));
+  V3D_CHECK(backward(c, 3 * nv, 0, 2, t3, st));
   k_lin_mean_curv<<<gblocks(nv * ng), 256, 0, st>>>(t1, t2, t3, nrm, E, F, G, nv, ng, out);
   V3D_CHECK(launch_status());
-  return low_pass(out, nv, st);
+  return low_pass_c(out, nv, cout, true, st);
+}
+
+int SurfaceG::lin_mean_curv(const double *x, double *out, hipStream_t st) {
+  V3D_CHECK(forward(x, 3 * nv, shc, st));
+  return lin_mean_curv_c(shc, out, c1, st);
 }
 
 // ------------------------------------------------------------- ForcesG
@@ -444,8 +473,26 @@ int ForcesG::init(int p_, int pu_, int f, long nv_, double kb_) {
   const int pm = std::min(p, pu);
   V3D_CHECK(rcoef.alloc(3 * nv * sh_ncoef(pm)));
   long w = std::max({grid2shc_work(3 * nv, p, pm), grid2shc_work(3 * nv, pu, pm),
-                     shc2grid_work(3 * nv, pm, p), shc2grid_work(3 * nv, pm, pu)});
+                     shc2grid_work(3 * nv, pm, p), shc2grid_work(3 * nv, pm, pu),
+                     grid2shc_work(3 * nv, p, p)});
+  if (pu >= p) {
+    // block j (order m = (j + 1) / 2) of the p_up layout starts with the
+    // degrees m .. p that block j of the order-p layout holds
+    std::vector<double> map(sh_ncoef(pu), -1.0);
+    for (int j = 0; j < 2 * p; ++j)
+      for (int k = 0; k < sh_blocklen(p, j); ++k)
+        map[sh_blockoff(pu, j) + k] = sh_blockoff(p, j) + k;
+    V3D_CHECK(upload(padmap, map));
+  }
   return rwork.alloc(w);
+}
+
+int ForcesG::up_coef(const double *in, long nf, double *cu_out, hipStream_t st) {
+  if (!padmap.p) return VES3D_INVALID_PARAMETER;
+  V3D_CHECK(sht_grid2shc<double>(in, nf, p, p, rcoef, rwork, st));
+  const long ncu = sh_ncoef(pu);
+  k_pad_coef<<<gblocks(nf * ncu), 256, 0, st>>>(rcoef, nf, sh_ncoef(p), padmap, ncu, cu_out);
+  return launch_status();
 }
 
 // Resample (src/HelperFuns.cc:436-493): truncate / zero-pad the SH expansion.
@@ -467,8 +514,10 @@ int ForcesG::bending(double *Fb, hipStream_t st) {  // src/InterfacialForce.cc:2
   const long mu = Su.ng;
   k_bend_s2<<<gblocks(nv * mu), 256, 0, st>>>(Su.H, Su.H, Su.K, nv * mu, su2);
   V3D_CHECK(launch_status());
-  V3D_CHECK(Su.grad(Su.H, gu, st));
-  V3D_CHECK(Su.div(gu, su1, st));
+  // div(grad H): the low-passed gradient's coefficients feed the divergence
+  V3D_CHECK(Su.forward(Su.H, nv, Su.c1, st));
+  V3D_CHECK(Su.grad_c(Su.c1, gu, Su.c3, false, st));
+  V3D_CHECK(Su.div_c(Su.c3, su1, st));
   k_bend_force<<<gblocks(nv * mu), 256, 0, st>>>(su1, su2, Su.nrm, bcoef, nv, mu, fu);
   V3D_CHECK(launch_status());
   return resample(fu, 3 * nv, pu, p, Fb, st);
@@ -476,12 +525,16 @@ int ForcesG::bending(double *Fb, hipStream_t st) {  // src/InterfacialForce.cc:2
 
 int ForcesG::linear_bending(const double *x_new, double *Fb, hipStream_t st) {  // :52-92
   const long mu = Su.ng;
-  V3D_CHECK(resample(x_new, 3 * nv, p, pu, xu, st));
-  V3D_CHECK(Su.lin_mean_curv(xu, su1, st));
+  // x_new upsampled, its linearised mean curvature, the gradient of that
+  // and its divergence, each operator fed the previous one's low-passed
+  // coefficients (the reference synthesises each on the upsampled grid and
+  // transforms it back: the same field, up to rounding)
+  V3D_CHECK(up_coef(x_new, 3 * nv, Su.c3, st));
+  V3D_CHECK(Su.lin_mean_curv_c(Su.c3, su1, Su.c1, st));
   k_bend_s2<<<gblocks(nv * mu), 256, 0, st>>>(su1, Su.H, Su.K, nv * mu, su2);
   V3D_CHECK(launch_status());
-  V3D_CHECK(Su.grad(su1, gu, st));
-  V3D_CHECK(Su.div(gu, su1, st));
+  V3D_CHECK(Su.grad_c(Su.c1, gu, Su.c3, false, st));
+  V3D_CHECK(Su.div_c(Su.c3, su1, st));
   k_bend_force<<<gblocks(nv * mu), 256, 0, st>>>(su1, su2, Su.nrm, bcoef, nv, mu, fu);
   V3D_CHECK(launch_status());
   return resample(fu, 3 * nv, pu, p, Fb, st);
@@ -513,8 +566,10 @@ int ForcesG::gravity_add(const double *x, double *F, hipStream_t st) {
 
 int ForcesG::tensile(const double *sigma, double *Fs, hipStream_t st) {  // :94-124
   const long mu = Su.ng;
-  V3D_CHECK(resample(sigma, nv, p, pu, su1, st));
-  V3D_CHECK(Su.grad(su1, gu, st));
+  // sigma upsampled (from its zero-padded coefficients) and its gradient
+  V3D_CHECK(up_coef(sigma, nv, Su.c1, st));
+  V3D_CHECK(Su.backward(Su.c1, nv, 0, 0, su1, st));
+  V3D_CHECK(Su.grad_c(Su.c1, gu, Su.c3, true, st));
   k_tens_force<<<gblocks(nv * mu), 256, 0, st>>>(su1, Su.H, Su.nrm, gu, nv, mu, fu);
   V3D_CHECK(launch_status());
   return resample(fu, 3 * nv, pu, p, Fs, st);

@@ -95,11 +95,24 @@ class SurfaceG {
   int low_pass(double *field, long nf, hipStream_t st);  // in place, [nf][ng]
   int forward(const double *X, long nf, double *S, hipStream_t st);
   int backward(const double *S, long nf, int du, int dv, double *X, hipStream_t st);
+  // The same operators from the input's SH coefficients (a chain of them
+  // skips each synthesis that the next operator would transform back):
+  // c holds the coefficients of the input field (nv functions for grad,
+  // 3 nv for div and lin_mean_curv).  low_pass_c leaves the filtered
+  // coefficients in coef and, with want_grid, the filtered field in field;
+  // grad_c leaves the low-passed gradient's coefficients in cout (3 nv) and,
+  // with want_grid, the gradient in out; lin_mean_curv_c leaves its
+  // low-passed result's coefficients in cout (nv).
+  int low_pass_c(double *field, long nf, double *coef, bool want_grid, hipStream_t st);
+  int grad_c(const double *c, double *out, double *cout, bool want_grid, hipStream_t st);
+  int div_c(const double *c, double *out, hipStream_t st);
+  int lin_mean_curv_c(const double *c, double *out, double *cout, hipStream_t st);
 
   int p = 0, f = 0;
   long nv = 0, ng = 0, nc = 0;
   DBuf E, F, G, W, cu, cv, nrm, H, K;
   DBuf shc, t1, t2, t3, s1, work, box;
+  DBuf c1, c3;  // coefficient buffers of the chains: nv and 3 nv functions
 };
 
 // InterfacialForce at order p with the surface upsampled to p_up.
@@ -120,6 +133,10 @@ class ForcesG {
   int set_props(const double *kb_host, const double *rho_host, const double *g_host);
   // resample [nf][grid(p_from)] -> [nf][grid(p_to)]
   int resample(const double *in, long nf, int p_from, int p_to, double *out, hipStream_t st);
+  // the SH coefficients at p_up of a field on the p grid (nf functions):
+  // its order-p coefficients, zero-padded (Resample up, then the forward
+  // transform at p_up, without the round trip through the upsampled grid)
+  int up_coef(const double *in, long nf, double *cu, hipStream_t st);
 
   SurfaceG S, Su;
   int p = 0, pu = 0;
@@ -129,6 +146,7 @@ class ForcesG {
   double grav[3] = {0.0, 0.0, -1.0};  // Parameters::gravity_field default (src/Parameters.cc:31-33)
   bool has_rho = false;
   DBuf xu, gu, fu, su1, su2, rcoef, rwork;
+  DBuf padmap;  // p_up coefficient -> order-p coefficient index, or -1
 };
 
 // StokesVelocity<Real>::operator() for surface targets, single layer, no

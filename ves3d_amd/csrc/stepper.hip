@@ -501,7 +501,8 @@ struct Stepper {
     if (prm.inner_max_iter < 0 || prm.inner_tol < 0 || prm.inner_tol >= 1 ||
         (prm.extrapolate != 0 && prm.extrapolate != 1) ||
         (prm.far_coarse != 0.0 && !(prm.far_coarse > 1.0)) ||
-        (prm.far_coarse2 != 0.0 && !(prm.far_coarse > 1.0 && prm.far_coarse2 > prm.far_coarse)))
+        (prm.far_coarse2 != 0.0 && !(prm.far_coarse > 1.0 && prm.far_coarse2 > prm.far_coarse)) ||
+        (prm.fallback != 0 && prm.fallback != 1))
       return VES3D_INVALID_PARAMETER;
     if (prm.p_up <= 0) prm.p_up = 2 * p;
     if (prm.filter_freq <= 0) prm.filter_freq = 2 * p / 3;
@@ -957,11 +958,12 @@ struct Stepper {
     if (learned && !(status == 100 && true_rel <= prm.tol * kTrueResSlack)) {
       // the learned preconditioner did not deliver: BiCGStab starts again
       // from the initial guess with the diagonal spectral preconditioner
-      // (ImplicitPrecond)
+      // (ImplicitPrecond), or unpreconditioned
       fell_back = true;
       V3D_CHECK(dev_copy(n, x0s, xsol, st));
       tol = prm.tol;
-      V3D_CHECK(solve(Pdiag_cb, prm.max_iter, &it, &tol, &status, &true_rel));
+      V3D_CHECK(solve(prm.fallback == 1 ? nullptr : Pdiag_cb, prm.max_iter, &it, &tol, &status,
+                      &true_rel));
     }
     // src/InterfacialVelocity.cc:176-188: a solve that failed, or whose
     // true residual misses the tolerance, is a DivergenceError; the state

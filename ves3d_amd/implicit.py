@@ -30,7 +30,7 @@ class StepParams(ctypes.Structure):
                 ("r_near", ctypes.c_double), ("post_off", ctypes.c_int),
                 ("inner_max_iter", ctypes.c_int), ("inner_tol", ctypes.c_double),
                 ("extrapolate", ctypes.c_int), ("far_coarse", ctypes.c_double),
-                ("far_coarse2", ctypes.c_double)]
+                ("far_coarse2", ctypes.c_double), ("fallback", ctypes.c_int)]
 
 
 class StepStats(ctypes.Structure):
@@ -60,11 +60,12 @@ class Stepper:
     def __init__(self, nves, p, dt=1e-3, shear_rate=0.1, bending_modulus=1e-2, tol=1e-6,
                  max_iter=200, precision=0, precond=1, p_up=0, filter_freq=0, near=True,
                  repul_dist=0.0, r_near=0.0, post=True, inner_max_iter=0, inner_tol=0.0,
-                 extrapolate=False, far_coarse=0.0, far_coarse2=0.0):
+                 extrapolate=False, far_coarse=0.0, far_coarse2=0.0, fallback="diagonal"):
         self.params = StepParams(nves, p, p_up, filter_freq, dt, shear_rate, bending_modulus, tol,
                                  max_iter, precision, precond, 0 if near else 1, repul_dist, r_near,
                                  0 if post else 1, inner_max_iter, inner_tol,
-                                 1 if extrapolate else 0, far_coarse, far_coarse2)
+                                 1 if extrapolate else 0, far_coarse, far_coarse2,
+                                 {"diagonal": 0, "none": 1}[fallback])
         self.nves, self.p = nves, p
         self.ng = 2 * p * (p + 1)
         h = ctypes.c_void_p()
