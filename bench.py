@@ -63,8 +63,8 @@ SL_FLOP_PER_PAIR = 28      # src/CPUKernels.cc:1008
 # HBM bytes per launch of k_far_groups_f32c at 512 vesicles, p = 16, from two
 # rocprofv3 --pmc runs (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a
 # 512 MiB device copy gives the two counters in the same runs), keyed by the
-# far_coarse setting they were measured at
-FAR32_TRAFFIC = {(3.0, 0.0): (2.0 * 258954.5 * 1024 + 52224.0 * 1024,
+# (far_coarse, far_coarse2) setting they were measured at
+FAR32_TRAFFIC = {(3.0, 6.0): (2.0 * 389723.9 * 1024 + 52224.0 * 1024,
                               "profiles/k_far_groups_f32c_hbm_counters_512ves_p16.md")}
 FAR_KINDS = {"lds": 0, "scalar": 1, "scalar2": 2}
 FAR32_NAMES = {"lds": "k_far_f32c<SL>", "scalar": "k_far_groups_f32c", "scalar2": "k_far_groups_f32c2"}
