@@ -810,10 +810,10 @@ def main():
                     help="the network's activations (BF16 on BF16 MFMA, or FP32)")
     ap.add_argument("--ref-tol", type=float, default=1e-10)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--inner-max-iter", type=int, default=6,
+    ap.add_argument("--inner-max-iter", type=int, default=3,
                     help="learned form: inner BiCGStab iterations on the vesicles' own operators "
                          "per preconditioner application (0: apply the network once)")
-    ap.add_argument("--inner-tol", type=float, default=1e-2)
+    ap.add_argument("--inner-tol", type=float, default=1e-1)
     ap.add_argument("--traj-steps", type=int, default=100,
                     help="trajectory workload: steps per form")
     ap.add_argument("--extrapolate", action="store_true",
