@@ -8,7 +8,10 @@ Bars (relative to the oracle output's max-norm unless stated):
                                                         the surfaces with a DL
                                                         density 5e-7 (interior
                                                         targets 0.05 below the
-                                                        surface reach 1.8e-7)
+                                                        surface reach 1.8e-7;
+                                                        the oracle itself moves
+                                                        by 1.4e-7 there when the
+                                                        targets move by 1e-12)
   step right-hand side and matvec with a contrast, FP64  2e-8
   step velocity with a contrast                        1e-5 vs the oracle's
                                                         solve to 1e-10
