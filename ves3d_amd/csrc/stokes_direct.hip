@@ -93,6 +93,8 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_f64(
   const long my_g = (grouped && tvalid) ? t / tpg : -1;
   const long tfirst = tb * FF_BLOCK;
   const long tlast = min(tfirst + FF_BLOCK, n_trg) - 1;
+  // the group holding all the block's targets (or -1), divided once
+  const long own_b = (grouped && tfirst / tpg == tlast / tpg) ? tfirst / tpg : -1;
 
   double Sx = 0, Sy = 0, Sz = 0, Qx = 0, Qy = 0, Qz = 0;
   long g0, g1;
@@ -100,7 +102,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_f64(
   for (long g = g0; g < g1; ++g) {
     long sb, se;
     if (grouped) {
-      if (tfirst / tpg == g && tlast / tpg == g) continue;  // block-uniform skip
+      if (g == own_b) continue;  // block-uniform skip
       sb = g * spg;
       se = sb + spg;
     } else {
@@ -190,6 +192,8 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_f32c(
   const long my_g = (grouped && tvalid) ? t / tpg : -1;
   const long tfirst = tb * FF_BLOCK;
   const long tlast = min(tfirst + FF_BLOCK, n_trg) - 1;
+  // the group holding all the block's targets (or -1), divided once
+  const long own_b = (grouped && tfirst / tpg == tlast / tpg) ? tfirst / tpg : -1;
 
   // (hi, lo) compensated running sums
   float Sh[3] = {0, 0, 0}, Sl[3] = {0, 0, 0};
@@ -199,7 +203,7 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_f32c(
   for (long g = g0; g < g1; ++g) {
     long sb, se;
     if (grouped) {
-      if (tfirst / tpg == g && tlast / tpg == g) continue;
+      if (g == own_b) continue;
       sb = g * spg;
       se = sb + spg;
     } else {
@@ -441,8 +445,11 @@ constexpr double kOwnGroupBias64 = 1e30;
 // scalar addressing.  A wave whose targets all belong to group g skips g.
 struct WaveSpan {
   long row, first, last;  // flag row (clamped to the last wave), first and last target
+  long own;               // the group that holds all the wave's targets, or -1
 };
-__device__ __forceinline__ WaveSpan wave_span(long tb, long n_trg) {
+// The divisions run once per wave here, not once per group in the source
+// loops (a 64-bit division is a long scalar sequence).
+__device__ __forceinline__ WaveSpan wave_span(long tb, long n_trg, long tpg) {
   const long wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const long w = tb * (FF_BLOCK / 64) + wv;
   const long n_waves = (n_trg + 63) / 64;
@@ -450,6 +457,8 @@ __device__ __forceinline__ WaveSpan wave_span(long tb, long n_trg) {
   r.row = w < n_waves ? w : n_waves - 1;
   r.first = w * 64;
   r.last = min(r.first + 64, n_trg) - 1;
+  const long gf = r.first / tpg, gl = r.last / tpg;
+  r.own = gf == gl ? gf : -1;
   return r;
 }
 
@@ -480,12 +489,12 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f64(
   const long tc = tvalid ? t : n_trg - 1;
   const double tx = tpos[3 * tc], ty = tpos[3 * tc + 1], tz = tpos[3 * tc + 2];
   const long my_g = tc / tpg;
-  const WaveSpan ws = wave_span(tb, n_trg);
+  const WaveSpan ws = wave_span(tb, n_trg, tpg);
   double Sx = 0, Sy = 0, Sz = 0;
   long g0, g1;
   slice_range(slice, nsplit, 0, spg, n_groups, g0, g1);
   for (long g = g0; g < g1; ++g) {
-    if (ws.first / tpg == g && ws.last / tpg == g) continue;  // wave-uniform skip
+    if (g == ws.own) continue;  // wave-uniform skip
     const double bias = my_g == g ? kOwnGroupBias64 : 0.0;
     const int lv = L.level(ws.row, n_groups, g);
     const FarRec64 *__restrict__ r = L.rec[lv] + g * L.cnt[lv];
@@ -539,12 +548,12 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c(
   const long tc = tvalid ? t : n_trg - 1;
   const double tx = tpos[3 * tc], ty = tpos[3 * tc + 1], tz = tpos[3 * tc + 2];
   const long my_g = tc / tpg;
-  const WaveSpan ws = wave_span(tb, n_trg);
+  const WaveSpan ws = wave_span(tb, n_trg, tpg);
   Acc32 S = {{0, 0, 0}, {0, 0, 0}};
   long g0, g1;
   slice_range(slice, nsplit, 0, spg, n_groups, g0, g1);
   for (long g = g0; g < g1; ++g) {
-    if (ws.first / tpg == g && ws.last / tpg == g) continue;  // wave-uniform skip
+    if (g == ws.own) continue;  // wave-uniform skip
     // a target's own group is not skipped lane by lane (that would make the
     // loop below divergent): its r^2 carries kOwnGroupBias instead
     const float bias = my_g == g ? kOwnGroupBias : 0.0f;
@@ -607,11 +616,12 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c2(
   const long ga = ca / tpg, gbb = cb / tpg;
   const long tfirst = tb * (2 * FF_BLOCK);
   const long tlast = min(tfirst + 2 * FF_BLOCK, n_trg) - 1;
+  const long own_b = tfirst / tpg == tlast / tpg ? tfirst / tpg : -1;
   Acc32 A = {{0, 0, 0}, {0, 0, 0}}, B = {{0, 0, 0}, {0, 0, 0}};
   long g0, g1;
   slice_range(slice, nsplit, 0, spg, n_groups, g0, g1);
   for (long g = g0; g < g1; ++g) {
-    if (tfirst / tpg == g && tlast / tpg == g) continue;  // block-uniform skip
+    if (g == own_b) continue;  // block-uniform skip
     const double *a = spos + 3 * g * spg;
     const float ax = (float)(Ax - a[0]), ay = (float)(Ay - a[1]), az = (float)(Az - a[2]);
     const float bx = (float)(Bx - a[0]), by = (float)(By - a[1]), bz = (float)(Bz - a[2]);
@@ -721,12 +731,12 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f32c_x2(
   const long tc = tvalid ? t : n_trg - 1;  // uniform control flow, as k_far_groups_f32c
   const double tx = tpos[3 * tc], ty = tpos[3 * tc + 1], tz = tpos[3 * tc + 2];
   const long my_g = tc / tpg;
-  const WaveSpan ws = wave_span(tb, n_trg);
+  const WaveSpan ws = wave_span(tb, n_trg, tpg);
   Acc32 A = {{0, 0, 0}, {0, 0, 0}}, B = {{0, 0, 0}, {0, 0, 0}};
   long g0, g1;
   slice_range(slice, nsplit, 0, spg, n_groups, g0, g1);
   for (long g = g0; g < g1; ++g) {
-    if (ws.first / tpg == g && ws.last / tpg == g) continue;  // wave-uniform skip
+    if (g == ws.own) continue;  // wave-uniform skip
     const float bias = my_g == g ? kOwnGroupBias : 0.0f;
     const double *a = spos + 3 * g * spg;
     const float lx = (float)(tx - a[0]), ly = (float)(ty - a[1]), lz = (float)(tz - a[2]);
@@ -782,12 +792,12 @@ __global__ __launch_bounds__(FF_BLOCK) void k_far_groups_f64_x2(
   const long tc = tvalid ? t : n_trg - 1;  // uniform control flow, as k_far_groups_f64
   const double tx = tpos[3 * tc], ty = tpos[3 * tc + 1], tz = tpos[3 * tc + 2];
   const long my_g = tc / tpg;
-  const WaveSpan ws = wave_span(tb, n_trg);
+  const WaveSpan ws = wave_span(tb, n_trg, tpg);
   double Sx = 0, Sy = 0, Sz = 0, Tx = 0, Ty = 0, Tz = 0;
   long g0, g1;
   slice_range(slice, nsplit, 0, spg, n_groups, g0, g1);
   for (long g = g0; g < g1; ++g) {
-    if (ws.first / tpg == g && ws.last / tpg == g) continue;  // wave-uniform skip
+    if (g == ws.own) continue;  // wave-uniform skip
     const double bias = my_g == g ? kOwnGroupBias64 : 0.0;
     const int lv = L.level(ws.row, n_groups, g);
     const FarRec64x2 *__restrict__ r = L.rec[lv] + g * L.cnt[lv];
