@@ -311,6 +311,19 @@ int ves3d_hip_rotate_all_f64(const double *S, size_t N, int p, int dof,
 int ves3d_hip_rotate_transpose_f64(const double *S_in, size_t N, int p, int dof,
                                    double *S_out, ves3d_stream_t stream);
 
+/* Kernel choice of the singular-operator build (no counterpart in the
+ * reference, global per process): 1 (the default) evaluates the FP64 build's
+ * pole kernel and its transposed synthesis (Fourier stage on FP64 MFMA,
+ * Legendre stage) in one kernel, where p1 = 2 p0 and p0 <= 16; 0 runs them
+ * as the pole kernel and two grouped GEMMs with the kernel values in HBM
+ * between them (the FP32 build always does). */
+int ves3d_hip_set_pole_kernel(int fused);
+/* The fused stage's launch on the current device for orders (p0, p1) of the
+ * FP64 build (no counterpart in the reference): latitude rows per pass,
+ * threads per workgroup and the persistent grid's workgroups; all 0 where
+ * the build runs the separate kernels. */
+int ves3d_hip_pole_kernel_shape(int p0, int p1, int *rows_host, int *threads_host,
+                                int *workgroups_host);
 /* StokesSingularInteg (include/SphericalHarmonics.h:40,
  * src/SphericalHarmonics.cc:862-886, :952-1180): per-vesicle single- and
  * double-layer self-interaction operators on SH coefficients.

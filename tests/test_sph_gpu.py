@@ -195,3 +195,35 @@ def test_self_apply(hip, dt, tol):
     ref1 = np.einsum("vr,vrc->vc", F, SL)
     G.self_apply(cu(SL, dt), cu(F), None, None, nv, p, V)
     assert rel(V.cpu().numpy(), ref1) < tol
+
+
+@pytest.mark.parametrize("p0", [16, 8])
+def test_singular_build_fused_pole_stage(hip, p0):
+    """The FP64 build's fused pole kernel + transposed synthesis (the default
+    where p1 = 2 p0, p0 <= 16) against the separate kernel and grouped GEMMs,
+    SL and DL.  Asserts that the fused stage applies on this device."""
+    import ctypes
+    from ves3d_amd import call
+    nv, p1, dt, tol = 3, 2 * p0, torch.float64, 1e-12
+    rows, threads, groups = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    call("ves3d_hip_pole_kernel_shape", p0, p1, ctypes.byref(rows), ctypes.byref(threads),
+         ctypes.byref(groups))
+    print("fused pole stage p0=%d: %d rows per pass, %d threads, %d workgroups"
+          % (p0, rows.value, threads.value, groups.value), flush=True)
+    assert rows.value >= 1 and threads.value >= 64 and groups.value >= 1
+    X = _singular_case(nv, p0, seed=41)
+    n = 3 * p0 * (p0 + 2)
+    out = {}
+    try:
+        for fused in (1, 0):
+            call("ves3d_hip_set_pole_kernel", fused)
+            sl = torch.zeros((nv, n, n), dtype=dt, device="cuda")
+            dl = torch.zeros((nv, n, n), dtype=dt, device="cuda")
+            G.singular_build(cu(X), nv, p0, p1, sl, dl)
+            torch.cuda.synchronize()
+            out[fused] = (sl.double().cpu().numpy(), dl.double().cpu().numpy())
+    finally:
+        call("ves3d_hip_set_pole_kernel", 1)
+    e_sl, e_dl = rel(out[1][0], out[0][0]), rel(out[1][1], out[0][1])
+    print("fused vs separate pole stage (p0=%d): SL %.2e, DL %.2e" % (p0, e_sl, e_dl), flush=True)
+    assert e_sl < tol and e_dl < tol

@@ -73,6 +73,9 @@ for _t, _real in (("f64", c_double), ("f32", ctypes.c_float)):
 _sig("ves3d_hip_set_gemm_kernel", c_int)
 _sig("ves3d_hip_set_gemm_tile", c_int)
 _sig("ves3d_hip_set_far_kernel", c_int)
+_sig("ves3d_hip_set_pole_kernel", c_int)
+_sig("ves3d_hip_pole_kernel_shape", c_int, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
+     ctypes.POINTER(c_int))
 _sig("ves3d_hip_valu_probe", c_int, P)
 _sig("ves3d_hip_stokes_all2all_f64", P, P, S, P, P)
 _sig("ves3d_hip_stokes_all2all_f32c", P, P, S, P, P)

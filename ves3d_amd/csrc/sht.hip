@@ -176,6 +176,22 @@ int sht_shc2grid_transpose(const T *X, long N, int p0, int p1, T *S, T *work,
   return run<T>(kTLeg, p0, p1, N, m, work, S, st);
 }
 
+template <typename T>
+const T *sht_mft(int p0, int p1) {
+  if (!valid_order(p0) || !valid_order(p1)) return nullptr;
+  const ShMatsDev *m = sh_mats(p0, p1);
+  if (!m) return nullptr;
+  return mats_ptr<T>(m) + m->h->MfT;
+}
+
+template <typename T>
+const T *sht_mlt(int p0, int p1) {
+  if (!valid_order(p0) || !valid_order(p1)) return nullptr;
+  const ShMatsDev *m = sh_mats(p0, p1);
+  if (!m) return nullptr;
+  return mats_ptr<T>(m) + m->h->MlT[0];
+}
+
 // ---------------------------------------------------------------- poles
 namespace {
 std::map<std::pair<int, int>, double *> g_qp;  // (device, p) -> [2][p+1]
@@ -230,7 +246,9 @@ int sht_shc2pole(const T *S, long N, int p0, T *P, hipStream_t st) {
   template int sht_grid2shc<T>(const T *, long, int, int, T *, T *, hipStream_t);          \
   template int sht_shc2grid_transpose<T>(const T *, long, int, int, T *, T *, hipStream_t); \
   template int sht_shc2pole<T>(const T *, long, int, T *, hipStream_t);                 \
-  template int sht_synth<T>(const T *, long, int, int, int, int, T *, T *, hipStream_t, T);
+  template int sht_synth<T>(const T *, long, int, int, int, int, T *, T *, hipStream_t, T); \
+  template const T *sht_mft<T>(int, int);                                                  \
+  template const T *sht_mlt<T>(int, int);
 V3D_INST(double)
 V3D_INST(float)
 #undef V3D_INST

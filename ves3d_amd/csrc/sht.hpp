@@ -34,4 +34,13 @@ int sht_synth(const T *S, long N, int p0, int p1, int dleg, int dfour, T *X, T *
 template <typename T>
 int sht_shc2pole(const T *S, long N, int p0, T *P, hipStream_t st);
 
+// The matrices of SHC2GridTranspose (p1 -> p0) on the device, for a caller
+// that runs its stages itself: sht_mft gives MatFourier(p1, p0) ([2p1][2p0]:
+// frequency row jj, longitude column), sht_mlt MatLegendre(p1, p0)[m] for
+// m = 0..p1, consecutive blocks [(p1+1-m)][p0+1].
+template <typename T>
+const T *sht_mft(int p0, int p1);
+template <typename T>
+const T *sht_mlt(int p0, int p1);
+
 }  // namespace v3d

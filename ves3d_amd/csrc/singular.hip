@@ -16,10 +16,12 @@
 // The FP32 form runs every GEMM stage in FP32 and keeps the pole kernel
 // (singular weights, 1/r near the rotated pole) in FP64.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "runtime.hpp"
@@ -30,6 +32,10 @@ namespace v3d {
 namespace {
 
 std::mutex g_mu;
+// 1: the singular build's pole kernel runs fused with the transposed
+// synthesis (k_pole_transpose) where its shapes allow; 0: as separate
+// kernel and grouped GEMMs (ves3d_hip_set_pole_kernel)
+std::atomic<int> g_pole_fused{1};
 struct RotPlan {
   GemmDesc *d;
   int n, tile;
@@ -309,6 +315,192 @@ __global__ void k_pole_kernel(const T *__restrict__ X, const T *__restrict__ Xt,
   }
 }
 
+// The pole kernel fused with SHC2GridTranspose(p1 -> p0) of its values:
+// K1[r][t*6+c][k] = sum over the grid (lat, lon) of the kernel value of
+// rotation r, pole t, tensor component c at (lat, lon), times MatFourier(p0,
+// p1)[jj][lon] (the Fourier stage) and MatLegendre(p0, p1)[m][n][lat] (the
+// Legendre stage), k = blockoff(p0, jj) + n, m = (jj + 1) / 2 -- the products
+// the two grouped GEMMs of sht_shc2grid_transpose form, without the kernel
+// values (12 functions on the upsampled grid per rotation) or the Fourier
+// stage's output passing through HBM.
+// Persistent workgroups walk the rotations r = blockIdx.x + i gridDim.x and,
+// within a rotation, the latitude rows in passes of R rows:
+//   1. thread (row, lon, t) evaluates pole t's 6 kernel values at (lat, lon)
+//      into Kv[row][lon][t * 6 + q] (kPtKs doubles per point, an odd stride);
+//   2. wave (row, tile) forms the 16 x 16 tile T[row][c][16 tile + j] =
+//      sum_lon Kv[row][lon][c] Mf[lon][16 tile + j] on v_mfma_f64_16x16x4f64
+//      (lane l: A(l % 16, l / 16), B(l / 16, l % 16), results at rows
+//      l / 16 + 4 i, column l % 16; rows c >= 12 read as zero), its B operand
+//      (the wave's 16 columns of MatFourier) in registers for the whole kernel;
+//   3. thread (c, jj) adds T[row][c][jj] times the rows' Legendre values into
+//      its p0 + 1 - m accumulators (FP64 registers).
+// Two barriers per pass; each kernel-value thread loads the geometry of its
+// point of the next pass while the current pass runs.  At a rotation's end
+// the accumulators go through LDS (over Kv and T) in the output layout and
+// leave as whole rows, as many rows per round as that space holds.  The Legendre blocks and the weights sit in LDS for the workgroup's
+// lifetime.
+constexpr int kPtKs = 13;
+
+// LDS doubles of k_pole_transpose: Kv, T, the weights, the Legendre blocks
+// and LMAX - 1 zero rows after them (the branch-free degree loop's reads
+// past a block's end).
+inline long pole_lds_doubles(int R, int p0, int p1, int lmax) {
+  const long L = 2 * p1, Nlat = p1 + 1, FP = 16 * ((2L * p0 + 15) / 16);
+  return R * L * kPtKs + R * 12 * FP + Nlat + (long)(p0 + 1) * (p0 + 2) / 2 * Nlat +
+         (lmax - 1) * Nlat;
+}
+
+template <int LMAX, int P1>
+__global__ __launch_bounds__(384) void k_pole_transpose(
+    const double *__restrict__ X, const double *__restrict__ Xt, const double *__restrict__ Xp,
+    const double *__restrict__ trg, const double *__restrict__ qw, long nrot, int p0, int R,
+    const double *__restrict__ mft, const double *__restrict__ mlt, int layer,
+    double *__restrict__ K1) {
+  typedef double v4d __attribute__((ext_vector_type(4)));
+  constexpr int L = 2 * P1, Nlat = P1 + 1, M1 = L * Nlat, KS = L / 4;
+  extern __shared__ double lds[];
+  const int F = 2 * p0, NT = (F + 15) / 16, FP = 16 * NT, Nc = p0 * (p0 + 2);
+  const int nmlt = (p0 + 1) * (p0 + 2) / 2 * Nlat;
+  double *Kv = lds;                 // [R][L][kPtKs]
+  double *Ts = Kv + R * L * kPtKs;  // [R][12][FP]
+  double *Qw = Ts + R * 12 * FP;    // [Nlat]
+  double *Ml = Qw + Nlat;           // blocks m = 0..p0: [(p0+1-m)][Nlat], then zeros
+  double *St = lds;                 // [cs][Nc], over Kv and Ts at a rotation's end
+  const int cs = min(12, (R * L * kPtKs + R * 12 * FP) / Nc);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int i = tid; i < nmlt + (LMAX - 1) * Nlat; i += blockDim.x)
+    Ml[i] = i < nmlt ? mlt[i] : 0.0;
+  for (int i = tid; i < Nlat; i += blockDim.x) Qw[i] = qw[i];
+  // kernel-value role: point (krow, lon) of the pass, pole t
+  const bool kvt = tid < R * 2 * L;
+  const int krow = kvt ? tid / (2 * L) : 0, lon = kvt ? (tid >> 1) % L : 0, t = tid & 1;
+  // Fourier role (wave-uniform): row frow, output columns 16 tile .. 16 tile + 15
+  const bool fw = wv < R * NT;
+  const int frow = fw ? wv / NT : 0, tile = fw ? wv % NT : 0;
+  const int li = lane & 15, lk = lane >> 4;
+  double mf[KS];
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    const int col = 16 * tile + li;
+    mf[s] = (fw && col < F) ? mft[(long)col * L + 4 * s + lk] : 0.0;
+  }
+  // Legendre role: component c, frequency row jj of the order-p0 layout,
+  // whose block starts at off (after block 0 of length p0 + 1, the blocks
+  // of order m' = 1 .. p0 come in pairs of length p0 + 1 - m')
+  const bool lt = tid < 12 * F;
+  const int c = lt ? tid / F : 0, jj = lt ? tid % F : 0;
+  const int m = (jj + 1) / 2, len = p0 + 1 - m;
+  const int mloff = Nlat * (m * (p0 + 1) - m * (m - 1) / 2);
+  const int off = jj == 0 ? 0 : (p0 + 1) + 2 * ((m - 1) * (p0 + 1) - (m - 1) * m / 2) +
+                                    ((jj % 2 == 0) ? (p0 + 1 - m) : 0);
+  const int ng = (Nlat + R - 1) / R;
+  const double eps = 2.220446049250313e-16;
+  const double csl = 1.0 / (8.0 * kPi), cdl = 3.0 / (4.0 * kPi);
+  // geometry and poles of the next (rotation, pass), loaded by every thread
+  // at indices clamped into range: unconditional loads leave no value to
+  // merge after a branch, so nothing waits for them before their use in the
+  // next pass
+  double g[9], tg[3];
+  auto load = [&](long r_, int g_) {
+    const long rc = r_ < nrot ? r_ : nrot - 1;
+    const int lat_ = min(g_ * R + krow, Nlat - 1);
+    const long b = rc * 3 * (long)M1 + (long)lat_ * L + lon;
+    g[0] = X[b]; g[1] = X[b + M1]; g[2] = X[b + 2 * M1];
+    g[3] = Xt[b]; g[4] = Xt[b + M1]; g[5] = Xt[b + 2 * M1];
+    g[6] = Xp[b]; g[7] = Xp[b + M1]; g[8] = Xp[b + 2 * M1];
+    for (int d = 0; d < 3; ++d) tg[d] = trg[(rc * 3 + d) * 2 + t];
+  };
+  load(blockIdx.x, 0);
+  __syncthreads();  // Ml, Qw in
+  for (long r = blockIdx.x; r < nrot; r += gridDim.x) {
+    const double tx = tg[0], ty = tg[1], tz = tg[2];
+    double acc[LMAX];
+#pragma unroll
+    for (int n = 0; n < LMAX; ++n) acc[n] = 0.0;
+    for (int gi = 0; gi < ng; ++gi) {
+      const double x0 = g[0], y0 = g[1], z0 = g[2];
+      const double xt = g[3], yt = g[4], zt = g[5];
+      const double xp = g[6], yp = g[7], zp = g[8];
+      if (gi + 1 < ng)
+        load(r, gi + 1);
+      else
+        load(r + gridDim.x, 0);
+      const int lat = gi * R + krow;
+      if (kvt && lat < Nlat) {
+        const double nx = yt * zp - zt * yp, ny = zt * xp - xt * zp, nz = xt * yp - yt * xp;
+        const double w = Qw[t ? lat : P1 - lat];
+        const double dx = tx - x0, dy = ty - y0, dz = tz - z0;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        double rinv = 1.0 / sqrt(r2);
+        if (r2 <= eps) rinv = 0;
+        const double rinv2 = rinv * rinv;
+        const double dd[6] = {dx * dx, dx * dy, dx * dz, dy * dy, dy * dz, dz * dz};
+        double *kv = Kv + (krow * L + lon) * kPtKs + t * 6;
+        if (layer == 1) {
+          const double rn5 = cdl * w * (nx * dx + ny * dy + nz * dz) * rinv2 * rinv2 * rinv;
+#pragma unroll
+          for (int q = 0; q < 6; ++q) kv[q] = dd[q] * rn5;
+        } else {
+          const double area = sqrt(nx * nx + ny * ny + nz * nz);
+          const double ar = csl * w * area * rinv;
+          const double ar2 = ar * rinv2;
+#pragma unroll
+          for (int q = 0; q < 6; ++q)
+            kv[q] = dd[q] * ar2 + ((q == 0 || q == 3 || q == 5) ? ar : 0.0);
+        }
+      }
+      __syncthreads();  // Kv of the pass is in; the previous pass's T is consumed
+      if (fw && gi * R + frow < Nlat) {
+        const double *kv = Kv + frow * L * kPtKs + lk * kPtKs + li;
+        v4d d = {0.0, 0.0, 0.0, 0.0}, e = {0.0, 0.0, 0.0, 0.0};
+        double a[KS];
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+          const double v = kv[4 * s * kPtKs];
+          a[s] = li < 12 ? v : 0.0;
+        }
+#pragma unroll
+        for (int s = 0; s < KS; s += 2) {  // two accumulation chains
+          d = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], mf[s], d, 0, 0, 0);
+          e = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s + 1], mf[s + 1], e, 0, 0, 0);
+        }
+        d += e;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) Ts[(frow * 12 + lk + 4 * i) * FP + 16 * tile + li] = d[i];
+      }
+      __syncthreads();  // T of the pass is in
+      if (lt) {
+#pragma unroll 1
+        for (int row = 0; row < R; ++row) {
+          const int lat = gi * R + row;
+          if (lat >= Nlat) break;
+          // branch-free: degrees n >= len read finite values past the block
+          // (the next blocks, then zeros) into accumulators that are never
+          // written out
+          const double f = Ts[(row * 12 + c) * FP + jj];
+          const double *ml = Ml + mloff + lat;
+#pragma unroll
+          for (int n = 0; n < LMAX; ++n) acc[n] += f * ml[n * Nlat];
+        }
+      }
+    }
+    // the rotation's 12 output rows, cs rows per round through St
+    double *out = K1 + r * 12 * (long)Nc;
+    for (int c0 = 0; c0 < 12; c0 += cs) {
+      __syncthreads();  // T, or the previous round of St, is read
+      if (lt && c >= c0 && c < c0 + cs) {
+#pragma unroll
+        for (int n = 0; n < LMAX; ++n)
+          if (n < len) St[(c - c0) * Nc + off + n] = acc[n];
+      }
+      __syncthreads();
+      const int nrow = min(cs, 12 - c0);
+      for (int i = tid; i < nrow * Nc; i += blockDim.x) out[c0 * Nc + i] = St[i];
+    }
+    __syncthreads();  // St read before the next rotation's kernel values
+  }
+}
+
 // M1arr[v][a*Ncoef+k][b][lat][lon] from K2[v][j1][j2][t2*6+c][k]
 // (src/SphericalHarmonics.cc:1090-1131): t2=0 pole -> grid (j1, j2),
 // t2=1 pole -> grid (p0-j1, p0+j2); c = sym(a, b).  For fixed (v, a, b) this
@@ -388,8 +580,10 @@ const double *sing_weights_dev(int p1) {
 }
 }  // namespace
 
-// Elements of scratch the build needs per vesicle of a chunk.
-static long singular_elems_per_ves(int p0, int p1) {
+// Elements of scratch the build needs per vesicle of a chunk; the fused
+// pole stage needs neither the kernel values K0 nor the transposed
+// synthesis's scratch.
+static long singular_elems_per_ves(int p0, int p1, bool fused) {
   const long Nc = sh_ncoef(p0), Ng = sh_ngrid(p0), M1 = sh_ngrid(p1);
   const long nr = (long)(p0 + 1) * p0;  // rotations per vesicle
   long e = 0;
@@ -398,15 +592,71 @@ static long singular_elems_per_ves(int p0, int p1) {
   e += nr * 3 * Nc;                     // S (rotated)
   e += 3 * nr * 3 * M1;                 // X, Xt, Xp
   e += nr * 3 * 2;                      // poles
-  e += nr * 12 * M1;                    // K0
+  if (!fused) e += nr * 12 * M1;        // K0
   e += nr * 12 * Nc;                    // K1
   e += nr * 12 * Nc;                    // K2
   e += 3 * Nc * 3 * Ng;                 // M1 rearranged
-  long w1 = std::max(shc2grid_work(nr * 3, p0, p1), grid2shc_work(nr * 12, p1, p0));
+  long w1 = shc2grid_work(nr * 3, p0, p1);
+  if (!fused) w1 = std::max(w1, grid2shc_work(nr * 12, p1, p0));
   w1 = std::max(w1, grid2shc_work(9 * Nc, p0, p0));
   w1 = std::max(w1, 3 * nr * 12 * Nc);  // rotation scratch
   e += w1;
   return e;
+}
+
+// Launch shape of k_pole_transpose for (p0, p1) on the current device:
+// R latitude rows per pass (3 where the LDS allows, else fewer), threads
+// covering the three roles, and the persistent grid's workgroups per CU;
+// threads == 0 where the fused stage does not apply.
+struct PoleShape {
+  int R = 0, threads = 0, blocks = 0;
+  size_t lds = 0;
+  bool small = false;  // the <9, 16> instance (p0 <= 8, p1 = 16)
+};
+
+PoleShape pole_fused_shape(int p0, int p1) {
+  PoleShape sh;
+  // the instances' shapes: p0 <= 8 with p1 = 16, p0 <= 16 with p1 = 32
+  // (the build's upsampled order 2 p0)
+  const bool small = p0 <= 8 && p1 == 16;
+  if (!small && !(p0 <= 16 && p1 == 32)) return sh;
+  const int L = 2 * p1, F = 2 * p0, NT = (F + 15) / 16, FP = 16 * NT;
+  const long Nc = sh_ncoef(p0);
+  int dev = 0, maxlds = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+          hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return sh;
+  }
+  const void *kern = small ? (const void *)k_pole_transpose<9, 16>
+                           : (const void *)k_pole_transpose<17, 32>;
+  for (int R = 3; R >= 1; --R) {
+    const int threads =
+        64 * std::max({(int)cdiv(2L * R * L, 64), R * NT, (int)cdiv(12L * F, 64)});
+    const long stage = (long)R * L * kPtKs + (long)R * 12 * FP;
+    const size_t lds = sizeof(double) * (size_t)pole_lds_doubles(R, p0, p1, small ? 9 : 17);
+    if (threads > 384 || stage < Nc || lds > (size_t)maxlds) continue;
+    if (lds > 64 * 1024 && hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)lds) != hipSuccess) {
+      (void)hipGetLastError();
+      continue;
+    }
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, threads, lds) != hipSuccess) {
+      (void)hipGetLastError();
+      continue;
+    }
+    if (nb < 1) continue;
+    sh.R = R;
+    sh.threads = threads;
+    sh.blocks = nb * cus;
+    sh.lds = lds;
+    sh.small = small;
+    return sh;
+  }
+  return sh;
 }
 
 template <typename T>
@@ -418,7 +668,13 @@ int singular_build(const double *X0, long Nv, int p0, int p1, T *SL, T *DL, hipS
   const long nr = (long)(p0 + 1) * p0;
   const double *qw = sing_weights_dev(p1);
   if (!qw) return VES3D_MEMORY_ERROR;
-  const long per = singular_elems_per_ves(p0, p1);
+  // the fused stage serves the FP64 build: with FP32 transforms the
+  // separate kernels' FP32 GEMMs are faster than its FP64 arithmetic
+  const PoleShape ps = std::is_same<T, double>::value && g_pole_fused.load(std::memory_order_relaxed)
+                           ? pole_fused_shape(p0, p1)
+                           : PoleShape();
+  const bool fused = ps.threads > 0;
+  const long per = singular_elems_per_ves(p0, p1, fused);
   const long budget = (6L << 30) / (long)sizeof(T);
   long chunk = std::max(1L, std::min(Nv, budget / per));
   T *ws = (T *)workspace(kWsSing0, sizeof(T) * per * chunk);
@@ -434,7 +690,7 @@ int singular_build(const double *X0, long Nv, int p0, int p1, T *SL, T *DL, hipS
     T *Xt = p; p += rot * 3 * M1;
     T *Xp = p; p += rot * 3 * M1;
     T *P = p; p += rot * 3 * 2;
-    T *K0 = p; p += rot * 12 * M1;
+    T *K0 = p; if (!fused) p += rot * 12 * M1;
     T *K1 = p; p += rot * 12 * Nc;
     T *K2 = p; p += rot * 12 * Nc;
     T *M1a = p; p += cv * 3 * Nc * 3 * Ng;
@@ -449,10 +705,27 @@ int singular_build(const double *X0, long Nv, int p0, int p1, T *SL, T *DL, hipS
     for (int layer = 0; layer < 2; ++layer) {
       T *dst = layer == 0 ? SL : DL;
       if (!dst) continue;
-      k_pole_kernel<T><<<grid_for(rot * M1), 256, 0, st>>>(
-          X, Xt, Xp, P, qw, rot, p1, layer == 0 ? K0 : nullptr, layer == 1 ? K0 : nullptr);
-      V3D_CHECK(launch_status());
-      V3D_CHECK(sht_shc2grid_transpose<T>(K0, rot * 12, p1, p0, K1, w, st));
+      if constexpr (std::is_same<T, double>::value) {
+        if (fused) {
+          const double *mft = sht_mft<double>(p1, p0);
+          const double *mlt = sht_mlt<double>(p1, p0);
+          if (!mft || !mlt) return VES3D_MEMORY_ERROR;
+          const unsigned nb = (unsigned)std::min<long>(rot, ps.blocks);
+          if (ps.small)
+            k_pole_transpose<9, 16><<<nb, ps.threads, ps.lds, st>>>(
+                X, Xt, Xp, P, qw, rot, p0, ps.R, mft, mlt, layer, K1);
+          else
+            k_pole_transpose<17, 32><<<nb, ps.threads, ps.lds, st>>>(
+                X, Xt, Xp, P, qw, rot, p0, ps.R, mft, mlt, layer, K1);
+          V3D_CHECK(launch_status());
+        }
+      }
+      if (!fused) {
+        k_pole_kernel<T><<<grid_for(rot * M1), 256, 0, st>>>(
+            X, Xt, Xp, P, qw, rot, p1, layer == 0 ? K0 : nullptr, layer == 1 ? K0 : nullptr);
+        V3D_CHECK(launch_status());
+        V3D_CHECK(sht_shc2grid_transpose<T>(K0, rot * 12, p1, p0, K1, w, st));
+      }
       V3D_CHECK(rotate_transpose<T>(K1, cv, p0, 12, K2, w, st));
       const long ra_blocks = cv * 9L * cdiv(Nc, kRaK) * cdiv(Ng, kRaG);
       if (ra_blocks > 0x7fffffffL) return VES3D_INVALID_PARAMETER;
@@ -552,6 +825,21 @@ int ves3d_hip_rotate_transpose_f64(const double *S_in, size_t N, int p, int dof,
   double *w = (double *)workspace(kWsSing1, sizeof(double) * 2L * N * (p + 1) * p * dof * nc);
   if (!w) return VES3D_MEMORY_ERROR;
   return rotate_transpose<double>(S_in, (long)N, p, dof, S_out, w, (hipStream_t)stream);
+}
+int ves3d_hip_pole_kernel_shape(int p0, int p1, int *rows_host, int *threads_host,
+                                int *workgroups_host) {
+  if (p0 < 1 || p1 < 1 || p0 > 128 || p1 > 255) return VES3D_INVALID_PARAMETER;
+  const PoleShape ps =
+      g_pole_fused.load(std::memory_order_relaxed) ? pole_fused_shape(p0, p1) : PoleShape();
+  if (rows_host) *rows_host = ps.R;
+  if (threads_host) *threads_host = ps.threads;
+  if (workgroups_host) *workgroups_host = ps.blocks;
+  return VES3D_SUCCESS;
+}
+int ves3d_hip_set_pole_kernel(int fused) {
+  if (fused != 0 && fused != 1) return VES3D_INVALID_PARAMETER;
+  g_pole_fused.store(fused);
+  return VES3D_SUCCESS;
 }
 int ves3d_hip_singular_build_f64(const double *X0, size_t nves, int p0, int p1, double *SL,
                                  double *DL, ves3d_stream_t stream) {
