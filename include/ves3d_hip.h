@@ -366,6 +366,18 @@ int ves3d_hip_bicgstab(size_t n, ves3d_apply_fn A, const void *A_ctx,
                        ves3d_apply_fn P, const void *P_ctx, double *x,
                        const double *b, int *max_iter, double *tol,
                        int *bicgs_status, ves3d_stream_t stream);
+/* ParallelLinSolver<T>::Solve of the PETSc implementation (KSP GMRES with a
+ * shell preconditioner, src/ParallelLinSolver_Petsc.cc:545-551, :600-634) as
+ * flexible GMRES: right-preconditioned, restarted every `restart` (1..63)
+ * iterations, the preconditioned vectors kept so that P may change between
+ * applications.  Classical Gram-Schmidt applied twice with FP64 dots.  On
+ * return *max_iter holds the iterations (operator applications after the
+ * initial residual), *tol the relative residual ||b - A x|| / ||b|| formed
+ * explicitly from the Arnoldi basis, *status 100 (converged), 101 (iteration
+ * limit) or 104 (NaN).  Allocates (2 restart + 1) n doubles of scratch. */
+int ves3d_hip_fgmres(size_t n, ves3d_apply_fn A, const void *A_ctx, ves3d_apply_fn P,
+                     const void *P_ctx, double *x, const double *b, int *max_iter, double *tol,
+                     int *status, int restart, ves3d_stream_t stream);
 
 /* ---- StokesVelocity<Real> (src/StokesVelocity.cc:146-539) --------------
  * The Stokes velocity of a single-layer density f and a double-layer
@@ -446,10 +458,18 @@ typedef struct ves3d_step_params {
                              tolerance, the step solves again from the same initial
                              guess: 0 with the diagonal spectral preconditioner
                              (ImplicitPrecond), 1 unpreconditioned. */
+  int solver;             /* outer solver: 0 right-preconditioned BiCGStab
+                             (src/BiCGStab.cc:45-138) with a true-residual check;
+                             1 flexible GMRES (ves3d_hip_fgmres), the GMRES of the
+                             reference's globally implicit solve
+                             (src/ParallelLinSolver_Petsc.cc:545-551) in the form a
+                             varying preconditioner needs. */
+  int restart;            /* FGMRES restart length (0: 30; at most 63) */
 } ves3d_step_params;
 
 typedef struct ves3d_step_stats {
-  int iterations;         /* BiCGStab iterations */
+  int iterations;         /* outer solver iterations (FGMRES: operator applications
+                             after the initial residual) */
   int matvecs;            /* operator applications (incl. residual checks) */
   int status;             /* BiCGSReturn code */
   int fell_back;          /* 1 if the learned solve failed or missed the tolerance

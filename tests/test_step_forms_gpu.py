@@ -14,14 +14,18 @@ Forms:
             pairs from plain solves of other suspensions) inside inner
             BiCGStab solves (6 iterations, 1e-2), as in the benchmark.
 
-Bars, both forms, against the oracle's solve of the same system to 1e-10
+Each form runs with either outer solver (BiCGStab with its true-residual
+check, or flexible GMRES with its explicitly formed residual).
+
+Bars, every form, against the oracle's solve of the same system to 1e-10
 (oracle/ves.py Implicit + bicgstab):
   velocity                        1e-5 relative (L2)
   residual under the FP64 operator 1e-6 (the step's own check, recomputed
                                   by a precision-0 stepper's residual())
   learned form                    no fallback
-At 512 vesicles, p = 16 (no oracle at that size): both forms converge, both
-meet 1e-6 under the FP64 operator, and their velocities agree to 1e-5.
+At 512 vesicles, p = 16 (no oracle at that size): the plain form and the
+headline form with either solver converge, meet 1e-6 under the FP64
+operator, and their velocities agree to 1e-5.
 """
 import numpy as np
 import pytest
@@ -108,10 +112,13 @@ def test_both_forms_vs_oracle(hip):
                                   epochs=300, batch=256)
     print("trained on %d samples, loss %.3e" % (info["samples"], info["loss"]), flush=True)
     forms = {"plain": dict(precision=0, precond=1),
-             "learned": dict(precision=2, precond=2, inner_max_iter=6, inner_tol=1e-2)}
+             "learned": dict(precision=2, precond=2, inner_max_iter=6, inner_tol=1e-2),
+             "plain fgmres": dict(precision=0, precond=1, solver="fgmres"),
+             "learned fgmres": dict(precision=2, precond=2, inner_max_iter=6, inner_tol=1e-2,
+                                    solver="fgmres")}
     for name, kw in forms.items():
         st = I.Stepper(nv, p, **kw, **COMMON)
-        if name == "learned":
+        if name.startswith("learned"):
             I.attach(st, pc, collect=False)
         st.set_state(cu(X))
         s = st.step()
@@ -222,11 +229,13 @@ def test_full_size_forms_agree(hip):
     print("stored network: %d samples from %d plain-solve steps" %
           (meta["samples"], meta["plain_solve_steps"]), flush=True)
     sol = {}
+    headline = dict(precision=2, precond=2, inner_max_iter=3, inner_tol=1e-1, far_coarse=3.0,
+                    far_coarse2=6.0)
     for name, kw in (("plain", dict(precision=0, precond=1)),
-                     ("learned", dict(precision=2, precond=2, inner_max_iter=6,
-                                      inner_tol=1e-2, far_coarse=3.0, far_coarse2=6.0))):
+                     ("learned", dict(headline)),
+                     ("learned fgmres", dict(headline, solver="fgmres"))):
         st = I.Stepper(nv, p, **kw, **full)
-        if name == "learned":
+        if name.startswith("learned"):
             I.attach(st, pc, collect=False)
         st.set_state(cu(X))
         s = st.step()
@@ -236,10 +245,11 @@ def test_full_size_forms_agree(hip):
         print("%s: %d iterations, %.0f ms, true relres %.2e, FP64 relres %.2e" %
               (name, s["iterations"], s["ms_total"], s["true_relres"], r64), flush=True)
         assert s["status"] == 100 and r64 <= 1e-6, (name, s, r64)
-        if name == "learned":
+        if name.startswith("learned"):
             assert s["fell_back"] == 0
         sol[name] = U
-    e = float(torch.linalg.norm(sol["learned"] - sol["plain"]) / torch.linalg.norm(sol["plain"]))
-    print("velocity difference learned vs plain %.2e" % e, flush=True)
-    assert e <= 1e-5
+    for name in ("learned", "learned fgmres"):
+        e = float(torch.linalg.norm(sol[name] - sol["plain"]) / torch.linalg.norm(sol["plain"]))
+        print("velocity difference %s vs plain %.2e" % (name, e), flush=True)
+        assert e <= 1e-5
     pc.close()

@@ -97,6 +97,7 @@ _sig("ves3d_hip_resample", c_int, c_int, S, P, P, P)
 _sig("ves3d_hip_rotate_all_f64", P, S, c_int, c_int, P, P)
 _sig("ves3d_hip_rotate_transpose_f64", P, S, c_int, c_int, P, P)
 _sig("ves3d_hip_bicgstab", S, P, P, P, P, P, P, P, P, P, P)
+_sig("ves3d_hip_fgmres", S, P, P, P, P, P, P, P, P, P, c_int, P)
 _sig("ves3d_hip_stokes_create", c_int, c_int, S, c_double, P)
 _sig("ves3d_hip_stokes_destroy", P)
 _sig("ves3d_hip_stokes_set_src", P, P, P)

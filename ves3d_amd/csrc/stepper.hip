@@ -10,6 +10,7 @@
 // with the Galerkin matvec  A = P [u - S f(dt u, sigma) ; -div S f] P^-1,
 // then x += dt u.  P packs grid fields into SH coefficients in the
 // reference's order-major Galerkin layout (src/InterfacialVelocity.cc:981-1052).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <vector>
@@ -451,6 +452,136 @@ int bicgstab(long n, apply_fn A, const void *actx, apply_fn P, const void *pctx,
   return VES3D_SUCCESS;
 }
 
+// Flexible GMRES (Saad, SIAM J. Sci. Comput. 14 (1993) 461-469), right
+// preconditioned, restarted every m iterations: the method of the
+// reference's globally implicit solve (PETSc KSP GMRES,
+// src/ParallelLinSolver_Petsc.cc:545-551, restart 1000, with a shell
+// preconditioner), in the flexible form that a preconditioner which changes
+// between applications (an inner solve, a network) needs: the preconditioned
+// vectors z_j are kept and the iterate is x_0 + Z_k y.  The Arnoldi vectors
+// are orthogonalised by classical Gram-Schmidt applied twice (batched FP64
+// dots, fixed reduction order); Givens rotations give each iterate's
+// residual norm |g_{j+1}|.  At the end of a cycle the residual vector is
+// formed as V_{k+1} (beta e_1 - Hbar_k y), which equals b - A x_k up to
+// rounding (the Arnoldi relation A Z_k = V_{k+1} Hbar_k holds for the
+// operator applications made), so neither a restart nor the final residual
+// costs an operator application; a cycle whose explicit residual misses the
+// tolerance is followed by another.
+// work: (2m + 1) n doubles (V: m + 1 vectors, Z: m), or null to allocate.
+// *status: 100 converged, 101 iteration limit, 104 NaN (the BiCGSReturn
+// codes, include/BiCGStab.h:20-24); *tol: the explicit residual ||b - A x||
+// / ||b|| of the returned x; *max_iter: iterations (operator applications
+// after the initial residual).
+int fgmres(long n, apply_fn A, const void *actx, apply_fn P, const void *pctx, double *x,
+           const double *b, int *max_iter, double *tol, int *status, hipStream_t st, int m,
+           const double *Ax0 = nullptr, double *work = nullptr) {
+  if (n < 1 || m < 1 || m >= kMaxVecCoefs) return VES3D_INVALID_PARAMETER;
+  DBuf own;
+  double *V = work;
+  if (!V) {
+    V3D_CHECK(own.alloc((2L * m + 1) * n));
+    V = own.p;
+  }
+  double *Z = V + (long)(m + 1) * n;
+  int err = 0;
+  double normb = dev_norm(n, b, st, &err);
+  V3D_CHECK(err);
+  if (normb == 0.0) normb = 1.0;
+  if (Ax0)  // A x computed by the caller (together with b)
+    V3D_CHECK(dev_copy(n, Ax0, V, st));
+  else
+    V3D_CHECK(A(actx, x, V));
+  V3D_CHECK(dev_axpby(n, 1.0, b, -1.0, V, st));  // V_0 = b - A x
+  double beta = dev_norm(n, V, st, &err);
+  V3D_CHECK(err);
+  double resid = beta / normb;
+  const double tol0 = *tol;
+  const int maxit = *max_iter;
+  int it = 0;
+  // R: the Hessenberg matrix rotated to upper triangular; Hb: as built
+  std::vector<double> R((m + 1) * m), Hb((m + 1) * m), cs(m), sn(m), g(m + 1), y(m),
+      h(m + 1), h2(m + 1), c(m + 1);
+  for (;;) {
+    if (resid != resid) {
+      *status = 104;
+      break;
+    }
+    if (resid <= tol0) {
+      *status = 100;
+      break;
+    }
+    if (it >= maxit) {
+      *status = 101;
+      break;
+    }
+    V3D_CHECK(dev_axpby(n, 1.0 / beta, V, 0.0, V, st));  // v_0 = r / beta
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int k = 0;
+    while (k < m && it < maxit) {
+      const double *vk = V + (long)k * n;
+      double *zk = Z + (long)k * n, *w = V + (long)(k + 1) * n;
+      if (P)
+        V3D_CHECK(P(pctx, vk, zk));
+      else
+        V3D_CHECK(dev_copy(n, vk, zk, st));
+      V3D_CHECK(A(actx, zk, w));
+      ++it;
+      // w -= V_k (V_k^T w), twice
+      V3D_CHECK(dev_mdots(n, k + 1, V, n, w, h.data(), st));
+      for (int i = 0; i <= k; ++i) c[i] = -h[i];
+      V3D_CHECK(dev_combine(n, k + 1, V, n, c.data(), 1.0, w, st));
+      V3D_CHECK(dev_mdots(n, k + 1, V, n, w, h2.data(), st));
+      for (int i = 0; i <= k; ++i) {
+        c[i] = -h2[i];
+        h[i] += h2[i];
+      }
+      V3D_CHECK(dev_combine(n, k + 1, V, n, c.data(), 1.0, w, st));
+      const double hn = dev_norm(n, w, st, &err);
+      V3D_CHECK(err);
+      h[k + 1] = hn;
+      for (int i = 0; i <= k + 1; ++i) Hb[i * m + k] = h[i];
+      for (int i = 0; i < k; ++i) {
+        const double a = cs[i] * h[i] + sn[i] * h[i + 1];
+        h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
+        h[i] = a;
+      }
+      const double den = std::hypot(h[k], h[k + 1]);
+      cs[k] = den == 0.0 ? 1.0 : h[k] / den;
+      sn[k] = den == 0.0 ? 0.0 : h[k + 1] / den;
+      h[k] = den;
+      g[k + 1] = -sn[k] * g[k];
+      g[k] = cs[k] * g[k];
+      for (int i = 0; i <= k; ++i) R[i * m + k] = h[i];
+      ++k;
+      if (hn > 0.0) V3D_CHECK(dev_axpby(n, 1.0 / hn, w, 0.0, w, st));  // v_k
+      resid = std::fabs(g[k]) / normb;
+      if (resid != resid || resid <= tol0 || hn == 0.0) break;
+    }
+    // y = R^-1 g, x += Z y
+    for (int i = k - 1; i >= 0; --i) {
+      double t = g[i];
+      for (int j = i + 1; j < k; ++j) t -= R[i * m + j] * y[j];
+      y[i] = R[i * m + i] != 0.0 ? t / R[i * m + i] : 0.0;
+    }
+    V3D_CHECK(dev_combine(n, k, Z, n, y.data(), 1.0, x, st));
+    // r = V_{k+1} (beta e_1 - Hbar y), formed in Z_0, then moved to V_0
+    for (int i = 0; i <= k; ++i) {
+      double t = i == 0 ? beta : 0.0;
+      for (int j = 0; j < k; ++j) t -= Hb[i * m + j] * y[j];
+      c[i] = t;
+    }
+    V3D_CHECK(dev_combine(n, k + 1, V, n, c.data(), 0.0, Z, st));
+    V3D_CHECK(dev_copy(n, Z, V, st));
+    beta = dev_norm(n, V, st, &err);
+    V3D_CHECK(err);
+    resid = beta / normb;
+  }
+  *tol = resid;
+  *max_iter = it;
+  return VES3D_SUCCESS;
+}
+
 // ------------------------------------------------------------- the step
 struct Stepper {
   ves3d_step_params prm;
@@ -482,6 +613,7 @@ struct Stepper {
   // inner solves of the preconditioner (inner_max_iter > 0)
   DBuf inner_work;                 // BiCGStab scratch of the inner solves, 8 n
   DBuf solve_work;                 // BiCGStab scratch of the outer solve, 8 n
+  DBuf gmres_work;                 // FGMRES scratch of the outer solve, (2 m + 1) n
   DBuf x0s, Axr;                   // the initial guess (for the fallback); A x of a restart
   // viscosity contrast (VesicleProperties::update, src/VesicleProps.cc:28-52):
   // vel_coeff = (1 + lambda) / 2, dl_coeff = 1 - lambda per vesicle
@@ -502,7 +634,8 @@ struct Stepper {
         (prm.extrapolate != 0 && prm.extrapolate != 1) ||
         (prm.far_coarse != 0.0 && !(prm.far_coarse > 1.0)) ||
         (prm.far_coarse2 != 0.0 && !(prm.far_coarse > 1.0 && prm.far_coarse2 > prm.far_coarse)) ||
-        (prm.fallback != 0 && prm.fallback != 1))
+        (prm.fallback != 0 && prm.fallback != 1) || (prm.solver != 0 && prm.solver != 1) ||
+        prm.restart < 0 || prm.restart >= kMaxVecCoefs)
       return VES3D_INVALID_PARAMETER;
     if (prm.p_up <= 0) prm.p_up = 2 * p;
     if (prm.filter_freq <= 0) prm.filter_freq = 2 * p / 3;
@@ -879,6 +1012,7 @@ struct Stepper {
   // A solution is accepted when BiCGStab converged and its true residual
   // is within 5% of the tolerance.
   static constexpr double kTrueResSlack = 1.05;
+  static constexpr int kDefaultRestart = 30;  // FGMRES restart length
   static constexpr int kRestarts = 2;
 
   // BiCGStab on the step's system from xsol (whose A x is Ax0), then the
@@ -888,6 +1022,18 @@ struct Stepper {
   // from the current solution, at most kRestarts times, within the same
   // iteration budget.  it: iterations used in total.
   int solve(apply_fn P, int budget, int *it, double *tol, int *status, double *true_rel) {
+    if (prm.solver == 1) {  // FGMRES: its residual is formed explicitly, no check matvec
+      const int m = prm.restart > 0 ? prm.restart : kDefaultRestart;
+      if (!gmres_work.p) V3D_CHECK(gmres_work.alloc((2L * m + 1) * n));
+      int k = budget;
+      double t = prm.tol;
+      V3D_CHECK(fgmres(n, A_cb, this, P, this, xsol, rhs, &k, &t, status, st, m, Ax0,
+                       gmres_work));
+      *it = k;
+      *tol = t;
+      *true_rel = t;
+      return VES3D_SUCCESS;
+    }
     const double *ax = Ax0;
     int used = 0;
     for (int r = 0;; ++r) {
@@ -1038,6 +1184,16 @@ int ves3d_hip_bicgstab(size_t n, ves3d_apply_fn A, const void *A_ctx, ves3d_appl
   if (!A || !x || !b || !max_iter || !tol || !bicgs_status) return VES3D_INVALID_PARAMETER;
   return bicgstab((long)n, (apply_fn)A, A_ctx, (apply_fn)P, P_ctx, x, b, max_iter, tol,
                   bicgs_status, (hipStream_t)stream);
+}
+
+int ves3d_hip_fgmres(size_t n, ves3d_apply_fn A, const void *A_ctx, ves3d_apply_fn P,
+                     const void *P_ctx, double *x, const double *b, int *max_iter, double *tol,
+                     int *status, int restart, ves3d_stream_t stream) {
+  if (!A || !x || !b || !max_iter || !tol || !status || n < 1 || restart < 1 ||
+      restart >= kMaxVecCoefs)
+    return VES3D_INVALID_PARAMETER;
+  return fgmres((long)n, (apply_fn)A, A_ctx, (apply_fn)P, P_ctx, x, b, max_iter, tol, status,
+                (hipStream_t)stream, restart);
 }
 
 int ves3d_hip_stepper_create(const ves3d_step_params *prm, void **handle) {

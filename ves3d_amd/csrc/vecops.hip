@@ -79,6 +79,53 @@ __global__ __launch_bounds__(RB) void k_dots_stage2(int nd, int nb, const double
   }
 }
 
+// Dots of nk vectors against one: part[k][b] = sum over block b's share of
+// V[k][i] w[i]; workgroup row k of the grid serves vector k, and w is re-read
+// (from cache) per vector.
+__global__ __launch_bounds__(RB) void k_mdot_stage1(long n, const double *__restrict__ V,
+                                                     long ldv, const double *__restrict__ w,
+                                                     double *__restrict__ part) {
+  __shared__ double sh[RB / 64];
+  const double *v = V + (long)blockIdx.y * ldv;
+  double acc = 0;
+  for (long i = (long)blockIdx.x * RB + threadIdx.x; i < n; i += (long)gridDim.x * RB)
+    acc += v[i] * w[i];
+  acc = wave_sum(acc);
+  if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int j = 0; j < RB / 64; ++j) s += sh[j];
+    part[(long)blockIdx.y * RBLOCKS + blockIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(RB) void k_mdot_stage2(int nb, const double *__restrict__ part,
+                                                     double *__restrict__ out) {
+  __shared__ double sh[RB / 64];
+  double s = 0;
+  for (int j = threadIdx.x; j < nb; j += RB) s += part[(long)blockIdx.x * RBLOCKS + j];
+  s = wave_sum(s);
+  if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int j = 0; j < RB / 64; ++j) t += sh[j];
+    out[blockIdx.x] = t;
+  }
+}
+
+// w = b w + sum_k c[k] V[k], the coefficients passed by value
+__global__ void k_combine(long n, int nk, const double *__restrict__ V, long ldv, VecCoefs c,
+                          double b, double *__restrict__ w) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    double s = b == 0.0 ? 0.0 : b * w[i];
+    for (int k = 0; k < nk; ++k) s += c.c[k] * V[k * ldv + i];
+    w[i] = s;
+  }
+}
+
 unsigned blocks_for(long n) {
   long b = (n + RB - 1) / RB;
   if (b > 4096) b = 4096;
@@ -126,6 +173,31 @@ int dev_dots(long n, int nd, const double *const *x, const double *const *y, dou
   V3D_CHECK(launch_status());
   V3D_HIP(hipMemcpyAsync(out_host, ws + 4 * RBLOCKS, nd * sizeof(double), hipMemcpyDeviceToHost, st));
   return hip_status(hipStreamSynchronize(st));
+}
+
+int dev_mdots(long n, int nk, const double *V, long ldv, const double *w, double *out_host,
+              hipStream_t st) {
+  if (nk < 1 || nk > kMaxVecCoefs || ldv < n) return VES3D_INVALID_PARAMETER;
+  double *ws = (double *)workspace(kWsSolver, sizeof(double) * ((long)kMaxVecCoefs * RBLOCKS + kMaxVecCoefs));
+  if (!ws) return VES3D_MEMORY_ERROR;
+  const int nb = (int)std::min<long>(RBLOCKS, std::max<long>(1, (n + RB - 1) / RB));
+  k_mdot_stage1<<<dim3(nb, nk), RB, 0, st>>>(n, V, ldv, w, ws);
+  V3D_CHECK(launch_status());
+  double *out = ws + (long)kMaxVecCoefs * RBLOCKS;
+  k_mdot_stage2<<<nk, RB, 0, st>>>(nb, ws, out);
+  V3D_CHECK(launch_status());
+  V3D_HIP(hipMemcpyAsync(out_host, out, nk * sizeof(double), hipMemcpyDeviceToHost, st));
+  return hip_status(hipStreamSynchronize(st));
+}
+
+int dev_combine(long n, int nk, const double *V, long ldv, const double *c_host, double b,
+                double *w, hipStream_t st) {
+  if (nk < 0 || nk > kMaxVecCoefs || ldv < n) return VES3D_INVALID_PARAMETER;
+  if (n <= 0) return VES3D_SUCCESS;
+  VecCoefs c{};
+  for (int k = 0; k < nk; ++k) c.c[k] = c_host[k];
+  k_combine<<<blocks_for(n), RB, 0, st>>>(n, nk, V, ldv, c, b, w);
+  return launch_status();
 }
 
 double dev_norm(long n, const double *x, hipStream_t st, int *err) {

@@ -18,4 +18,17 @@ int dev_dots(long n, int nd, const double *const *x, const double *const *y,
              double *out_host, hipStream_t st);
 double dev_norm(long n, const double *x, hipStream_t st, int *err);
 
+// Vectors V[k] = V + k ldv, k < nk <= kMaxVecCoefs (an Arnoldi basis).
+constexpr int kMaxVecCoefs = 64;
+struct VecCoefs {
+  double c[kMaxVecCoefs];
+};
+// out[k] = V[k] . w for k < nk, fixed-order FP64 reductions, into host
+// memory (synchronises the stream).
+int dev_mdots(long n, int nk, const double *V, long ldv, const double *w, double *out_host,
+              hipStream_t st);
+// w = b w + sum_k c[k] V[k] (b == 0: w is not read)
+int dev_combine(long n, int nk, const double *V, long ldv, const double *c_host, double b,
+                double *w, hipStream_t st);
+
 }  // namespace v3d

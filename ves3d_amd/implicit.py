@@ -30,7 +30,8 @@ class StepParams(ctypes.Structure):
                 ("r_near", ctypes.c_double), ("post_off", ctypes.c_int),
                 ("inner_max_iter", ctypes.c_int), ("inner_tol", ctypes.c_double),
                 ("extrapolate", ctypes.c_int), ("far_coarse", ctypes.c_double),
-                ("far_coarse2", ctypes.c_double), ("fallback", ctypes.c_int)]
+                ("far_coarse2", ctypes.c_double), ("fallback", ctypes.c_int),
+                ("solver", ctypes.c_int), ("restart", ctypes.c_int)]
 
 
 class StepStats(ctypes.Structure):
@@ -60,12 +61,14 @@ class Stepper:
     def __init__(self, nves, p, dt=1e-3, shear_rate=0.1, bending_modulus=1e-2, tol=1e-6,
                  max_iter=200, precision=0, precond=1, p_up=0, filter_freq=0, near=True,
                  repul_dist=0.0, r_near=0.0, post=True, inner_max_iter=0, inner_tol=0.0,
-                 extrapolate=False, far_coarse=0.0, far_coarse2=0.0, fallback="diagonal"):
+                 extrapolate=False, far_coarse=0.0, far_coarse2=0.0, fallback="diagonal",
+                 solver="bicgstab", restart=0):
         self.params = StepParams(nves, p, p_up, filter_freq, dt, shear_rate, bending_modulus, tol,
                                  max_iter, precision, precond, 0 if near else 1, repul_dist, r_near,
                                  0 if post else 1, inner_max_iter, inner_tol,
                                  1 if extrapolate else 0, far_coarse, far_coarse2,
-                                 {"diagonal": 0, "none": 1}[fallback])
+                                 {"diagonal": 0, "none": 1}[fallback],
+                                 {"bicgstab": 0, "fgmres": 1}[solver], restart)
         self.nves, self.p = nves, p
         self.ng = 2 * p * (p + 1)
         h = ctypes.c_void_p()
@@ -331,6 +334,22 @@ class LearnedPC:
 def attach(stepper, pc, collect):
     call("ves3d_hip_stepper_attach_pc", stepper.h, pc.h if pc is not None else None,
          1 if collect else 0)
+
+
+def fgmres(apply_a, x, b, tol, max_iter, restart=30, apply_p=None, stream=None):
+    """Device flexible GMRES with Python-side operator callbacks receiving raw
+    device pointers (ctx, x, y).  Returns (iterations, relres, status)."""
+    a_cb = APPLY_FN(apply_a)
+    p_cb = APPLY_FN(apply_p) if apply_p is not None else None
+    it = ctypes.c_int(max_iter)
+    tl = ctypes.c_double(tol)
+    status = ctypes.c_int(0)
+    torch.cuda.synchronize()
+    fa = ctypes.cast(a_cb, ctypes.c_void_p)
+    fp = ctypes.cast(p_cb, ctypes.c_void_p) if p_cb is not None else None
+    call("ves3d_hip_fgmres", x.numel(), fa, None, fp, None, ptr(x), ptr(b),
+         ctypes.byref(it), ctypes.byref(tl), ctypes.byref(status), restart, stream_ptr(stream))
+    return it.value, tl.value, status.value
 
 
 def bicgstab(apply_a, x, b, tol, max_iter, apply_p=None, stream=None):
