@@ -185,14 +185,21 @@ def _mean(stats, key):
     return float(np.mean([s[key] for s in stats]))
 
 
+def _init(args):
+    """Device 0 and the process-wide kernel choices of the run."""
+    import ves3d_amd
+    ves3d_amd.init(0)
+    ves3d_amd.call("ves3d_hip_set_far_kernel", FAR_KINDS[args.far_kernel])
+    ves3d_amd.call("ves3d_hip_set_pole_kernel", 1 if args.pole_kernel == "fused" else 0)
+
+
 def bench_step(args):
     import torch
     import ves3d_amd
     from ves3d_amd import implicit as I
     from ves3d_amd import workloads as W
 
-    ves3d_amd.init(0)
-    ves3d_amd.call("ves3d_hip_set_far_kernel", FAR_KINDS[args.far_kernel])
+    _init(args)
     nv, p = args.nves, args.p
     X0 = W.implicit_step_case(nv=nv, p=p)
     Xg = torch.from_numpy(X0).cuda()
@@ -365,8 +372,7 @@ def bench_trajectory(args):
     from ves3d_amd import implicit as I
     from ves3d_amd import workloads as W
 
-    ves3d_amd.init(0)
-    ves3d_amd.call("ves3d_hip_set_far_kernel", FAR_KINDS[args.far_kernel])
+    _init(args)
     nv, p, nsteps = args.nves, args.p, args.traj_steps
     X0 = W.implicit_step_case(nv=nv, p=p)
     Xg = torch.from_numpy(X0).cuda()
@@ -526,8 +532,7 @@ def bench_matvec(args):
     from ves3d_amd import implicit as I
     from ves3d_amd import workloads as W
 
-    ves3d_amd.init(0)
-    ves3d_amd.call("ves3d_hip_set_far_kernel", FAR_KINDS[args.far_kernel])
+    _init(args)
     nv, p = args.nves, args.p
     prec = args.mixed_precision if args.dtype == "f32c" else 0
     # the mixed form's kernel with its multi-level far field, the FP64 form's
@@ -598,7 +603,7 @@ def bench_stages(args):
     import ves3d_amd
     from ves3d_amd import sph as S
     from ves3d_amd import workloads as W
-    ves3d_amd.init(0)
+    _init(args)
     out = {}
     rng = np.random.default_rng(11)
     for nv, p in ((2048, 16), (512, 24)):
@@ -665,8 +670,7 @@ def bench_learned(args):
     import ves3d_amd
     from ves3d_amd import implicit as I
     from ves3d_amd import workloads as W
-    ves3d_amd.init(0)
-    ves3d_amd.call("ves3d_hip_set_far_kernel", FAR_KINDS[args.far_kernel])
+    _init(args)
     nv, p = args.nves, args.p
     Xg = torch.from_numpy(W.implicit_step_case(nv=nv, p=p)).cuda()
     common = dict(dt=1e-3, shear_rate=1.0, bending_modulus=1e-2, tol=1e-6, max_iter=args.max_iter,
@@ -696,7 +700,7 @@ def bench_build(args):
     import ves3d_amd
     from ves3d_amd import sph as S
     from ves3d_amd import workloads as W
-    ves3d_amd.init(0)
+    _init(args)
     nv, p = args.nves, args.p
     X = torch.from_numpy(W.perturbed_shapes(nv, p, amp=0.1, seed=21)).cuda()
     n = 3 * S.ncoef(p)
@@ -719,7 +723,7 @@ def bench_probe(args):
     and FP64): the attainable ceiling beside the published peaks."""
     import ctypes
     import ves3d_amd
-    ves3d_amd.init(0)
+    _init(args)
     r32, r64 = ctypes.c_double(), ctypes.c_double()
     ves3d_amd.call("ves3d_hip_valu_probe", 0, ctypes.byref(r32))
     ves3d_amd.call("ves3d_hip_valu_probe", 1, ctypes.byref(r64))
@@ -735,7 +739,7 @@ def bench_far(args):
     from ves3d_amd import stokes as K
     from ves3d_amd import workloads as W
 
-    ves3d_amd.init(0)
+    _init(args)
     pos, sld, dld, npv = W.stokes_direct_sum_case(nv=512, p=16)
     n = pos.shape[0]
     g = lambda a: torch.from_numpy(a).cuda()
@@ -844,6 +848,9 @@ def main():
                          "beyond this many radii (0: two levels)")
     ap.add_argument("--no-attribution", action="store_true",
                     help="step workload: skip the attribution forms and the near-default line")
+    ap.add_argument("--pole-kernel", default="fused", choices=["fused", "separate"],
+                    help="the FP64 singular build's pole kernel: fused with its transposed "
+                         "synthesis, or the separate kernel and grouped GEMMs")
     ap.add_argument("--solver", default="fgmres", choices=["bicgstab", "fgmres"],
                     help="outer solver of every form: flexible GMRES (the reference's implicit "
                          "solve is PETSc GMRES) or BiCGStab with a true-residual check")
