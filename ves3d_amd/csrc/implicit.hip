@@ -577,6 +577,12 @@ int ForcesG::tensile(const double *sigma, double *Fs, hipStream_t st) {  // :94-
 
 // ------------------------------------------------------------- StokesG
 StokesG::~StokesG() {
+  if (bst) {  // a build in flight writes SL
+    (void)hipStreamSynchronize(bst);
+    (void)hipStreamDestroy(bst);
+  }
+  if (ev_in) (void)hipEventDestroy(ev_in);
+  if (ev_built) (void)hipEventDestroy(ev_built);
   if (SL32) (void)hipFree(SL32);
   if (DL32) (void)hipFree(DL32);
   if (cflags) (void)hipFree(cflags);
@@ -626,6 +632,11 @@ int StokesG::init(int p_, int pu_, long nv_, int precision_, int near_, double r
   far_coarse2 = far_coarse2_;
   const long nc = sh_ncoef(p), ng = sh_ngrid(p), mu = sh_ngrid(pu);
   const long nmat = nv * 9 * nc * nc;
+  if (!bst) {
+    V3D_HIP(hipStreamCreateWithFlags(&bst, hipStreamNonBlocking));
+    V3D_HIP(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
+    V3D_HIP(hipEventCreateWithFlags(&ev_built, hipEventDisableTiming));
+  }
   if (precision != 1) V3D_CHECK(SL.alloc(nmat));
   // precision 1: the FP32 build; precision 2: an FP32 copy of the FP64
   // operator for the local (inner-solve) applications
@@ -679,23 +690,15 @@ int StokesG::init(int p_, int pu_, long nv_, int precision_, int near_, double r
 
 int StokesG::setup(const double *X, hipStream_t st) {
   const long ng = sh_ngrid(p), mu = sh_ngrid(pu);
-  if (precision != 1)
-    V3D_CHECK(singular_build<double>(X, nv, p, pu, SL.p, want_dl ? DL.p : nullptr, st));
-  else
-    V3D_CHECK(singular_build<float>(X, nv, p, pu, SL32, want_dl ? DL32 : nullptr, st));
-  if (precision == 2) {
-    const long nmat = nv * 9 * (long)sh_ncoef(p) * sh_ncoef(p);
-    k_round_f32<<<gblocks(nmat), 256, 0, st>>>(SL.p, nmat, SL32);
-    V3D_CHECK(launch_status());
-  }
   near_off_ready = false;  // new sources
   V3D_CHECK(sht_grid2shc<double>(X, 3 * nv, p, p, shc, work, st));
   V3D_CHECK(sht_shc2grid<double>(shc, 3 * nv, p, pu, tmpx, tmpt, tmpp, work, st));
   V3D_CHECK(sht_shc2pole<double>(shc, 3 * nv, p, pole, st));
-  std::vector<double> qw;
-  legendre_weights(pu, qw);
-  DBuf dqw;
-  V3D_CHECK(upload(dqw, qw));
+  if (!dqw.p) {
+    std::vector<double> qw;
+    legendre_weights(pu, qw);
+    V3D_CHECK(upload(dqw, qw));
+  }
   k_far_setup<<<gblocks(nv * (sh_ngrid(pu) + 2)), 256, 0, st>>>(
       tmpx, tmpt, tmpp, pole, dqw, nv, pu, far_src, aqw, want_dl ? nrm_up.p : nullptr);
   V3D_CHECK(launch_status());
@@ -705,22 +708,24 @@ int StokesG::setup(const double *X, hipStream_t st) {
                                  coarse ? tmpp.p : nullptr, work, st));
   k_to_points<<<gblocks(nv * ng), 256, 0, st>>>(tmpx, nv, ng, trg);
   V3D_CHECK(launch_status());
-  DBuf dqwp;
   if (coarse) {  // the targets double as the coarse far-field sources
-    std::vector<double> qwp;
-    legendre_weights(p, qwp);
-    V3D_CHECK(upload(dqwp, qwp));
+    if (!dqwp.p) {
+      std::vector<double> qwp;
+      legendre_weights(p, qwp);
+      V3D_CHECK(upload(dqwp, qwp));
+    }
     k_coarse_weights<<<gblocks(nv * ng), 256, 0, st>>>(tmpt, tmpp, dqwp, nv, p, aqwp);
     V3D_CHECK(launch_status());
     long ng2 = 0;
-    DBuf dinvs;
     if (far_coarse2 > 0) {
       // the area element per unit-sphere measure, |x_t x x_p| / sin(theta),
       // and the degree-p surface at the order-q2 grid points
-      std::vector<double> x, w, invs(p + 1);
-      gauss_legendre(p + 1, x, w);
-      for (int i = 0; i <= p; ++i) invs[i] = 1.0 / std::sqrt(1.0 - x[i] * x[i]);
-      V3D_CHECK(upload(dinvs, invs));
+      if (!dinvs.p) {
+        std::vector<double> x, w, invs(p + 1);
+        gauss_legendre(p + 1, x, w);
+        for (int i = 0; i <= p; ++i) invs[i] = 1.0 / std::sqrt(1.0 - x[i] * x[i]);
+        V3D_CHECK(upload(dinvs, invs));
+      }
       k_coarse_weights<<<gblocks(nv * ng), 256, 0, st>>>(tmpt, tmpp, dinvs, nv, p, c2gw);
       V3D_CHECK(launch_status());
       ng2 = sh_ngrid(ord2);
@@ -739,7 +744,29 @@ int StokesG::setup(const double *X, hipStream_t st) {
     far_pairs = (double)(nv * ng) * (double)(nv - 1) * (double)(mu + 2);
   }
   if (use_near) V3D_CHECK(near.setup(far_src, trg, st));
-  return hip_status(hipStreamSynchronize(st));  // dqw, dqwp are freed on return
+  // the singular operators on bst, after everything st has queued (the
+  // positions, and the last step's applications of the operators)
+  V3D_HIP(hipEventRecord(ev_in, st));
+  V3D_HIP(hipStreamWaitEvent(bst, ev_in, 0));
+  if (precision != 1)
+    V3D_CHECK(singular_build<double>(X, nv, p, pu, SL.p, want_dl ? DL.p : nullptr, bst));
+  else
+    V3D_CHECK(singular_build<float>(X, nv, p, pu, SL32, want_dl ? DL32 : nullptr, bst));
+  if (precision == 2) {
+    const long nmat = nv * 9 * (long)sh_ncoef(p) * sh_ncoef(p);
+    k_round_f32<<<gblocks(nmat), 256, 0, bst>>>(SL.p, nmat, SL32);
+    V3D_CHECK(launch_status());
+  }
+  V3D_HIP(hipEventRecord(ev_built, bst));
+  build_pending = true;
+  return hip_status(hipStreamSynchronize(st));
+}
+
+int StokesG::wait_build(hipStream_t st) {
+  if (!build_pending) return VES3D_SUCCESS;
+  V3D_HIP(hipStreamWaitEvent(st, ev_built, 0));
+  build_pending = false;
+  return VES3D_SUCCESS;
 }
 
 int StokesG::pack_coarse(const double *f, double *qcoarse, hipStream_t st) {
@@ -790,33 +817,51 @@ FarCoarse StokesG::coarse_levels(const double *f1, const double *f2, hipStream_t
   return fc;
 }
 
-int StokesG::self_and_density(const double *f, double *Fc, double *Vc, double *vel, double *upc,
-                              double *qc, bool pack, hipStream_t st, const double *d) {
-  const long mu = sh_ngrid(pu);
+int StokesG::density_coef(const double *f, double *Fc, hipStream_t st, const double *d) {
   if (f) V3D_CHECK(sht_grid2shc<double>(f, 3 * nv, p, p, Fc, work, st));
   if (d) V3D_CHECK(sht_grid2shc<double>(d, 3 * nv, p, p, G, work, st));
+  return VES3D_SUCCESS;
+}
+
+int StokesG::self_velocity(bool sl, const double *Fc, double *Vc, double *vel, bool pack,
+                           hipStream_t st, bool dl) {
+  V3D_CHECK(wait_build(st));
   // precision 2, the self-interaction alone (an inner solve's operator, a
   // preconditioner's): the FP32 copy halves the HBM-bound apply
-  if (precision == 2 && !pack && !d)
-    V3D_CHECK(self_apply<float>(f ? SL32 : nullptr, Fc, nullptr, G.p, nv, p, Vc, st));
+  if (precision == 2 && !pack && !dl)
+    V3D_CHECK(self_apply<float>(sl ? SL32 : nullptr, Fc, nullptr, G.p, nv, p, Vc, st));
   else if (precision != 1)
-    V3D_CHECK(self_apply<double>(f ? SL.p : nullptr, Fc, d ? DL.p : nullptr, G.p, nv, p, Vc, st));
+    V3D_CHECK(self_apply<double>(sl ? SL.p : nullptr, Fc, dl ? DL.p : nullptr, G.p, nv, p, Vc,
+                                 st));
   else
-    V3D_CHECK(self_apply<float>(f ? SL32 : nullptr, Fc, d ? DL32 : nullptr, G.p, nv, p, Vc, st));
-  V3D_CHECK(sht_shc2grid<double>(Vc, 3 * nv, p, p, vel, nullptr, nullptr, work, st));
-  if (!pack) return VES3D_SUCCESS;
-  if (f) {
+    V3D_CHECK(self_apply<float>(sl ? SL32 : nullptr, Fc, dl ? DL32 : nullptr, G.p, nv, p, Vc,
+                                st));
+  return sht_shc2grid<double>(Vc, 3 * nv, p, p, vel, nullptr, nullptr, work, st);
+}
+
+int StokesG::pack_density(bool sl, const double *Fc, double *upc, double *qc, hipStream_t st,
+                          bool dl) {
+  const long mu = sh_ngrid(pu);
+  if (sl) {
     V3D_CHECK(sht_shc2grid<double>(Fc, 3 * nv, p, pu, upc, nullptr, nullptr, work, st));
     k_pack_density<<<gblocks(nv * (mu + 2)), 256, 0, st>>>(upc, aqw, nv, mu, qc);
     V3D_CHECK(launch_status());
   }
-  if (d) {  // uforce_double / qforce_double (src/StokesVelocity.cc:338-378)
+  if (dl) {  // uforce_double / qforce_double (src/StokesVelocity.cc:338-378)
     V3D_CHECK(sht_shc2grid<double>(G, 3 * nv, p, pu, dup, nullptr, nullptr, work, st));
     V3D_CHECK(sht_shc2pole<double>(G, 3 * nv, p, dpole, st));
     k_pack_dl<<<gblocks(nv * (mu + 2)), 256, 0, st>>>(dup, aqw, nrm_up, nv, mu, qd);
     V3D_CHECK(launch_status());
   }
   return VES3D_SUCCESS;
+}
+
+int StokesG::self_and_density(const double *f, double *Fc, double *Vc, double *vel, double *upc,
+                              double *qc, bool pack, hipStream_t st, const double *d) {
+  V3D_CHECK(density_coef(f, Fc, st, d));
+  V3D_CHECK(self_velocity(f != nullptr, Fc, Vc, vel, pack, st, d != nullptr));
+  if (!pack) return VES3D_SUCCESS;
+  return pack_density(f != nullptr, Fc, upc, qc, st, d != nullptr);
 }
 
 int StokesG::near_correct(const double *qc, const double *Vc, double *vel, hipStream_t st,
@@ -959,8 +1004,12 @@ int StokesG::apply2(const double *f1, double *vel1, bool add_repul1, const doubl
     V3D_CHECK(near.add_repulsion(frep, st));
     f1 = frep;
   }
-  V3D_CHECK(self_and_density(f1, F, V, vel1, up, q, true, st));
-  V3D_CHECK(self_and_density(f2, F2, V2, vel2, up2, q2, true, st));
+  // the far-field pass first: it needs no self interaction, so it runs
+  // while the singular build of the last setup finishes on bst
+  V3D_CHECK(density_coef(f1, F, st, nullptr));
+  V3D_CHECK(density_coef(f2, F2, st, nullptr));
+  V3D_CHECK(pack_density(true, F, up, q, st, false));
+  V3D_CHECK(pack_density(true, F2, up2, q2, st, false));
   FarCoarse fc{};
   if (far_coarse > 0) {
     int e = VES3D_SUCCESS;
@@ -972,6 +1021,8 @@ int StokesG::apply2(const double *f1, double *vel1, bool add_repul1, const doubl
                                       farv2, nv * ng, mu + 2, ng, st,
                                       far_coarse > 0 ? &fc : nullptr));
   V3D_CHECK(event_end(st));
+  V3D_CHECK(self_velocity(true, F, V, vel1, true, st, false));
+  V3D_CHECK(self_velocity(true, F2, V2, vel2, true, st, false));
   k_add_points<<<gblocks(nv * ng), 256, 0, st>>>(farv, nv, ng, vel1);
   V3D_CHECK(launch_status());
   k_add_points<<<gblocks(nv * ng), 256, 0, st>>>(farv2, nv, ng, vel2);

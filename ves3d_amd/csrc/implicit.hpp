@@ -246,6 +246,16 @@ class StokesG {
   float *DL32 = nullptr;
   NearG near_off;              // near-singular correction of targets off the surfaces
   bool near_off_ready = false;
+  // The singular operators are built on a stream of their own (bst,
+  // non-blocking, default priority), ordered after the setup stream's
+  // earlier work (ev_in), so that the far-field pass that needs no self
+  // interaction (the right-hand side's) overlaps the build; the first self
+  // apply after a setup makes its stream wait for ev_built.
+  hipStream_t bst = nullptr;
+  hipEvent_t ev_in = nullptr, ev_built = nullptr;
+  bool build_pending = false;
+  int wait_build(hipStream_t st);
+  DBuf dqw, dqwp, dinvs;  // quadrature weights, uploaded at the first setup
   ~StokesG();
 
  private:
@@ -253,6 +263,12 @@ class StokesG {
   // with pack, the upsampled far-field densities qc (and qd, dup, dpole)
   int self_and_density(const double *f, double *Fc, double *Vc, double *vel, double *upc,
                        double *qc, bool pack, hipStream_t st, const double *d = nullptr);
+  // its three parts: the densities' coefficients (Fc, G), the self velocity
+  // (waits for the build), the far-field packing
+  int density_coef(const double *f, double *Fc, hipStream_t st, const double *d);
+  int self_velocity(bool sl, const double *Fc, double *Vc, double *vel, bool pack,
+                    hipStream_t st, bool dl);
+  int pack_density(bool sl, const double *Fc, double *upc, double *qc, hipStream_t st, bool dl);
   int near_correct(const double *qc, const double *Vc, double *vel, hipStream_t st,
                    bool dl = false);
   int event_begin(int kind, hipStream_t st);

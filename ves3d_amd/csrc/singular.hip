@@ -32,6 +32,13 @@ namespace v3d {
 namespace {
 
 std::mutex g_mu;
+// The build's scratch (kWsSing0) is one per device: each build waits on its
+// stream for the device's previous build, wherever that ran
+struct BuildOrder {
+  hipEvent_t done = nullptr;
+  bool recorded = false;
+};
+std::map<int, BuildOrder> g_build_order;
 // 1: the singular build's pole kernel runs fused with the transposed
 // synthesis (k_pole_transpose) where its shapes allow; 0: as separate
 // kernel and grouped GEMMs (ves3d_hip_set_pole_kernel)
@@ -679,6 +686,17 @@ int singular_build(const double *X0, long Nv, int p0, int p1, T *SL, T *DL, hipS
   long chunk = std::max(1L, std::min(Nv, budget / per));
   T *ws = (T *)workspace(kWsSing0, sizeof(T) * per * chunk);
   if (!ws) return VES3D_MEMORY_ERROR;
+  hipEvent_t done = nullptr;
+  {
+    int dev = 0;
+    V3D_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_mu);
+    BuildOrder &o = g_build_order[dev];
+    if (!o.done) V3D_HIP(hipEventCreateWithFlags(&o.done, hipEventDisableTiming));
+    if (o.recorded) V3D_HIP(hipStreamWaitEvent(st, o.done, 0));
+    done = o.done;
+    o.recorded = true;
+  }
   for (long v0 = 0; v0 < Nv; v0 += chunk) {
     const long cv = std::min(chunk, Nv - v0);
     const long rot = cv * nr;
@@ -734,7 +752,7 @@ int singular_build(const double *X0, long Nv, int p0, int p1, T *SL, T *DL, hipS
       V3D_CHECK(sht_grid2shc<T>(M1a, cv * 9 * Nc, p0, p0, dst + v0 * 9 * Nc * Nc, w, st));
     }
   }
-  return VES3D_SUCCESS;
+  return hip_status(hipEventRecord(done, st));
 }
 
 // ------------------------------------------------------------- self apply
