@@ -434,6 +434,14 @@ int SurfaceG::div_c(const double *c, double *out, hipStream_t st) {
   return low_pass(out, nv, st);
 }
 
+int SurfaceG::div_coef(const double *c, double *out, double *cout, hipStream_t st) {
+  V3D_CHECK(backward(c, 3 * nv, 1, 0, t1, st));
+  V3D_CHECK(backward(c, 3 * nv, 0, 1, t2, st));
+  k_div_comb<<<gblocks(nv * ng), 256, 0, st>>>(t1, t2, cu, cv, nv, ng, out);
+  V3D_CHECK(launch_status());
+  return low_pass_c(out, nv, cout, false, st);
+}
+
 int SurfaceG::div(const double *V, double *out, hipStream_t st) {
   V3D_CHECK(forward(V, 3 * nv, shc, st));
   return div_c(shc, out, st);
@@ -490,8 +498,13 @@ int ForcesG::init(int p_, int pu_, int f, long nv_, double kb_) {
 int ForcesG::up_coef(const double *in, long nf, double *cu_out, hipStream_t st) {
   if (!padmap.p) return VES3D_INVALID_PARAMETER;
   V3D_CHECK(sht_grid2shc<double>(in, nf, p, p, rcoef, rwork, st));
+  return up_coef_c(rcoef, nf, cu_out, st);
+}
+
+int ForcesG::up_coef_c(const double *c, long nf, double *cu_out, hipStream_t st) {
+  if (!padmap.p) return VES3D_INVALID_PARAMETER;
   const long ncu = sh_ncoef(pu);
-  k_pad_coef<<<gblocks(nf * ncu), 256, 0, st>>>(rcoef, nf, sh_ncoef(p), padmap, ncu, cu_out);
+  k_pad_coef<<<gblocks(nf * ncu), 256, 0, st>>>(c, nf, sh_ncoef(p), padmap, ncu, cu_out);
   return launch_status();
 }
 
@@ -524,12 +537,17 @@ int ForcesG::bending(double *Fb, hipStream_t st) {  // src/InterfacialForce.cc:2
 }
 
 int ForcesG::linear_bending(const double *x_new, double *Fb, hipStream_t st) {  // :52-92
+  V3D_CHECK(sht_grid2shc<double>(x_new, 3 * nv, p, p, rcoef, rwork, st));
+  return linear_bending_c(rcoef, Fb, st);
+}
+
+int ForcesG::linear_bending_c(const double *c, double *Fb, hipStream_t st) {
   const long mu = Su.ng;
   // x_new upsampled, its linearised mean curvature, the gradient of that
   // and its divergence, each operator fed the previous one's low-passed
   // coefficients (the reference synthesises each on the upsampled grid and
   // transforms it back: the same field, up to rounding)
-  V3D_CHECK(up_coef(x_new, 3 * nv, Su.c3, st));
+  V3D_CHECK(up_coef_c(c, 3 * nv, Su.c3, st));
   V3D_CHECK(Su.lin_mean_curv_c(Su.c3, su1, Su.c1, st));
   k_bend_s2<<<gblocks(nv * mu), 256, 0, st>>>(su1, Su.H, Su.K, nv * mu, su2);
   V3D_CHECK(launch_status());
@@ -565,9 +583,14 @@ int ForcesG::gravity_add(const double *x, double *F, hipStream_t st) {
 }
 
 int ForcesG::tensile(const double *sigma, double *Fs, hipStream_t st) {  // :94-124
+  V3D_CHECK(sht_grid2shc<double>(sigma, nv, p, p, rcoef, rwork, st));
+  return tensile_c(rcoef, Fs, st);
+}
+
+int ForcesG::tensile_c(const double *c, double *Fs, hipStream_t st) {
   const long mu = Su.ng;
   // sigma upsampled (from its zero-padded coefficients) and its gradient
-  V3D_CHECK(up_coef(sigma, nv, Su.c1, st));
+  V3D_CHECK(up_coef_c(c, nv, Su.c1, st));
   V3D_CHECK(Su.backward(Su.c1, nv, 0, 0, su1, st));
   V3D_CHECK(Su.grad_c(Su.c1, gu, Su.c3, true, st));
   k_tens_force<<<gblocks(nv * mu), 256, 0, st>>>(su1, Su.H, Su.nrm, gu, nv, mu, fu);
@@ -836,7 +859,13 @@ int StokesG::self_velocity(bool sl, const double *Fc, double *Vc, double *vel, b
   else
     V3D_CHECK(self_apply<float>(sl ? SL32 : nullptr, Fc, dl ? DL32 : nullptr, G.p, nv, p, Vc,
                                 st));
-  return sht_shc2grid<double>(Vc, 3 * nv, p, p, vel, nullptr, nullptr, work, st);
+  return vel ? sht_shc2grid<double>(Vc, 3 * nv, p, p, vel, nullptr, nullptr, work, st)
+             : VES3D_SUCCESS;
+}
+
+int StokesG::self_coef(const double *f, double *Vc, hipStream_t st) {
+  V3D_CHECK(density_coef(f, F, st, nullptr));
+  return self_velocity(true, F, Vc, nullptr, false, st, false);
 }
 
 int StokesG::pack_density(bool sl, const double *Fc, double *upc, double *qc, hipStream_t st,

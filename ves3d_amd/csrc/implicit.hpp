@@ -106,6 +106,8 @@ class SurfaceG {
   int low_pass_c(double *field, long nf, double *coef, bool want_grid, hipStream_t st);
   int grad_c(const double *c, double *out, double *cout, bool want_grid, hipStream_t st);
   int div_c(const double *c, double *out, hipStream_t st);
+  // the divergence's low-passed coefficients [nv][nc] (out: grid scratch)
+  int div_coef(const double *c, double *out, double *cout, hipStream_t st);
   int lin_mean_curv_c(const double *c, double *out, double *cout, hipStream_t st);
 
   int p = 0, f = 0;
@@ -123,6 +125,9 @@ class ForcesG {
   int bending(double *Fb, hipStream_t st);                            // explicit
   int linear_bending(const double *x_new, double *Fb, hipStream_t st);
   int tensile(const double *sigma, double *Fs, hipStream_t st);
+  // the same from the order-p coefficients [nf][nc] of x_new / sigma
+  int linear_bending_c(const double *c, double *Fb, hipStream_t st);
+  int tensile_c(const double *c, double *Fs, hipStream_t st);
   // F += rho_v (g . x) n on the p grid (InterfacialForce::gravityForce,
   // src/InterfacialForce.cc:126-160); nothing unless some vesicle has an
   // excess density (VesicleProperties::has_excess_density)
@@ -137,6 +142,8 @@ class ForcesG {
   // its order-p coefficients, zero-padded (Resample up, then the forward
   // transform at p_up, without the round trip through the upsampled grid)
   int up_coef(const double *in, long nf, double *cu, hipStream_t st);
+  // the same from the order-p coefficients [nf][nc]
+  int up_coef_c(const double *c, long nf, double *cu, hipStream_t st);
 
   SurfaceG S, Su;
   int p = 0, pu = 0;
@@ -194,6 +201,9 @@ class StokesG {
   // solver's first matvec's (f2)
   int apply2(const double *f1, double *vel1, bool add_repul1, const double *f2, double *vel2,
              hipStream_t st);
+  // the self interaction alone in SH coefficients: Vc [3 nv][nc] = S_self f
+  // (apply with local, without the synthesis of the velocity)
+  int self_coef(const double *f, double *Vc, hipStream_t st);
 
   // HIP events around every far-field launch (on the launch stream); the
   // elapsed times are summed by far_time() after the stream has drained:

@@ -591,6 +591,7 @@ struct Stepper {
   StokesG stokes;
   DBuf X, U, sig;                  // state
   DBuf f, f2, Sf, ten, ug, sg, cf, rhs, xsol, yres, tmpD, yloc;
+  DBuf lcu, lcd, lcv, lcs, lct;  // coefficients of the local operator (matvec_local_c)
   DBuf ub, fb, Sfb, Ax0;           // right-hand side fused with the first matvec
   DBuf bprec, tprec;               // diagonal-spectral preconditioner
   DBuf fgrid, frames;              // canonical frames of the vesicles (learned PC)
@@ -732,6 +733,7 @@ struct Stepper {
   // the learned preconditioner its training pairs
   int matvec(const double *x, double *y, bool local = false) {
     if (!prepared) return VES3D_INVALID_PARAMETER;
+    if (local && !contrast) return matvec_local_c(x, y);
     if (!local) ++matvecs;
     V3D_CHECK(unpack(x, ug, sg));
     V3D_CHECK(dev_axpby(3 * nv * ng, prm.dt, ug, 0.0, f2, st));  // Du = dt u
@@ -754,6 +756,42 @@ struct Stepper {
       V3D_CHECK(dev_axpby(3 * nv * ng, -1.0, Sf, 1.0, ug, st));  // vox = u - Sf
     }
     return pack(ug, ten, y);
+  }
+
+  // Every vesicle's own operator (the self interaction alone), the same map
+  // as matvec(x, y, true) carried in SH coefficients: the Galerkin vector
+  // holds the coefficients of u and sigma, so the forces start from them
+  // (no synthesis and re-analysis of dt u and sigma), the self velocity
+  // stays in coefficients, the tension row takes the divergence's low-passed
+  // coefficients and the velocity row is u - S f coefficient by coefficient:
+  // the same values up to rounding, five transforms fewer (an inner solve
+  // applies it at every iteration).
+  int matvec_local_c(const double *x, double *y) {
+    if (!lcu.p)
+      for (DBuf *b : {&lcu, &lcd, &lcv}) V3D_CHECK(b->alloc(3 * nv * nc));
+    if (!lcs.p)
+      for (DBuf *b : {&lcs, &lct}) V3D_CHECK(b->alloc(nv * nc));
+    k_om_to_fm<<<gb(3 * nv * nc), 256, 0, st>>>(x, 3 * nv, (int)nc, cinfo, lcu);
+    V3D_CHECK(launch_status());
+    k_om_to_fm<<<gb(nv * nc), 256, 0, st>>>(x + 3 * nv * nc, nv, (int)nc, cinfo, lcs);
+    V3D_CHECK(launch_status());
+    V3D_CHECK(dev_axpby(3 * nv * nc, prm.dt, lcu, 0.0, lcd, st));  // Du = dt u
+    // implicitTractionJump (src/InterfacialForce.cc:172-184)
+    V3D_CHECK(forces.linear_bending_c(lcd, f, st));
+    if (forces.has_rho) {  // the gravity traction needs dt u on the grid
+      V3D_CHECK(forces.S.backward(lcd, 3 * nv, 0, 0, f2, st));
+      V3D_CHECK(forces.gravity_add(f2, f, st));
+    }
+    V3D_CHECK(forces.tensile_c(lcs, f2, st));
+    V3D_CHECK(dev_axpby(3 * nv * ng, 1.0, f2, 1.0, f, st));
+    V3D_CHECK(stokes.self_coef(f, lcv, st));
+    V3D_CHECK(forces.S.div_coef(lcv, ten, lct, st));                  // div (S f), low-passed
+    V3D_CHECK(dev_axpby(3 * nv * nc, -1.0, lcv, 1.0, lcu, st));      // u - S f
+    V3D_CHECK(dev_axpby(nv * nc, -1.0, lct, 0.0, lct, st));          // -div (S f)
+    k_fm_to_om<<<gb(3 * nv * nc), 256, 0, st>>>(lcu, 3 * nv, (int)nc, cinfo, y);
+    V3D_CHECK(launch_status());
+    k_fm_to_om<<<gb(nv * nc), 256, 0, st>>>(lct, nv, (int)nc, cinfo, y + 3 * nv * nc);
+    return launch_status();
   }
 
   // AssembleRhsVel (src/InterfacialVelocity.cc:715-787): the explicit
