@@ -367,11 +367,16 @@ int SurfaceG::init(int p_, int f_, long nv_) {
   ng = sh_ngrid(p);
   nc = sh_ncoef(p);
   for (DBuf *b : {&E, &F, &G, &W, &H, &K, &s1}) V3D_CHECK(b->alloc(nv * ng));
-  for (DBuf *b : {&cu, &cv, &nrm, &t1, &t2, &t3}) V3D_CHECK(b->alloc(3 * nv * ng));
+  for (DBuf *b : {&cu, &cv, &nrm}) V3D_CHECK(b->alloc(3 * nv * ng));
+  V3D_CHECK(tt.alloc(9 * nv * ng));
+  t1 = tt.p;
+  t2 = tt.p + 3 * nv * ng;
+  t3 = tt.p + 6 * nv * ng;
   V3D_CHECK(shc.alloc(3 * nv * nc));
   V3D_CHECK(c1.alloc(nv * nc));
   V3D_CHECK(c3.alloc(3 * nv * nc));
-  V3D_CHECK(work.alloc(std::max(shc2grid_work(3 * nv, p, p), grid2shc_work(3 * nv, p, p))));
+  V3D_CHECK(work.alloc(
+      std::max(shc2grid_multi_work(3 * nv, p, p, 3), grid2shc_work(3 * nv, p, p))));
   return upload(box, box_filter(p, f));
 }
 
@@ -384,6 +389,17 @@ int SurfaceG::forward(const double *X, long nf, double *S, hipStream_t st) {
 // the negative of SphericalHarmonics' LegPolyDeriv (d1legTrans files).
 int SurfaceG::backward(const double *S, long nf, int du, int dv, double *X, hipStream_t st) {
   return sht_synth<double>(S, nf, p, p, du, dv, X, work, st, du == 1 ? -1.0 : 1.0);
+}
+
+int SurfaceG::backward_multi(const double *S, long nf, int nvar, const int (*d)[2],
+                             hipStream_t st) {
+  int du[3], dv[3], neg[3];
+  for (int v = 0; v < nvar; ++v) {
+    du[v] = d[v][0];
+    dv[v] = d[v][1];
+    neg[v] = du[v] == 1;  // SHTrans's first latitude derivative (see backward)
+  }
+  return sht_synth_multi<double>(S, nf, p, p, nvar, du, dv, neg, t1, 3 * nv * ng, work, st);
 }
 
 int SurfaceG::low_pass_c(double *field, long nf, double *coef, bool want_grid, hipStream_t st) {
@@ -399,23 +415,22 @@ int SurfaceG::low_pass(double *field, long nf, hipStream_t st) {
 
 int SurfaceG::update(const double *X, hipStream_t st) {
   V3D_CHECK(forward(X, 3 * nv, shc, st));
-  V3D_CHECK(backward(shc, 3 * nv, 1, 0, t1, st));
-  V3D_CHECK(backward(shc, 3 * nv, 0, 1, t2, st));
+  static const int d1[2][2] = {{1, 0}, {0, 1}}, d2[3][2] = {{1, 1}, {2, 0}, {0, 2}};
+  V3D_CHECK(backward_multi(shc, 3 * nv, 2, d1, st));
   k_first_forms<<<gblocks(nv * ng), 256, 0, st>>>(t1, t2, nv, ng, E, F, G, W, cu, cv, nrm);
   V3D_CHECK(launch_status());
-  V3D_CHECK(backward(shc, 3 * nv, 1, 1, t1, st));
-  V3D_CHECK(backward(shc, 3 * nv, 2, 0, t2, st));
-  V3D_CHECK(backward(shc, 3 * nv, 0, 2, t3, st));
+  V3D_CHECK(backward_multi(shc, 3 * nv, 3, d2, st));
   k_second_forms<<<gblocks(nv * ng), 256, 0, st>>>(t1, t2, t3, nrm, E, F, G, W, nv, ng, H, K);
   V3D_CHECK(launch_status());
   V3D_CHECK(low_pass(K, nv, st));
   return low_pass(H, nv, st);
 }
 
+static const int kD1[2][2] = {{1, 0}, {0, 1}};
+
 int SurfaceG::grad_c(const double *c, double *out, double *cout, bool want_grid,
                      hipStream_t st) {
-  V3D_CHECK(backward(c, nv, 1, 0, t1, st));
-  V3D_CHECK(backward(c, nv, 0, 1, t2, st));
+  V3D_CHECK(backward_multi(c, nv, 2, kD1, st));
   k_grad_comb<<<gblocks(nv * ng), 256, 0, st>>>(t1, t2, cu, cv, nv, ng, out);
   V3D_CHECK(launch_status());
   return low_pass_c(out, 3 * nv, cout, want_grid, st);
@@ -427,16 +442,14 @@ int SurfaceG::grad(const double *fin, double *out, hipStream_t st) {
 }
 
 int SurfaceG::div_c(const double *c, double *out, hipStream_t st) {
-  V3D_CHECK(backward(c, 3 * nv, 1, 0, t1, st));
-  V3D_CHECK(backward(c, 3 * nv, 0, 1, t2, st));
+  V3D_CHECK(backward_multi(c, 3 * nv, 2, kD1, st));
   k_div_comb<<<gblocks(nv * ng), 256, 0, st>>>(t1, t2, cu, cv, nv, ng, out);
   V3D_CHECK(launch_status());
   return low_pass(out, nv, st);
 }
 
 int SurfaceG::div_coef(const double *c, double *out, double *cout, hipStream_t st) {
-  V3D_CHECK(backward(c, 3 * nv, 1, 0, t1, st));
-  V3D_CHECK(backward(c, 3 * nv, 0, 1, t2, st));
+  V3D_CHECK(backward_multi(c, 3 * nv, 2, kD1, st));
   k_div_comb<<<gblocks(nv * ng), 256, 0, st>>>(t1, t2, cu, cv, nv, ng, out);
   V3D_CHECK(launch_status());
   return low_pass_c(out, nv, cout, false, st);
@@ -448,9 +461,8 @@ int SurfaceG::div(const double *V, double *out, hipStream_t st) {
 }
 
 int SurfaceG::lin_mean_curv_c(const double *c, double *out, double *cout, hipStream_t st) {
-  V3D_CHECK(backward(c, 3 * nv, 2, 0, t1, st));
-  V3D_CHECK(backward(c, 3 * nv, 1, 1, t2, st));
-  V3D_CHECK(backward(c, 3 * nv, 0, 2, t3, st));
+  static const int d[3][2] = {{2, 0}, {1, 1}, {0, 2}};
+  V3D_CHECK(backward_multi(c, 3 * nv, 3, d, st));
   k_lin_mean_curv<<<gblocks(nv * ng), 256, 0, st>>>(t1, t2, t3, nrm, E, F, G, nv, ng, out);
   V3D_CHECK(launch_status());
   return low_pass_c(out, nv, cout, true, st);

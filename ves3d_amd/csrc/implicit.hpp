@@ -113,7 +113,12 @@ class SurfaceG {
   int p = 0, f = 0;
   long nv = 0, ng = 0, nc = 0;
   DBuf E, F, G, W, cu, cv, nrm, H, K;
-  DBuf shc, t1, t2, t3, s1, work, box;
+  DBuf shc, s1, work, box;
+  DBuf tt;  // t1, t2, t3: consecutive [3 nv][ng] slices (grouped syntheses write all three)
+  double *t1 = nullptr, *t2 = nullptr, *t3 = nullptr;
+  // backward of S (nf functions) for nvar (du, dv) pairs in one grouped
+  // synthesis, into t1, t2, t3 in order
+  int backward_multi(const double *S, long nf, int nvar, const int (*d)[2], hipStream_t st);
   DBuf c1, c3;  // coefficient buffers of the chains: nv and 3 nv functions
 };
 

@@ -216,15 +216,18 @@ void build_sh_mats(int p0, int p1, ShMatsHost &h) {
     long src = 0;
     h.Ml.resize(p0 + 1);
     h.Mdl.resize(p0 + 1);
+    h.Mdln.resize(p0 + 1);
     h.Md2l.resize(p0 + 1);
     for (int m = 0; m <= p0; ++m) {
       long n = (long)(p0 + 1 - m) * (p1 + 1);
       h.Ml[m] = alloc(n);
       h.Mdl[m] = alloc(n);
+      h.Mdln[m] = alloc(n);
       h.Md2l[m] = alloc(n);
       for (long i = 0; i < n; ++i) {
         D[h.Ml[m] + i] = alp[src + i];
         D[h.Mdl[m] + i] = dlp[src + i];
+        D[h.Mdln[m] + i] = -dlp[src + i];
         D[h.Md2l[m] + i] = d2lp[src + i];
       }
       src += n;

@@ -37,6 +37,7 @@ struct ShMatsHost {
   long MfT;                     // MatFourier(p1,p0): [2p1][2p0] (read transposed by SHC2GridTranspose)
   long Md2f;                    // second longitude derivative of MatFourier: [2p0][2p1]
   std::vector<long> Ml, Mdl;    // MatLegendre(p0,p1)[m], MatLegendreGrad: [(p0+1-m)][(p1+1)], m=0..p0
+  std::vector<long> Mdln;       // -MatLegendreGrad (SHTrans's first latitude derivative), same shape
   std::vector<long> Md2l;       // second latitude derivative (ladder of d2legTrans), same shape
   std::vector<long> Mlinv;      // MatLegendreInv(p0,p1)[m]: [(p0+1)][(p1+1-m)], m=0..p1
   std::vector<long> MlT;        // MatLegendre(p1,p0)[m]: [(p1+1-m)][(p0+1)], m=0..p1

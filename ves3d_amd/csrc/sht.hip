@@ -123,7 +123,91 @@ int run(int kind, int p0, int p1, long N, const ShMatsDev *m, const T *A, T *C,
 
 bool valid_order(int p) { return p >= 1 && p <= 255; }
 
+// plans of sht_synth_multi: the Legendre and the Fourier stage of every
+// variant, keyed by (device, p0, p1, N, variants, x_stride)
+struct MultiPlan {
+  Plan leg, four;
+};
+std::map<std::tuple<int, int, int, long, int, long>, MultiPlan> g_mplans;
+
+Plan finish_plan(std::vector<GemmDesc> &v) {
+  Plan p;
+  p.n = (int)v.size();
+  for (auto &d : v) {
+    p.max_m = std::max(p.max_m, d.M);
+    p.max_n = std::max(p.max_n, d.N);
+  }
+  p.tile = gemm_pick_tile(v.data(), p.n);
+  p.d = upload_descs(v.data(), p.n);
+  return p;
+}
+
+const MultiPlan *get_multi_plan(int p0, int p1, long N, int nvar, const int *dleg,
+                                const int *dfour, const int *neg, long x_stride,
+                                const ShMatsHost &h) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  int code = nvar;
+  for (int v = 0; v < nvar; ++v)
+    code = code * 32 + dleg[v] * 8 + dfour[v] * 2 + (neg[v] ? 1 : 0);
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto key = std::make_tuple(dev, p0, p1, N, code, x_stride);
+  auto it = g_mplans.find(key);
+  if (it != g_mplans.end()) return &it->second;
+  const int nc0 = sh_ncoef(p0);
+  const long slab = shc2grid_work(N, p0, p1);
+  std::vector<GemmDesc> leg, four;
+  auto desc = [](int M, int Nn, int K, long ao, long ars, long acs, long bo, long brs, long bcs,
+                 long co, long crs, long ccs) {
+    GemmDesc d{};
+    d.M = M; d.N = Nn; d.K = K;
+    d.a_off = ao; d.a_rs = ars; d.a_cs = acs;
+    d.b_off = bo; d.b_rs = brs; d.b_cs = bcs;
+    d.c_off = co; d.c_rs = crs; d.c_cs = ccs;
+    return d;
+  };
+  for (int v = 0; v < nvar; ++v) {
+    for (int j = 0; j < 2 * p0; ++j) {
+      const int m = (j + 1) / 2;
+      const long bo = dleg[v] == 0   ? h.Ml[m]
+                      : dleg[v] == 1 ? (neg[v] ? h.Mdln[m] : h.Mdl[m])
+                                     : h.Md2l[m];
+      leg.push_back(desc((int)N, p1 + 1, sh_blocklen(p0, j), sh_blockoff(p0, j), nc0, 1, bo,
+                         p1 + 1, 1, v * slab + (long)j * N * (p1 + 1), p1 + 1, 1));
+    }
+    four.push_back(desc((int)(N * (p1 + 1)), 2 * p1, 2 * p0, v * slab, 1, N * (p1 + 1),
+                        dfour[v] == 0 ? h.Mf : (dfour[v] == 1 ? h.Mdf : h.Md2f), 2 * p1, 1,
+                        v * x_stride, 2 * p1, 1));
+  }
+  MultiPlan mp;
+  mp.leg = finish_plan(leg);
+  mp.four = finish_plan(four);
+  if (!mp.leg.d || !mp.four.d) return nullptr;
+  return &(g_mplans[key] = mp);
+}
+
 }  // namespace
+
+template <typename T>
+int sht_synth_multi(const T *S, long N, int p0, int p1, int nvar, const int *dleg,
+                    const int *dfour, const int *neg, T *X, long x_stride, T *work,
+                    hipStream_t st) {
+  if (!valid_order(p0) || !valid_order(p1) || N < 0 || nvar < 1 || nvar > 4 ||
+      x_stride < N * sh_ngrid(p1))
+    return VES3D_INVALID_PARAMETER;
+  for (int v = 0; v < nvar; ++v)
+    if (dleg[v] < 0 || dleg[v] > 2 || dfour[v] < 0 || dfour[v] > 2 || (neg[v] && dleg[v] != 1))
+      return VES3D_INVALID_PARAMETER;
+  if (N == 0) return VES3D_SUCCESS;
+  const ShMatsDev *m = sh_mats(p0, p1);
+  if (!m) return VES3D_MEMORY_ERROR;
+  const MultiPlan *mp = get_multi_plan(p0, p1, N, nvar, dleg, dfour, neg, x_stride, *m->h);
+  if (!mp) return VES3D_MEMORY_ERROR;
+  V3D_CHECK(gemm_grouped<T>(mp->leg.d, mp->leg.n, 1, mp->leg.max_m, mp->leg.max_n, S,
+                            mats_ptr<T>(m), work, T(1), T(0), st, mp->leg.tile));
+  return gemm_grouped<T>(mp->four.d, mp->four.n, 1, mp->four.max_m, mp->four.max_n, work,
+                         mats_ptr<T>(m), X, T(1), T(0), st, mp->four.tile);
+}
 
 template <typename T>
 int sht_shc2grid(const T *S, long N, int p0, int p1, T *X, T *Xt, T *Xp, T *work,
@@ -247,6 +331,8 @@ int sht_shc2pole(const T *S, long N, int p0, T *P, hipStream_t st) {
   template int sht_shc2grid_transpose<T>(const T *, long, int, int, T *, T *, hipStream_t); \
   template int sht_shc2pole<T>(const T *, long, int, T *, hipStream_t);                 \
   template int sht_synth<T>(const T *, long, int, int, int, int, T *, T *, hipStream_t, T); \
+  template int sht_synth_multi<T>(const T *, long, int, int, int, const int *, const int *,    \
+                                  const int *, T *, long, T *, hipStream_t);                   \
   template const T *sht_mft<T>(int, int);                                                  \
   template const T *sht_mlt<T>(int, int);
 V3D_INST(double)
