@@ -364,9 +364,16 @@ __global__ __launch_bounds__(384) void k_pole_transpose(
     const double *__restrict__ mft, const double *__restrict__ mlt, int layer,
     double *__restrict__ K1) {
   typedef double v4d __attribute__((ext_vector_type(4)));
-  constexpr int L = 2 * P1, Nlat = P1 + 1, M1 = L * Nlat, KS = L / 4;
+  // FOLD (p1 = 32): longitudes lon and lon + L/2 differ by pi, where the
+  // Fourier functions of order m change sign by (-1)^m, so a wave forms the
+  // 16 columns of one parity of m from Kv[lon] +- Kv[lon + L/2], lon < L/2:
+  // half the MFMA steps.  Wave tiles: the two parities (FOLD), else 16
+  // consecutive columns.
+  constexpr bool FOLD = P1 == 32;
+  constexpr int L = 2 * P1, Nlat = P1 + 1, M1 = L * Nlat, KS = FOLD ? L / 8 : L / 4;
   extern __shared__ double lds[];
-  const int F = 2 * p0, NT = (F + 15) / 16, FP = 16 * NT, Nc = p0 * (p0 + 2);
+  const int F = 2 * p0, FP = 16 * ((F + 15) / 16), NT = FOLD ? 2 : FP / 16,
+            Nc = p0 * (p0 + 2);
   const int nmlt = (p0 + 1) * (p0 + 2) / 2 * Nlat;
   double *Kv = lds;                 // [R][L][kPtKs]
   double *Ts = Kv + R * L * kPtKs;  // [R][12][FP]
@@ -385,12 +392,20 @@ __global__ __launch_bounds__(384) void k_pole_transpose(
   const bool fw = wv < R * NT;
   const int frow = fw ? wv / NT : 0, tile = fw ? wv % NT : 0;
   const int li = lane & 15, lk = lane >> 4;
+  // the wave's output column for lane li (-1: a padding column)
+  int fcol = 16 * tile + li;
+  if constexpr (FOLD) {
+    fcol = -1;
+    for (int cc = 0, seen = 0; cc < F; ++cc)
+      if ((((cc + 1) / 2) & 1) == tile) {
+        if (seen == li) fcol = cc;
+        ++seen;
+      }
+  }
+  const bool fcv = fw && fcol >= 0 && fcol < F;
   double mf[KS];
 #pragma unroll
-  for (int s = 0; s < KS; ++s) {
-    const int col = 16 * tile + li;
-    mf[s] = (fw && col < F) ? mft[(long)col * L + 4 * s + lk] : 0.0;
-  }
+  for (int s = 0; s < KS; ++s) mf[s] = fcv ? mft[(long)fcol * L + 4 * s + lk] : 0.0;
   // Legendre role: component c, frequency row jj of the order-p0 layout,
   // whose block starts at off (after block 0 of length p0 + 1, the blocks
   // of order m' = 1 .. p0 come in pairs of length p0 + 1 - m')
@@ -463,7 +478,11 @@ __global__ __launch_bounds__(384) void k_pole_transpose(
         double a[KS];
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
-          const double v = kv[4 * s * kPtKs];
+          double v = kv[4 * s * kPtKs];
+          if constexpr (FOLD) {
+            const double w = kv[(4 * s + L / 2) * kPtKs];
+            v = tile ? v - w : v + w;
+          }
           a[s] = li < 12 ? v : 0.0;
         }
 #pragma unroll
@@ -472,8 +491,10 @@ __global__ __launch_bounds__(384) void k_pole_transpose(
           e = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s + 1], mf[s + 1], e, 0, 0, 0);
         }
         d += e;
+        if (fcol >= 0) {
 #pragma unroll
-        for (int i = 0; i < 3; ++i) Ts[(frow * 12 + lk + 4 * i) * FP + 16 * tile + li] = d[i];
+          for (int i = 0; i < 3; ++i) Ts[(frow * 12 + lk + 4 * i) * FP + fcol] = d[i];
+        }
       }
       __syncthreads();  // T of the pass is in
       if (lt) {
@@ -627,7 +648,9 @@ PoleShape pole_fused_shape(int p0, int p1) {
   // (the build's upsampled order 2 p0)
   const bool small = p0 <= 8 && p1 == 16;
   if (!small && !(p0 <= 16 && p1 == 32)) return sh;
-  const int L = 2 * p1, F = 2 * p0, NT = (F + 15) / 16, FP = 16 * NT;
+  // MFMA wave tiles per row: the two parities of m at p1 = 32 (the kernel's
+  // FOLD), else 16 consecutive columns
+  const int L = 2 * p1, F = 2 * p0, FP = 16 * ((F + 15) / 16), NT = small ? FP / 16 : 2;
   const long Nc = sh_ncoef(p0);
   int dev = 0, maxlds = 0, cus = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
