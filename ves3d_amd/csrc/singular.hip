@@ -244,9 +244,11 @@ int rotate_all(const T *S, long Nv, int p, int dof, T *out, T *work, hipStream_t
   return launch_status();
 }
 
-// RotateTranspose: in/out [Nv][p+1][p][dof][Ncoef]; work: 2 * that size.
+// RotateTranspose: in/out [Nv][p+1][p][dof][Ncoef]; work: 2 * that size
+// (that size when gathered: the input rows are already in degree-major order).
 template <typename T>
-int rotate_transpose(const T *in, long Nv, int p, int dof, T *out, T *work, hipStream_t st) {
+int rotate_transpose(const T *in, long Nv, int p, int dof, T *out, T *work, hipStream_t st,
+                     bool gathered) {
   const RotMatsDev *r = rot_mats(p);
   if (!r) return VES3D_MEMORY_ERROR;
   const auto *pl = rot_plan(p, dof, true, *r->h);
@@ -254,9 +256,14 @@ int rotate_transpose(const T *in, long Nv, int p, int dof, T *out, T *work, hipS
   const int nc = sh_ncoef(p);
   const long rows = Nv * (p + 1) * (long)p * dof;
   if (rows / 4 >= 0x7fffffffL) return VES3D_INVALID_PARAMETER;
-  T *Ap = work, *Bp = work + rows * nc;
-  k_gather_coef<T><<<row_blocks(rows), 256, 0, st>>>(in, rows, nc, r->perm, Ap);
-  V3D_CHECK(launch_status());
+  const T *Ap = in;
+  T *Bp = work;
+  if (!gathered) {
+    k_gather_coef<T><<<row_blocks(rows), 256, 0, st>>>(in, rows, nc, r->perm, work);
+    V3D_CHECK(launch_status());
+    Ap = work;
+    Bp = work + rows * nc;
+  }
   V3D_CHECK(gemm_grouped<T>(pl->d, pl->n, (int)Nv, p * dof, 2 * p, Ap,
                             rot_ptr<T>(r), Bp, T(1), T(0), st, pl->tile));
   const size_t stage = 4 * nc * sizeof(T);
@@ -323,13 +330,15 @@ __global__ void k_pole_kernel(const T *__restrict__ X, const T *__restrict__ Xt,
 }
 
 // The pole kernel fused with SHC2GridTranspose(p1 -> p0) of its values:
-// K1[r][t*6+c][k] = sum over the grid (lat, lon) of the kernel value of
+// K1[r][t*6+c][dpos[k]] = sum over the grid (lat, lon) of the kernel value of
 // rotation r, pole t, tensor component c at (lat, lon), times MatFourier(p0,
 // p1)[jj][lon] (the Fourier stage) and MatLegendre(p0, p1)[m][n][lat] (the
 // Legendre stage), k = blockoff(p0, jj) + n, m = (jj + 1) / 2 -- the products
 // the two grouped GEMMs of sht_shc2grid_transpose form, without the kernel
 // values (12 functions on the upsampled grid per rotation) or the Fourier
-// stage's output passing through HBM.
+// stage's output passing through HBM.  dpos (RotateTranspose's degree-major
+// position of each coefficient) lays the rows out as RotateTranspose's
+// per-degree products read them, so its gather pass is not needed.
 // Persistent workgroups walk the rotations r = blockIdx.x + i gridDim.x and,
 // within a rotation, the latitude rows in passes of R rows:
 //   1. thread (row, lon, t) evaluates pole t's 6 kernel values at (lat, lon)
@@ -362,7 +371,7 @@ __global__ __launch_bounds__(384) void k_pole_transpose(
     const double *__restrict__ X, const double *__restrict__ Xt, const double *__restrict__ Xp,
     const double *__restrict__ trg, const double *__restrict__ qw, long nrot, int p0, int R,
     const double *__restrict__ mft, const double *__restrict__ mlt, int layer,
-    double *__restrict__ K1) {
+    const int *__restrict__ dpos, double *__restrict__ K1) {
   typedef double v4d __attribute__((ext_vector_type(4)));
   // FOLD (p1 = 32): longitudes lon and lon + L/2 differ by pi, where the
   // Fourier functions of order m change sign by (-1)^m, so a wave forms the
@@ -519,7 +528,7 @@ __global__ __launch_bounds__(384) void k_pole_transpose(
       if (lt && c >= c0 && c < c0 + cs) {
 #pragma unroll
         for (int n = 0; n < LMAX; ++n)
-          if (n < len) St[(c - c0) * Nc + off + n] = acc[n];
+          if (n < len) St[(c - c0) * Nc + dpos[off + n]] = acc[n];
       }
       __syncthreads();
       const int nrow = min(cs, 12 - c0);
@@ -750,14 +759,15 @@ int singular_build(const double *X0, long Nv, int p0, int p1, T *SL, T *DL, hipS
         if (fused) {
           const double *mft = sht_mft<double>(p1, p0);
           const double *mlt = sht_mlt<double>(p1, p0);
-          if (!mft || !mlt) return VES3D_MEMORY_ERROR;
+          const RotMatsDev *rm = rot_mats(p0);
+          if (!mft || !mlt || !rm) return VES3D_MEMORY_ERROR;
           const unsigned nb = (unsigned)std::min<long>(rot, ps.blocks);
           if (ps.small)
             k_pole_transpose<9, 16><<<nb, ps.threads, ps.lds, st>>>(
-                X, Xt, Xp, P, qw, rot, p0, ps.R, mft, mlt, layer, K1);
+                X, Xt, Xp, P, qw, rot, p0, ps.R, mft, mlt, layer, rm->iperm, K1);
           else
             k_pole_transpose<17, 32><<<nb, ps.threads, ps.lds, st>>>(
-                X, Xt, Xp, P, qw, rot, p0, ps.R, mft, mlt, layer, K1);
+                X, Xt, Xp, P, qw, rot, p0, ps.R, mft, mlt, layer, rm->iperm, K1);
           V3D_CHECK(launch_status());
         }
       }
@@ -767,7 +777,7 @@ int singular_build(const double *X0, long Nv, int p0, int p1, T *SL, T *DL, hipS
         V3D_CHECK(launch_status());
         V3D_CHECK(sht_shc2grid_transpose<T>(K0, rot * 12, p1, p0, K1, w, st));
       }
-      V3D_CHECK(rotate_transpose<T>(K1, cv, p0, 12, K2, w, st));
+      V3D_CHECK(rotate_transpose<T>(K1, cv, p0, 12, K2, w, st, fused));
       const long ra_blocks = cv * 9L * cdiv(Nc, kRaK) * cdiv(Ng, kRaG);
       if (ra_blocks > 0x7fffffffL) return VES3D_INVALID_PARAMETER;
       k_singular_rearrange<T><<<(unsigned)ra_blocks, 256, 0, st>>>(K2, cv, p0, M1a);
@@ -838,7 +848,8 @@ int self_apply(const TM *SL, const double *F, const TM *DL, const double *G, lon
 }
 
 template int rotate_all<double>(const double *, long, int, int, double *, double *, hipStream_t);
-template int rotate_transpose<double>(const double *, long, int, int, double *, double *, hipStream_t);
+template int rotate_transpose<double>(const double *, long, int, int, double *, double *,
+                                       hipStream_t, bool);
 template int singular_build<double>(const double *, long, int, int, double *, double *, hipStream_t);
 template int singular_build<float>(const double *, long, int, int, float *, float *, hipStream_t);
 template int self_apply<double>(const double *, const double *, const double *, const double *,
@@ -865,7 +876,7 @@ int ves3d_hip_rotate_transpose_f64(const double *S_in, size_t N, int p, int dof,
   const long nc = sh_ncoef(p);
   double *w = (double *)workspace(kWsSing1, sizeof(double) * 2L * N * (p + 1) * p * dof * nc);
   if (!w) return VES3D_MEMORY_ERROR;
-  return rotate_transpose<double>(S_in, (long)N, p, dof, S_out, w, (hipStream_t)stream);
+  return rotate_transpose<double>(S_in, (long)N, p, dof, S_out, w, (hipStream_t)stream, false);
 }
 int ves3d_hip_pole_kernel_shape(int p0, int p1, int *rows_host, int *threads_host,
                                 int *workgroups_host) {
