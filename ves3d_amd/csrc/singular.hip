@@ -591,6 +591,141 @@ __global__ __launch_bounds__(256) void k_singular_rearrange(const T *__restrict_
   }
 }
 
+// The rearrangement above and Grid2SHC(p0 -> p0) of its rows in one pass
+// (FP64, p0 <= 16): out[v][a Nc + k][b Nc + c] = Grid2SHC of the grid function
+// M1(v, a, k, b)[lat][lon] that k_singular_rearrange writes (the same K2
+// element per grid point), then the Fourier and Legendre stages of
+// sht_grid2shc (src/SphericalHarmonics.cc:151-235).  A workgroup takes
+// (v, a, b) and kRgK consecutive k.  In rounds of four latitudes, wave w forms
+//   Fs[w][k][j] = sum_lon M1(k)[lat][lon] MatFourierInv[lon][j],  lat = l0 + w,
+// on v_mfma_f64_16x16x4f64 (lane l: A(l % 16, l / 16), B(l / 16, l % 16),
+// results at rows l / 16 + 4 i, column l % 16), its A operand read from K2
+// (16 consecutive k of one grid point: a 128-byte run per 16 lanes, the next
+// round's loaded during this one) and MatFourierInv in registers; after a
+// barrier every thread adds the round's Legendre terms
+//   Fs[u][k][j_c] MatLegendreInv[m_c][lat][n_c]
+// to its outputs (k, c), k = k0 + wave + 4 r, c = lane + 64 q.  Neither the
+// rearranged grid values nor the Fourier sums pass through HBM.
+constexpr int kRgK = 16;
+constexpr int kRgMl = 17 * 153;  // MatLegendreInv doubles at p0 = 16
+__global__ __launch_bounds__(256, 3) void k_rearrange_g2s(const double *__restrict__ K2,
+                                                          long Nv, int p0,
+                                                          const double *__restrict__ mfinv,
+                                                          const double *__restrict__ mlinv,
+                                                          double *__restrict__ out) {
+  typedef double v4d __attribute__((ext_vector_type(4)));
+  __shared__ double Fs[4][kRgK][33];
+  __shared__ double Ml[kRgMl];
+  const int Nc = p0 * (p0 + 2), Nlat = p0 + 1, L = 2 * p0;
+  const int nml = Nlat * (p0 + 1) * (p0 + 2) / 2;
+  const int nkt = (Nc + kRgK - 1) / kRgK;
+  long bid = blockIdx.x;
+  const int kt = (int)(bid % nkt);
+  bid /= nkt;
+  const int ab = (int)(bid % 9);
+  const long v = bid / 9;
+  if (v >= Nv) return;
+  const int a = ab / 3, b = ab % 3;
+  const int lo = a < b ? a : b, hi = a < b ? b : a;
+  const int c6 = lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
+  const int k0 = kt * kRgK;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  for (int i = tid; i < nml; i += 256) Ml[i] = mlinv[i];
+  // B operand: MatFourierInv[lon = 4 s + lk][j = 16 ct + li]
+  double bf[2][8];
+#pragma unroll
+  for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int lon = 4 * s + lk, j = 16 * ct + li;
+      bf[ct][s] = (lon < L && j < L) ? mfinv[lon * L + j] : 0.0;
+    }
+  // outputs c = lane + 64 q: block column j, MatLegendreInv offset of (m, n)
+  // at latitude 0, and the block's row length (degrees)
+  int cj[5], cb[5], cl[5];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    const int c = lane + 64 * q;
+    cj[q] = -1;
+    cb[q] = 0;
+    cl[q] = 0;
+    for (int jj = 0, off = 0; jj < L && c < Nc; ++jj) {
+      const int m = (jj + 1) / 2, len = p0 + 1 - m;
+      if (c < off + len) {
+        cj[q] = jj;
+        cl[q] = len;
+        cb[q] = Nlat * (m * (p0 + 1) - m * (m - 1) / 2) + (c - off);
+        break;
+      }
+      off += len;
+    }
+  }
+  double acc[4][5];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int q = 0; q < 5; ++q) acc[r][q] = 0.0;
+  const double *kv2 = K2 + v * (long)(p0 + 1) * p0 * 12 * Nc;
+  const int kk = k0 + li;
+  const bool kvalid = kk < Nc;
+  // A operand of latitude lat: M1(kk)[lat][4 s + lk]
+  double xa[8];
+  auto load = [&](int lat) {
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int lon = 4 * s + lk;
+      double x = 0.0;
+      if (kvalid && lon < L && lat < Nlat) {
+        const bool t2 = lon >= p0;
+        const int j1 = t2 ? p0 - lat : lat, j2 = t2 ? lon - p0 : lon;
+        x = kv2[(((long)j1 * p0 + j2) * 12 + (t2 ? 6 : 0) + c6) * Nc + kk];
+      }
+      xa[s] = x;
+    }
+  };
+  load(w);
+  __syncthreads();  // Ml in
+  for (int l0 = 0; l0 < Nlat; l0 += 4) {
+    if (l0 + w < Nlat) {
+      v4d d0 = {0.0, 0.0, 0.0, 0.0}, d1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        d0 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[s], bf[0][s], d0, 0, 0, 0);
+        d1 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[s], bf[1][s], d1, 0, 0, 0);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        Fs[w][lk + 4 * i][li] = d0[i];
+        Fs[w][lk + 4 * i][16 + li] = d1[i];
+      }
+    }
+    load(l0 + 4 + w);
+    __syncthreads();  // the round's Fourier sums are in
+    const int nu = min(4, Nlat - l0);
+    for (int u = 0; u < nu; ++u) {
+      const int lat = l0 + u;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        if (cj[q] < 0) continue;
+        const double ml = Ml[cb[q] + lat * cl[q]];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[r][q] += Fs[u][w + 4 * r][cj[q]] * ml;
+      }
+    }
+    __syncthreads();  // Fs read before the next round writes it
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int k = k0 + w + 4 * r;
+    if (k >= Nc) continue;
+    double *o = out + ((((v * 3 + a) * (long)Nc + k) * 3 + b) * Nc);
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+      if (cj[q] >= 0) o[lane + 64 * q] = acc[r][q];
+  }
+}
+
 template <typename T>
 __global__ void k_copy_cast_grid(const double *__restrict__ in, long n, T *__restrict__ out) {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -778,6 +913,19 @@ int singular_build(const double *X0, long Nv, int p0, int p1, T *SL, T *DL, hipS
         V3D_CHECK(sht_shc2grid_transpose<T>(K0, rot * 12, p1, p0, K1, w, st));
       }
       V3D_CHECK(rotate_transpose<T>(K1, cv, p0, 12, K2, w, st, fused));
+      if constexpr (std::is_same<T, double>::value) {
+        if (fused) {  // rearrangement and Grid2SHC in one pass
+          const ShMatsDev *sm = sh_mats(p0, p0);
+          if (!sm) return VES3D_MEMORY_ERROR;
+          const long g_blocks = cv * 9L * cdiv(Nc, kRgK);
+          if (g_blocks > 0x7fffffffL) return VES3D_INVALID_PARAMETER;
+          k_rearrange_g2s<<<(unsigned)g_blocks, 256, 0, st>>>(
+              K2, cv, p0, sm->d64 + sm->h->Mfinv, sm->d64 + sm->h->Mlinv[0],
+              dst + v0 * 9 * Nc * Nc);
+          V3D_CHECK(launch_status());
+          continue;
+        }
+      }
       const long ra_blocks = cv * 9L * cdiv(Nc, kRaK) * cdiv(Ng, kRaG);
       if (ra_blocks > 0x7fffffffL) return VES3D_INVALID_PARAMETER;
       k_singular_rearrange<T><<<(unsigned)ra_blocks, 256, 0, st>>>(K2, cv, p0, M1a);
