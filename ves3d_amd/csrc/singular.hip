@@ -595,8 +595,10 @@ __global__ __launch_bounds__(256) void k_singular_rearrange(const T *__restrict_
 // (FP64, p0 <= 16): out[v][a Nc + k][b Nc + c] = Grid2SHC of the grid function
 // M1(v, a, k, b)[lat][lon] that k_singular_rearrange writes (the same K2
 // element per grid point), then the Fourier and Legendre stages of
-// sht_grid2shc (src/SphericalHarmonics.cc:151-235).  A workgroup takes
-// (v, a, b) and kRgK consecutive k.  In rounds of four latitudes, wave w forms
+// sht_grid2shc (src/SphericalHarmonics.cc:151-235).  M1(v, a, k, b) depends on
+// (a, b) only through the symmetric tensor component c6(a, b), so a workgroup
+// takes (v, c6) and kRgK consecutive k and writes the rows of (a, b) and
+// (b, a).  In rounds of four latitudes, wave w forms
 //   Fs[w][k][j] = sum_lon M1(k)[lat][lon] MatFourierInv[lon][j],  lat = l0 + w,
 // on v_mfma_f64_16x16x4f64 (lane l: A(l % 16, l / 16), B(l / 16, l % 16),
 // results at rows l / 16 + 4 i, column l % 16), its A operand read from K2
@@ -622,12 +624,10 @@ __global__ __launch_bounds__(256, 3) void k_rearrange_g2s(const double *__restri
   long bid = blockIdx.x;
   const int kt = (int)(bid % nkt);
   bid /= nkt;
-  const int ab = (int)(bid % 9);
-  const long v = bid / 9;
+  const int c6 = (int)(bid % 6);  // xx, xy, xz, yy, yz, zz
+  const long v = bid / 6;
   if (v >= Nv) return;
-  const int a = ab / 3, b = ab % 3;
-  const int lo = a < b ? a : b, hi = a < b ? b : a;
-  const int c6 = lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
+  const int a = c6 < 3 ? 0 : (c6 < 5 ? 1 : 2), b = c6 < 3 ? c6 : (c6 < 5 ? c6 - 2 : 2);
   const int k0 = kt * kRgK;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
@@ -720,9 +720,13 @@ __global__ __launch_bounds__(256, 3) void k_rearrange_g2s(const double *__restri
     const int k = k0 + w + 4 * r;
     if (k >= Nc) continue;
     double *o = out + ((((v * 3 + a) * (long)Nc + k) * 3 + b) * Nc);
+    double *o2 = out + ((((v * 3 + b) * (long)Nc + k) * 3 + a) * Nc);
 #pragma unroll
     for (int q = 0; q < 5; ++q)
-      if (cj[q] >= 0) o[lane + 64 * q] = acc[r][q];
+      if (cj[q] >= 0) {
+        o[lane + 64 * q] = acc[r][q];
+        if (a != b) o2[lane + 64 * q] = acc[r][q];
+      }
   }
 }
 
@@ -917,7 +921,7 @@ int singular_build(const double *X0, long Nv, int p0, int p1, T *SL, T *DL, hipS
         if (fused) {  // rearrangement and Grid2SHC in one pass
           const ShMatsDev *sm = sh_mats(p0, p0);
           if (!sm) return VES3D_MEMORY_ERROR;
-          const long g_blocks = cv * 9L * cdiv(Nc, kRgK);
+          const long g_blocks = cv * 6L * cdiv(Nc, kRgK);
           if (g_blocks > 0x7fffffffL) return VES3D_INVALID_PARAMETER;
           k_rearrange_g2s<<<(unsigned)g_blocks, 256, 0, st>>>(
               K2, cv, p0, sm->d64 + sm->h->Mfinv, sm->d64 + sm->h->Mlinv[0],
