@@ -227,3 +227,32 @@ def test_singular_build_fused_pole_stage(hip, p0):
     e_sl, e_dl = rel(out[1][0], out[0][0]), rel(out[1][1], out[0][1])
     print("fused vs separate pole stage (p0=%d): SL %.2e, DL %.2e" % (p0, e_sl, e_dl), flush=True)
     assert e_sl < tol and e_dl < tol
+
+
+@pytest.mark.parametrize("p0,fused,dt", [(16, 1, torch.float64), (16, 0, torch.float64),
+                                         (12, 1, torch.float64), (16, 1, torch.float32)])
+def test_singular_operator_block_symmetry(hip, p0, fused, dt):
+    """The built SL and DL operators' 3 x 3 blocks of Ncoef x Ncoef satisfy
+    B_ab = B_ba exactly (block (a, b) holds the rows of the symmetric tensor
+    component c(a, b)), which the step's self apply relies on to read only
+    the blocks a <= b.  Fused FP64 build, separate FP64 build, and the FP32
+    build (p0 = 12 takes the separate path whatever the setting)."""
+    from ves3d_amd import call
+    nv = 2
+    nc = p0 * (p0 + 2)
+    X = _singular_case(nv, p0, seed=43)
+    try:
+        call("ves3d_hip_set_pole_kernel", fused)
+        sl = torch.zeros((nv, 3 * nc, 3 * nc), dtype=dt, device="cuda")
+        dl = torch.zeros((nv, 3 * nc, 3 * nc), dtype=dt, device="cuda")
+        G.singular_build(cu(X), nv, p0, 2 * p0, sl, dl)
+        torch.cuda.synchronize()
+    finally:
+        call("ves3d_hip_set_pole_kernel", 1)
+    for M in (sl.cpu().numpy(), dl.cpu().numpy()):
+        assert np.max(np.abs(M)) > 0
+        for a in range(3):
+            for b in range(a + 1, 3):
+                B_ab = M[:, a * nc:(a + 1) * nc, b * nc:(b + 1) * nc]
+                B_ba = M[:, b * nc:(b + 1) * nc, a * nc:(a + 1) * nc]
+                assert np.array_equal(B_ab, B_ba), (a, b)

@@ -861,16 +861,18 @@ int StokesG::density_coef(const double *f, double *Fc, hipStream_t st, const dou
 int StokesG::self_velocity(bool sl, const double *Fc, double *Vc, double *vel, bool pack,
                            hipStream_t st, bool dl) {
   V3D_CHECK(wait_build(st));
+  // the operators come from singular_build, whose (a, b) and (b, a) blocks
+  // are equal: the apply reads the six blocks a <= b (sym)
   // precision 2, the self-interaction alone (an inner solve's operator, a
   // preconditioner's): the FP32 copy halves the HBM-bound apply
   if (precision == 2 && !pack && !dl)
-    V3D_CHECK(self_apply<float>(sl ? SL32 : nullptr, Fc, nullptr, G.p, nv, p, Vc, st));
+    V3D_CHECK(self_apply<float>(sl ? SL32 : nullptr, Fc, nullptr, G.p, nv, p, Vc, st, true));
   else if (precision != 1)
     V3D_CHECK(self_apply<double>(sl ? SL.p : nullptr, Fc, dl ? DL.p : nullptr, G.p, nv, p, Vc,
-                                 st));
+                                 st, true));
   else
     V3D_CHECK(self_apply<float>(sl ? SL32 : nullptr, Fc, dl ? DL32 : nullptr, G.p, nv, p, Vc,
-                                st));
+                                st, true));
   return vel ? sht_shc2grid<double>(Vc, 3 * nv, p, p, vel, nullptr, nullptr, work, st)
              : VES3D_SUCCESS;
 }

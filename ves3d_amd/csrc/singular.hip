@@ -986,14 +986,65 @@ __global__ __launch_bounds__(256) void k_self_apply(const TM *__restrict__ SL,
   V[v * n + c] = (acc0 + acc1) + (acc2 + acc3);
 }
 
+// As k_self_apply for the singular build's operators, whose 3 x 3 blocks of
+// Ncoef x Ncoef satisfy B_ab = B_ba (block (a, b) holds the rows of the
+// symmetric tensor component c(a, b), formed once and written to both):
+// thread cc reads the six blocks a <= b at column cc once and forms
+// V[b Nc + cc] = sum_a sum_k F[a Nc + k] B_ab[k][cc] for b = 0, 1, 2 --
+// two thirds of the general kernel's matrix reads.
+template <typename TM>
+__global__ __launch_bounds__(512) void k_self_apply_sym(const TM *__restrict__ SL,
+                                                         const double *__restrict__ F,
+                                                         const TM *__restrict__ DL,
+                                                         const double *__restrict__ G, int nc,
+                                                         double *__restrict__ V) {
+  extern __shared__ double sh[];
+  const int n = 3 * nc;
+  const long v = blockIdx.y;
+  const int cc = blockIdx.x * blockDim.x + threadIdx.x;
+  double *f = sh, *g = sh + n;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    f[i] = SL ? F[v * n + i] : 0.0;
+    g[i] = DL ? G[v * n + i] : 0.0;
+  }
+  __syncthreads();
+  if (cc >= nc) return;
+  double v0 = 0, v1 = 0, v2 = 0;
+  auto run = [&](const TM *M, const double *x) {
+    const TM *m = M + v * (long)n * n + cc;  // element (row, b nc + cc) at row n + b nc
+#pragma unroll 2
+    for (int k = 0; k < nc; ++k) {
+      const TM *r0 = m + (long)k * n, *r1 = m + (long)(nc + k) * n,
+               *r2 = m + (long)(2 * nc + k) * n;
+      const double b00 = (double)r0[0], b01 = (double)r0[nc], b02 = (double)r0[2 * nc];
+      const double b11 = (double)r1[nc], b12 = (double)r1[2 * nc], b22 = (double)r2[2 * nc];
+      const double x0 = x[k], x1 = x[nc + k], x2 = x[2 * nc + k];
+      v0 += x0 * b00 + x1 * b01 + x2 * b02;
+      v1 += x0 * b01 + x1 * b11 + x2 * b12;
+      v2 += x0 * b02 + x1 * b12 + x2 * b22;
+    }
+  };
+  if (SL) run(SL, f);
+  if (DL) run(DL, g);
+  V[v * n + cc] = v0;
+  V[v * n + nc + cc] = v1;
+  V[v * n + 2 * nc + cc] = v2;
+}
+
 template <typename TM>
 int self_apply(const TM *SL, const double *F, const TM *DL, const double *G, long Nv, int p,
-               double *V, hipStream_t st) {
+               double *V, hipStream_t st, bool sym) {
   if (p < 1 || Nv < 0 || Nv > 65535) return VES3D_INVALID_PARAMETER;
   if (Nv == 0) return VES3D_SUCCESS;
   if (!V || (SL && !F) || (DL && !G)) return VES3D_INVALID_PARAMETER;
   const int n = 3 * sh_ncoef(p);
   if (!SL && !DL) return hip_status(hipMemsetAsync(V, 0, sizeof(double) * n * Nv, st));
+  if (sym) {
+    const int nc = sh_ncoef(p), bs = 64 * (int)cdiv(std::min(nc, 512), 64);
+    dim3 g2(cdiv(nc, bs), (unsigned)Nv);
+    k_self_apply_sym<TM><<<g2, bs, 2 * n * sizeof(double), st>>>(SL, F, DL, G, nc, V);
+    return launch_status();
+  }
   dim3 grid(cdiv(n, 256), (unsigned)Nv);
   k_self_apply<TM><<<grid, 256, 2 * n * sizeof(double), st>>>(SL, F, DL, G, n, V);
   return launch_status();
@@ -1005,9 +1056,9 @@ template int rotate_transpose<double>(const double *, long, int, int, double *, 
 template int singular_build<double>(const double *, long, int, int, double *, double *, hipStream_t);
 template int singular_build<float>(const double *, long, int, int, float *, float *, hipStream_t);
 template int self_apply<double>(const double *, const double *, const double *, const double *,
-                                long, int, double *, hipStream_t);
+                                long, int, double *, hipStream_t, bool);
 template int self_apply<float>(const float *, const double *, const float *, const double *,
-                               long, int, double *, hipStream_t);
+                               long, int, double *, hipStream_t, bool);
 
 }  // namespace v3d
 
@@ -1056,11 +1107,11 @@ int ves3d_hip_singular_build_f32(const double *X0, size_t nves, int p0, int p1, 
 int ves3d_hip_self_apply_f64(const double *SL, const double *F, const double *DL,
                              const double *G, size_t nves, int p, double *V,
                              ves3d_stream_t stream) {
-  return self_apply<double>(SL, F, DL, G, (long)nves, p, V, (hipStream_t)stream);
+  return self_apply<double>(SL, F, DL, G, (long)nves, p, V, (hipStream_t)stream, false);
 }
 int ves3d_hip_self_apply_f32(const float *SL, const double *F, const float *DL,
                              const double *G, size_t nves, int p, double *V,
                              ves3d_stream_t stream) {
-  return self_apply<float>(SL, F, DL, G, (long)nves, p, V, (hipStream_t)stream);
+  return self_apply<float>(SL, F, DL, G, (long)nves, p, V, (hipStream_t)stream, false);
 }
 }
