@@ -13,8 +13,13 @@
 //   Grid2SHC -> RotateAll -> SHC2Grid(p0 -> p1, +derivatives) & SHC2Pole
 //   -> pole kernel (FP64) -> SHC2GridTranspose(p1 -> p0) -> RotateTranspose
 //   -> rearrange -> Grid2SHC = [Nves][3 Ncoef][3 Ncoef]
-// The FP32 form runs every GEMM stage in FP32 and keeps the pole kernel
-// (singular weights, 1/r near the rotated pole) in FP64.
+// The FP64 build of p1 = 2 p0 <= 32 fuses the pole kernel with
+// SHC2GridTranspose and RotateTranspose's gather (k_pole_transpose), and the
+// rearrangement with the last Grid2SHC (k_rearrange_g2s).  The FP32 form runs
+// every GEMM stage in FP32 and keeps the pole kernel (singular weights, 1/r
+// near the rotated pole) in FP64.  Both write the operator's (a, b) and
+// (b, a) blocks equal (the rows of one symmetric tensor component), which
+// the step's self apply relies on (k_self_apply_sym).
 #include <algorithm>
 #include <atomic>
 #include <cmath>
